@@ -1,0 +1,290 @@
+/* tectonic.h -- C ABI of libtectonic, the MI355X (gfx950) implementation of the
+ * per-step terrain pipeline of MarWo22/master_thesis.
+ *
+ * The reference has no FFI layer: its boundary for this path is the C++ class
+ * PlateTectonicSim (src/cuda_compute/plate_tectonic_sim.h:56-179) plus the
+ * global settings objects it reads (KernelSettings, kernel_settings.cuh:4-81;
+ * SimulationSettings, generation_settings.h:67-76; Config, config.h:15-29).
+ * Every entry point below names the reference interface it replaces.
+ *
+ * Conventions: plain C types only; every function returns an int status
+ * (TT_OK == 0, negative on error) and never exits the process (the reference
+ * calls exit() from CUDA_ERROR_CHECK, plate_tectonic_sim.h:40-54).  The caller
+ * owns host buffers; the library owns device memory.  One world handle may be
+ * used by one thread at a time; different handles may be used concurrently.
+ */
+#ifndef TECTONIC_H
+#define TECTONIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+enum {
+    TT_OK = 0,
+    TT_ERR_INVALID = -1,      /* bad argument (null pointer, bad size, bad enum) */
+    TT_ERR_DEVICE = -2,       /* HIP runtime error (message in tt_last_error) */
+    TT_ERR_NOMEM = -3,        /* device allocation failed */
+    TT_ERR_STATE = -4,        /* call out of order (e.g. step before init) */
+    TT_ERR_UNSUPPORTED = -5,  /* feature or size not supported */
+    TT_ERR_BUDGET = -6        /* serving admission: request exceeds the HBM budget */
+};
+
+/* ---- parameters --------------------------------------------------------- */
+
+/* Mirrors struct KernelSettings (src/cuda_compute/kernel_settings.cuh:4-81):
+ * same 50 fields, same order, same names and defaults (tt_settings_default).
+ * The reference keeps these in __constant__ memory that is only written when
+ * the GUI changes a slider (kernel_settings.cu:7-9, gui.cpp:434-441); this
+ * library always uses the values passed here. */
+typedef struct tt_kernel_settings {
+    float inelasticCollisionMultiplierContinental;
+    float inelasticCollisionMultiplierSubduction;
+    float inelasticCollisionMultiplierAccretion;
+    float frictionCoefficientContinental;
+    float frictionCoefficientSubduction;
+    float frictionCoefficientAccretion;
+    float environmentalDragCoefficient;
+    float velocitySmoothingFactor;
+    float directionSmoothingFactor;
+    float breakScoreSmoothingFactor;
+    float mergeDotDirectionThreshold;
+    float mergeVelocityDiffThreshold;
+    float mergeMinVelocity;
+    int32_t mergeWaitAfterSplit;
+    int32_t minPlateSize;
+    float continentalCrustThreshold;
+    float divergence_height_target_min;
+    float divergence_height_target_max;
+    float divergence_interpolation_factor;
+    int32_t collisionTypeUpdateCooldown;
+    int32_t minCollisionSize;
+    float subductionOverUpliftMultiplier;
+    float subductionUnderUpliftMultiplier;
+    float continentalUpliftMultiplier;
+    float minUpliftClamp;
+    float maxUpliftClamp;
+    int32_t upliftRange;
+    int32_t subductionOverRange;
+    int32_t continentalPropMaxIters;
+    float continentalPropThreshold;
+    float hydrationPipeCrossSection;
+    float hydrationPipeLength;
+    float hydrationRainfall;
+    float hydrationEvaporation;
+    float gravity;
+    float sedimentCapacity;
+    float sedimentErosionRate;
+    float sedimentDepositionRate;
+    float pressureAccumulation;
+    float pressureDecayRate;
+    int32_t pressureBlurRange;
+    float pressureMultiplier;
+    float curlNoiseMultiplier;
+    float stressSplitThreshold;
+    int32_t targetMinimumPlateArea;
+    int32_t targetMaximumPlateArea;
+    float thermalErosionStrength;
+    float thermalErosionAmplitude;
+    float thermalCellSize;
+    float thermalThresholdAngle;
+} tt_kernel_settings;
+
+/* How the initial state is produced (untimed). */
+enum {
+    /* BASELINE-style synthetic world: heights U[0,1) smoothed by a toroidal
+     * Gaussian (sigma 8, radius 32, fp64), plates = nearest of num_plates
+     * uniform sites, speeds U[speed_min, speed_max]. */
+    TT_INIT_SYNTHETIC = 0,
+    /* the reference's initializeTectonics (plate_tectonic_sim.cu:705-773):
+     * simplex fBm heights in [5000, 15000] (initHeightmap,
+     * plate_tectonics_kernel.cu:68-99), nested Voronoi seed generations, speed
+     * (U(-1,1)+1)/2; the MSVC RNG stream is replaced by SplitMix64. */
+    TT_INIT_REFERENCE = 1,
+    /* all fields zero; the caller uploads a state with tt_world_upload. */
+    TT_INIT_EMPTY = 2
+};
+
+/* Stage mask bits (tests may switch stages off to localise a difference;
+ * the default TT_STAGE_ALL is the reference step). */
+enum {
+    TT_STAGE_COLLIDE = 1 << 0,   /* advection, collision typing and resolution */
+    TT_STAGE_UPLIFT = 1 << 1,    /* continental propagation, JFA, applyUplift */
+    TT_STAGE_THERMAL = 1 << 2,
+    TT_STAGE_HYDRAULIC = 1 << 3,
+    TT_STAGE_KINEMATICS = 1 << 4, /* applyPlateMovementChanges + applyAccretion */
+    TT_STAGE_CCL = 1 << 5,
+    TT_STAGE_MERGE = 1 << 6,     /* mergePlates + break score + centres + stats */
+    TT_STAGE_SPLIT = 1 << 7,
+    TT_STAGE_ALL = 0xff
+};
+
+/* World geometry and schedule.  Replaces Config (config.h:15-29:
+ * heightmap_dimensions) and SimulationSettings (generation_settings.h:67-76:
+ * seed, numStartingPlates, numVoronoiSeeds, iterationsForCCL), plus explicit
+ * fields for counts the reference hard-codes (10 hydraulic sub-iterations at
+ * plate_tectonic_sim.cu:336, one thermal pass at :73). */
+typedef struct tt_sim_config {
+    int32_t width;                 /* multiple of 64, >= 64 */
+    int32_t height;                /* multiple of 4, >= 64 */
+    uint32_t seed;
+    int32_t num_plates;            /* 1..254 */
+    int32_t num_voronoi_gens;      /* TT_INIT_REFERENCE only, 0..4 */
+    int32_t voronoi_gens[4];
+    int32_t iterations_for_ccl;    /* CCL when step % iterations_for_ccl == 0 */
+    int32_t hydraulic_iterations;  /* default 10 */
+    int32_t thermal_iterations;    /* default 1 */
+    int32_t init_mode;             /* TT_INIT_* */
+    float speed_min;               /* TT_INIT_SYNTHETIC plate speeds */
+    float speed_max;
+    int32_t stage_mask;            /* TT_STAGE_* */
+    int32_t reserved[7];
+} tt_sim_config;
+
+/* struct PlateData (src/cuda_compute/types/plate_data.h:5-38), 84 bytes, same
+ * field order and layout; the plate table has 255 entries (MAX_PLATE_COUNT,
+ * plate_tectonics_kernel.cuh:13). */
+typedef struct tt_plate_data {
+    float velocity;
+    float velocitySmoothed;
+    float direction[2];
+    float directionSmoothed[2];
+    float pixelCenter[2];
+    float velocityChange[2];
+    float geometricCenter[2];
+    float asthenosphereVelocity[2];
+    int32_t size;
+    float mass;
+    int32_t perimeter;
+    float breakScore;
+    float circularity;
+    uint8_t used;
+    uint8_t hasMoved;
+    uint8_t pad_[2];
+    int32_t mergeWaitTime;
+} tt_plate_data;
+
+#define TT_MAX_PLATES 255
+
+/* Fields that can be moved between host and device.  Row-major, index y*W+x. */
+enum {
+    TT_FIELD_HEIGHT = 0,        /* float[N]        m_heightMapTexture */
+    TT_FIELD_PLATE_IDS = 1,     /* uint8[N]        m_plateIdsTexture */
+    TT_FIELD_WATER = 2,         /* float[N]        m_hydrationLevel */
+    TT_FIELD_FLUX = 3,          /* float4[N]       m_hydrationFlux (x=-x, y=+y, z=+x, w=-y) */
+    TT_FIELD_VELOCITY = 4,      /* float2[N]       m_hydrationVelocity */
+    TT_FIELD_SEDIMENT = 5,      /* float[N]        m_sedimentLevel */
+    TT_FIELD_ASTHENOSPHERE = 6, /* float2[N]       m_asthenosphere (read-only) */
+    TT_FIELD_COLLISIONS = 7,    /* uint32[N]       m_plateCollisions of the last step */
+    TT_FIELD_ACCRETION = 8,     /* uint8[N]        m_accretionTexture of the last step */
+    TT_FIELD_UPLIFT = 9,        /* {float, uint8, pad[3]}[N]   m_upliftData */
+    TT_FIELD_PROPAGATED = 10,   /* {float, uint32, uint8, pad[3]}[N] m_propagatedUpliftData */
+    TT_FIELD_PLATE_DATA = 11,   /* tt_plate_data[255] m_plateDataLookup */
+    TT_FIELD_DIVERGENCE_BITMAP = 12, /* uint32[1013]  m_divergenceBitmap */
+    TT_FIELD_COLLISION_TYPES = 13,   /* uint8[32385]  m_collisionTypeBitmap */
+    TT_FIELD_SORTED_KEYS = 14,  /* uint64[N] sorted (dest << 8 | plate) keys of the last step */
+    TT_FIELD_LABELS = 15,       /* uint32[N] CCL labels (min cell index per component) of the last CCL */
+    TT_FIELD_DISTANCE = 16,     /* {int32 dist, uint32 origIndex}[N] JFA distance field of the last step */
+    TT_FIELD_COUNT_
+};
+
+/* Per-step statistics (replaces IterationStatistics, iteration_statistics.h,
+ * and the per-step stdout prints of the reference). */
+typedef struct tt_step_stats {
+    uint32_t iteration;          /* m_iterations after the step */
+    int32_t largest_value;       /* IterationStatistics */
+    int32_t largest_plate_id;
+    float heaviest_value;
+    int32_t heaviest_plate_id;
+    int32_t number_of_plates;    /* plates with size > 0 */
+    int32_t overflow_cells;      /* cells reached by > 4 plates (4 smallest ids kept) */
+    int32_t num_labels;          /* CCL components in the last CCL */
+    int32_t unassigned_cells;    /* cells filled by neighbour vote in the last CCL */
+    int32_t fill_rounds;
+    int32_t merges;              /* merge pairs accepted in the last step */
+    int32_t split_plate;         /* plate split in the last step, -1 if none */
+    int32_t split_new_plate;
+    int32_t split_iterations;    /* relaxation sweeps until convergence */
+    int32_t complex_sources;     /* continental sources handled in canonical serial order */
+    int32_t reserved[3];
+    double mass_ledger;          /* sum of height over the grid, exact fixed-point, fp64 */
+} tt_step_stats;
+
+typedef struct tt_world tt_world;
+
+/* ---- library -------------------------------------------------------------- */
+const char *tt_status_string(int status);
+/* Text of the last error on the calling thread. */
+const char *tt_last_error(void);
+/* Device count visible to the library (0 on a host without GPU). */
+int tt_device_count(int *count);
+
+/* KernelSettings defaults, kernel_settings.cuh:6-71. */
+int tt_settings_default(tt_kernel_settings *out);
+/* SimulationSettings / Config defaults (generation_settings.h:67-76,
+ * config.json:1-6): 1024x1024, seed 1000, 16 plates, gens {200, 500}, CCL every
+ * 10 steps, 10 hydraulic and 1 thermal iteration, reference init. */
+int tt_config_default(tt_sim_config *out);
+/* Parse config.json / generation-settings JSON text into the two structs.
+ * Recognised keys: heightmap_dimensions.{x,y} (config.json:3-5), seed,
+ * numStartingPlates, numVoronoiSeeds, iterationsForCCL, hydraulicIterations,
+ * thermalIterations, and any KernelSettings field by name. */
+int tt_config_parse_json(const char *text, tt_sim_config *cfg, tt_kernel_settings *ks);
+
+/* ---- world (replaces PlateTectonicSim) ---------------------------------- */
+/* PlateTectonicSim(int width, int height, unsigned seed, CudaGlInteropManager*)
+ * plate_tectonic_sim.h:111; allocates device state for one world on the
+ * current device. */
+int tt_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, tt_world **out);
+/* initialize(numStartingPlates, numVoronoiSeeds), plate_tectonic_sim.h:115 */
+int tt_world_init(tt_world *w);
+/* resetSim(seed, numStartingPlates, numVoronoiSeeds), plate_tectonic_sim.h:119 */
+int tt_world_reset(tt_world *w, uint32_t seed, int32_t num_plates);
+/* executeIteration() x n, plate_tectonic_sim.h:117; returns after the device
+ * work is complete (the reference synchronises at sim.cu:117-119). */
+int tt_world_step(tt_world *w, int32_t n);
+/* Upload of new KernelSettings (copyHostKernelSettingsToDevice, kernel_settings.cu:7-9). */
+int tt_world_set_settings(tt_world *w, const tt_kernel_settings *ks);
+int tt_world_download(tt_world *w, int32_t field, void *host, size_t bytes);
+int tt_world_upload(tt_world *w, int32_t field, const void *host, size_t bytes);
+int tt_world_stats(tt_world *w, tt_step_stats *out);
+int tt_world_iteration(tt_world *w, uint32_t *iteration);
+/* Device bytes held by the world (persistent + transient), for admission. */
+int tt_world_bytes(const tt_sim_config *cfg, size_t *bytes);
+int tt_world_destroy(tt_world *w);
+/* Headless export (replaces saveFloatCudaTextureToDiskAsGray16 and
+ * saveGrayscale8BitCudaTextureToDiskAsRgb, texture_save.cuh:12-18): a 16-bit
+ * grey PNG of height normalised to [minUpliftClamp, maxUpliftClamp], or the
+ * RGB plate palette.  Untimed. */
+int tt_world_export_png(tt_world *w, int32_t what, const char *path);
+enum { TT_EXPORT_HEIGHT16 = 0, TT_EXPORT_PLATES_RGB = 1 };
+
+/* ---- timing (hipEvent around the stages of the last tt_world_step) -------- */
+typedef struct tt_stage_times {
+    float ms_total;
+    float ms_collide;
+    float ms_uplift;
+    float ms_thermal;
+    float ms_hydraulic;
+    float ms_post;     /* kinematics, accretion, CCL, merge, stats */
+    float ms_split;
+    float reserved[9];
+} tt_stage_times;
+int tt_world_set_timing(tt_world *w, int32_t enabled);
+int tt_world_stage_times(tt_world *w, tt_stage_times *out);
+
+/* ---- device self-test of the deterministic math (no reference counterpart:
+ * used by the parity tests to show device and host math agree bit for bit) -- */
+enum { TT_MATH_EXPF = 0, TT_MATH_LOG10F = 1, TT_MATH_SINF = 2, TT_MATH_COSF = 3,
+       TT_MATH_SIMPLEX = 4, TT_MATH_DISCRETE = 5, TT_MATH_FMODF = 6, TT_MATH_ATAN2 = 7 };
+int tt_math_eval_device(int32_t fn, const float *in, float *out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TECTONIC_H */

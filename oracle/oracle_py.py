@@ -1,0 +1,108 @@
+"""Python handle on the CPU oracle (liboracle.so).  Test infrastructure only:
+loaded by tests/, by bench.py's cpu_baseline leg and by the smoke check, never
+by the library."""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from master_thesis_amd import _abi
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        vp = C.c_void_p
+        L.oracle_world_create.restype = vp
+        L.oracle_world_create.argtypes = [C.POINTER(_abi.SimConfig), C.POINTER(_abi.KernelSettings), C.c_char_p]
+        L.oracle_world_init.argtypes = [vp]
+        L.oracle_world_step.argtypes = [vp, C.c_int]
+        L.oracle_world_get.argtypes = [vp, C.c_int, vp, C.c_size_t]
+        L.oracle_world_set.argtypes = [vp, C.c_int, vp, C.c_size_t]
+        L.oracle_world_set_iteration.argtypes = [vp, C.c_uint32]
+        L.oracle_world_stats.argtypes = [vp, C.POINTER(_abi.StepStats)]
+        L.oracle_world_ccl.argtypes = [vp]
+        L.oracle_world_destroy.argtypes = [vp]
+        L.oracle_math_eval.argtypes = [C.c_int, vp, vp, C.c_longlong]
+        _lib = L
+    return _lib
+
+
+class OracleWorld:
+    def __init__(self, cfg: _abi.SimConfig, ks: _abi.KernelSettings, noise: bytes | None = None):
+        self.cfg = cfg.copy()
+        self.ks = ks.copy()
+        self.n = cfg.width * cfg.height
+        noise = noise if noise is not None else _abi.load_noise()
+        self.h = lib().oracle_world_create(C.byref(self.cfg), C.byref(self.ks), noise)
+        if not self.h:
+            raise RuntimeError("oracle_world_create failed")
+
+    def init(self):
+        self._ok(lib().oracle_world_init(self.h), "init")
+
+    def step(self, n: int = 1):
+        self._ok(lib().oracle_world_step(self.h, n), "step")
+
+    def get(self, field: int) -> np.ndarray:
+        dt, shape = _abi.field_spec(field, self.n)
+        a = np.empty(shape, dtype=dt)
+        self._ok(lib().oracle_world_get(self.h, field, a.ctypes.data, a.nbytes), f"get {field}")
+        return a
+
+    def set(self, field: int, a: np.ndarray):
+        dt, shape = _abi.field_spec(field, self.n)
+        a = np.ascontiguousarray(a, dtype=dt).reshape(shape)
+        self._ok(lib().oracle_world_set(self.h, field, a.ctypes.data, a.nbytes), f"set {field}")
+
+    def set_iteration(self, it: int):
+        self._ok(lib().oracle_world_set_iteration(self.h, it), "set_iteration")
+
+    def stats(self) -> _abi.StepStats:
+        s = _abi.StepStats()
+        self._ok(lib().oracle_world_stats(self.h, C.byref(s)), "stats")
+        return s
+
+    def ccl(self):
+        self._ok(lib().oracle_world_ccl(self.h), "ccl")
+
+    @staticmethod
+    def _ok(st, what):
+        if st != 0:
+            raise RuntimeError(f"oracle {what} failed: {st}")
+
+    def close(self):
+        if self.h:
+            lib().oracle_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def math_eval(fn: int, x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    per = {_abi.TT_MATH_SIMPLEX: 3, _abi.TT_MATH_DISCRETE: 3, _abi.TT_MATH_FMODF: 2, _abi.TT_MATH_ATAN2: 2}.get(fn, 1)
+    n = x.size // per
+    out = np.empty(n, dtype=np.float32)
+    st = lib().oracle_math_eval(fn, x.ctypes.data, out.ctypes.data, n)
+    if st != 0:
+        raise RuntimeError(f"oracle math {fn} failed")
+    return out

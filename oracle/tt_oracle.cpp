@@ -1,0 +1,1742 @@
+// tt_oracle.cpp -- CPU restatement of the reference terrain step (test
+// infrastructure only).
+//
+// This file is the oracle: plain C++17 loops that follow the reference CUDA
+// sources of MarWo22/master_thesis line by line, with every race of the
+// reference replaced by the canonical serial semantics described in the
+// comments (outcome of running the kernel's threads one after another in
+// ascending cell order, with the listed exceptions).  Only tests/, bench.py's
+// cpu_baseline leg and __graft_entry__.smoke() load it, and only as the checker;
+// the product (libtectonic) never links or calls it.
+//
+// It shares two headers with the product: tt_math.h (deterministic exp / log10
+// / sin / atan2 / fmod written from published methods) and tt_noise.h (the
+// cudaNoise hash and simplex formulas).  Both are pinned separately by tests
+// (host libm error bounds; an independent Python restatement of the noise).
+//
+// Parity status per function is listed in DESIGN.md.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <deque>
+#include <queue>
+#include <vector>
+
+#include "../include/tectonic.h"
+#include "../master_thesis_amd/csrc/tt_math.h"
+#include "../master_thesis_amd/csrc/tt_noise.h"
+
+namespace {
+
+constexpr int MAXP = 255;                          // MAX_PLATE_COUNT
+constexpr int NUM_TRI = MAXP * (MAXP - 1) / 2;     // 32385 pair entries
+constexpr int NUM_TRI_WORDS = (NUM_TRI + 31) / 32; // 1013 bitmap words
+constexpr int MASS_BITS = 20;   // fixed-point scale of height sums
+constexpr int ASTH_BITS = 32;   // asthenosphere sums
+constexpr int FRIC_BITS = 48;   // friction sums
+constexpr int ANG_BITS = 40;    // angular (sin/cos) sums
+
+struct UpliftData { float cum; uint8_t types; uint8_t pad[3]; };
+struct PropData { float cum; uint32_t inv; uint8_t types; uint8_t pad[3]; };
+struct DistBuf { int32_t dist; uint32_t orig; };
+struct V2 { float x, y; };
+struct F4 { float x, y, z, w; };
+struct TypeCounts { int subACont, subAOce, subBCont, subBOce, cont; };
+
+// upperTriangleIndexUnchecked (cuda_helper.cuh:98-101) with matrixSize 255
+static inline unsigned tri(unsigned i, unsigned j) { return i * (2u * MAXP - i - 1u) / 2u + (j - i - 1u); }
+
+static inline int wrapc(int v, int n)
+{
+    int r = v % n;
+    return r + (n & (r >> 31));  // CudaTexture::wrapCoordinate, cuda_texture.cuh:67-76
+}
+
+// Vec2 helpers with the reference's operation order (vec2.cuh)
+static inline float mag(V2 v) { return sqrtf(v.x * v.x + v.y * v.y); }
+static inline V2 norm_zs(V2 v)  // normalizedZeroSafe, vec2.cuh:106-112
+{
+    const float m = tt_fmaxf(mag(v), 1e-20f);
+    return {v.x / m, v.y / m};
+}
+
+struct SplitMix {
+    uint64_t s;
+    uint64_t next()
+    {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    float u01() { return (float)((double)(next() >> 40) * (1.0 / 16777216.0)); }
+};
+
+static inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct World {
+    tt_sim_config cfg;
+    tt_kernel_settings ks;
+    int W, H, N;
+    uint32_t iter = 0;
+
+    std::vector<float> height, water, sediment;
+    std::vector<uint8_t> ids;
+    std::vector<F4> flux;
+    std::vector<V2> velocity, asth;
+    std::vector<uint32_t> collisions;
+    std::vector<uint8_t> accretion;
+    std::vector<UpliftData> uplift;
+    std::vector<PropData> prop;
+    std::vector<tt_plate_data> pd;
+    std::vector<uint32_t> divBitmap;
+    std::vector<uint8_t> typeBitmap;
+    std::vector<uint64_t> keys;
+    std::vector<uint32_t> labels;
+    std::vector<DistBuf> dist;
+    tt_step_stats stats;
+    std::vector<uint8_t> noise;  // 256x256 asthenosphere source (R channel of noise.png)
+    int noiseW = 256, noiseH = 256;
+
+    int idx(int x, int y) const { return wrapc(y, H) * W + wrapc(x, W); }
+};
+
+tt_plate_data default_plate()
+{
+    // PlateData() constructor, plate_data.h:26-37
+    tt_plate_data p;
+    memset(&p, 0, sizeof(p));
+    p.breakScore = 1.0f;
+    return p;
+}
+
+// per-plate pixel offset floor(pixelCenter + direction * velocity)
+// (getPixelMovements, plate_tectonics_kernel.cu:255-260)
+static inline void plate_offset(const tt_plate_data &p, int &ox, int &oy)
+{
+    const float mx = p.direction[0] * p.velocity;
+    const float my = p.direction[1] * p.velocity;
+    ox = (int)tt_floorf(p.pixelCenter[0] + mx);
+    oy = (int)tt_floorf(p.pixelCenter[1] + my);
+}
+
+// ---------------------------------------------------------------------------
+// Initialisation (untimed)
+// ---------------------------------------------------------------------------
+
+// asthenosphere field = computeCurl(noise.png R/255) (plate_tectonics_kernel.cu:2788-2822)
+void init_asthenosphere(World &w)
+{
+    const double xs = (double)w.noiseW / (double)w.W;
+    const double ys = (double)w.noiseH / (double)w.H;
+    auto nz = [&](int x, int y) {
+        const int xx = wrapc(x, w.noiseW), yy = wrapc(y, w.noiseH);
+        return (float)w.noise[yy * w.noiseW + xx] / 255.0f;
+    };
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < w.H; ++y)
+        for (int x = 0; x < w.W; ++x) {
+            const int sx = (int)((double)x * xs), sy = (int)((double)y * ys);
+            const float left = nz(sx - 1, sy), right = nz(sx + 1, sy);
+            const float bottom = nz(sx, sy - 1), top = nz(sx, sy + 1);
+            V2 d;
+            d.y = (float)((double)(right - left) * 0.5);
+            d.x = (float)(-((double)(top - bottom) * 0.5));
+            w.asth[(size_t)y * w.W + x] = d;
+        }
+}
+
+// initPlateIDs (plate_tectonics_kernel.cu:31-66): nearest seed on the torus,
+// strict < keeps the first seed.
+void init_plate_ids(World &w, const std::vector<V2> &pos, const std::vector<uint8_t> &sid)
+{
+    const float fw = (float)w.W, fh = (float)w.H;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < w.H; ++y)
+        for (int x = 0; x < w.W; ++x) {
+            float best = 3.402823466e38f;
+            int id = 0;
+            for (size_t i = 0; i < pos.size(); ++i) {
+                const float dx = tt_fabsf((float)x - pos[i].x);
+                const float dy = tt_fabsf((float)y - pos[i].y);
+                const float wx = tt_fminf(dx, fw - dx);
+                const float wy = tt_fminf(dy, fh - dy);
+                const float d2 = wx * wx + wy * wy;
+                if (d2 < best) { best = d2; id = sid[i]; }
+            }
+            w.ids[(size_t)y * w.W + x] = (uint8_t)id;
+        }
+}
+
+// initPixelDependantPlateData (plate_tectonics_kernel.cu:199-230); float
+// atomics replaced by an exact fixed-point sum.
+void init_plate_counts(World &w)
+{
+    std::vector<long long> mass(MAXP, 0);
+    std::vector<int> size(MAXP, 0);
+    for (int i = 0; i < w.N; ++i) {
+        const int p = w.ids[i];
+        if (p >= MAXP) continue;
+        size[p] += 1;
+        mass[p] += tt_fix(w.height[i], MASS_BITS);
+    }
+    for (int p = 0; p < MAXP; ++p) {
+        if (size[p] == 0) continue;
+        w.pd[p].size += size[p];
+        w.pd[p].mass = w.pd[p].mass + tt_unfix(mass[p], MASS_BITS);
+    }
+}
+
+void ccl_and_relabel(World &w);
+
+void init_synthetic(World &w)
+{
+    SplitMix rng{(uint64_t)w.cfg.seed};
+    const int P = w.cfg.num_plates;
+    std::vector<V2> pos(P);
+    std::vector<uint8_t> sid(P);
+    // generatePlateCenters (plate_tectonic_sim.cu:874-890): x drawn over the
+    // height range, y over the width range, as in the reference.
+    for (int i = 0; i < P; ++i) {
+        pos[i].x = rng.u01() * (float)w.H;
+        pos[i].y = rng.u01() * (float)w.W;
+        sid[i] = (uint8_t)i;
+    }
+    // generatePlateData (plate_tectonic_sim.cu:816-842) with a configurable speed range
+    for (int i = 0; i < P; ++i) {
+        tt_plate_data &p = w.pd[i];
+        p.pixelCenter[0] = pos[i].x - tt_floorf(pos[i].x);
+        p.pixelCenter[1] = pos[i].y - tt_floorf(pos[i].y);
+        p.velocity = w.cfg.speed_min + (w.cfg.speed_max - w.cfg.speed_min) * rng.u01();
+        p.velocitySmoothed = p.velocity;
+        const float xd = rng.u01() * 2.0f - 1.0f;
+        const float yd = rng.u01() * 2.0f - 1.0f;
+        const float m = sqrtf(xd * xd + yd * yd);
+        if (m > 0.0f) { p.direction[0] = xd / m; p.direction[1] = yd / m; }
+        else { p.direction[0] = 1.0f; p.direction[1] = 0.0f; }
+        p.directionSmoothed[0] = p.direction[0];
+        p.directionSmoothed[1] = p.direction[1];
+    }
+    // heights: U[0,1) per cell from a counter-based SplitMix64 stream, then a
+    // separable toroidal Gaussian (sigma 8, radius 32) in fp64
+    const uint64_t hseed = mix64((uint64_t)w.cfg.seed ^ 0x6A09E667F3BCC909ull);
+    std::vector<double> raw((size_t)w.N), tmp((size_t)w.N);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < w.N; ++i) {
+        const uint64_t z = mix64(hseed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull);
+        raw[i] = (double)(z >> 40) * (1.0 / 16777216.0);
+    }
+    const int R = 32;
+    double wts[2 * R + 1], tot = 0.0;
+    for (int k = -R; k <= R; ++k) { wts[k + R] = tt_exp_core(-(double)(k * k) / 128.0); tot = tot + wts[k + R]; }
+    for (int k = 0; k <= 2 * R; ++k) wts[k] = wts[k] / tot;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < w.H; ++y)
+        for (int x = 0; x < w.W; ++x) {
+            double s = 0.0;
+            for (int k = -R; k <= R; ++k) s = s + wts[k + R] * raw[(size_t)y * w.W + wrapc(x + k, w.W)];
+            tmp[(size_t)y * w.W + x] = s;
+        }
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < w.H; ++y)
+        for (int x = 0; x < w.W; ++x) {
+            double s = 0.0;
+            for (int k = -R; k <= R; ++k) s = s + wts[k + R] * tmp[(size_t)wrapc(y + k, w.H) * w.W + x];
+            w.height[(size_t)y * w.W + x] = (float)s;
+        }
+    init_plate_ids(w, pos, sid);
+    init_asthenosphere(w);
+    init_plate_counts(w);
+    ccl_and_relabel(w);
+}
+
+// initHeightmap (plate_tectonics_kernel.cu:68-99): 4 octaves of simplex fBm
+void init_reference_heights(World &w)
+{
+    const int seed = (int)w.cfg.seed;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < w.H; ++y)
+        for (int x = 0; x < w.W; ++x) {
+            const float d = 0.001f;
+            float value = 0.0f, amp = 0.9f, freq = 0.01f;
+            for (int o = 0; o < 4; ++o) {
+                const float v = tt_simplex((float)x, (float)y, 0.0f, freq, seed + o);
+                const float vx = tt_simplex((float)x + d, (float)y, 0.0f, freq, seed + o);
+                const float vy = tt_simplex((float)x, (float)y + d, 0.0f, freq, seed + o);
+                V2 der{(vx - v) / d, (vy - v) / d};
+                const float q = v / (1.0f + mag(der));
+                // remap(q, -1, 1, 0, 1) = 0 + (1 - 0) * ((q - -1) / (1 - -1))
+                value = value + (0.0f + 1.0f * ((q - -1.0f) / (1.0f - -1.0f))) * amp;
+                amp = amp * 0.1f;
+                freq = freq * 10.0f;
+            }
+            w.height[(size_t)y * w.W + x] = 5000.0f + value * (15000.0f - 5000.0f);
+        }
+}
+
+void init_reference(World &w)
+{
+    SplitMix rng{(uint64_t)w.cfg.seed};
+    const int P = w.cfg.num_plates;
+    std::vector<V2> cpos(P);
+    std::vector<uint8_t> cid(P);
+    for (int i = 0; i < P; ++i) {
+        cpos[i].x = rng.u01() * (float)w.H;
+        cpos[i].y = rng.u01() * (float)w.W;
+        cid[i] = (uint8_t)i;
+    }
+    std::vector<V2> spos = cpos;
+    std::vector<uint8_t> sid = cid;
+    // generateVoronoiSeeds generations (plate_tectonic_sim.cu:892-928)
+    for (int g = 0; g < w.cfg.num_voronoi_gens; ++g) {
+        const int n = w.cfg.voronoi_gens[g];
+        std::vector<V2> np(n);
+        std::vector<uint8_t> ni(n);
+        for (int i = 0; i < n; ++i) {
+            V2 s;
+            s.x = rng.u01() * (float)w.H;
+            s.y = rng.u01() * (float)w.W;
+            float best = 3.402823466e38f;
+            uint8_t cl = 0;
+            for (size_t j = 0; j < spos.size(); ++j) {
+                const float dx = tt_fabsf(s.x - spos[j].x), dy = tt_fabsf(s.y - spos[j].y);
+                const float wx = tt_fminf(dx, (float)w.W - dx), wy = tt_fminf(dy, (float)w.H - dy);
+                const float d2 = wx * wx + wy * wy;
+                if (d2 < best) { cl = sid[j]; best = d2; }
+            }
+            np[i] = s;
+            ni[i] = cl;
+        }
+        spos.swap(np);
+        sid.swap(ni);
+    }
+    if (spos.size() < cpos.size()) { spos = cpos; sid = cid; }
+    for (int i = 0; i < P; ++i) {
+        tt_plate_data &p = w.pd[i];
+        p.pixelCenter[0] = cpos[i].x - tt_floorf(cpos[i].x);
+        p.pixelCenter[1] = cpos[i].y - tt_floorf(cpos[i].y);
+        p.velocity = ((rng.u01() * 2.0f - 1.0f) + 1.0f) / 2.0f;
+        p.velocitySmoothed = p.velocity;
+        const float xd = rng.u01() * 2.0f - 1.0f;
+        const float yd = rng.u01() * 2.0f - 1.0f;
+        const float m = sqrtf(xd * xd + yd * yd);
+        if (m > 0.0f) { p.direction[0] = xd / m; p.direction[1] = yd / m; }
+        else { p.direction[0] = 1.0f; p.direction[1] = 0.0f; }
+        p.directionSmoothed[0] = p.direction[0];
+        p.directionSmoothed[1] = p.direction[1];
+    }
+    init_plate_ids(w, spos, sid);
+    init_reference_heights(w);
+    init_asthenosphere(w);
+    init_plate_counts(w);
+    ccl_and_relabel(w);
+}
+
+// ---------------------------------------------------------------------------
+// Stage A: advection + (cell, plate) key sort + collision words
+// (getPixelMovements :232-271, registerPlateCollisions :273-298; sort design of
+//  plate_tectonic_sim.cu:188-201).  Canonical slots: ascending plate id, the
+//  four smallest kept.
+// ---------------------------------------------------------------------------
+void stage_advect(World &w, std::vector<int> &offx, std::vector<int> &offy)
+{
+    offx.assign(MAXP, 0);
+    offy.assign(MAXP, 0);
+    for (int p = 0; p < MAXP; ++p) plate_offset(w.pd[p], offx[p], offy[p]);
+    // keys (dest << 8 | plate) for every source cell, then sort
+    std::vector<uint64_t> k((size_t)w.N);
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < w.H; ++y)
+        for (int x = 0; x < w.W; ++x) {
+            const size_t s = (size_t)y * w.W + x;
+            const int p = w.ids[s];
+            const uint64_t d = (uint64_t)w.idx(x + offx[p], y + offy[p]);
+            k[s] = (d << 8) | (uint64_t)p;
+        }
+    std::sort(k.begin(), k.end());
+    w.keys = k;
+    std::fill(w.collisions.begin(), w.collisions.end(), 0u);
+    int overflow = 0;
+    size_t i = 0;
+    while (i < k.size()) {
+        const uint64_t d = k[i] >> 8;
+        int slot = 0;
+        size_t j = i;
+        uint32_t word = 0;
+        while (j < k.size() && (k[j] >> 8) == d) {
+            if (slot < 4) word |= (uint32_t)(0xFFu - (uint32_t)(k[j] & 0xFF)) << (8 * slot);
+            ++slot;
+            ++j;
+        }
+        if (slot > 4) ++overflow;
+        w.collisions[d] = word;
+        i = j;
+    }
+    w.stats.overflow_cells = overflow;
+}
+
+// ---------------------------------------------------------------------------
+// Stage B1/B2: collision typing (determineCollisionType :838-888 +
+// writeCollision :794-836; createCollisionTypeMatrix :890-972).  Integer
+// counts are order-free.
+// ---------------------------------------------------------------------------
+void write_collision(const World &w, float hA, float hB, int pA, int pB, std::vector<TypeCounts> &cnt)
+{
+    const bool cond = pA < pB;
+    const int mn = cond ? pA : pB;
+    const float hmn = cond ? hA : hB;
+    const int mx = cond ? pB : pA;
+    const float hmx = cond ? hB : hA;
+    const unsigned t = tri((unsigned)mn, (unsigned)mx);
+    const uint8_t packed = w.typeBitmap[t];
+    // `packed >> 0 & 1 && packed >> 4 & 0x0F < cooldown` (:813): the shift
+    // result is and-ed with the boolean (0x0F < cooldown).
+    const unsigned cmpbit = (0x0F < w.ks.collisionTypeUpdateCooldown) ? 1u : 0u;
+    if ((packed & 1u) && ((unsigned)(packed >> 4) & cmpbit)) return;
+    const bool minCont = hmn >= w.ks.continentalCrustThreshold;
+    const bool maxCont = hmx >= w.ks.continentalCrustThreshold;
+    if (minCont && maxCont) cnt[t].cont++;
+    else if (minCont) cnt[t].subBCont++;
+    else if (maxCont) cnt[t].subACont++;
+    else if (hmn <= hmx) cnt[t].subAOce++;
+    else cnt[t].subBOce++;
+}
+
+void stage_collision_types(World &w, const std::vector<int> &offx, const std::vector<int> &offy)
+{
+    std::vector<TypeCounts> cnt(NUM_TRI, TypeCounts{0, 0, 0, 0, 0});
+    for (int y = 0; y < w.H; ++y)
+        for (int x = 0; x < w.W; ++x) {
+            const uint32_t cp = w.collisions[(size_t)y * w.W + x];
+            if (cp <= 0xFF) continue;
+            int p[4];
+            for (int s = 0; s < 4; ++s) p[s] = (uint8_t)(MAXP - ((cp >> (8 * s)) & 0xFF));
+            auto hgt = [&](int q) { return w.height[w.idx(x - offx[q], y - offy[q])]; };
+            const float hA = hgt(p[0]), hB = hgt(p[1]);
+            if (p[3] != MAXP) {
+                const float hC = hgt(p[2]), hD = hgt(p[3]);
+                write_collision(w, hA, hB, p[0], p[1], cnt);
+                write_collision(w, hA, hC, p[0], p[2], cnt);
+                write_collision(w, hA, hD, p[0], p[3], cnt);
+                write_collision(w, hB, hC, p[1], p[2], cnt);
+                write_collision(w, hB, hD, p[1], p[3], cnt);
+                write_collision(w, hC, hD, p[2], p[3], cnt);
+            } else if (p[2] != MAXP) {
+                const float hC = hgt(p[2]);
+                write_collision(w, hA, hB, p[0], p[1], cnt);
+                write_collision(w, hA, hC, p[0], p[2], cnt);
+                write_collision(w, hB, hC, p[1], p[2], cnt);
+            } else {
+                write_collision(w, hA, hB, p[0], p[1], cnt);
+            }
+        }
+    // createCollisionTypeMatrix; the pair decode with double sqrt (:900-905)
+    // feeds nothing and is omitted.
+    for (int i = 0; i < NUM_TRI; ++i) {
+        const unsigned packed = w.typeBitmap[i];
+        const bool hasType = packed & 1u;
+        const unsigned count = (packed >> 4) & 0x0Fu;
+        const TypeCounts c = cnt[i];
+        const uint32_t total = (uint32_t)c.subACont + (uint32_t)c.subAOce + (uint32_t)c.subBCont +
+                               (uint32_t)c.subBOce + (uint32_t)c.cont;
+        if (total == 0) continue;
+        if (hasType && (int)count < w.ks.collisionTypeUpdateCooldown) {
+            w.typeBitmap[i] = (uint8_t)((packed & 0x0Fu) | (((count + 1u) & 0x0Fu) << 4));
+            continue;
+        }
+        if (total < (uint32_t)w.ks.minCollisionSize) continue;
+        uint8_t out = (uint8_t)(packed & 0x0Fu);
+        const bool hasPol = (packed >> 2) & 1u;
+        const bool polA = (packed >> 3) & 1u;
+        const int contOppA = c.subBCont + c.cont;
+        const int contOppB = c.subACont + c.cont;
+        const int powA = c.subACont + c.subAOce - contOppA;
+        const int powB = c.subBCont + c.subBOce - contOppB;
+        if (c.cont >= c.subACont + c.subAOce && c.cont >= c.subBCont + c.subBOce) out |= 0x03;
+        else if (powA > 0 && powA > powB && (!hasPol || polA)) out = (uint8_t)((out & ~0x02) | 0x0D);
+        else if (powB > 0 && powB > powA && (!hasPol || !polA)) out = (uint8_t)((out & ~0x0A) | 0x05);
+        w.typeBitmap[i] = out;
+    }
+}
+
+// device determineCollisionType (:611-649)
+unsigned det_type(const World &w, int pa, int pb, float ha, float hb)
+{
+    const bool cond = pa < pb;
+    const int mn = cond ? pa : pb, mx = cond ? pb : pa;
+    const float hmn = cond ? ha : hb, hmx = cond ? hb : ha;
+    const uint8_t packed = w.typeBitmap[tri((unsigned)mn, (unsigned)mx)];
+    const bool hasType = packed & 1, type = (packed >> 1) & 1, pol = (packed >> 3) & 1;
+    unsigned out = 0;
+    if (hasType && type == 0) {
+        if (pol == 1) out = hmn >= w.ks.continentalCrustThreshold ? (mn == pa ? 3 : 4) : (mn == pa ? 1 : 2);
+        if (pol == 0) out = hmx >= w.ks.continentalCrustThreshold ? (mx == pa ? 3 : 4) : (mx == pa ? 1 : 2);
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Stage B3/B4: processCollisions (:677-745) with processDivergence (:301-362),
+// processMovement (:586-609), applyInelasticCollision (:397-467, friction only:
+// the inelastic direction change is never read, :1881,1897), processConvergence
+// (:469-584); flipDivergedBitmap (:772-780).
+// Canonical: accretion marks -> the highest writer cell wins; friction sums are
+// exact fixed-point.
+// ---------------------------------------------------------------------------
+void stage_resolve(World &w, const std::vector<int> &offx, const std::vector<int> &offy,
+                   std::vector<float> &hW, std::vector<uint8_t> &idW, std::vector<long long> &fric)
+{
+    std::vector<uint32_t> hasDiv(NUM_TRI_WORDS, 0u);
+    std::fill(w.accretion.begin(), w.accretion.end(), 0);
+    fric.assign(MAXP, 0);
+    const int seed = (int)w.cfg.seed;
+    static const int pairL[6][2] = {{0, 1}, {0, 2}, {1, 2}, {0, 3}, {1, 3}, {2, 3}};
+    for (int y = 0; y < w.H; ++y)
+        for (int x = 0; x < w.W; ++x) {
+            const size_t c = (size_t)y * w.W + x;
+            const uint32_t cv = w.collisions[c];
+            if (cv == 0) {
+                // divergence
+                const int prev = w.ids[c];
+                const uint32_t ov = w.collisions[w.idx(x - offx[prev], y - offy[prev])];
+                const uint8_t opp = (uint8_t)((MAXP - ov) & 0xFF);
+                int nid;
+                if (ov != 0 && ov <= 0xFF && opp != prev) {
+                    const int mn = prev < opp ? prev : opp, mx = prev < opp ? opp : prev;
+                    const unsigned bit = tri((unsigned)mn, (unsigned)mx);
+                    nid = (w.divBitmap[bit / 32] & (1u << (bit % 32))) ? mn : mx;
+                    hasDiv[bit / 32] |= 1u << (bit % 32);
+                } else {
+                    nid = prev;
+                }
+                const float nz = tt_simplex((float)x, (float)y, (float)w.iter, 0.01f, seed);
+                const float target = w.ks.divergence_height_target_min +
+                                     nz * (w.ks.divergence_height_target_max - w.ks.divergence_height_target_min);
+                hW[c] = tt_lerp(w.height[c], target, w.ks.divergence_interpolation_factor);
+                idW[c] = (uint8_t)nid;
+                continue;
+            }
+            int P[4];
+            for (int s = 0; s < 4; ++s) P[s] = (uint8_t)(MAXP - ((cv >> (8 * s)) & 0xFF));
+            if (P[1] == MAXP) {
+                // movement
+                idW[c] = (uint8_t)P[0];
+                hW[c] = w.height[w.idx(x - offx[P[0]], y - offy[P[0]])];
+                continue;
+            }
+            // convergence
+            tt_plate_data PD[4];
+            PD[0] = w.pd[P[0]];
+            PD[1] = w.pd[P[1]];
+            PD[2] = P[2] != MAXP ? w.pd[P[2]] : default_plate();
+            PD[3] = P[3] != MAXP ? w.pd[P[3]] : default_plate();
+            int ox[4], oy[4];
+            for (int s = 0; s < 4; ++s) plate_offset(PD[s], ox[s], oy[s]);
+            float prevH[4];
+            prevH[0] = w.height[w.idx(x - ox[0], y - oy[0])];
+            prevH[1] = w.height[w.idx(x - ox[1], y - oy[1])];
+            prevH[2] = P[2] != MAXP ? w.height[w.idx(x - ox[2], y - oy[2])] : 0.0f;
+            prevH[3] = P[3] != MAXP ? w.height[w.idx(x - ox[3], y - oy[3])] : 0.0f;
+            unsigned types[6] = {0, 0, 0, 0, 0, 0};
+            types[0] = det_type(w, P[0], P[1], prevH[0], prevH[1]);
+            if (P[2] != MAXP) {
+                types[1] = det_type(w, P[0], P[2], prevH[0], prevH[2]);
+                types[2] = det_type(w, P[1], P[2], prevH[1], prevH[2]);
+                if (P[3] != MAXP) {
+                    types[3] = det_type(w, P[0], P[3], prevH[0], prevH[3]);
+                    types[4] = det_type(w, P[1], P[3], prevH[1], prevH[3]);
+                    types[5] = det_type(w, P[2], P[3], prevH[2], prevH[3]);
+                }
+            }
+            const int nColl = P[3] != MAXP ? 4 : P[2] != MAXP ? 3 : 2;
+            // friction (applyInelasticCollision -> applyElasticVelocityChange)
+            unsigned contB = 0, accB = 0;
+            for (int i = 0; i != nColl - 1; ++i)
+                for (int j = i + 1; j != nColl; ++j) {
+                    const unsigned t = types[(j * (j - 1)) / 2 + i];
+                    contB |= (unsigned)(t == 0) << i;
+                    accB |= (unsigned)(t == 3 || t == 4) << i;
+                    contB |= (unsigned)(t == 0) << j;
+                    accB |= (unsigned)(t == 3 || t == 4) << j;
+                }
+            for (int k = 0; k < nColl; ++k) {
+                const float massDiff = prevH[k] / PD[k].mass;
+                const bool hc = (contB >> k) & 1u, ha = (accB >> k) & 1u;
+                const float coef = hc ? w.ks.frictionCoefficientContinental
+                                      : ha ? w.ks.frictionCoefficientAccretion : w.ks.frictionCoefficientSubduction;
+                const float f = (PD[k].velocity * coef) * massDiff;
+                fric[P[k]] += tt_fix(f, FRIC_BITS);
+            }
+            // processConvergence
+            const int pairLen = nColl == 4 ? 6 : nColl == 3 ? 3 : 1;
+            bool sub[4] = {false, false, false, false};
+            for (int a = 0; a != pairLen; ++a) {
+                const unsigned t = types[a];
+                if (t == 1 || t == 3) sub[pairL[a][0]] = true;
+                else if (t == 2 || t == 4) sub[pairL[a][1]] = true;
+            }
+            int own = 0;
+            for (int i = 0; i != nColl; ++i)
+                if (!sub[i]) own = i;
+            const int newOwner = P[own];
+            float writeH = prevH[own];
+            float add = 0.0f;
+            unsigned typeByte = 0;
+            for (int a = 0; a != pairLen; ++a) {
+                const int ia = pairL[a][0], ib = pairL[a][1];
+                float h;
+                if (ia == own) h = prevH[ib];
+                else if (ib == own) h = prevH[ia];
+                else continue;
+                const unsigned t = types[a];
+                if (h > writeH) { const float tmp = h; h = writeH; writeH = tmp; }
+                add = add + h;
+                const uint8_t accVal = (uint8_t)(MAXP - newOwner);
+                if (ia == own && t == 4) {
+                    const int bx = x - ox[ib], by = y - oy[ib];
+                    if (bx != x || by != y) w.accretion[w.idx(bx, by)] = accVal;
+                    else w.accretion[w.idx(x + ox[ia], y + oy[ia])] = accVal;
+                } else if (ib == own && t == 3) {
+                    const int axx = x - ox[ia], ayy = y - oy[ia];
+                    if (axx != x || ayy != y) w.accretion[w.idx(axx, ayy)] = accVal;
+                    else w.accretion[w.idx(x + ox[ib], y + oy[ib])] = accVal;
+                }
+                if (ia == own) {
+                    const unsigned wv = t == 2 ? 0u : 1u;
+                    typeByte |= wv & 1u;
+                    typeByte |= (unsigned)(ia & 3) << 1;
+                    typeByte |= (unsigned)(ib & 3) << 3;
+                } else if (ib == own) {
+                    const unsigned wv = t == 1 ? 0u : 1u;
+                    typeByte |= wv & 1u;
+                    typeByte |= (unsigned)(ib & 3) << 1;
+                    typeByte |= (unsigned)(ia & 3) << 3;
+                }
+            }
+            hW[c] = writeH;
+            w.uplift[c].cum = add;
+            w.uplift[c].types = (uint8_t)typeByte;
+            idW[c] = (uint8_t)newOwner;
+        }
+    for (int i = 0; i < NUM_TRI_WORDS; ++i) w.divBitmap[i] ^= hasDiv[i];
+}
+
+// ---------------------------------------------------------------------------
+// Stage C1/C2: moveContinentalCollisions (:1087-1185) with
+// continentalPropagation (:975-1027), atomicInsertPlatePair (:1029-1069),
+// countCollision (:1071-1084); combinePropagatedCollisions (:1187-1218).
+// Canonical: source cells in ascending order, each with its program order.
+// ---------------------------------------------------------------------------
+struct IV2 { int x, y; bool operator==(const IV2 &o) const { return x == o.x && y == o.y; } };
+
+IV2 cont_prop(const World &w, IV2 pos, const float dir[2], const float pc[2], float startH)
+{
+    if (startH < w.ks.continentalPropThreshold) return pos;
+    const float px = (float)pos.x + pc[0], py = (float)pos.y + pc[1];
+    const float ndx = -dir[0], ndy = -dir[1];
+    const int stepX = ndx > 0 ? 1 : -1;
+    const int stepY = ndy > 0 ? 1 : -1;
+    const float tDeltaX = tt_fabsf(1.0f / ndx);
+    const float tDeltaY = tt_fabsf(1.0f / ndy);
+    const int nextX = stepX > 0 ? pos.x + 1 : pos.x;
+    const int nextY = stepY > 0 ? pos.y + 1 : pos.y;
+    float tMaxX = ((float)nextX - px) / ndx;
+    float tMaxY = ((float)nextY - py) / ndy;
+    IV2 prev = pos;
+    for (int i = 0; i < w.ks.continentalPropMaxIters; ++i) {
+        if (tMaxX < tMaxY) { pos.x += stepX; tMaxX = tMaxX + tDeltaX; }
+        else { pos.y += stepY; tMaxY = tMaxY + tDeltaY; }
+        const float ch = w.height[w.idx(pos.x, pos.y)];
+        if (ch < w.ks.continentalPropThreshold) return pos;
+        prev = pos;
+    }
+    return prev;
+}
+
+bool insert_pair(PropData &pr, int a, int b)
+{
+    uint32_t ins[2] = {(uint32_t)(MAXP - a), (uint32_t)(MAXP - b)};
+    const uint32_t old = pr.inv;
+    uint32_t nw = old;
+    const uint32_t plates[4] = {old & 0xFF, (old >> 8) & 0xFF, (old >> 16) & 0xFF, (old >> 24) & 0xFF};
+    for (int i = 0; i != 4; ++i)
+        for (int k = 0; k < 2; ++k) {
+            if (plates[i] == 0 && ins[k] != 0) { nw |= ins[k] << (i * 8); ins[k] = 0; break; }
+            if (plates[i] == ins[k]) ins[k] = 0;
+        }
+    if (ins[0] != 0 || ins[1] != 0) return false;
+    pr.inv = nw;
+    return true;
+}
+
+void count_coll(IV2 p, IV2 *pos, int *cnt, int &total, int inc)
+{
+    for (int i = 0; i != 6; ++i)
+        if (cnt[i] == 0 || pos[i] == p) { pos[i] = p; cnt[i] += inc; total += inc; break; }
+}
+
+void stage_propagate(World &w)
+{
+    for (auto &p : w.prop) p = PropData{0.0f, 0u, 0, {0, 0, 0}};
+    int complexSources = 0;
+    for (int y = 0; y < w.H; ++y)
+        for (int x = 0; x < w.W; ++x) {
+            const size_t c = (size_t)y * w.W + x;
+            const uint32_t cp = w.collisions[c];
+            if ((cp >> 8) == 0) continue;
+            const UpliftData U = w.uplift[c];
+            if ((U.types & 1u) != 1u) continue;
+            const int primIdx = (U.types >> 1) & 3;
+            const int prim = MAXP - (int)((cp >> (8 * primIdx)) & 0xFF);
+            const tt_plate_data &PP = w.pd[prim];
+            IV2 pos[6];
+            int cnt[6] = {0, 0, 0, 0, 0, 0};
+            int total = 0;
+            const float ch = w.height[c];
+            const IV2 here{x, y};
+            const IV2 pp = cont_prop(w, here, PP.direction, PP.pixelCenter, ch);
+            bool remote = !(pp == here);
+            for (int ci = 0; ci != 4; ++ci) {
+                const int sec = (uint8_t)(MAXP - ((cp >> (8 * ci)) & 0xFF));
+                if (sec == MAXP || sec == prim) continue;
+                const tt_plate_data &SP = w.pd[sec];
+                const IV2 sp = cont_prop(w, here, SP.direction, SP.pixelCenter, ch);
+                if (!(sp == here)) remote = true;
+                if (pp == sp) {
+                    if (insert_pair(w.prop[w.idx(pp.x, pp.y)], prim, sec)) count_coll(pp, pos, cnt, total, 2);
+                } else {
+                    if (insert_pair(w.prop[w.idx(pp.x, pp.y)], prim, sec)) count_coll(pp, pos, cnt, total, 1);
+                    if (insert_pair(w.prop[w.idx(sp.x, sp.y)], prim, sec)) count_coll(sp, pos, cnt, total, 1);
+                }
+                count_coll(pp, pos, cnt, total, 1);
+                count_coll(sp, pos, cnt, total, 1);
+            }
+            if (remote) ++complexSources;
+            for (int i = 0; i != 6; ++i) {
+                if (cnt[i] == 0) break;
+                PropData &t = w.prop[w.idx(pos[i].x, pos[i].y)];
+                t.cum = t.cum + (U.cum / (float)total) * (float)cnt[i];
+            }
+        }
+    w.stats.complex_sources = complexSources;
+    // combinePropagatedCollisions
+    for (size_t c = 0; c < (size_t)w.N; ++c) {
+        PropData P = w.prop[c];
+        if (P.inv != 0) {
+            P.types = 1;
+        } else {
+            const UpliftData U = w.uplift[c];
+            if ((U.types & 1u) == 0) {
+                P.types = U.types;
+                P.cum = U.cum;
+                P.inv = w.collisions[c];
+            } else {
+                P.inv = 1;
+            }
+        }
+        w.prop[c] = P;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage C3/C4: jump flood (JFAInitialize :1498-1510, JFAPass :1512-1571,
+// JFAComputeDistance :1573-1600, host loop sim.cu:295-308) and applyUplift
+// (:1662-1738 with the formulas at :1646-1660).
+// ---------------------------------------------------------------------------
+static inline int tdist2(int ax, int ay, int bx, int by, int W, int H)
+{
+    int dx = std::abs(ax - bx), dy = std::abs(ay - by);
+    dx = std::min(dx, W - dx);
+    dy = std::min(dy, H - dy);
+    return dx * dx + dy * dy;
+}
+
+static inline float cont_formula(float x)
+{
+    const float ax = tt_fabsf(x);
+    return tt_expf((-0.02f * x) * x) + tt_expf((-0.2f * x) * x) * (0.5f - tt_fabsf(tt_fmodf(ax, 1.0f) - 0.5f));
+}
+static inline float sub_under(float x)
+{
+    const float l = tt_log10f(1.0f + 8.0f * (x - 1.0f));
+    return tt_fminf(l / 1.908485019f - 1.0f, 0.0f);
+}
+
+void stage_jfa_uplift(World &w, std::vector<float> &hW, const std::vector<uint8_t> &idW)
+{
+    const int W = w.W, H = w.H;
+    std::vector<int> a((size_t)w.N), b((size_t)w.N);
+    for (size_t c = 0; c < (size_t)w.N; ++c) a[c] = (w.prop[c].inv >> 8) != 0 ? (int)c : -1;
+    const int maxDim = std::max(W, H);
+    int step = 1;
+    while (step < maxDim) step <<= 1;
+    step >>= 1;
+    while (step > 0) {
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                int best = a[c];
+                int bd = 0x7FFFFFFF;
+                if (best >= 0) bd = tdist2(x, y, best % W, best / W, W, H);
+                for (int oy = -1; oy <= 1; ++oy)
+                    for (int ox = -1; ox <= 1; ++ox) {
+                        if (ox == 0 && oy == 0) continue;
+                        const int cand = a[w.idx(x + ox * step, y + oy * step)];
+                        if (cand < 0) continue;
+                        const int d = tdist2(x, y, cand % W, cand / W, W, H);
+                        if (d < bd) { bd = d; best = cand; }
+                    }
+                b[c] = best;
+            }
+        a.swap(b);
+        step >>= 1;
+    }
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t c = (size_t)y * W + x;
+            const int s = a[c];
+            if (s < 0) { w.dist[c] = DistBuf{-1, 0u}; continue; }
+            const int d2 = tdist2(x, y, s % W, s / W, W, H);
+            w.dist[c] = DistBuf{(int)sqrtf((float)d2), (uint32_t)s};
+        }
+    const float sizeX = (float)W, sizeY = (float)H;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t c = (size_t)y * W + x;
+            const DistBuf buf = w.dist[c];
+            const PropData U = w.prop[buf.orig];
+            const int pid = idW[c];
+            if (buf.dist <= -1) continue;
+            const int type = U.types & 1;
+            const int pi = (U.types >> 1) & 3, si = (U.types >> 3) & 3;
+            const int prim = (uint8_t)(MAXP - ((U.inv >> (8 * pi)) & 0xFF));
+            const int sec = (uint8_t)(MAXP - ((U.inv >> (8 * si)) & 0xFF));
+            if (type == 0 && sec == pid) {
+                const float v = sub_under((float)buf.dist);
+                const float m = 1.0f + v * w.ks.subductionUnderUpliftMultiplier;
+                hW[c] = tt_fmaxf(hW[c] * m, w.ks.minUpliftClamp);
+            } else if (type == 0 && prim == pid) {
+                const int ox = (int)(buf.orig % (uint32_t)W), oy = (int)(buf.orig / (uint32_t)W);
+                const int dxi = x - ox, dyi = y - oy;
+                const float dx = tt_fmodf((float)dxi + sizeX * 0.5f, sizeX) - sizeX * 0.5f;
+                const float dy = tt_fmodf((float)dyi + sizeY * 0.5f, sizeY) - sizeY * 0.5f;
+                const V2 dir = norm_zs(V2{dx, dy});
+                const tt_plate_data &OP = w.pd[sec];
+                const float dot = tt_fmaxf(dir.x * OP.direction[0] + dir.y * OP.direction[1], 0.0f);
+                const float v = cont_formula((((float)buf.dist - (float)w.ks.subductionOverRange)) - 10.0f);
+                const float m = 1.0f + (v * dot) * w.ks.subductionOverUpliftMultiplier;
+                hW[c] = tt_fmaxf(hW[c] * m, w.ks.minUpliftClamp);
+            } else if (type == 1) {
+                const float v = cont_formula((float)buf.dist);
+                const float hc = (v * U.cum) * w.ks.continentalUpliftMultiplier;
+                hW[c] = tt_fminf(hW[c] + hc, w.ks.maxUpliftClamp);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Stage D: thermal erosion (thermalErosionKernel :2735-2786), canonical Jacobi
+// form: every cell reads the heights of the previous pass; its new height is
+// old - own outflows (direction order 0..7) + inflows (direction order 0..7).
+// ---------------------------------------------------------------------------
+static const int TOFF[8][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}, {-1, -1}, {-1, 1}, {1, -1}, {1, 1}};
+
+void stage_thermal(World &w, std::vector<float> &h)
+{
+    const int W = w.W, H = w.H;
+    const float talus = w.ks.thermalThresholdAngle * w.ks.thermalCellSize;
+    const float amp = w.ks.thermalErosionAmplitude, str = w.ks.thermalErosionStrength;
+    std::vector<float> tot((size_t)w.N), nh((size_t)w.N);
+    for (int it = 0; it < w.cfg.thermal_iterations; ++it) {
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const float hc = h[(size_t)y * W + x];
+                float t = 0.0f;
+                for (int d = 0; d < 8; ++d) {
+                    const float diff = hc - h[w.idx(x + TOFF[d][0], y + TOFF[d][1])];
+                    if (diff > talus) t = t + diff;
+                }
+                tot[(size_t)y * W + x] = t;
+            }
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                const float hc = h[c];
+                float v = hc;
+                if (tot[c] > 0.0f)
+                    for (int d = 0; d < 8; ++d) {
+                        const float diff = hc - h[w.idx(x + TOFF[d][0], y + TOFF[d][1])];
+                        if (diff > talus) v = v - (amp * (diff / tot[c])) * str;
+                    }
+                for (int d = 0; d < 8; ++d) {
+                    const int s = w.idx(x - TOFF[d][0], y - TOFF[d][1]);
+                    const float diff = h[s] - hc;
+                    if (diff > talus && tot[s] > 0.0f) v = v + (amp * (diff / tot[s])) * str;
+                }
+                nh[c] = v;
+            }
+        h.swap(nh);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage E: hydraulic erosion (applyHydraulicErosion, plate_tectonic_sim.cu:
+// 331-373): rain :2430-2437, flux :2439-2485, flow :2487-2533, sediment
+// :2535-2570 with Slope (cuda_texture.cuh:78-89), transport :2572-2585 with
+// interpolate (cuda_helper.cuh:49-65), evaporate :2587-2593; dt = 1.
+// ---------------------------------------------------------------------------
+void stage_hydraulic(World &w, std::vector<float> &m)
+{
+    const int W = w.W, H = w.H;
+    const tt_kernel_settings &K = w.ks;
+    const float dt = 1.0f;
+    const int rseed = (int)(w.cfg.seed + w.iter);
+    std::vector<F4> nf((size_t)w.N);
+    std::vector<float> snap((size_t)w.N), sb((size_t)w.N);
+    for (int it = 0; it < w.cfg.hydraulic_iterations; ++it) {
+        // rain
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const float n = tt_discrete_noise((float)x, 0.0f, (float)y, 1.0f, rseed);
+                w.water[(size_t)y * W + x] = w.water[(size_t)y * W + x] + (tt_fmaxf(0.0f, n) * dt) * K.hydrationRainfall;
+            }
+        // flux
+        const float fk = dt * K.hydrationPipeCrossSection;
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                auto sub = [&](float f, int nx, int ny) {
+                    const size_t n = (size_t)w.idx(nx, ny);
+                    const float dh = ((m[c] + w.water[c]) - m[n]) - w.water[n];
+                    return tt_fmaxf(0.0f, f + fk * ((K.gravity * dh) / K.hydrationPipeLength));
+                };
+                const F4 cf = w.flux[c];
+                F4 o;
+                o.x = sub(cf.x, x - 1, y);
+                o.y = sub(cf.y, x, y + 1);
+                o.z = sub(cf.z, x + 1, y);
+                o.w = sub(cf.w, x, y - 1);
+                const float tot = (((o.x + o.y) + o.z) + o.w) + 1e-6f;
+                float Kf = tt_fminf(1.0f, w.water[c] / (tot * dt + 1e-6f));
+                Kf = tt_fmaxf(Kf, 0.01f);
+                o.x = o.x * Kf; o.y = o.y * Kf; o.z = o.z * Kf; o.w = o.w * Kf;
+                nf[c] = o;
+            }
+        // flow
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                const F4 L = nf[w.idx(x - 1, y)], U = nf[w.idx(x, y + 1)], R = nf[w.idx(x + 1, y)], D = nf[w.idx(x, y - 1)];
+                const F4 S = nf[c];
+                float in = 0.0f;
+                in = in + L.z;
+                in = in + U.w;
+                in = in + R.x;
+                in = in + D.y;
+                const float out = ((S.x + S.y) + S.z) + S.w;
+                const float dv = dt * (in - out);
+                const float wn = tt_fmaxf(0.0f, w.water[c] + dv);
+                w.water[c] = wn;
+                const float nx = (((L.z - S.x) + S.z) - R.x) * 0.5f;
+                const float ny = (((U.w - S.y) + S.y) - D.w) * 0.5f;
+                const float hh = tt_fmaxf(wn, 0.01f);
+                V2 v{nx / hh, ny / hh};
+                const float vm = mag(v);
+                if (vm > 50.0f) { const float s = 50.0f / vm; v.x = v.x * s; v.y = v.y * s; }
+                w.velocity[c] = v;
+            }
+        w.flux.swap(nf);
+        // material snapshot + sediment
+        snap = m;
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                const float dzdx = (float)((double)(snap[w.idx(x + 1, y)] - snap[w.idx(x - 1, y)]) / 2.0);
+                const float dzdy = (float)((double)(snap[w.idx(x, y + 1)] - snap[w.idx(x, y - 1)]) / 2.0);
+                const float slope = sqrtf(dzdx * dzdx + dzdy * dzdy);
+                float C = (K.sedimentCapacity * tt_sinf(tt_fmaxf(slope, 0.03f))) * mag(w.velocity[c]);
+                C = tt_fminf(C, 5.0f);
+                const float sed = w.sediment[c];
+                if (C > sed) {
+                    float s = (K.sedimentErosionRate * (C - sed)) * dt;
+                    s = tt_fminf(s, snap[c]);
+                    m[c] = snap[c] - s;
+                    w.sediment[c] = sed + s;
+                } else {
+                    float s = (K.sedimentDepositionRate * (sed - C)) * dt;
+                    s = tt_fminf(s, sed);
+                    w.sediment[c] = sed - s;
+                    m[c] = snap[c] + s;
+                }
+            }
+        // transport
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                const V2 v = w.velocity[c];
+                const float tx = (float)x - v.x * dt, ty = (float)y - v.y * dt;
+                const int x0 = (int)tt_floorf(tx), y0 = (int)tt_floorf(ty);
+                const int x1 = x0 + 1, y1 = y0 + 1;
+                const float wx = tx - (float)x0, wy = ty - (float)y0;
+                const float I0 = (1.0f - wx) * w.sediment[w.idx(x0, y0)] + wx * w.sediment[w.idx(x1, y0)];
+                const float I1 = (1.0f - wx) * w.sediment[w.idx(x0, y1)] + wx * w.sediment[w.idx(x1, y1)];
+                sb[c] = (1.0f - wy) * I0 + wy * I1;
+            }
+        // evaporate
+#pragma omp parallel for schedule(static)
+        for (long long c = 0; c < w.N; ++c) {
+            const float wv = w.water[c];
+            w.water[c] = tt_fmaxf(0.0f, wv - (tt_fminf(wv, 20.0f) * K.hydrationEvaporation) * dt);
+        }
+        w.sediment.swap(sb);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage G1: applyPlateMovementChanges (:1863-1925)
+// ---------------------------------------------------------------------------
+void stage_plate_movement(World &w, const std::vector<long long> &fric)
+{
+    const tt_kernel_settings &K = w.ks;
+    for (int p = 0; p < MAXP; ++p) {
+        tt_plate_data cur = w.pd[p];
+        const float cx = cur.pixelCenter[0] + cur.direction[0] * cur.velocity;
+        const float cy = cur.pixelCenter[1] + cur.direction[1] * cur.velocity;
+        const float nx = cx - tt_floorf(cx), ny = cy - tt_floorf(cy);
+        cur.hasMoved = (nx != cx || ny != cy) ? 1 : 0;
+        cur.pixelCenter[0] = nx;
+        cur.pixelCenter[1] = ny;
+        const float frictionLoss = tt_unfix(fric[p], FRIC_BITS);
+        const V2 orig{cur.direction[0] * cur.velocity, cur.direction[1] * cur.velocity};
+        const int sx = (int)(cur.geometricCenter[0] + 0.5f), sy = (int)(cur.geometricCenter[1] + 0.5f);
+        const V2 as = w.asth[w.idx(sx, sy)];
+        auto clampf = [](float v, float lo, float hi) { return tt_fmaxf(tt_fminf(v, hi), lo); };
+        const V2 av{clampf((float)((double)(as.x * K.curlNoiseMultiplier) + (as.x < 0 ? -0.2 : 0.2)), -1.0f, 1.0f),
+                    clampf((float)((double)(as.y * K.curlNoiseMultiplier) + (as.y < 0 ? -0.2 : 0.2)), -1.0f, 1.0f)};
+        const float m0 = tt_fmaxf(mag(orig), 1e-20f);
+        const float post = tt_fmaxf(m0 - frictionLoss, 0.0f);
+        const V2 pv{cur.direction[0] * post, cur.direction[1] * post};
+        const V2 rel{av.x - pv.x, av.y - pv.y};
+        const float rm = mag(rel);
+        const float d = (K.environmentalDragCoefficient * rm) * rm;
+        const V2 rn = norm_zs(rel);
+        const V2 nv{pv.x + rn.x * d, pv.y + rn.y * d};
+        const float nm = tt_fmaxf(mag(nv), 1e-20f);
+        cur.velocity = nm;
+        cur.direction[0] = nv.x / nm;
+        cur.direction[1] = nv.y / nm;
+        cur.velocitySmoothed = cur.velocitySmoothed * K.velocitySmoothingFactor + cur.velocity * (1.0f - K.velocitySmoothingFactor);
+        const V2 sd{cur.directionSmoothed[0] * K.directionSmoothingFactor + cur.direction[0] * (1.0f - K.directionSmoothingFactor),
+                    cur.directionSmoothed[1] * K.directionSmoothingFactor + cur.direction[1] * (1.0f - K.directionSmoothingFactor)};
+        const V2 sdn = norm_zs(sd);
+        cur.directionSmoothed[0] = sdn.x;
+        cur.directionSmoothed[1] = sdn.y;
+        cur.mergeWaitTime = cur.mergeWaitTime > 0 ? cur.mergeWaitTime - 1 : 0;
+        w.pd[p] = cur;
+    }
+}
+
+// applyAccretion (:747-770)
+void stage_accretion(World &w)
+{
+    for (size_t c = 0; c < (size_t)w.N; ++c) {
+        const uint8_t aid = (uint8_t)(MAXP - w.accretion[c]);
+        if (aid == MAXP) continue;
+        if (w.height[c] >= w.ks.continentalCrustThreshold) w.ids[c] = aid;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage F: CCL (ccl.cu:90-177) + relabel (plate_tectonic_sim.cu:375-468,
+// createLabelIdLookup :3130-3185, postCClIdReassign :3187-3206,
+// assignUnassignedIdsToNeighbor :2052-2105).
+// Labels: components of the reference's edge set, label = minimum cell index.
+// Fragment order: (count desc, label desc).  Free ids ascending among
+// {i < min(255, #labels) : size == 0}.  Neighbour fill in Jacobi rounds.
+// ---------------------------------------------------------------------------
+struct DSU {
+    std::vector<uint32_t> p;
+    explicit DSU(size_t n) : p(n) { std::iota(p.begin(), p.end(), 0u); }
+    uint32_t find(uint32_t a)
+    {
+        while (p[a] != a) { p[a] = p[p[a]]; a = p[a]; }
+        return a;
+    }
+    void unite(uint32_t a, uint32_t b)
+    {
+        a = find(a); b = find(b);
+        if (a == b) return;
+        if (a < b) p[b] = a; else p[a] = b;
+    }
+};
+
+void ccl_labels(World &w)
+{
+    const int W = w.W, H = w.H;
+    DSU d((size_t)w.N);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const uint32_t c = (uint32_t)(y * W + x);
+            const uint8_t id = w.ids[c];
+            const uint32_t tl = (uint32_t)w.idx(x - 1, y - 1), t = (uint32_t)w.idx(x, y - 1);
+            const uint32_t tr = (uint32_t)w.idx(x + 1, y - 1), l = (uint32_t)w.idx(x - 1, y);
+            // init edge: first match among tl, t, tr, l (ccl.cu:107-116)
+            if (id == w.ids[tl]) d.unite(c, tl);
+            else if (id == w.ids[t]) d.unite(c, t);
+            else if (id == w.ids[tr]) d.unite(c, tr);
+            else if (id == w.ids[l]) d.unite(c, l);
+            // reduce edges (ccl.cu:161-176)
+            if (y == 0) {
+                if (id == w.ids[t]) d.unite(c, t);
+                if (id == w.ids[tr]) d.unite(c, tr);
+            }
+            if (id == w.ids[tl]) d.unite(c, tl);
+            if (id == w.ids[l]) d.unite(c, l);
+        }
+    for (uint32_t c = 0; c < (uint32_t)w.N; ++c) w.labels[c] = d.find(c);
+}
+
+void ccl_and_relabel(World &w)
+{
+    ccl_labels(w);
+    const size_t N = (size_t)w.N;
+    std::vector<uint32_t> count(N, 0);
+    for (size_t c = 0; c < N; ++c) count[w.labels[c]]++;
+    struct Frag { uint32_t count, label; uint8_t pid; };
+    std::vector<Frag> fr;
+    for (size_t c = 0; c < N; ++c)
+        if (w.labels[c] == c) fr.push_back(Frag{count[c], (uint32_t)c, w.ids[c]});
+    std::sort(fr.begin(), fr.end(), [](const Frag &a, const Frag &b) {
+        if (a.count != b.count) return a.count > b.count;
+        return a.label > b.label;
+    });
+    const int numLabels = (int)fr.size();
+    w.stats.num_labels = numLabels;
+    std::vector<int> maxCount(MAXP + 1, 0);
+    for (const Frag &f : fr) maxCount[f.pid] = std::max(maxCount[f.pid], (int)f.count);
+    const int M = std::min(MAXP, numLabels);
+    std::vector<uint8_t> avail;
+    for (int i = 0; i < M; ++i)
+        if (w.pd[i].size == 0) avail.push_back((uint8_t)i);
+    std::vector<uint8_t> lookup(N, (uint8_t)MAXP);
+    size_t ai = 0;
+    for (int k = 0; k < M; ++k) {
+        const Frag &f = fr[k];
+        if ((int)f.count == maxCount[f.pid]) {
+            lookup[f.label] = f.pid;
+        } else if ((int)f.count >= w.ks.minPlateSize) {
+            const size_t idx = ai++;
+            if (idx < avail.size()) {
+                const uint8_t nid = avail[idx];
+                w.pd[nid] = w.pd[f.pid];
+                lookup[f.label] = nid;
+            }
+        }
+    }
+    std::vector<uint32_t> un;
+    for (size_t c = 0; c < N; ++c) {
+        const uint8_t id = lookup[w.labels[c]];
+        w.ids[c] = id;
+        if (id == MAXP) un.push_back((uint32_t)c);
+    }
+    w.stats.unassigned_cells = (int)un.size();
+    static const int NO[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+    int rounds = 0;
+    std::vector<uint8_t> snap;
+    while (!un.empty()) {
+        ++rounds;
+        snap = w.ids;
+        std::vector<uint32_t> rem;
+        bool progress = false;
+        for (uint32_t c : un) {
+            const int x = (int)(c % (uint32_t)w.W), y = (int)(c / (uint32_t)w.W);
+            uint8_t nb[8];
+            int n = 0;
+            for (int k = 0; k < 8; ++k) {
+                const uint8_t v = snap[w.idx(x + NO[k][0], y + NO[k][1])];
+                if (v != MAXP) nb[n++] = v;
+            }
+            if (n == 0) { rem.push_back(c); continue; }
+            uint8_t sel = 0;
+            int best = 0;
+            for (int i = 0; i != n; ++i) {
+                int lc = 1;
+                for (int j = 0; j != n; ++j)
+                    if (i != j && nb[j] == nb[i]) lc++;
+                if (lc > best) { best = lc; sel = nb[i]; }
+            }
+            w.ids[c] = sel;
+            progress = true;
+        }
+        un.swap(rem);
+        if (!progress) break;  // whole map unassigned: cannot happen (largest fragments keep ids)
+    }
+    w.stats.fill_rounds = rounds;
+}
+
+// ---------------------------------------------------------------------------
+// Stage G2: mergePlates (plate_tectonic_sim.cu:470-494): getNeighboringPlates
+// :1283-1311, getPlateMerges :1313-1378 (canonical: pairs in ascending matrix
+// index, greedy), clearCollisionTypes :1381-1392 (indexes the 1-row pair table
+// with (min, max) -> clears entry min), resetPlateDataPreCount :3208-3225,
+// mergeAndCountSizeMass :102-197 (neighbours from the pre-merge snapshot;
+// parent id -1 read as uint8 255 -> neighbour counted as plate 0).
+// ---------------------------------------------------------------------------
+void stage_merge(World &w)
+{
+    const int W = w.W, H = w.H;
+    const tt_kernel_settings &K = w.ks;
+    std::vector<uint8_t> nbm(MAXP * MAXP, 0);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const uint8_t cur = w.ids[(size_t)y * W + x];
+            const uint8_t r = w.ids[w.idx(x + 1, y)], t = w.ids[w.idx(x, y + 1)];
+            if (cur != r) {
+                const int mn = r < cur ? r : cur, mx = r < cur ? cur : r;
+                nbm[(size_t)wrapc(mx, MAXP) * MAXP + wrapc(mn, MAXP)] = 1;
+            }
+            if (cur != t) {
+                const int mn = t < cur ? t : cur, mx = t < cur ? cur : t;
+                nbm[(size_t)wrapc(mx, MAXP) * MAXP + wrapc(mn, MAXP)] = 1;
+            }
+        }
+    std::vector<int> mergeIds(MAXP, 0);
+    int merges = 0;
+    for (int i = 0; i < MAXP * MAXP; ++i) {
+        const int x = i % MAXP, y = i / MAXP;
+        if (x >= y || !nbm[i]) continue;
+        const tt_plate_data &X = w.pd[x], &Y = w.pd[y];
+        const float dot = X.directionSmoothed[0] * Y.directionSmoothed[0] + X.directionSmoothed[1] * Y.directionSmoothed[1];
+        const float vd = tt_fabsf(X.velocitySmoothed - Y.velocitySmoothed);
+        if (!(X.velocity <= K.mergeMinVelocity && Y.velocity <= K.mergeMinVelocity) &&
+            !(dot >= K.mergeDotDirectionThreshold && vd <= K.mergeVelocityDiffThreshold))
+            continue;
+        if (X.mergeWaitTime > 0 || Y.mergeWaitTime > 0) continue;
+        if (X.size + Y.size > K.targetMaximumPlateArea) continue;
+        const bool xIsParent = Y.mass <= X.mass;
+        if (mergeIds[x] != 0) continue;
+        mergeIds[x] = xIsParent ? -1 : MAXP - y;
+        if (mergeIds[y] != 0) { mergeIds[x] = 0; continue; }
+        mergeIds[y] = xIsParent ? MAXP - x : -1;
+        ++merges;
+        const int parent = xIsParent ? x : y;
+        for (int j = 0; j < MAXP; ++j) {
+            if (j == parent) continue;
+            const int mn = parent < j ? parent : j;
+            w.typeBitmap[mn] = 0;
+        }
+    }
+    w.stats.merges = merges;
+    for (int p = 0; p < MAXP; ++p) {
+        tt_plate_data &c = w.pd[p];
+        c.asthenosphereVelocity[0] = 0.0f;
+        c.asthenosphereVelocity[1] = 0.0f;
+        c.mass = 0.0f;
+        c.size = 0;
+        c.perimeter = 0;
+        c.breakScore = 0.0f;
+        c.used = 0;
+    }
+    const std::vector<uint8_t> snap = w.ids;
+    std::vector<long long> mass(MAXP + 1, 0), ax(MAXP + 1, 0), ay(MAXP + 1, 0);
+    std::vector<int> size(MAXP + 1, 0), per(MAXP + 1, 0);
+    static const int PO[4][2] = {{0, -1}, {1, 0}, {0, 1}, {-1, 0}};
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t c = (size_t)y * W + x;
+            int pid = snap[c];
+            if (mergeIds[pid] > 0) { pid = MAXP - mergeIds[pid]; w.ids[c] = (uint8_t)pid; }
+            mass[pid] += tt_fix(w.height[c], MASS_BITS);
+            size[pid] += 1;
+            ax[pid] += tt_fix(w.asth[c].x, ASTH_BITS);
+            ay[pid] += tt_fix(w.asth[c].y, ASTH_BITS);
+            int cnt = 0;
+            for (int k = 0; k < 4; ++k) {
+                int nid = snap[w.idx(x + PO[k][0], y + PO[k][1])];
+                const uint8_t mm = (uint8_t)mergeIds[nid];
+                if (mm != 0) nid = (uint8_t)(MAXP - mm);
+                if (nid != pid) cnt++;
+            }
+            per[pid] += cnt;
+        }
+    for (int p = 0; p < MAXP; ++p) {
+        if (size[p] == 0 && mass[p] == 0 && per[p] == 0 && ax[p] == 0 && ay[p] == 0) continue;
+        tt_plate_data &c = w.pd[p];
+        c.mass = tt_unfix(mass[p], MASS_BITS);
+        c.size = size[p];
+        c.perimeter = per[p];
+        c.asthenosphereVelocity[0] = tt_unfix(ax[p], ASTH_BITS);
+        c.asthenosphereVelocity[1] = tt_unfix(ay[p], ASTH_BITS);
+    }
+}
+
+// computeBreakScore (:2706-2733); the smoothed value is overwritten in the
+// reference and has no effect.
+void stage_break_score(World &w)
+{
+    const tt_kernel_settings &K = w.ks;
+    for (int p = 0; p < MAXP; ++p) {
+        tt_plate_data &d = w.pd[p];
+        if (d.size > 0) {
+            const float area = (float)d.size, per = (float)d.perimeter;
+            d.circularity = 1.0f - (((2.0f * 6.2831855f) * area) / tt_fmaxf(per * per, 1.0f));
+            const float mn = (float)K.targetMinimumPlateArea, mx = (float)K.targetMaximumPlateArea;
+            const float as = tt_fmaxf(tt_fminf((area - mn) / (mx - mn), 1.0f), 0.0f);
+            d.breakScore = as + d.circularity;
+        } else {
+            d.breakScore = 0.0f;
+        }
+    }
+}
+
+// accumulatePlateAngularCoords (:2154-2189) + calculatePlateCenters
+// (:2191-2226): every 10th cell in x and y; fp64 angles and exact fixed-point
+// sums, fp64 atan2.
+void stage_centers(World &w)
+{
+    const int W = w.W, H = w.H;
+    std::vector<long long> s[4];
+    for (auto &v : s) v.assign(MAXP + 1, 0);
+    std::vector<int> cnt(MAXP + 1, 0);
+    const double twopi = 6.283185307179586;
+    for (int y = 0; y < H; y += 10)
+        for (int x = 0; x < W; x += 10) {
+            const int p = w.ids[(size_t)y * W + x];
+            const double axg = (twopi * (double)x) / (double)W, ayg = (twopi * (double)y) / (double)H;
+            s[0][p] += tt_fix_d(tt_sin_d(axg), ANG_BITS);
+            s[1][p] += tt_fix_d(tt_cos_d(axg), ANG_BITS);
+            s[2][p] += tt_fix_d(tt_sin_d(ayg), ANG_BITS);
+            s[3][p] += tt_fix_d(tt_cos_d(ayg), ANG_BITS);
+            cnt[p]++;
+        }
+    for (int p = 0; p < MAXP; ++p) {
+        if (cnt[p] <= 0) continue;
+        const double c = (double)cnt[p];
+        const double a0 = tt_unfix_d(s[0][p], ANG_BITS) / c, a1 = tt_unfix_d(s[1][p], ANG_BITS) / c;
+        const double a2 = tt_unfix_d(s[2][p], ANG_BITS) / c, a3 = tt_unfix_d(s[3][p], ANG_BITS) / c;
+        double cx = (tt_atan2(a0, a1) * (double)W) / twopi;
+        double cy = (tt_atan2(a2, a3) * (double)H) / twopi;
+        if (cx < 0.0) cx = cx + (double)W;
+        if (cy < 0.0) cy = cy + (double)H;
+        w.pd[p].geometricCenter[0] = (float)cx;
+        w.pd[p].geometricCenter[1] = (float)cy;
+    }
+}
+
+// statisticsPass (:2377-2402)
+void stage_statistics(World &w)
+{
+    float heavy = 0.0f;
+    int largest = 0, hid = 0, lid = 0, nplates = 0;
+    for (int i = 0; i < MAXP; ++i) {
+        if (heavy < w.pd[i].mass) { heavy = w.pd[i].mass; hid = i; }
+        if (largest < w.pd[i].size) { largest = w.pd[i].size; lid = i; }
+        w.pd[i].used = w.pd[i].size > 0 ? 1 : 0;
+        nplates += w.pd[i].used;
+    }
+    w.stats.heaviest_value = heavy;
+    w.stats.heaviest_plate_id = hid;
+    w.stats.largest_value = largest;
+    w.stats.largest_plate_id = lid;
+    w.stats.number_of_plates = nplates;
+}
+
+// ---------------------------------------------------------------------------
+// Stage G6: processPlateSplitting (plate_tectonic_sim.cu:496-527) +
+// SplitPlateV2 (:933-1040): initEffortToBoundary :2824-2836,
+// propagateEffortToBoundary :2838-2881 (the fixed point of the monotone
+// relaxation; any order reaches it), selectUnusedPlateId :2404-2421,
+// BacktrackPath :2883-2983, floodFillPlate :2985-3024 (fixed point = the
+// 4-connected component of the seed), finalizePlateSplit :3026-3045,
+// resetPlateDataPreCountPostSplit :3227-3247, countSizePostSplit :3047-3112.
+// ---------------------------------------------------------------------------
+void stage_split(World &w)
+{
+    w.stats.split_plate = -1;
+    w.stats.split_new_plate = -1;
+    w.stats.split_iterations = 0;
+    int plate = -1;
+    for (int i = 0; i < MAXP; ++i)
+        if (w.pd[i].used && w.pd[i].breakScore >= 1.2f) { plate = i; break; }
+    if (plate < 0) return;
+    const int W = w.W, H = w.H;
+    const size_t N = (size_t)w.N;
+    const uint8_t old = (uint8_t)plate;
+    const int ptx = (int)(w.pd[plate].geometricCenter[0] + 0.5f);
+    const int pty = (int)(w.pd[plate].geometricCenter[1] + 0.5f);
+    // effort to boundary: 0 outside the plate, +inf inside, relaxed to the fixed point
+    const float INF = tt_u2f(0x7f800000u);
+    std::vector<float> eff(N, 0.0f);
+    for (size_t c = 0; c < N; ++c)
+        if (w.ids[c] == old) eff[c] = INF;
+    const float SQRT2 = 1.41421356237f;
+    // label-correcting worklist (order-independent fixed point)
+    std::vector<uint8_t> inq(N, 0);
+    std::deque<uint32_t> q;
+    for (size_t c = 0; c < N; ++c)
+        if (w.ids[c] == old) { q.push_back((uint32_t)c); inq[c] = 1; }
+    long long relax = 0;
+    while (!q.empty()) {
+        const uint32_t c = q.front();
+        q.pop_front();
+        inq[c] = 0;
+        ++relax;
+        const int x = (int)(c % (uint32_t)W), y = (int)(c / (uint32_t)W);
+        float md = eff[c];
+        const float cost = w.height[c];
+        for (int i = 0; i < 8; ++i) {
+            const float mul = i >= 4 ? SQRT2 : 1.0f;
+            const float t = eff[w.idx(x + TOFF[i][0], y + TOFF[i][1])] + cost * mul;
+            if (t < md) md = t;
+        }
+        if (md != eff[c]) {
+            eff[c] = md;
+            // neighbours whose tentative value may drop
+            for (int i = 0; i < 8; ++i) {
+                const uint32_t n = (uint32_t)w.idx(x + TOFF[i][0], y + TOFF[i][1]);
+                if (!inq[n] && w.ids[n] == old) { inq[n] = 1; q.push_back(n); }
+            }
+        }
+    }
+    w.stats.split_iterations = (int)std::min<long long>(relax, 0x7fffffff);
+    // selectUnusedPlateId
+    int nid = -1;
+    for (int i = 0; i < MAXP; ++i)
+        if (!w.pd[i].used) { nid = i; w.pd[i].used = 1; break; }
+    if (nid < 0) return;  // no free id: no split
+    const uint8_t newId = (uint8_t)nid;
+    w.stats.split_plate = plate;
+    w.stats.split_new_plate = nid;
+    auto E = [&](int x, int y) { return eff[w.idx(x, y)]; };
+    auto ID = [&](int x, int y) -> uint8_t & { return w.ids[w.idx(x, y)]; };
+    // BacktrackPath
+    bool deadEnd = false;
+    const uint8_t oldAtPoint = ID(ptx, pty);
+    int cx = ptx, cy = pty;
+    long long guard = 0;
+    while (E(cx, cy) > 0.0f) {
+        ID(cx, cy) = (uint8_t)MAXP;
+        float best = INF;
+        int nx = cx, ny = cy;
+        for (int i = 0; i < 8; ++i) {
+            const float v = E(cx + TOFF[i][0], cy + TOFF[i][1]);
+            if (v < best) { best = v; nx = cx + TOFF[i][0]; ny = cy + TOFF[i][1]; }
+        }
+        if (best == INF || ++guard > (long long)N) { deadEnd = true; break; }
+        cx = nx; cy = ny;
+    }
+    float px = 0.0f, py = 0.0f;
+    if (!deadEnd) {
+        const V2 fb{(float)(cx - ptx), (float)(cy - pty)};
+        const V2 od = norm_zs(V2{fb.x * -1.0f, fb.y * -1.0f});
+        px = -od.y; py = od.x;
+        cx = ptx; cy = pty;
+        while (E(cx, cy) > 0.0f) {
+            ID(cx, cy) = (uint8_t)MAXP;
+            float best = INF;
+            int nx = cx, ny = cy;
+            for (int i = 0; i < 8; ++i) {
+                const int qx = cx + TOFF[i][0], qy = cy + TOFF[i][1];
+                if (ID(qx, qy) == MAXP) continue;
+                const V2 tn{(float)(qx - ptx), (float)(qy - pty)};
+                const float pdot = tn.x * od.x + tn.y * od.y;
+                if (pdot < 0.0f) continue;
+                const float dv = E(qx, qy);
+                const float ofx = (float)TOFF[i][0], ofy = (float)TOFF[i][1];
+                const float om = sqrtf(ofx * ofx + ofy * ofy);
+                const V2 tnd{ofx / om, ofy / om};
+                const float dot = tnd.x * od.x + tnd.y * od.y;
+                const float bias = -dot * 50.0f;
+                const float score = dv + bias;
+                if (score < best) { best = score; nx = qx; ny = qy; }
+            }
+            if (best != INF) { cx = nx; cy = ny; }
+            else { deadEnd = true; break; }
+        }
+    }
+    if (!deadEnd) {
+        int sx = ptx, sy = pty;
+        for (int i = 1; i <= 100 && ID(sx, sy) != oldAtPoint; ++i) {
+            const int xo = (i % 2 == 1) ? (i + 1) / 2 : -(i / 2);
+            const float fx = (float)ptx + px * (float)xo;
+            const float fy = (float)pty + py * (float)xo;
+            sx = (int)(fx + 0.5f);
+            sy = (int)(fy + 0.5f);
+        }
+        if (ID(sx, sy) == oldAtPoint) ID(sx, sy) = newId;
+    }
+    if (deadEnd) {
+        for (size_t c = 0; c < N; ++c)
+            if (w.ids[c] == MAXP) w.ids[c] = old;
+        return;
+    }
+    // floodFillPlate to its fixed point: 4-connected component of newId cells through old cells
+    {
+        std::vector<uint32_t> st;
+        for (size_t c = 0; c < N; ++c)
+            if (w.ids[c] == newId) st.push_back((uint32_t)c);
+        static const int F4O[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+        while (!st.empty()) {
+            const uint32_t c = st.back();
+            st.pop_back();
+            const int x = (int)(c % (uint32_t)W), y = (int)(c / (uint32_t)W);
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t n = (uint32_t)w.idx(x + F4O[k][0], y + F4O[k][1]);
+                if (w.ids[n] == old) { w.ids[n] = newId; st.push_back(n); }
+            }
+        }
+    }
+    // finalizePlateSplit
+    for (size_t c = 0; c < N; ++c)
+        if (w.ids[c] == MAXP) w.ids[c] = newId;
+    if (newId != old) {
+        w.pd[newId].velocity = w.pd[old].velocity;
+        w.pd[newId].velocitySmoothed = w.pd[old].velocitySmoothed;
+        w.pd[newId].direction[0] = w.pd[old].direction[0];
+        w.pd[newId].direction[1] = w.pd[old].direction[1];
+        w.pd[newId].directionSmoothed[0] = w.pd[old].directionSmoothed[0];
+        w.pd[newId].directionSmoothed[1] = w.pd[old].directionSmoothed[1];
+    }
+    for (int p = 0; p < MAXP; ++p) {
+        tt_plate_data &c = w.pd[p];
+        c.asthenosphereVelocity[0] = 0.0f;
+        c.asthenosphereVelocity[1] = 0.0f;
+        c.mass = 0.0f;
+        c.size = 0;
+        c.perimeter = 0;
+        c.breakScore = 0.0f;
+        c.used = 0;
+        if (p == old || p == newId) c.mergeWaitTime = w.ks.mergeWaitAfterSplit;
+    }
+    std::vector<long long> mass(MAXP + 1, 0);
+    std::vector<int> size(MAXP + 1, 0), per(MAXP + 1, 0);
+    static const int PO[4][2] = {{0, -1}, {1, 0}, {0, 1}, {-1, 0}};
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t c = (size_t)y * W + x;
+            const int pid = w.ids[c];
+            mass[pid] += tt_fix(w.height[c], MASS_BITS);
+            size[pid] += 1;
+            bool b = false;
+            for (int k = 0; k < 4; ++k)
+                if (w.ids[w.idx(x + PO[k][0], y + PO[k][1])] != pid) { b = true; break; }
+            if (b) per[pid] += 1;
+        }
+    for (int p = 0; p < MAXP; ++p) {
+        if (size[p] == 0) continue;
+        w.pd[p].mass = tt_unfix(mass[p], MASS_BITS);
+        w.pd[p].size = size[p];
+        w.pd[p].perimeter = per[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// executeIteration (plate_tectonic_sim.cu:54-131)
+// ---------------------------------------------------------------------------
+void step(World &w)
+{
+    const int mask = w.cfg.stage_mask;
+    std::vector<int> offx, offy;
+    std::vector<float> hW = w.height;
+    std::vector<uint8_t> idW = w.ids;
+    std::vector<long long> fric(MAXP, 0);
+    w.stats.overflow_cells = 0;
+    w.stats.complex_sources = 0;
+    if (mask & TT_STAGE_COLLIDE) {
+        stage_advect(w, offx, offy);
+        stage_collision_types(w, offx, offy);
+        stage_resolve(w, offx, offy, hW, idW, fric);
+        if (mask & TT_STAGE_UPLIFT) {
+            stage_propagate(w);
+            stage_jfa_uplift(w, hW, idW);
+        }
+    }
+    if (mask & TT_STAGE_THERMAL) stage_thermal(w, hW);
+    if (mask & TT_STAGE_HYDRAULIC) stage_hydraulic(w, hW);
+    w.height.swap(hW);
+    w.ids.swap(idW);
+    if (mask & TT_STAGE_KINEMATICS) {
+        stage_plate_movement(w, fric);
+        stage_accretion(w);
+    }
+    if ((mask & TT_STAGE_CCL) && (w.iter % (uint32_t)w.cfg.iterations_for_ccl) == 0) ccl_and_relabel(w);
+    if (mask & TT_STAGE_MERGE) {
+        stage_merge(w);
+        stage_break_score(w);
+        stage_centers(w);
+        stage_statistics(w);
+    }
+    if (mask & TT_STAGE_SPLIT) stage_split(w);
+    w.iter++;
+    w.stats.iteration = w.iter;
+    long long ledger = 0;
+    for (float h : w.height) ledger += tt_fix(h, MASS_BITS);
+    w.stats.mass_ledger = tt_unfix_d(ledger, MASS_BITS);
+}
+
+bool field_info(World &w, int f, void **ptr, size_t *bytes)
+{
+    const size_t N = (size_t)w.N;
+    switch (f) {
+        case TT_FIELD_HEIGHT: *ptr = w.height.data(); *bytes = N * 4; return true;
+        case TT_FIELD_PLATE_IDS: *ptr = w.ids.data(); *bytes = N; return true;
+        case TT_FIELD_WATER: *ptr = w.water.data(); *bytes = N * 4; return true;
+        case TT_FIELD_FLUX: *ptr = w.flux.data(); *bytes = N * 16; return true;
+        case TT_FIELD_VELOCITY: *ptr = w.velocity.data(); *bytes = N * 8; return true;
+        case TT_FIELD_SEDIMENT: *ptr = w.sediment.data(); *bytes = N * 4; return true;
+        case TT_FIELD_ASTHENOSPHERE: *ptr = w.asth.data(); *bytes = N * 8; return true;
+        case TT_FIELD_COLLISIONS: *ptr = w.collisions.data(); *bytes = N * 4; return true;
+        case TT_FIELD_ACCRETION: *ptr = w.accretion.data(); *bytes = N; return true;
+        case TT_FIELD_UPLIFT: *ptr = w.uplift.data(); *bytes = N * 8; return true;
+        case TT_FIELD_PROPAGATED: *ptr = w.prop.data(); *bytes = N * 12; return true;
+        case TT_FIELD_PLATE_DATA: *ptr = w.pd.data(); *bytes = sizeof(tt_plate_data) * MAXP; return true;
+        case TT_FIELD_DIVERGENCE_BITMAP: *ptr = w.divBitmap.data(); *bytes = NUM_TRI_WORDS * 4; return true;
+        case TT_FIELD_COLLISION_TYPES: *ptr = w.typeBitmap.data(); *bytes = NUM_TRI; return true;
+        case TT_FIELD_SORTED_KEYS: *ptr = w.keys.data(); *bytes = N * 8; return true;
+        case TT_FIELD_LABELS: *ptr = w.labels.data(); *bytes = N * 4; return true;
+        case TT_FIELD_DISTANCE: *ptr = w.dist.data(); *bytes = N * 8; return true;
+        default: return false;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+struct ow_handle { World w; };
+
+void *oracle_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, const uint8_t *noise256)
+{
+    if (!cfg || !ks || cfg->width <= 0 || cfg->height <= 0) return nullptr;
+    auto *h = new ow_handle();
+    World &w = h->w;
+    w.cfg = *cfg;
+    w.ks = *ks;
+    w.W = cfg->width;
+    w.H = cfg->height;
+    w.N = w.W * w.H;
+    const size_t N = (size_t)w.N;
+    w.height.assign(N, 0.0f);
+    w.water.assign(N, 0.0f);
+    w.sediment.assign(N, 0.0f);
+    w.ids.assign(N, 0);
+    w.flux.assign(N, F4{0, 0, 0, 0});
+    w.velocity.assign(N, V2{0, 0});
+    w.asth.assign(N, V2{0, 0});
+    w.collisions.assign(N, 0u);
+    w.accretion.assign(N, 0);
+    w.uplift.assign(N, UpliftData{0.0f, 0, {0, 0, 0}});
+    w.prop.assign(N, PropData{0.0f, 0u, 0, {0, 0, 0}});
+    w.pd.assign(MAXP, default_plate());
+    w.divBitmap.assign(NUM_TRI_WORDS, 0u);
+    w.typeBitmap.assign(NUM_TRI, 0);
+    w.keys.assign(N, 0ull);
+    w.labels.assign(N, 0u);
+    w.dist.assign(N, DistBuf{-1, 0u});
+    memset(&w.stats, 0, sizeof(w.stats));
+    w.stats.split_plate = -1;
+    w.stats.split_new_plate = -1;
+    w.noise.assign(256 * 256, 0);
+    if (noise256) memcpy(w.noise.data(), noise256, 256 * 256);
+    return h;
+}
+
+int oracle_world_init(void *hh)
+{
+    if (!hh) return TT_ERR_INVALID;
+    World &w = ((ow_handle *)hh)->w;
+    if (w.cfg.init_mode == TT_INIT_SYNTHETIC) init_synthetic(w);
+    else if (w.cfg.init_mode == TT_INIT_REFERENCE) init_reference(w);
+    else init_asthenosphere(w);
+    return TT_OK;
+}
+
+int oracle_world_step(void *hh, int n)
+{
+    if (!hh || n < 0) return TT_ERR_INVALID;
+    World &w = ((ow_handle *)hh)->w;
+    for (int i = 0; i < n; ++i) step(w);
+    return TT_OK;
+}
+
+int oracle_world_get(void *hh, int field, void *dst, size_t bytes)
+{
+    if (!hh || !dst) return TT_ERR_INVALID;
+    void *p;
+    size_t b;
+    if (!field_info(((ow_handle *)hh)->w, field, &p, &b) || b != bytes) return TT_ERR_INVALID;
+    memcpy(dst, p, b);
+    return TT_OK;
+}
+
+int oracle_world_set(void *hh, int field, const void *src, size_t bytes)
+{
+    if (!hh || !src) return TT_ERR_INVALID;
+    void *p;
+    size_t b;
+    if (!field_info(((ow_handle *)hh)->w, field, &p, &b) || b != bytes) return TT_ERR_INVALID;
+    memcpy(p, src, b);
+    return TT_OK;
+}
+
+int oracle_world_set_iteration(void *hh, uint32_t it)
+{
+    if (!hh) return TT_ERR_INVALID;
+    ((ow_handle *)hh)->w.iter = it;
+    return TT_OK;
+}
+
+int oracle_world_stats(void *hh, tt_step_stats *out)
+{
+    if (!hh || !out) return TT_ERR_INVALID;
+    *out = ((ow_handle *)hh)->w.stats;
+    return TT_OK;
+}
+
+int oracle_world_ccl(void *hh)
+{
+    if (!hh) return TT_ERR_INVALID;
+    ccl_labels(((ow_handle *)hh)->w);
+    return TT_OK;
+}
+
+void oracle_world_destroy(void *hh) { delete (ow_handle *)hh; }
+
+int oracle_math_eval(int fn, const float *in, float *out, long long n)
+{
+    for (long long i = 0; i < n; ++i) {
+        switch (fn) {
+            case TT_MATH_EXPF: out[i] = tt_expf(in[i]); break;
+            case TT_MATH_LOG10F: out[i] = tt_log10f(in[i]); break;
+            case TT_MATH_SINF: out[i] = tt_sinf(in[i]); break;
+            case TT_MATH_COSF: out[i] = tt_cosf(in[i]); break;
+            case TT_MATH_SIMPLEX: out[i] = tt_simplex(in[3 * i], in[3 * i + 1], in[3 * i + 2], 0.01f, 1234); break;
+            case TT_MATH_DISCRETE: out[i] = tt_discrete_noise(in[3 * i], in[3 * i + 1], in[3 * i + 2], 1.0f, 1001); break;
+            case TT_MATH_FMODF: out[i] = tt_fmodf(in[2 * i], in[2 * i + 1]); break;
+            case TT_MATH_ATAN2: out[i] = (float)tt_atan2((double)in[2 * i], (double)in[2 * i + 1]); break;
+            default: return TT_ERR_INVALID;
+        }
+    }
+    return TT_OK;
+}
+
+}  // extern "C"
