@@ -130,7 +130,7 @@ enum {
  * plate_tectonic_sim.cu:336, one thermal pass at :73). */
 typedef struct tt_sim_config {
     int32_t width;                 /* multiple of 64, >= 64 */
-    int32_t height;                /* multiple of 4, >= 64 */
+    int32_t height;                /* multiple of 16, >= 64 */
     uint32_t seed;
     int32_t num_plates;            /* 1..254 */
     int32_t num_voronoi_gens;      /* TT_INIT_REFERENCE only, 0..4 */
@@ -253,6 +253,8 @@ int tt_world_download(tt_world *w, int32_t field, void *host, size_t bytes);
 int tt_world_upload(tt_world *w, int32_t field, const void *host, size_t bytes);
 int tt_world_stats(tt_world *w, tt_step_stats *out);
 int tt_world_iteration(tt_world *w, uint32_t *iteration);
+/* Sets m_iterations (used when a state is moved between worlds). */
+int tt_world_set_iteration(tt_world *w, uint32_t iteration);
 /* Device bytes held by the world (persistent + transient), for admission. */
 int tt_world_bytes(const tt_sim_config *cfg, size_t *bytes);
 int tt_world_destroy(tt_world *w);

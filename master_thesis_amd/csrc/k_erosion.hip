@@ -1,0 +1,236 @@
+// k_erosion.hip -- stage D (thermal erosion) and stage E (hydraulic erosion)
+// on gfx950, LDS-tiled stencils over 64x16 tiles (one wavefront per row).
+//
+// Thermal: thermalErosionKernel (plate_tectonics_kernel.cu:2735-2786) in the
+// canonical Jacobi form -- each pass reads the previous heights; a cell's new
+// height is its old height minus its own outflows (directions 0..7) plus the
+// inflows it receives (directions 0..7).  The tile holds a two-cell halo.
+//
+// Hydraulic: one kernel per sub-iteration fuses rain (:2430-2437), flux
+// (:2439-2485), flow (:2487-2533), evaporate (:2587-2593) and the
+// erosion/deposition exchange (:2535-2570 with CudaTexture::Slope), and begins
+// with the semi-Lagrangian transport (:2572-2585, cuda_helper.cuh:49-65) of the
+// previous sub-iteration, gathered from its sediment and velocity.  The order of
+// float operations is the reference's; dt = 1 (plate_tectonic_sim.cu:338-363).
+#include "tt_kernels.h"
+
+static_assert(TT_TILE_W == 64 && TT_TILE_H == 16, "tiles are 64 x 16");
+
+#define TH_W (TT_TILE_W + 4)
+#define TH_H (TT_TILE_H + 4)
+#define TT_W (TT_TILE_W + 2)
+#define TT_H (TT_TILE_H + 2)
+
+__constant__ int c_toff[8][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}, {-1, -1}, {-1, 1}, {1, -1}, {1, 1}};
+
+__global__ __launch_bounds__(256) void k_thermal(tt_dev d, const float *hin, float *hout)
+{
+    __shared__ float hs[TH_H][TH_W];
+    __shared__ float tot[TT_H][TT_W];
+    const int tid = threadIdx.x;
+    const int nbx = d.W / TT_TILE_W, nby = d.H / TT_TILE_H;
+    const int b = tt_xcd_tile(blockIdx.x, nbx * nby);
+    const int x0 = (b % nbx) * TT_TILE_W, y0 = (b / nbx) * TT_TILE_H;
+    for (int i = tid; i < TH_H * TH_W; i += 256) {
+        const int ly = i / TH_W, lx = i - ly * TH_W;
+        const int gx = tt_wrap1(x0 + lx - 2, d.W), gy = tt_wrap1(y0 + ly - 2, d.H);
+        hs[ly][lx] = hin[(size_t)gy * d.W + gx];
+    }
+    __syncthreads();
+    const float talus = d.ks.thermalThresholdAngle * d.ks.thermalCellSize;
+    const float amp = d.ks.thermalErosionAmplitude, str = d.ks.thermalErosionStrength;
+    for (int i = tid; i < TT_H * TT_W; i += 256) {
+        const int ty = i / TT_W, tx = i - ty * TT_W;
+        const int ly = ty + 1, lx = tx + 1;
+        const float hc = hs[ly][lx];
+        float t = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float diff = hc - hs[ly + c_toff[k][1]][lx + c_toff[k][0]];
+            if (diff > talus) t = t + diff;
+        }
+        tot[ty][tx] = t;
+    }
+    __syncthreads();
+    const int lane = tid & 63, wv = tid >> 6;
+    for (int r = 0; r < 4; ++r) {
+        const int ry = wv * 4 + r;
+        const int ly = ry + 2, lx = lane + 2;
+        const float hc = hs[ly][lx];
+        const float tc = tot[ry + 1][lane + 1];
+        float v = hc;
+        if (tc > 0.0f) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float diff = hc - hs[ly + c_toff[k][1]][lx + c_toff[k][0]];
+                if (diff > talus) v = v - (amp * (diff / tc)) * str;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int sy = ly - c_toff[k][1], sx = lx - c_toff[k][0];
+            const float diff = hs[sy][sx] - hc;
+            const float ts = tot[sy - 1][sx - 1];
+            if (diff > talus && ts > 0.0f) v = v + (amp * (diff / ts)) * str;
+        }
+        hout[(size_t)(y0 + ry) * d.W + x0 + lane] = v;
+    }
+}
+
+hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, const float *hin, float *hout)
+{
+    const unsigned blocks = (unsigned)((d.W / TT_TILE_W) * (d.H / TT_TILE_H));
+    hipLaunchKernelGGL(k_thermal, dim3(blocks), dim3(256), 0, s, d, hin, hout);
+    return hipGetLastError();
+}
+
+// bilinear sample with toroidal wrap (interpolate, cuda_helper.cuh:49-65);
+// |v| <= 50 keeps the sample within one period of the grid.
+__device__ __forceinline__ float tt_bilinear(const float *s, float tx, float ty, int W, int H)
+{
+    const int x0 = (int)tt_floorf(tx), y0 = (int)tt_floorf(ty);
+    const float wx = tx - (float)x0, wy = ty - (float)y0;
+    const int xa = tt_wrapg(x0, W), xb = tt_wrapg(x0 + 1, W);
+    const size_t ra = (size_t)tt_wrapg(y0, H) * W, rb = (size_t)tt_wrapg(y0 + 1, H) * W;
+    const float I0 = (1.0f - wx) * s[ra + xa] + wx * s[ra + xb];
+    const float I1 = (1.0f - wx) * s[rb + xa] + wx * s[rb + xb];
+    return (1.0f - wy) * I0 + wy * I1;
+}
+
+__global__ __launch_bounds__(256) void k_hydraulic(tt_dev d, int first, int rseed, const float *__restrict__ hin,
+                                                   float *__restrict__ hout, const float *__restrict__ win,
+                                                   float *__restrict__ wout, const float4 *__restrict__ fin,
+                                                   float4 *__restrict__ fout, float2 *vel, const float *sprev,
+                                                   const float *sedState, float *sout)
+{
+    __shared__ float M[TH_H][TH_W];
+    __shared__ float Wt[TH_H][TH_W];
+    __shared__ float4 F[TT_H][TT_W];
+    const int tid = threadIdx.x;
+    const int W = d.W, H = d.H;
+    const int nbx = W / TT_TILE_W, nby = H / TT_TILE_H;
+    const int b = tt_xcd_tile(blockIdx.x, nbx * nby);
+    const int x0 = (b % nbx) * TT_TILE_W, y0 = (b / nbx) * TT_TILE_H;
+    const float rain = d.ks.hydrationRainfall;
+    // heights and water (after rain) on the tile plus a two-cell halo
+    for (int i = tid; i < TH_H * TH_W; i += 256) {
+        const int ly = i / TH_W, lx = i - ly * TH_W;
+        const int gx = tt_wrap1(x0 + lx - 2, W), gy = tt_wrap1(y0 + ly - 2, H);
+        const size_t g = (size_t)gy * W + gx;
+        M[ly][lx] = hin[g];
+        const float n = tt_discrete_noise((float)gx, 0.0f, (float)gy, 1.0f, rseed);
+        Wt[ly][lx] = win[g] + (tt_fmaxf(0.0f, n) * 1.0f) * rain;
+    }
+    __syncthreads();
+    // outflow flux on the tile plus a one-cell halo
+    const float fk = 1.0f * d.ks.hydrationPipeCrossSection;
+    const float g = d.ks.gravity, L = d.ks.hydrationPipeLength;
+    for (int i = tid; i < TT_H * TT_W; i += 256) {
+        const int fy = i / TT_W, fx = i - fy * TT_W;
+        const int ly = fy + 1, lx = fx + 1;
+        const int gx = tt_wrap1(x0 + fx - 1, W), gy = tt_wrap1(y0 + fy - 1, H);
+        const float4 cf = fin[(size_t)gy * W + gx];
+        const float sc = M[ly][lx] + Wt[ly][lx];
+        float4 o;
+        o.x = tt_fmaxf(0.0f, cf.x + fk * ((g * ((sc - M[ly][lx - 1]) - Wt[ly][lx - 1])) / L));
+        o.y = tt_fmaxf(0.0f, cf.y + fk * ((g * ((sc - M[ly + 1][lx]) - Wt[ly + 1][lx])) / L));
+        o.z = tt_fmaxf(0.0f, cf.z + fk * ((g * ((sc - M[ly][lx + 1]) - Wt[ly][lx + 1])) / L));
+        o.w = tt_fmaxf(0.0f, cf.w + fk * ((g * ((sc - M[ly - 1][lx]) - Wt[ly - 1][lx])) / L));
+        const float tot = (((o.x + o.y) + o.z) + o.w) + 1e-6f;
+        float K = tt_fminf(1.0f, Wt[ly][lx] / (tot * 1.0f + 1e-6f));
+        K = tt_fmaxf(K, 0.01f);
+        o.x = o.x * K;
+        o.y = o.y * K;
+        o.z = o.z * K;
+        o.w = o.w * K;
+        F[fy][fx] = o;
+    }
+    __syncthreads();
+    const int lane = tid & 63, wv = tid >> 6;
+    const float evap = d.ks.hydrationEvaporation;
+    const float cap = d.ks.sedimentCapacity, ke = d.ks.sedimentErosionRate, kd = d.ks.sedimentDepositionRate;
+    for (int r = 0; r < 4; ++r) {
+        const int ry = wv * 4 + r;
+        const int gx = x0 + lane, gy = y0 + ry;
+        const size_t c = (size_t)gy * W + gx;
+        const int fy = ry + 1, fx = lane + 1;
+        const float4 S = F[fy][fx], Lf = F[fy][fx - 1], Uf = F[fy + 1][fx], Rf = F[fy][fx + 1], Df = F[fy - 1][fx];
+        float in = 0.0f;
+        in = in + Lf.z;
+        in = in + Uf.w;
+        in = in + Rf.x;
+        in = in + Df.y;
+        const float out = ((S.x + S.y) + S.z) + S.w;
+        const float dv = 1.0f * (in - out);
+        const int ly = ry + 2, lx = lane + 2;
+        const float wn = tt_fmaxf(0.0f, Wt[ly][lx] + dv);
+        const float nx = (((Lf.z - S.x) + S.z) - Rf.x) * 0.5f;
+        const float ny = (((Uf.w - S.y) + S.y) - Df.w) * 0.5f;
+        const float hh = tt_fmaxf(wn, 0.01f);
+        float vx = nx / hh, vy = ny / hh;
+        const float vm = sqrtf(vx * vx + vy * vy);
+        if (vm > 50.0f) {
+            const float sc = 50.0f / vm;
+            vx = vx * sc;
+            vy = vy * sc;
+        }
+        wout[c] = tt_fmaxf(0.0f, wn - (tt_fminf(wn, 20.0f) * evap) * 1.0f);
+        fout[c] = S;
+        float sed;
+        if (first) {
+            sed = sedState[c];
+        } else {
+            const float2 pv = vel[c];
+            sed = tt_bilinear(sprev, (float)gx - pv.x * 1.0f, (float)gy - pv.y * 1.0f, W, H);
+        }
+        vel[c] = make_float2(vx, vy);
+        const float mc = M[ly][lx];
+        // Slope: (m[+1] - m[-1]) / 2.0 in the reference; halving is exact in float
+        const float dzdx = (M[ly][lx + 1] - M[ly][lx - 1]) * 0.5f;
+        const float dzdy = (M[ly + 1][lx] - M[ly - 1][lx]) * 0.5f;
+        const float slope = sqrtf(dzdx * dzdx + dzdy * dzdy);
+        const float vmag = sqrtf(vx * vx + vy * vy);
+        float C = (cap * tt_sinf(tt_fmaxf(slope, 0.03f))) * vmag;
+        C = tt_fminf(C, 5.0f);
+        float hnew;
+        if (C > sed) {
+            float s = (ke * (C - sed)) * 1.0f;
+            s = tt_fminf(s, mc);
+            hnew = mc - s;
+            sed = sed + s;
+        } else {
+            float s = (kd * (sed - C)) * 1.0f;
+            s = tt_fminf(s, sed);
+            sed = sed - s;
+            hnew = mc + s;
+        }
+        hout[c] = hnew;
+        sout[c] = sed;
+    }
+}
+
+hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int first, int rseed, const float *hin, float *hout,
+                               const float *win, float *wout, const float4 *fin, float4 *fout, float2 *vel,
+                               const float *sprev, const float *sedState, float *sout)
+{
+    const unsigned blocks = (unsigned)((d.W / TT_TILE_W) * (d.H / TT_TILE_H));
+    hipLaunchKernelGGL(k_hydraulic, dim3(blocks), dim3(256), 0, s, d, first, rseed, hin, hout, win, wout, fin, fout,
+                       vel, sprev, sedState, sout);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_transport(tt_dev d, const float *sprev, const float2 *vel, float *sout)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= d.N) return;
+    const int x = (int)(c % d.W), y = (int)(c / d.W);
+    const float2 v = vel[c];
+    sout[c] = tt_bilinear(sprev, (float)x - v.x * 1.0f, (float)y - v.y * 1.0f, d.W, d.H);
+}
+
+hipError_t tt_launch_transport(hipStream_t s, const tt_dev &d, const float *sprev, const float2 *vel, float *sout)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_transport, dim3(blocks), dim3(256), 0, s, d, sprev, vel, sout);
+    return hipGetLastError();
+}

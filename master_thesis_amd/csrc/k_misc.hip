@@ -1,0 +1,322 @@
+// k_misc.hip -- device-wide scan and ordered compaction (wavefront ballot /
+// prefix primitives), world initialisation kernels and the device math
+// self-test.
+#include "tt_kernels.h"
+
+// ---------------------------------------------------------------------------
+// exclusive scan of uint32, 1024 elements per workgroup (256 threads x 4)
+// ---------------------------------------------------------------------------
+#define SCAN_B 1024
+
+__device__ __forceinline__ uint32_t tt_wave_incl_scan(uint32_t v, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// exclusive scan of the 256 per-thread values of a workgroup; returns the
+// workgroup total through *total
+__device__ __forceinline__ uint32_t tt_block_excl_scan(uint32_t v, uint32_t *wsum, uint32_t *total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t incl = tt_wave_incl_scan(v, lane);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < w) base += wsum[k];
+        tot += wsum[k];
+    }
+    *total = tot;
+    __syncthreads();
+    return base + incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_block(const uint32_t *in, uint32_t *out, long long n, uint32_t *bsum)
+{
+    __shared__ uint32_t wsum[4];
+    const long long base = (long long)blockIdx.x * SCAN_B + threadIdx.x * 4;
+    uint32_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = base + k < n ? in[base + k] : 0u;
+    const uint32_t t = v[0] + v[1] + v[2] + v[3];
+    uint32_t tot;
+    uint32_t ex = tt_block_excl_scan(t, wsum, &tot);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0 && bsum) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t *out, long long n, const uint32_t *boff)
+{
+    const long long base = (long long)blockIdx.x * SCAN_B + threadIdx.x * 4;
+    const uint32_t a = boff[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + k < n) out[base + k] += a;
+}
+
+__global__ void k_scan_total(const uint32_t *in, const uint32_t *out, long long n, uint32_t *total)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *total = n > 0 ? out[n - 1] + in[n - 1] : 0u;
+}
+
+size_t tt_scan_tmp_words(long long n)
+{
+    size_t words = 0;
+    while (n > SCAN_B) {
+        n = (n + SCAN_B - 1) / SCAN_B;
+        words += (size_t)n;
+    }
+    return words + 16;
+}
+
+static hipError_t scan_rec(hipStream_t s, const uint32_t *in, uint32_t *out, long long n, uint32_t *tmp)
+{
+    const long long nb = (n + SCAN_B - 1) / SCAN_B;
+    if (nb <= 1) {
+        hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(256), 0, s, in, out, n, (uint32_t *)nullptr);
+        return hipGetLastError();
+    }
+    uint32_t *bsum = tmp;
+    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(256), 0, s, in, out, n, bsum);
+    hipError_t e = scan_rec(s, bsum, bsum, nb, tmp + nb);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(256), 0, s, out, n, (const uint32_t *)bsum);
+    return hipGetLastError();
+}
+
+hipError_t tt_scan_u32(hipStream_t s, const uint32_t *in, uint32_t *out, long long n, uint32_t *tmp, uint32_t *total)
+{
+    if (n <= 0) return hipSuccess;
+    // the last input is needed for the total; keep a copy when scanning in place
+    if (in == out && total) {
+        hipError_t e = hipMemcpyAsync(tmp, in + n - 1, 4, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        e = scan_rec(s, in, out, n, tmp + 1);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, s, (const uint32_t *)tmp, out + n - 1, 1, total);
+        return hipGetLastError();
+    }
+    hipError_t e = scan_rec(s, in, out, n, tmp);
+    if (e != hipSuccess) return e;
+    if (total) {
+        hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, s, in, (const uint32_t *)out, n, total);
+        return hipGetLastError();
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// ordered compaction: out_idx[k] = k-th index i (ascending) with flags[i] != 0
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_compact_count(const uint8_t *flags, long long n, uint32_t *bcount)
+{
+    __shared__ uint32_t wsum[4];
+    const long long base = (long long)blockIdx.x * SCAN_B + threadIdx.x * 4;
+    uint32_t t = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t += (base + k < n && flags[base + k] != 0) ? 1u : 0u;
+    uint32_t tot;
+    tt_block_excl_scan(t, wsum, &tot);
+    if (threadIdx.x == 0) bcount[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_compact_write(const uint8_t *flags, long long n, const uint32_t *boff,
+                                                       uint32_t *out)
+{
+    __shared__ uint32_t wsum[4];
+    const long long base = (long long)blockIdx.x * SCAN_B + threadIdx.x * 4;
+    uint32_t f[4], t = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        f[k] = (base + k < n && flags[base + k] != 0) ? 1u : 0u;
+        t += f[k];
+    }
+    uint32_t tot;
+    uint32_t pos = boff[blockIdx.x] + tt_block_excl_scan(t, wsum, &tot);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (f[k]) out[pos++] = (uint32_t)(base + k);
+}
+
+hipError_t tt_compact_u8(hipStream_t s, const uint8_t *flags, long long n, uint32_t *out_idx, uint32_t *count,
+                         uint32_t *tmp)
+{
+    const long long nb = (n + SCAN_B - 1) / SCAN_B;
+    uint32_t *bc = tmp;
+    uint32_t *stmp = tmp + nb;
+    hipLaunchKernelGGL(k_compact_count, dim3((unsigned)nb), dim3(256), 0, s, flags, n, bc);
+    hipError_t e = tt_scan_u32(s, bc, bc, nb, stmp, count);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_compact_write, dim3((unsigned)nb), dim3(256), 0, s, flags, n, (const uint32_t *)bc, out_idx);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, uint32_t v, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t tt_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, long long n)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// initialisation
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t tt_mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_init_raw(tt_dev d, uint64_t hseed, double *raw)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= d.N) return;
+    const uint64_t z = tt_mix64(hseed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull);
+    raw[i] = (double)(z >> 40) * (1.0 / 16777216.0);
+}
+
+__global__ __launch_bounds__(256) void k_blur_x(tt_dev d, const double *in, double *out, const double *wts)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= d.N) return;
+    const int x = (int)(i % d.W), y = (int)(i / d.W);
+    double s = 0.0;
+    for (int k = -32; k <= 32; ++k) s = s + wts[k + 32] * in[(size_t)y * d.W + tt_wrapg(x + k, d.W)];
+    out[i] = s;
+}
+
+__global__ __launch_bounds__(256) void k_blur_y(tt_dev d, const double *in, float *out, const double *wts)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= d.N) return;
+    const int x = (int)(i % d.W), y = (int)(i / d.W);
+    double s = 0.0;
+    for (int k = -32; k <= 32; ++k) s = s + wts[k + 32] * in[(size_t)tt_wrapg(y + k, d.H) * d.W + x];
+    out[i] = (float)s;
+}
+
+hipError_t tt_launch_init_synthetic_heights(hipStream_t s, const tt_dev &d, uint64_t hseed, double *raw, double *tmp,
+                                            const double *wts, float *height)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_init_raw, dim3(blocks), dim3(256), 0, s, d, hseed, raw);
+    hipLaunchKernelGGL(k_blur_x, dim3(blocks), dim3(256), 0, s, d, (const double *)raw, tmp, wts);
+    hipLaunchKernelGGL(k_blur_y, dim3(blocks), dim3(256), 0, s, d, (const double *)tmp, height, wts);
+    return hipGetLastError();
+}
+
+// initHeightmap (plate_tectonics_kernel.cu:68-99)
+__global__ __launch_bounds__(256) void k_init_ref_heights(tt_dev d, float *height)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= d.N) return;
+    const int x = (int)(i % d.W), y = (int)(i / d.W);
+    const float dlt = 0.001f;
+    float value = 0.0f, amp = 0.9f, freq = 0.01f;
+    for (int o = 0; o < 4; ++o) {
+        const float v = tt_simplex((float)x, (float)y, 0.0f, freq, d.seed + o);
+        const float vx = tt_simplex((float)x + dlt, (float)y, 0.0f, freq, d.seed + o);
+        const float vy = tt_simplex((float)x, (float)y + dlt, 0.0f, freq, d.seed + o);
+        const float dx = (vx - v) / dlt, dy = (vy - v) / dlt;
+        const float q = v / (1.0f + sqrtf(dx * dx + dy * dy));
+        value = value + (0.0f + 1.0f * ((q - -1.0f) / (1.0f - -1.0f))) * amp;
+        amp = amp * 0.1f;
+        freq = freq * 10.0f;
+    }
+    height[i] = 5000.0f + value * (15000.0f - 5000.0f);
+}
+
+hipError_t tt_launch_init_reference_heights(hipStream_t s, const tt_dev &d, float *height)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_init_ref_heights, dim3(blocks), dim3(256), 0, s, d, height);
+    return hipGetLastError();
+}
+
+// initPlateIDs (plate_tectonics_kernel.cu:31-66)
+__global__ __launch_bounds__(256) void k_init_plate_ids(tt_dev d, const float *sx, const float *sy, const uint8_t *sid,
+                                                        int nseeds, uint8_t *ids)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= d.N) return;
+    const float x = (float)(int)(i % d.W), y = (float)(int)(i / d.W);
+    const float fw = (float)d.W, fh = (float)d.H;
+    float best = 3.402823466e38f;
+    int id = 0;
+    for (int k = 0; k < nseeds; ++k) {
+        const float dx = tt_fabsf(x - sx[k]), dy = tt_fabsf(y - sy[k]);
+        const float wx = tt_fminf(dx, fw - dx), wy = tt_fminf(dy, fh - dy);
+        const float d2 = wx * wx + wy * wy;
+        if (d2 < best) { best = d2; id = sid[k]; }
+    }
+    ids[i] = (uint8_t)id;
+}
+
+hipError_t tt_launch_init_plate_ids(hipStream_t s, const tt_dev &d, const float *sx, const float *sy, const uint8_t *sid,
+                                    int nseeds, uint8_t *ids)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_init_plate_ids, dim3(blocks), dim3(256), 0, s, d, sx, sy, sid, nseeds, ids);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_asthenosphere(tt_dev d, const uint8_t *noise, float2 *out)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= d.N) return;
+    float ax, ay;
+    tt_asth_at(noise, (int)(i % d.W), (int)(i / d.W), d.axs, d.ays, ax, ay);
+    out[i] = make_float2(ax, ay);
+}
+
+hipError_t tt_launch_asthenosphere(hipStream_t s, const tt_dev &d, const uint8_t *noise, float2 *out)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_asthenosphere, dim3(blocks), dim3(256), 0, s, d, noise, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// math self-test
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_math_eval(int fn, const float *in, float *out, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float r = 0.0f;
+    switch (fn) {
+        case TT_MATH_EXPF: r = tt_expf(in[i]); break;
+        case TT_MATH_LOG10F: r = tt_log10f(in[i]); break;
+        case TT_MATH_SINF: r = tt_sinf(in[i]); break;
+        case TT_MATH_COSF: r = tt_cosf(in[i]); break;
+        case TT_MATH_SIMPLEX: r = tt_simplex(in[3 * i], in[3 * i + 1], in[3 * i + 2], 0.01f, 1234); break;
+        case TT_MATH_DISCRETE: r = tt_discrete_noise(in[3 * i], in[3 * i + 1], in[3 * i + 2], 1.0f, 1001); break;
+        case TT_MATH_FMODF: r = tt_fmodf(in[2 * i], in[2 * i + 1]); break;
+        case TT_MATH_ATAN2: r = (float)tt_atan2((double)in[2 * i], (double)in[2 * i + 1]); break;
+        default: break;
+    }
+    out[i] = r;
+}
+
+hipError_t tt_launch_math_eval(hipStream_t s, int fn, const float *in, float *out, long long n)
+{
+    hipLaunchKernelGGL(k_math_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fn, in, out, n);
+    return hipGetLastError();
+}

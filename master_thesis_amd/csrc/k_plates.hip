@@ -1,0 +1,949 @@
+// k_plates.hip -- stages F and G on gfx950: plate kinematics, accretion, CCL of
+// plate masks with relabelling, merging, per-plate sums and plate splitting.
+//
+// Reference: applyPlateMovementChanges (plate_tectonics_kernel.cu:1863-1925),
+// applyAccretion (:747-770), ccl.cu:35-177 with applyCCL
+// (plate_tectonic_sim.cu:375-468), createLabelIdLookup / postCClIdReassign /
+// assignUnassignedIdsToNeighbor (:3130-3206, :2052-2105), mergePlates
+// (plate_tectonic_sim.cu:470-494, :1283-1392, :3208-3225, :102-197),
+// computeBreakScore (:2706-2733), accumulatePlateAngularCoords /
+// calculatePlateCenters (:2154-2226), statisticsPass (:2377-2402) and
+// SplitPlateV2 (plate_tectonic_sim.cu:933-1040, :2824-3112, :3227-3247).
+//
+// Float sums that feed integer decisions are exact int64 fixed-point sums
+// (order-free), so every grid decomposition gives the serial result.
+#include "tt_kernels.h"
+#include "tt_stage.h"
+
+// ---------------------------------------------------------------------------
+// G1: applyPlateMovementChanges
+// ---------------------------------------------------------------------------
+__global__ void k_plate_movement(tt_dev d, tt_plate_data *pd, const long long *fric, const uint8_t *noise)
+{
+    const int p = threadIdx.x;
+    if (p >= TT_MAXP) return;
+    const tt_kernel_settings &K = d.ks;
+    tt_plate_data cur = pd[p];
+    const float cx = cur.pixelCenter[0] + cur.direction[0] * cur.velocity;
+    const float cy = cur.pixelCenter[1] + cur.direction[1] * cur.velocity;
+    const float nx = cx - tt_floorf(cx), ny = cy - tt_floorf(cy);
+    cur.hasMoved = (nx != cx || ny != cy) ? 1 : 0;
+    cur.pixelCenter[0] = nx;
+    cur.pixelCenter[1] = ny;
+    const float frictionLoss = tt_unfix(fric[p], TT_FRIC_BITS);
+    const float ovx = cur.direction[0] * cur.velocity, ovy = cur.direction[1] * cur.velocity;
+    const int sx = (int)(cur.geometricCenter[0] + 0.5f), sy = (int)(cur.geometricCenter[1] + 0.5f);
+    float asx, asy;
+    tt_asth_at(noise, tt_wrapg(sx, d.W), tt_wrapg(sy, d.H), d.axs, d.ays, asx, asy);
+    const float avx = tt_fmaxf(tt_fminf((float)((double)(asx * K.curlNoiseMultiplier) + (asx < 0 ? -0.2 : 0.2)), 1.0f), -1.0f);
+    const float avy = tt_fmaxf(tt_fminf((float)((double)(asy * K.curlNoiseMultiplier) + (asy < 0 ? -0.2 : 0.2)), 1.0f), -1.0f);
+    const float m0 = tt_fmaxf(sqrtf(ovx * ovx + ovy * ovy), 1e-20f);
+    const float post = tt_fmaxf(m0 - frictionLoss, 0.0f);
+    const float pvx = cur.direction[0] * post, pvy = cur.direction[1] * post;
+    const float rx = avx - pvx, ry = avy - pvy;
+    const float rm = sqrtf(rx * rx + ry * ry);
+    const float dd = (K.environmentalDragCoefficient * rm) * rm;
+    const float rmz = tt_fmaxf(sqrtf(rx * rx + ry * ry), 1e-20f);
+    const float rnx = rx / rmz, rny = ry / rmz;
+    const float nvx = pvx + rnx * dd, nvy = pvy + rny * dd;
+    const float nm = tt_fmaxf(sqrtf(nvx * nvx + nvy * nvy), 1e-20f);
+    cur.velocity = nm;
+    cur.direction[0] = nvx / nm;
+    cur.direction[1] = nvy / nm;
+    cur.velocitySmoothed = cur.velocitySmoothed * K.velocitySmoothingFactor + cur.velocity * (1.0f - K.velocitySmoothingFactor);
+    const float sdx = cur.directionSmoothed[0] * K.directionSmoothingFactor + cur.direction[0] * (1.0f - K.directionSmoothingFactor);
+    const float sdy = cur.directionSmoothed[1] * K.directionSmoothingFactor + cur.direction[1] * (1.0f - K.directionSmoothingFactor);
+    const float sm = tt_fmaxf(sqrtf(sdx * sdx + sdy * sdy), 1e-20f);
+    cur.directionSmoothed[0] = sdx / sm;
+    cur.directionSmoothed[1] = sdy / sm;
+    cur.mergeWaitTime = cur.mergeWaitTime > 0 ? cur.mergeWaitTime - 1 : 0;
+    pd[p] = cur;
+}
+
+hipError_t tt_launch_plate_movement(hipStream_t s, const tt_dev &d, tt_plate_data *pd, const long long *fric,
+                                    const uint8_t *noise)
+{
+    hipLaunchKernelGGL(k_plate_movement, dim3(1), dim3(256), 0, s, d, pd, fric, noise);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// B5: applyAccretion.  accw holds (highest writer cell + 1); the value the
+// writer recorded is MAX_PLATE_COUNT - its new owner, i.e. the accreted id is
+// the writer's owner after resolution.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_accretion_apply(tt_dev d, const uint32_t *accw, const float *height,
+                                                         const uint8_t *idsIn, uint8_t *idsOut, uint8_t *accretion)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= d.N) return;
+    const uint32_t a = accw[c];
+    uint8_t id = idsIn[c];
+    uint8_t val = 0;
+    if (a != 0u) {
+        const uint8_t owner = idsIn[a - 1u];
+        val = (uint8_t)(TT_MAXP - owner);
+        const uint8_t aid = (uint8_t)(TT_MAXP - val);
+        if (aid != TT_MAXP && height[c] >= d.ks.continentalCrustThreshold) id = aid;
+    }
+    idsOut[c] = id;
+    accretion[c] = val;
+}
+
+hipError_t tt_launch_accretion_apply(hipStream_t s, const tt_dev &d, const uint32_t *accw, const float *height,
+                                     const uint8_t *idsIn, uint8_t *idsOut, uint8_t *accretion)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_accretion_apply, dim3(blocks), dim3(256), 0, s, d, accw, height, idsIn, idsOut, accretion);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// F1-F4: connected components of the reference's edge set.  For cell c with
+// neighbours tl, t, tr, l (wrapped): the first same-id neighbour in that order
+// (init, ccl.cu:107-116), plus tl and l, plus t and tr on row 0 (reduce,
+// ccl.cu:161-176).  Label = minimum cell index of the component.
+// ---------------------------------------------------------------------------
+// edge bits: 1 = tl, 2 = t, 4 = tr, 8 = l
+__device__ __forceinline__ unsigned tt_ccl_edges(uint8_t id, uint8_t tl, uint8_t t, uint8_t tr, uint8_t l, bool row0)
+{
+    unsigned m = 0;
+    if (id == tl) m |= 1u;
+    else if (id == t) m |= 2u;
+    else if (id == tr) m |= 4u;
+    else if (id == l) m |= 8u;
+    if (id == tl) m |= 1u;
+    if (id == l) m |= 8u;
+    if (row0) {
+        if (id == t) m |= 2u;
+        if (id == tr) m |= 4u;
+    }
+    return m;
+}
+
+// union-find in LDS (32-bit parents), linking the larger root under the
+// smaller with atomicMin as in unionStep (ccl.cu:60-85)
+__device__ __forceinline__ unsigned tt_find32(const unsigned *par, unsigned a)
+{
+    unsigned p = par[a];
+    while (p != a) { a = p; p = par[a]; }
+    return a;
+}
+
+__device__ __forceinline__ void tt_union32(unsigned *par, unsigned a, unsigned b)
+{
+    bool done = false;
+    while (!done) {
+        a = tt_find32(par, a);
+        b = tt_find32(par, b);
+        if (a < b) {
+            const unsigned old = atomicMin(&par[b], a);
+            done = old == b;
+            b = old;
+        } else if (b < a) {
+            const unsigned old = atomicMin(&par[a], b);
+            done = old == a;
+            a = old;
+        } else {
+            done = true;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ccl_local(tt_dev d, const uint8_t *ids, uint32_t *labels)
+{
+    __shared__ unsigned par[TT_TILE_H * TT_TILE_W];
+    __shared__ uint8_t tid8[(TT_TILE_H + 1) * (TT_TILE_W + 2)];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TT_TILE_W, y0 = blockIdx.y * TT_TILE_H;
+    const int TW2 = TT_TILE_W + 2;
+    for (int i = tid; i < (TT_TILE_H + 1) * TW2; i += 256) {
+        const int ly = i / TW2, lx = i - ly * TW2;
+        const int gx = tt_wrap1(x0 + lx - 1, d.W), gy = tt_wrap1(y0 + ly - 1, d.H);
+        tid8[i] = ids[(size_t)gy * d.W + gx];
+    }
+    for (int i = tid; i < TT_TILE_H * TT_TILE_W; i += 256) par[i] = (unsigned)i;
+    __syncthreads();
+    for (int i = tid; i < TT_TILE_H * TT_TILE_W; i += 256) {
+        const int ly = i / TT_TILE_W, lx = i - ly * TT_TILE_W;
+        const int r = ly + 1, cc = lx + 1;
+        const uint8_t id = tid8[r * TW2 + cc];
+        const unsigned m = tt_ccl_edges(id, tid8[(r - 1) * TW2 + cc - 1], tid8[(r - 1) * TW2 + cc],
+                                        tid8[(r - 1) * TW2 + cc + 1], tid8[r * TW2 + cc - 1], (y0 + ly) == 0);
+        if ((m & 1u) && ly > 0 && lx > 0) tt_union32(par, (unsigned)i, (unsigned)(i - TT_TILE_W - 1));
+        if ((m & 2u) && ly > 0) tt_union32(par, (unsigned)i, (unsigned)(i - TT_TILE_W));
+        if ((m & 4u) && ly > 0 && lx < TT_TILE_W - 1) tt_union32(par, (unsigned)i, (unsigned)(i - TT_TILE_W + 1));
+        if ((m & 8u) && lx > 0) tt_union32(par, (unsigned)i, (unsigned)(i - 1));
+    }
+    __syncthreads();
+    for (int i = tid; i < TT_TILE_H * TT_TILE_W; i += 256) {
+        const unsigned r = tt_find32(par, (unsigned)i);
+        const int ry = (int)(r / TT_TILE_W), rx = (int)(r % TT_TILE_W);
+        const int ly = i / TT_TILE_W, lx = i - ly * TT_TILE_W;
+        labels[(size_t)(y0 + ly) * d.W + x0 + lx] = (uint32_t)((size_t)(y0 + ry) * d.W + x0 + rx);
+    }
+}
+
+hipError_t tt_launch_ccl_local(hipStream_t s, const tt_dev &d, const uint8_t *ids, uint32_t *labels)
+{
+    hipLaunchKernelGGL(k_ccl_local, dim3(d.W / TT_TILE_W, d.H / TT_TILE_H), dim3(256), 0, s, d, ids, labels);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ uint32_t tt_gfind(const uint32_t *lab, uint32_t a)
+{
+    uint32_t p = lab[a];
+    while (p != a) { a = p; p = lab[a]; }
+    return a;
+}
+
+__device__ __forceinline__ void tt_gunion(uint32_t *lab, uint32_t a, uint32_t b)
+{
+    bool done = false;
+    while (!done) {
+        a = tt_gfind(lab, a);
+        b = tt_gfind(lab, b);
+        if (a < b) {
+            const uint32_t old = atomicMin(&lab[b], a);
+            done = old == b;
+            b = old;
+        } else if (b < a) {
+            const uint32_t old = atomicMin(&lab[a], b);
+            done = old == a;
+            a = old;
+        } else {
+            done = true;
+        }
+    }
+}
+
+// edges that cross tile borders: top row (tl, t, tr), left column (tl, l),
+// right column (tr)
+__global__ __launch_bounds__(256) void k_ccl_merge(tt_dev d, const uint8_t *ids, uint32_t *labels)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= d.N) return;
+    const int x = (int)(c % d.W), y = (int)(c / d.W);
+    const int lx = x % TT_TILE_W, ly = y % TT_TILE_H;
+    if (ly != 0 && lx != 0 && lx != TT_TILE_W - 1) return;
+    const int W = d.W, H = d.H;
+    const int xl = tt_wrap1(x - 1, W), xr = tt_wrap1(x + 1, W), yu = tt_wrap1(y - 1, H);
+    const uint8_t id = ids[c];
+    const uint32_t ntl = (uint32_t)yu * W + xl, nt = (uint32_t)yu * W + x, ntr = (uint32_t)yu * W + xr;
+    const uint32_t nl = (uint32_t)y * W + xl;
+    const unsigned m = tt_ccl_edges(id, ids[ntl], ids[nt], ids[ntr], ids[nl], y == 0);
+    const uint32_t cc = (uint32_t)c;
+    if ((m & 1u) && (ly == 0 || lx == 0)) tt_gunion(labels, cc, ntl);
+    if ((m & 2u) && ly == 0) tt_gunion(labels, cc, nt);
+    if ((m & 4u) && (ly == 0 || lx == TT_TILE_W - 1)) tt_gunion(labels, cc, ntr);
+    if ((m & 8u) && lx == 0) tt_gunion(labels, cc, nl);
+}
+
+hipError_t tt_launch_ccl_merge(hipStream_t s, const tt_dev &d, const uint8_t *ids, uint32_t *labels)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_ccl_merge, dim3(blocks), dim3(256), 0, s, d, ids, labels);
+    return hipGetLastError();
+}
+
+// final labels, component sizes (wavefront-aggregated atomics) and root flags
+__global__ __launch_bounds__(256) void k_ccl_flatten(tt_dev d, uint32_t *labels, uint32_t *rootCount,
+                                                     uint8_t *rootFlag)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = c < d.N;
+    uint32_t lab = 0;
+    if (valid) {
+        lab = tt_gfind(labels, (uint32_t)c);
+        rootFlag[c] = lab == (uint32_t)c ? 1 : 0;
+    }
+    // the forest is static here (all unions are done); writing a root into
+    // labels[c] only shortens paths other threads may be following
+    unsigned long long pending = __ballot(valid);
+    const int lane = threadIdx.x & 63;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t ll = __shfl(lab, leader, 64);
+        const unsigned long long same = __ballot(valid && lab == ll) & pending;
+        if (lane == leader) atomicAdd(&rootCount[ll], (uint32_t)__popcll(same));
+        pending &= ~same;
+    }
+    if (valid) labels[c] = lab;
+}
+
+hipError_t tt_launch_ccl_flatten(hipStream_t s, const tt_dev &d, uint32_t *labels, uint32_t *rootCount,
+                                 uint8_t *rootFlag)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_ccl_flatten, dim3(blocks), dim3(256), 0, s, d, labels, rootCount, rootFlag);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// F5: relabel.  Fragment key sorts ascending as (count desc, label desc).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_frag_build(tt_dev d, const uint32_t *rootList, const uint32_t *nroots,
+                                                    const uint32_t *rootCount, const uint8_t *ids,
+                                                    unsigned long long *fragKey, int *maxCount, uint8_t *newId)
+{
+    const uint32_t n = *nroots;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t r = rootList[i];
+    const uint32_t cnt = rootCount[r];
+    fragKey[i] = ((unsigned long long)(0xFFFFFFFFu - cnt) << 32) | (unsigned long long)(0xFFFFFFFFu - r);
+    atomicMax(&maxCount[ids[r]], (int)cnt);
+    newId[r] = (uint8_t)TT_MAXP;
+}
+
+hipError_t tt_launch_frag_build(hipStream_t s, const tt_dev &d, const uint32_t *rootList, const uint32_t *nroots,
+                                uint32_t nroots_host, const uint32_t *rootCount, const uint8_t *ids,
+                                unsigned long long *fragKey, int *maxCount, uint8_t *newId)
+{
+    if (nroots_host == 0) return hipSuccess;
+    const unsigned blocks = (nroots_host + 255) / 256;
+    hipLaunchKernelGGL(k_frag_build, dim3(blocks), dim3(256), 0, s, d, rootList, nroots, rootCount, ids, fragKey,
+                       maxCount, newId);
+    return hipGetLastError();
+}
+
+// single-workgroup bitonic sort of up to `capacity` (power of two) keys
+__global__ __launch_bounds__(1024) void k_frag_sort(unsigned long long *key, const uint32_t *nroots, int capacity)
+{
+    extern __shared__ unsigned long long sk[];
+    const uint32_t n = *nroots;
+    for (int i = threadIdx.x; i < capacity; i += 1024) sk[i] = (uint32_t)i < n ? key[i] : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= capacity; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < capacity; i += 1024) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned long long a = sk[i], b = sk[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) { sk[i] = b; sk[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < capacity; i += 1024)
+        if ((uint32_t)i < n) key[i] = sk[i];
+}
+
+hipError_t tt_launch_frag_sort(hipStream_t s, unsigned long long *fragKey, const uint32_t *nroots, int capacity)
+{
+    hipLaunchKernelGGL(k_frag_sort, dim3(1), dim3(1024), (size_t)capacity * 8, s, fragKey, nroots, capacity);
+    return hipGetLastError();
+}
+
+// createLabelIdLookup in canonical serial order
+__global__ __launch_bounds__(256) void k_frag_assign(tt_dev d, const unsigned long long *fragKey,
+                                                     const uint32_t *nroots, const int *maxCount, const uint8_t *ids,
+                                                     tt_plate_data *pd, uint8_t *newId)
+{
+    __shared__ uint8_t avail[256];
+    __shared__ int navail;
+    const uint32_t n = *nroots;
+    const int M = n < (uint32_t)TT_MAXP ? (int)n : TT_MAXP;
+    const int i = threadIdx.x;
+    // available ids: i < M with size == 0, ascending (`!size > 0`, :3151)
+    const bool av = i < M && pd[i].size == 0;
+    const unsigned long long b = __ballot(av);
+    const int w = i >> 6, lane = i & 63;
+    __shared__ int wcount[4];
+    if (lane == 0) wcount[w] = __popcll(b);
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < w; ++k) base += wcount[k];
+    if (av) avail[base + __popcll(b & ((1ull << lane) - 1ull))] = (uint8_t)i;
+    if (i == 0) navail = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    __syncthreads();
+    if (i != 0) return;
+    int ai = 0;
+    for (int k = 0; k < M; ++k) {
+        const unsigned long long key = fragKey[k];
+        const uint32_t cnt = 0xFFFFFFFFu - (uint32_t)(key >> 32);
+        const uint32_t label = 0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFu);
+        const int pid = ids[label];
+        if ((int)cnt == maxCount[pid]) {
+            newId[label] = (uint8_t)pid;
+        } else if ((int)cnt >= d.ks.minPlateSize) {
+            const int idx = ai++;
+            if (idx < navail) {
+                const int nid = avail[idx];
+                pd[nid] = pd[pid];
+                newId[label] = (uint8_t)nid;
+            }
+        }
+    }
+}
+
+hipError_t tt_launch_frag_assign(hipStream_t s, const tt_dev &d, const unsigned long long *fragKey,
+                                 const uint32_t *nroots, const int *maxCount, const uint8_t *ids, tt_plate_data *pd,
+                                 uint8_t *newId)
+{
+    hipLaunchKernelGGL(k_frag_assign, dim3(1), dim3(256), 0, s, d, fragKey, nroots, maxCount, ids, pd, newId);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_relabel(tt_dev d, const uint32_t *labels, const uint8_t *newId, uint8_t *ids,
+                                                 uint8_t *unFlag)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= d.N) return;
+    const uint8_t id = newId[labels[c]];
+    ids[c] = id;
+    unFlag[c] = id == TT_MAXP ? 1 : 0;
+}
+
+hipError_t tt_launch_relabel(hipStream_t s, const tt_dev &d, const uint32_t *labels, const uint8_t *newId,
+                             uint8_t *ids, uint8_t *unFlag)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_relabel, dim3(blocks), dim3(256), 0, s, d, labels, newId, ids, unFlag);
+    return hipGetLastError();
+}
+
+// neighbour vote of assignUnassignedIdsToNeighbor, Jacobi rounds
+__global__ __launch_bounds__(256) void k_fill_vote(tt_dev d, const uint32_t *list, const uint32_t *n,
+                                                   const uint8_t *ids, uint8_t *vote)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= *n) return;
+    const uint32_t c = list[k];
+    const int x = (int)(c % (uint32_t)d.W), y = (int)(c / (uint32_t)d.W);
+    const int NO[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+    uint8_t nb[8];
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const uint8_t v = ids[(size_t)tt_wrap1(y + NO[q][1], d.H) * d.W + tt_wrap1(x + NO[q][0], d.W)];
+        nb[q] = v;
+        cnt += v != TT_MAXP;
+    }
+    uint8_t sel = TT_MAXP;
+    if (cnt > 0) {
+        int best = 0;
+        sel = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (nb[i] == TT_MAXP) continue;
+            int lc = 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) lc += nb[j] == nb[i];
+            if (lc > best) { best = lc; sel = nb[i]; }
+        }
+    }
+    vote[k] = sel;
+}
+
+hipError_t tt_launch_fill_vote(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n,
+                               const uint8_t *ids, uint8_t *vote)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_fill_vote, dim3(blocks), dim3(256), 0, s, d, list, n, ids, vote);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_fill_apply(tt_dev d, const uint32_t *list, const uint32_t *n,
+                                                    const uint8_t *vote, uint8_t *ids, uint32_t *nextList,
+                                                    uint32_t *nextN)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= *n) return;
+    const uint8_t v = vote[k];
+    if (v != TT_MAXP) ids[list[k]] = v;
+    else nextList[atomicAdd(nextN, 1u)] = list[k];
+}
+
+hipError_t tt_launch_fill_apply(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n,
+                                const uint8_t *vote, uint8_t *ids, uint32_t *nextList, uint32_t *nextN)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_fill_apply, dim3(blocks), dim3(256), 0, s, d, list, n, vote, ids, nextList, nextN);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// G2: merge
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_neighbor_matrix(tt_dev d, const uint8_t *ids, uint8_t *nbm)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= d.N) return;
+    const int x = (int)(c % d.W), y = (int)(c / d.W);
+    const uint8_t cur = ids[c];
+    const uint8_t r = ids[(size_t)y * d.W + tt_wrap1(x + 1, d.W)];
+    const uint8_t t = ids[(size_t)tt_wrap1(y + 1, d.H) * d.W + x];
+    if (cur != r) {
+        const int mn = r < cur ? r : cur, mx = r < cur ? cur : r;
+        nbm[(size_t)(mx % TT_MAXP) * TT_MAXP + (mn % TT_MAXP)] = 1;
+    }
+    if (cur != t) {
+        const int mn = t < cur ? t : cur, mx = t < cur ? cur : t;
+        nbm[(size_t)(mx % TT_MAXP) * TT_MAXP + (mn % TT_MAXP)] = 1;
+    }
+}
+
+hipError_t tt_launch_neighbor_matrix(hipStream_t s, const tt_dev &d, const uint8_t *ids, uint8_t *nbm)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_neighbor_matrix, dim3(blocks), dim3(256), 0, s, d, ids, nbm);
+    return hipGetLastError();
+}
+
+#define TT_NBM (TT_MAXP * TT_MAXP)
+#define TT_NBM_WORDS ((TT_NBM + 31) / 32)
+
+// getPlateMerges + clearCollisionTypes: candidates evaluated in parallel,
+// pairing applied by one thread in ascending matrix index
+__global__ __launch_bounds__(1024) void k_merge_decide(tt_dev d, const uint8_t *nbm, const tt_plate_data *pd,
+                                                       int *mergeIds, uint8_t *typeBitmap, tt_step_ctl *ctl)
+{
+    __shared__ unsigned cand[TT_NBM_WORDS];
+    __shared__ int mids[256];
+    const tt_kernel_settings &K = d.ks;
+    for (int i = threadIdx.x; i < TT_NBM_WORDS; i += 1024) cand[i] = 0u;
+    for (int i = threadIdx.x; i < 256; i += 1024) mids[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < TT_NBM; i += 1024) {
+        const int x = i % TT_MAXP, y = i / TT_MAXP;
+        if (x >= y || !nbm[i]) continue;
+        const tt_plate_data &X = pd[x], &Y = pd[y];
+        const float dot = X.directionSmoothed[0] * Y.directionSmoothed[0] + X.directionSmoothed[1] * Y.directionSmoothed[1];
+        const float vd = tt_fabsf(X.velocitySmoothed - Y.velocitySmoothed);
+        if (!(X.velocity <= K.mergeMinVelocity && Y.velocity <= K.mergeMinVelocity) &&
+            !(dot >= K.mergeDotDirectionThreshold && vd <= K.mergeVelocityDiffThreshold))
+            continue;
+        if (X.mergeWaitTime > 0 || Y.mergeWaitTime > 0) continue;
+        if (X.size + Y.size > K.targetMaximumPlateArea) continue;
+        atomicOr(&cand[i >> 5], 1u << (i & 31));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int merges = 0;
+        for (int wi = 0; wi < TT_NBM_WORDS; ++wi) {
+            unsigned bits = cand[wi];
+            while (bits) {
+                const int bit = __ffs((int)bits) - 1;
+                bits &= bits - 1u;
+                const int i = wi * 32 + bit;
+                const int x = i % TT_MAXP, y = i / TT_MAXP;
+                const bool xIsParent = pd[y].mass <= pd[x].mass;
+                if (mids[x] != 0) continue;
+                mids[x] = xIsParent ? -1 : TT_MAXP - y;
+                if (mids[y] != 0) { mids[x] = 0; continue; }
+                mids[y] = xIsParent ? TT_MAXP - x : -1;
+                ++merges;
+                const int parent = xIsParent ? x : y;
+                // clearCollisionTypes writes entry min(parent, j) of the 1-row table
+                const int hi = parent < TT_MAXP - 1 ? parent : parent - 1;
+                for (int j = 0; j <= hi; ++j) typeBitmap[j] = 0;
+            }
+        }
+        ctl->merges = merges;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TT_MAXP; i += 1024) mergeIds[i] = mids[i];
+}
+
+hipError_t tt_launch_merge_decide(hipStream_t s, const tt_dev &d, const uint8_t *nbm, const tt_plate_data *pd,
+                                  int *mergeIds, uint8_t *typeBitmap, tt_step_ctl *ctl)
+{
+    hipLaunchKernelGGL(k_merge_decide, dim3(1), dim3(1024), 0, s, d, nbm, pd, mergeIds, typeBitmap, ctl);
+    return hipGetLastError();
+}
+
+// per-plate accumulator layout (int64 each)
+#define ACC_MASS 0
+#define ACC_AX 1
+#define ACC_AY 2
+#define ACC_S0 3
+#define ACC_S1 4
+#define ACC_S2 5
+#define ACC_S3 6
+#define ACC_SIZE 7
+#define ACC_PER 8
+#define ACC_ACNT 9
+#define ACC_N 10
+
+// mergeAndCountSizeMass (mode 0) / countSizePostSplit (mode 1), plus the
+// angular samples of accumulatePlateAngularCoords in mode 0.  4096 cells per
+// workgroup, per-plate partial sums in LDS.
+__global__ __launch_bounds__(256) void k_count(tt_dev d, const int *mergeIds, const uint8_t *idsIn, uint8_t *idsOut,
+                                               const float *height, const uint8_t *noise, long long *acc, int mode)
+{
+    __shared__ unsigned long long sa[256 * ACC_N];
+    __shared__ int mmap[256];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 256 * ACC_N; i += 256) sa[i] = 0ull;
+    mmap[tid] = (mode == 0 && tid < TT_MAXP) ? mergeIds[tid] : 0;
+    __syncthreads();
+    const int W = d.W, H = d.H;
+    const double twopi = 6.283185307179586;
+    for (int k = 0; k < 16; ++k) {
+        const long long c = (long long)blockIdx.x * 4096 + k * 256 + tid;
+        if (c >= d.N) break;
+        const int x = (int)(c % W), y = (int)(c / W);
+        int pid = idsIn[c];
+        if (mode == 0) {
+            const int mg = mmap[pid];
+            if (mg > 0) pid = TT_MAXP - mg;
+        }
+        idsOut[c] = (uint8_t)pid;
+        atomicAdd(&sa[pid * ACC_N + ACC_MASS], (unsigned long long)tt_fix(height[c], TT_MASS_BITS));
+        atomicAdd(&sa[pid * ACC_N + ACC_SIZE], 1ull);
+        const int NX[4] = {x, tt_wrap1(x + 1, W), x, tt_wrap1(x - 1, W)};
+        const int NY[4] = {tt_wrap1(y - 1, H), y, tt_wrap1(y + 1, H), y};
+        int cnt = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int nid = idsIn[(size_t)NY[q] * W + NX[q]];
+            if (mode == 0) {
+                const uint8_t mm = (uint8_t)mmap[nid];
+                if (mm != 0) nid = (uint8_t)(TT_MAXP - mm);
+            }
+            cnt += nid != pid;
+        }
+        if (mode == 0) {
+            if (cnt) atomicAdd(&sa[pid * ACC_N + ACC_PER], (unsigned long long)cnt);
+            float ax, ay;
+            tt_asth_at(noise, x, y, d.axs, d.ays, ax, ay);
+            atomicAdd(&sa[pid * ACC_N + ACC_AX], (unsigned long long)tt_fix(ax, TT_ASTH_BITS));
+            atomicAdd(&sa[pid * ACC_N + ACC_AY], (unsigned long long)tt_fix(ay, TT_ASTH_BITS));
+            if (x % 10 == 0 && y % 10 == 0) {
+                const double agx = (twopi * (double)x) / (double)W, agy = (twopi * (double)y) / (double)H;
+                atomicAdd(&sa[pid * ACC_N + ACC_S0], (unsigned long long)tt_fix_d(tt_sin_d(agx), TT_ANG_BITS));
+                atomicAdd(&sa[pid * ACC_N + ACC_S1], (unsigned long long)tt_fix_d(tt_cos_d(agx), TT_ANG_BITS));
+                atomicAdd(&sa[pid * ACC_N + ACC_S2], (unsigned long long)tt_fix_d(tt_sin_d(agy), TT_ANG_BITS));
+                atomicAdd(&sa[pid * ACC_N + ACC_S3], (unsigned long long)tt_fix_d(tt_cos_d(agy), TT_ANG_BITS));
+                atomicAdd(&sa[pid * ACC_N + ACC_ACNT], 1ull);
+            }
+        } else if (cnt) {
+            atomicAdd(&sa[pid * ACC_N + ACC_PER], 1ull);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < 256 * ACC_N; i += 256)
+        if (sa[i] != 0ull) atomicAdd((unsigned long long *)&acc[i], sa[i]);
+}
+
+hipError_t tt_launch_count(hipStream_t s, const tt_dev &d, const int *mergeIds, const uint8_t *idsIn, uint8_t *idsOut,
+                           const float *height, const uint8_t *noise, long long *acc, int split_mode)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 4095) / 4096);
+    hipLaunchKernelGGL(k_count, dim3(blocks), dim3(256), 0, s, d, mergeIds, idsIn, idsOut, height, noise, acc,
+                       split_mode);
+    return hipGetLastError();
+}
+
+// mode 0: resetPlateDataPreCount + sums, computeBreakScore, plate centres,
+//         statisticsPass and the split decision (mode 2: without the split);
+// mode 1: resetPlateDataPreCountPostSplit + sums of countSizePostSplit;
+// mode 3: sizes and masses of initPixelDependantPlateData.
+__global__ __launch_bounds__(256) void k_finalize(tt_dev d, tt_plate_data *pd, const long long *acc, int mode,
+                                                  tt_step_ctl *ctl)
+{
+    const int p = threadIdx.x;
+    const tt_kernel_settings &K = d.ks;
+    __shared__ long long ledger[256];
+    if (p < TT_MAXP && mode == 3) {
+        // initPixelDependantPlateData (:199-230): sizes and masses added
+        tt_plate_data c = pd[p];
+        const long long *a = &acc[p * ACC_N];
+        if (a[ACC_SIZE] != 0) {
+            c.size += (int)a[ACC_SIZE];
+            c.mass = c.mass + tt_unfix(a[ACC_MASS], TT_MASS_BITS);
+        }
+        pd[p] = c;
+        ledger[p] = a[ACC_MASS];
+    } else if (p < TT_MAXP) {
+        tt_plate_data c = pd[p];
+        const long long *a = &acc[p * ACC_N];
+        c.asthenosphereVelocity[0] = 0.0f;
+        c.asthenosphereVelocity[1] = 0.0f;
+        c.mass = 0.0f;
+        c.size = 0;
+        c.perimeter = 0;
+        c.breakScore = 0.0f;
+        c.used = 0;
+        if (mode == 1 && (p == ctl->split_plate || p == ctl->split_new)) c.mergeWaitTime = K.mergeWaitAfterSplit;
+        if (a[ACC_SIZE] != 0) {
+            c.mass = tt_unfix(a[ACC_MASS], TT_MASS_BITS);
+            c.size = (int)a[ACC_SIZE];
+            c.perimeter = (int)a[ACC_PER];
+            if (mode == 0 || mode == 2) {
+                c.asthenosphereVelocity[0] = tt_unfix(a[ACC_AX], TT_ASTH_BITS);
+                c.asthenosphereVelocity[1] = tt_unfix(a[ACC_AY], TT_ASTH_BITS);
+            }
+        }
+        if (mode == 0 || mode == 2) {
+            // computeBreakScore
+            if (c.size > 0) {
+                const float area = (float)c.size, per = (float)c.perimeter;
+                c.circularity = 1.0f - (((2.0f * 6.2831855f) * area) / tt_fmaxf(per * per, 1.0f));
+                const float mn = (float)K.targetMinimumPlateArea, mx = (float)K.targetMaximumPlateArea;
+                const float as = tt_fmaxf(tt_fminf((area - mn) / (mx - mn), 1.0f), 0.0f);
+                c.breakScore = as + c.circularity;
+            } else {
+                c.breakScore = 0.0f;
+            }
+            // calculatePlateCenters
+            if (a[ACC_ACNT] > 0) {
+                const double n = (double)a[ACC_ACNT];
+                const double a0 = tt_unfix_d(a[ACC_S0], TT_ANG_BITS) / n, a1 = tt_unfix_d(a[ACC_S1], TT_ANG_BITS) / n;
+                const double a2 = tt_unfix_d(a[ACC_S2], TT_ANG_BITS) / n, a3 = tt_unfix_d(a[ACC_S3], TT_ANG_BITS) / n;
+                const double twopi = 6.283185307179586;
+                double cx = (tt_atan2(a0, a1) * (double)d.W) / twopi;
+                double cy = (tt_atan2(a2, a3) * (double)d.H) / twopi;
+                if (cx < 0.0) cx = cx + (double)d.W;
+                if (cy < 0.0) cy = cy + (double)d.H;
+                c.geometricCenter[0] = (float)cx;
+                c.geometricCenter[1] = (float)cy;
+            }
+        }
+        pd[p] = c;
+        ledger[p] = a[ACC_MASS];
+    } else {
+        ledger[p] = 0;
+    }
+    __syncthreads();
+    if (p != 0) return;
+    long long led = 0;
+    for (int i = 0; i < 256; ++i) led += ledger[i];
+    ctl->ledger = led;
+    if (mode == 1 || mode == 3) return;
+    // statisticsPass
+    float heavy = 0.0f;
+    int largest = 0, hid = 0, lid = 0, np = 0;
+    for (int i = 0; i < TT_MAXP; ++i) {
+        if (heavy < pd[i].mass) { heavy = pd[i].mass; hid = i; }
+        if (largest < pd[i].size) { largest = pd[i].size; lid = i; }
+        pd[i].used = pd[i].size > 0 ? 1 : 0;
+        np += pd[i].used;
+    }
+    ctl->heaviest_value = heavy;
+    ctl->heaviest_id = hid;
+    ctl->largest_value = largest;
+    ctl->largest_id = lid;
+    ctl->n_plates = np;
+    if (mode == 2) {
+        ctl->split_plate = -1;
+        ctl->split_new = -1;
+        return;
+    }
+    // processPlateSplitting: first used plate with breakScore >= 1.2, and the
+    // id selectUnusedPlateId would pick
+    int sp = -1, sn = -1;
+    for (int i = 0; i < TT_MAXP; ++i)
+        if (pd[i].used && pd[i].breakScore >= 1.2f) { sp = i; break; }
+    if (sp >= 0)
+        for (int i = 0; i < TT_MAXP; ++i)
+            if (!pd[i].used) { sn = i; break; }
+    if (sn < 0) sp = -1;  // no free id: no split
+    if (sp >= 0) pd[sn].used = 1;  // selectUnusedPlateId marks the id used
+    ctl->split_plate = sp;
+    ctl->split_new = sn;
+}
+
+hipError_t tt_launch_finalize(hipStream_t s, const tt_dev &d, tt_plate_data *pd, const long long *acc, int mode,
+                              tt_step_ctl *ctl)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, d, pd, acc, mode, ctl);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// G6: split
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_split_prepare(tt_dev d, const uint8_t *ids, int plate, float *eff,
+                                                       uint8_t *flag)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= d.N) return;
+    const bool in = ids[c] == plate;
+    eff[c] = in ? tt_u2f(0x7f800000u) : 0.0f;
+    flag[c] = in ? 1 : 0;
+}
+
+hipError_t tt_launch_split_prepare(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, float *eff,
+                                   uint8_t *flag)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_split_prepare, dim3(blocks), dim3(256), 0, s, d, ids, plate, eff, flag);
+    return hipGetLastError();
+}
+
+// propagateEffortToBoundary, relaxed in place (any order reaches the same
+// fixed point: the update is monotone and starts from an upper bound)
+__global__ __launch_bounds__(256) void k_relax(tt_dev d, const uint32_t *list, const uint32_t *n, const float *height,
+                                               float *eff, tt_step_ctl *ctl)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    int changed = 0;
+    if (k < *n) {
+        const uint32_t c = list[k];
+        const int x = (int)(c % (uint32_t)d.W), y = (int)(c / (uint32_t)d.W);
+        const float old = eff[c];
+        float md = old;
+        const float cost = height[c];
+        const float SQRT2 = 1.41421356237f;
+        const int OFF[8][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}, {-1, -1}, {-1, 1}, {1, -1}, {1, 1}};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float mul = i >= 4 ? SQRT2 : 1.0f;
+            const float t = eff[(size_t)tt_wrap1(y + OFF[i][1], d.H) * d.W + tt_wrap1(x + OFF[i][0], d.W)] + cost * mul;
+            if (t < md) md = t;
+        }
+        if (md != old) { eff[c] = md; changed = 1; }
+    }
+    const unsigned long long b = __ballot(changed);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&ctl->changed, __popcll(b));
+}
+
+hipError_t tt_launch_relax(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n, const float *height,
+                           float *eff, tt_step_ctl *ctl, int sweeps)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    for (int i = 0; i < sweeps; ++i) {
+        hipLaunchKernelGGL(k_relax, dim3(blocks), dim3(256), 0, s, d, list, n, height, eff, ctl);
+    }
+    return hipGetLastError();
+}
+
+// BacktrackPath (plate_tectonics_kernel.cu:2883-2983), one thread
+__global__ void k_backtrack(tt_dev d, const float *eff, uint8_t *ids, int px, int py, int newId, tt_step_ctl *ctl)
+{
+    if (threadIdx.x != 0) return;
+    const int W = d.W, H = d.H;
+    const float INF = tt_u2f(0x7f800000u);
+    const int OFF[8][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}, {-1, -1}, {-1, 1}, {1, -1}, {1, 1}};
+    auto at = [&](int x, int y) { return (size_t)tt_wrapg(y, H) * W + tt_wrapg(x, W); };
+    const uint8_t oldAtPoint = ids[at(px, py)];
+    int cx = px, cy = py;
+    long long guard = 0;
+    bool dead = false;
+    while (eff[at(cx, cy)] > 0.0f) {
+        ids[at(cx, cy)] = (uint8_t)TT_MAXP;
+        float best = INF;
+        int nx = cx, ny = cy;
+        for (int i = 0; i < 8; ++i) {
+            const float v = eff[at(cx + OFF[i][0], cy + OFF[i][1])];
+            if (v < best) { best = v; nx = cx + OFF[i][0]; ny = cy + OFF[i][1]; }
+        }
+        if (best == INF || ++guard > (long long)d.N) { dead = true; break; }
+        cx = nx;
+        cy = ny;
+    }
+    float pxf = 0.0f, pyf = 0.0f;
+    if (!dead) {
+        const float fbx = (float)(cx - px), fby = (float)(cy - py);
+        const float obx = fbx * -1.0f, oby = fby * -1.0f;
+        const float om = tt_fmaxf(sqrtf(obx * obx + oby * oby), 1e-20f);
+        const float odx = obx / om, ody = oby / om;
+        pxf = -ody;
+        pyf = odx;
+        cx = px;
+        cy = py;
+        while (eff[at(cx, cy)] > 0.0f) {
+            ids[at(cx, cy)] = (uint8_t)TT_MAXP;
+            float best = INF;
+            int nx = cx, ny = cy;
+            for (int i = 0; i < 8; ++i) {
+                const int qx = cx + OFF[i][0], qy = cy + OFF[i][1];
+                if (ids[at(qx, qy)] == TT_MAXP) continue;
+                const float tnx = (float)(qx - px), tny = (float)(qy - py);
+                const float pdot = tnx * odx + tny * ody;
+                if (pdot < 0.0f) continue;
+                const float dv = eff[at(qx, qy)];
+                const float ofx = (float)OFF[i][0], ofy = (float)OFF[i][1];
+                const float ofm = sqrtf(ofx * ofx + ofy * ofy);
+                const float tdx = ofx / ofm, tdy = ofy / ofm;
+                const float dot = tdx * odx + tdy * ody;
+                const float bias = -dot * 50.0f;
+                const float score = dv + bias;
+                if (score < best) { best = score; nx = qx; ny = qy; }
+            }
+            if (best != INF) { cx = nx; cy = ny; }
+            else { dead = true; break; }
+        }
+    }
+    if (!dead) {
+        int sx = px, sy = py;
+        for (int i = 1; i <= 100 && ids[at(sx, sy)] != oldAtPoint; ++i) {
+            const int xo = (i % 2 == 1) ? (i + 1) / 2 : -(i / 2);
+            const float fx = (float)px + pxf * (float)xo;
+            const float fy = (float)py + pyf * (float)xo;
+            sx = (int)(fx + 0.5f);
+            sy = (int)(fy + 0.5f);
+        }
+        if (ids[at(sx, sy)] == oldAtPoint) ids[at(sx, sy)] = (uint8_t)newId;
+    }
+    ctl->dead_end = dead ? 1 : 0;
+}
+
+hipError_t tt_launch_backtrack(hipStream_t s, const tt_dev &d, const float *eff, uint8_t *ids, int px, int py,
+                               int newId, tt_step_ctl *ctl)
+{
+    hipLaunchKernelGGL(k_backtrack, dim3(1), dim3(64), 0, s, d, eff, ids, px, py, newId, ctl);
+    return hipGetLastError();
+}
+
+// floodFillPlate, in place (monotone; the fixed point is the 4-connected
+// component of the seed through old-plate cells)
+__global__ __launch_bounds__(256) void k_flood(tt_dev d, const uint32_t *list, const uint32_t *n, uint8_t *ids,
+                                               int oldId, int newId, tt_step_ctl *ctl)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    int changed = 0;
+    if (k < *n) {
+        const uint32_t c = list[k];
+        if (ids[c] == oldId) {
+            const int x = (int)(c % (uint32_t)d.W), y = (int)(c / (uint32_t)d.W);
+            const bool nb = ids[(size_t)y * d.W + tt_wrap1(x - 1, d.W)] == newId ||
+                            ids[(size_t)y * d.W + tt_wrap1(x + 1, d.W)] == newId ||
+                            ids[(size_t)tt_wrap1(y - 1, d.H) * d.W + x] == newId ||
+                            ids[(size_t)tt_wrap1(y + 1, d.H) * d.W + x] == newId;
+            if (nb) { ids[c] = (uint8_t)newId; changed = 1; }
+        }
+    }
+    const unsigned long long b = __ballot(changed);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&ctl->changed, __popcll(b));
+}
+
+hipError_t tt_launch_flood(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n, uint8_t *ids,
+                           int oldId, int newId, tt_step_ctl *ctl, int sweeps)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    for (int i = 0; i < sweeps; ++i)
+        hipLaunchKernelGGL(k_flood, dim3(blocks), dim3(256), 0, s, d, list, n, ids, oldId, newId, ctl);
+    return hipGetLastError();
+}
+
+// finalizePlateSplit (:3026-3045): cut-line cells take the new id
+__global__ __launch_bounds__(256) void k_split_finalize(tt_dev d, const uint32_t *list, const uint32_t *n,
+                                                        uint8_t *ids, int oldId, int newId, tt_plate_data *pd)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k < *n) {
+        const uint32_t c = list[k];
+        if (ids[c] == TT_MAXP) ids[c] = (uint8_t)newId;
+    }
+    if (k == 0 && newId != oldId) {
+        pd[newId].velocity = pd[oldId].velocity;
+        pd[newId].velocitySmoothed = pd[oldId].velocitySmoothed;
+        pd[newId].direction[0] = pd[oldId].direction[0];
+        pd[newId].direction[1] = pd[oldId].direction[1];
+        pd[newId].directionSmoothed[0] = pd[oldId].directionSmoothed[0];
+        pd[newId].directionSmoothed[1] = pd[oldId].directionSmoothed[1];
+    }
+}
+
+hipError_t tt_launch_split_finalize(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n,
+                                    uint8_t *ids, int oldId, int newId, tt_plate_data *pd)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_split_finalize, dim3(blocks), dim3(256), 0, s, d, list, n, ids, oldId, newId, pd);
+    return hipGetLastError();
+}

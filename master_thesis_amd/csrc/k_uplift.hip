@@ -1,0 +1,293 @@
+// k_uplift.hip -- stage C: continental propagation (moveContinentalCollisions,
+// plate_tectonics_kernel.cu:1087-1185), combinePropagatedCollisions
+// (:1187-1218), jump flood (JFAInitialize / JFAPass / JFAComputeDistance,
+// :1498-1600, host loop plate_tectonic_sim.cu:289-311) and applyUplift
+// (:1662-1738, formulas :1646-1660).
+//
+// Canonical order of the propagation (the reference races through CAS inserts
+// and float atomics): sources are applied one after another in ascending cell
+// order.  Sources whose requests all target their own cell and whose cell no
+// other source targets are independent and run in parallel; the rest (remote
+// voxel-march targets, or cells that receive remote requests) are applied by
+// one serial pass in cell order.
+#include "tt_kernels.h"
+#include "tt_stage.h"
+
+__device__ __forceinline__ tt_prop tt_combine(tt_prop P, const tt_uplift &U, uint32_t word)
+{
+    if (P.inv != 0) {
+        P.types = 1;
+    } else if ((U.types & 1u) == 0) {
+        P.types = U.types;
+        P.cum = U.cum;
+        P.inv = word;
+    } else {
+        P.inv = 1;
+    }
+    return P;
+}
+
+__device__ __forceinline__ int tt_seed_of(const tt_prop &P, int x, int y)
+{
+    return (P.inv >> 8) != 0 ? ((y << 16) | x) : -1;
+}
+
+__global__ __launch_bounds__(256) void k_propagate_local(tt_dev d, const uint32_t *word, const tt_uplift *uplift,
+                                                         const uint8_t *srcFlag, const uint8_t *contested, tt_prop *prop,
+                                                         int *jfa, uint8_t *serialFlag)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= d.N) return;
+    const int x = (int)(c % d.W), y = (int)(c / d.W);
+    const uint8_t sf = srcFlag[c];
+    const uint8_t ct = contested[c];
+    tt_prop P;
+    P.cum = 0.0f;
+    P.inv = 0u;
+    P.types = 0;
+    P.pad[0] = P.pad[1] = P.pad[2] = 0;
+    serialFlag[c] = (ct && sf) ? 1 : 0;
+    if (ct) {
+        // applied by the serial pass and combined afterwards
+        prop[c] = P;
+        return;
+    }
+    const uint32_t cv = word[c];
+    const tt_uplift U = uplift[c];
+    if (sf == 1) {
+        const int prim = tt_slot_plate(cv, (int)((U.types >> 1) & 3u));
+        int total = 0;
+        uint32_t inv = 0;
+#pragma unroll
+        for (int ci = 0; ci < 4; ++ci) {
+            const int sec = tt_slot_plate(cv, ci);
+            if (sec == TT_MAXP || sec == prim) continue;
+            if (tt_insert_pair(inv, prim, sec)) total += 2;
+            total += 2;  // the unconditional counts of both positions (same cell)
+        }
+        P.inv = inv;
+        if (total > 0) P.cum = 0.0f + (U.cum / (float)total) * (float)total;
+    }
+    P = tt_combine(P, U, cv);
+    prop[c] = P;
+    jfa[c] = tt_seed_of(P, x, y);
+}
+
+hipError_t tt_launch_propagate_local(hipStream_t s, const tt_dev &d, const uint32_t *word, const tt_uplift *uplift,
+                                     const uint8_t *srcFlag, const uint8_t *contested, tt_prop *prop, int *jfa,
+                                     uint8_t *serialFlag)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_propagate_local, dim3(blocks), dim3(256), 0, s, d, word, uplift, srcFlag, contested, prop,
+                       jfa, serialFlag);
+    return hipGetLastError();
+}
+
+// One thread applies the listed sources in ascending cell order, exactly as the
+// reference program would run them one after another.
+__global__ void k_propagate_serial(tt_dev d, const uint32_t *list, const uint32_t *count, const uint32_t *word,
+                                   const tt_uplift *uplift, const float *height, const tt_plate_step *ps,
+                                   tt_prop *prop)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t n = *count;
+    for (uint32_t li = 0; li < n; ++li) {
+        const uint32_t c = list[li];
+        const int x = (int)(c % (uint32_t)d.W), y = (int)(c / (uint32_t)d.W);
+        const uint32_t cv = word[c];
+        const tt_uplift U = uplift[c];
+        const int prim = tt_slot_plate(cv, (int)((U.types >> 1) & 3u));
+        const float hc = height[c];
+        int ppx, ppy;
+        tt_cont_prop(d, height, x, y, ps[prim].dirx, ps[prim].diry, ps[prim].pcx, ps[prim].pcy, hc, ppx, ppy);
+        int posx[6], posy[6], cnt[6] = {0, 0, 0, 0, 0, 0};
+        int total = 0;
+        auto count_at = [&](int qx, int qy, int inc) {
+            for (int i = 0; i < 6; ++i)
+                if (cnt[i] == 0 || (posx[i] == qx && posy[i] == qy)) {
+                    posx[i] = qx; posy[i] = qy; cnt[i] += inc; total += inc;
+                    break;
+                }
+        };
+        for (int ci = 0; ci < 4; ++ci) {
+            const int sec = tt_slot_plate(cv, ci);
+            if (sec == TT_MAXP || sec == prim) continue;
+            int spx, spy;
+            tt_cont_prop(d, height, x, y, ps[sec].dirx, ps[sec].diry, ps[sec].pcx, ps[sec].pcy, hc, spx, spy);
+            tt_prop *tp = &prop[(size_t)tt_wrapg(ppy, d.H) * d.W + tt_wrapg(ppx, d.W)];
+            if (ppx == spx && ppy == spy) {
+                uint32_t inv = tp->inv;
+                if (tt_insert_pair(inv, prim, sec)) { tp->inv = inv; count_at(ppx, ppy, 2); }
+            } else {
+                uint32_t inv = tp->inv;
+                if (tt_insert_pair(inv, prim, sec)) { tp->inv = inv; count_at(ppx, ppy, 1); }
+                tt_prop *ts = &prop[(size_t)tt_wrapg(spy, d.H) * d.W + tt_wrapg(spx, d.W)];
+                uint32_t inv2 = ts->inv;
+                if (tt_insert_pair(inv2, prim, sec)) { ts->inv = inv2; count_at(spx, spy, 1); }
+            }
+            count_at(ppx, ppy, 1);
+            count_at(spx, spy, 1);
+        }
+        for (int i = 0; i < 6; ++i) {
+            if (cnt[i] == 0) break;
+            tt_prop *t = &prop[(size_t)tt_wrapg(posy[i], d.H) * d.W + tt_wrapg(posx[i], d.W)];
+            t->cum = t->cum + (U.cum / (float)total) * (float)cnt[i];
+        }
+    }
+}
+
+hipError_t tt_launch_propagate_serial(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *count,
+                                      const uint32_t *word, const tt_uplift *uplift, const float *height,
+                                      const tt_plate_step *ps, tt_prop *prop)
+{
+    hipLaunchKernelGGL(k_propagate_serial, dim3(1), dim3(64), 0, s, d, list, count, word, uplift, height, ps, prop);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_combine_contested(tt_dev d, const uint32_t *word, const tt_uplift *uplift,
+                                                           const uint8_t *contested, const uint8_t *srcFlag,
+                                                           tt_prop *prop, int *jfa, tt_step_ctl *ctl)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    int remote = 0;
+    if (c < d.N) {
+        remote = srcFlag[c] == 2;
+        if (contested[c]) {
+            const int x = (int)(c % d.W), y = (int)(c / d.W);
+            const tt_prop P = tt_combine(prop[c], uplift[c], word[c]);
+            prop[c] = P;
+            jfa[c] = tt_seed_of(P, x, y);
+        }
+    }
+    // wavefront-aggregated count of sources with remote targets
+    const unsigned long long b = __ballot(remote);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&ctl->complex_sources, __popcll(b));
+}
+
+hipError_t tt_launch_combine_contested(hipStream_t s, const tt_dev &d, const uint32_t *word, const tt_uplift *uplift,
+                                       const uint8_t *contested, const uint8_t *srcFlag, tt_prop *prop, int *jfa,
+                                       tt_step_ctl *ctl)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_combine_contested, dim3(blocks), dim3(256), 0, s, d, word, uplift, contested, srcFlag, prop,
+                       jfa, ctl);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// jump flood: seeds are packed (y << 16 | x), -1 = none
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int tt_tdist2(int ax, int ay, int bx, int by, int W, int H)
+{
+    int dx = abs(ax - bx), dy = abs(ay - by);
+    dx = min(dx, W - dx);
+    dy = min(dy, H - dy);
+    return dx * dx + dy * dy;
+}
+
+__global__ __launch_bounds__(256) void k_jfa_pass(tt_dev d, const int *in, int *out, int step)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= d.N) return;
+    const int x = (int)(c % d.W), y = (int)(c / d.W);
+    int best = in[c];
+    int bd = 0x7FFFFFFF;
+    if (best >= 0) bd = tt_tdist2(x, y, best & 0xFFFF, best >> 16, d.W, d.H);
+    // (x + k * step) mod W == (x + k * (step mod W)) mod W, and the reduced
+    // offset keeps the single add / subtract wrap in range
+    const int sx = step % d.W, sy = step % d.H;
+#pragma unroll
+    for (int oy = -1; oy <= 1; ++oy) {
+        const size_t row = (size_t)tt_wrap1(y + oy * sy, d.H) * d.W;
+#pragma unroll
+        for (int ox = -1; ox <= 1; ++ox) {
+            if (ox == 0 && oy == 0) continue;
+            const int cand = in[row + tt_wrap1(x + ox * sx, d.W)];
+            if (cand < 0) continue;
+            const int dd = tt_tdist2(x, y, cand & 0xFFFF, cand >> 16, d.W, d.H);
+            if (dd < bd) { bd = dd; best = cand; }
+        }
+    }
+    out[c] = best;
+}
+
+hipError_t tt_launch_jfa_pass(hipStream_t s, const tt_dev &d, const int *in, int *out, int step)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_jfa_pass, dim3(blocks), dim3(256), 0, s, d, in, out, step);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// JFAComputeDistance + applyUplift
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float tt_cont_formula(float x)
+{
+    const float ax = tt_fabsf(x);
+    return tt_expf((-0.02f * x) * x) + tt_expf((-0.2f * x) * x) * (0.5f - tt_fabsf(tt_fmodf(ax, 1.0f) - 0.5f));
+}
+
+__device__ __forceinline__ float tt_sub_under(float x)
+{
+    const float l = tt_log10f(1.0f + 8.0f * (x - 1.0f));
+    return tt_fminf(l / 1.908485019f - 1.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_uplift(tt_dev d, const int *jfa, const tt_prop *prop, const uint8_t *idW,
+                                                const tt_plate_step *ps, float *hW, tt_dist *dist)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= d.N) return;
+    const int x = (int)(c % d.W), y = (int)(c / d.W);
+    const int s = jfa[c];
+    if (s < 0) {
+        tt_dist z;
+        z.dist = -1;
+        z.orig = 0u;
+        dist[c] = z;
+        return;
+    }
+    const int sx = s & 0xFFFF, sy = s >> 16;
+    const int d2 = tt_tdist2(x, y, sx, sy, d.W, d.H);
+    tt_dist db;
+    db.dist = (int)sqrtf((float)d2);
+    db.orig = (uint32_t)sy * (uint32_t)d.W + (uint32_t)sx;
+    dist[c] = db;
+    const tt_prop U = prop[db.orig];
+    const int pid = idW[c];
+    const int type = U.types & 1;
+    const int pi = (U.types >> 1) & 3, si = (U.types >> 3) & 3;
+    const int prim = tt_slot_plate(U.inv, pi);
+    const int sec = tt_slot_plate(U.inv, si);
+    float h = hW[c];
+    if (type == 0 && sec == pid) {
+        const float v = tt_sub_under((float)db.dist);
+        const float m = 1.0f + v * d.ks.subductionUnderUpliftMultiplier;
+        h = tt_fmaxf(h * m, d.ks.minUpliftClamp);
+    } else if (type == 0 && prim == pid) {
+        const float sizeX = (float)d.W, sizeY = (float)d.H;
+        const float dx = tt_fmodf((float)(x - sx) + sizeX * 0.5f, sizeX) - sizeX * 0.5f;
+        const float dy = tt_fmodf((float)(y - sy) + sizeY * 0.5f, sizeY) - sizeY * 0.5f;
+        const float mg = tt_fmaxf(sqrtf(dx * dx + dy * dy), 1e-20f);
+        const float nx = dx / mg, ny = dy / mg;
+        const float dot = tt_fmaxf(nx * ps[sec].dirx + ny * ps[sec].diry, 0.0f);
+        const float v = tt_cont_formula(((float)db.dist - (float)d.ks.subductionOverRange) - 10.0f);
+        const float m = 1.0f + (v * dot) * d.ks.subductionOverUpliftMultiplier;
+        h = tt_fmaxf(h * m, d.ks.minUpliftClamp);
+    } else if (type == 1) {
+        const float v = tt_cont_formula((float)db.dist);
+        const float hc = (v * U.cum) * d.ks.continentalUpliftMultiplier;
+        h = tt_fminf(h + hc, d.ks.maxUpliftClamp);
+    } else {
+        return;
+    }
+    hW[c] = h;
+}
+
+hipError_t tt_launch_uplift(hipStream_t s, const tt_dev &d, const int *jfa, const tt_prop *prop, const uint8_t *idW,
+                            const tt_plate_step *ps, float *hW, tt_dist *dist)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_uplift, dim3(blocks), dim3(256), 0, s, d, jfa, prop, idW, ps, hW, dist);
+    return hipGetLastError();
+}

@@ -1,0 +1,119 @@
+// tt_device.h -- definitions shared by the gfx950 kernels of libtectonic:
+// device-side layouts, the per-step plate table, index helpers and the launch
+// wrappers' declarations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tectonic.h"
+#include "tt_math.h"
+#include "tt_noise.h"
+
+#define TT_MAXP 255
+#define TT_NUM_TRI (TT_MAXP * (TT_MAXP - 1) / 2)
+#define TT_NUM_TRI_WORDS ((TT_NUM_TRI + 31) / 32)
+
+// fixed-point scales of the exact per-plate sums
+#define TT_MASS_BITS 20
+#define TT_ASTH_BITS 32
+#define TT_FRIC_BITS 48
+#define TT_ANG_BITS 40
+
+// 2-D tiles of the stencil kernels: 64 columns (one wavefront per row) by 16
+// rows, 256 threads per block (4 wavefronts, each owning 4 rows).
+#define TT_TILE_W 64
+#define TT_TILE_H 16
+#define TT_BLOCK 256
+
+struct tt_uplift { float cum; uint8_t types; uint8_t pad[3]; };
+struct tt_prop { float cum; uint32_t inv; uint8_t types; uint8_t pad[3]; };
+struct tt_dist { int32_t dist; uint32_t orig; };
+
+// Per-plate values derived once per step from the plate table.
+struct tt_plate_step {
+    int ox, oy;       // floor(pixelCenter + direction * velocity)
+    float vel, mass;
+    float dirx, diry;
+    float pcx, pcy;
+};
+
+// Per-step control block in device memory (one per world).
+struct tt_step_ctl {
+    // advection
+    int n_distinct;
+    int max_r;
+    int dox[256], doy[256];
+    // outputs of the step read back by the host
+    int overflow_cells;
+    int complex_sources;
+    int n_roots;
+    int n_unassigned;
+    int merges;
+    int split_plate;
+    int split_new;
+    int changed;         // relaxation / flood-fill change counter
+    int dead_end;
+    int n_plate_cells;
+    int largest_value, largest_id, heaviest_id, n_plates;
+    float heaviest_value;
+    int pad0;
+    long long ledger;
+};
+
+// uint8 plate id arithmetic of the reference (MAX_PLATE_COUNT - byte)
+__device__ __forceinline__ int tt_slot_plate(uint32_t word, int slot)
+{
+    return (int)(uint8_t)(TT_MAXP - ((word >> (8 * slot)) & 0xFFu));
+}
+
+__device__ __forceinline__ unsigned tt_tri(unsigned i, unsigned j) { return i * (2u * TT_MAXP - i - 1u) / 2u + (j - i - 1u); }
+
+// wrap for |v| < 2n (conditional add / subtract)
+__device__ __forceinline__ int tt_wrap1(int v, int n)
+{
+    v = v < 0 ? v + n : v;
+    return v >= n ? v - n : v;
+}
+// general wrap (CudaTexture::wrapCoordinate, cuda_texture.cuh:67-76)
+__device__ __forceinline__ int tt_wrapg(int v, int n)
+{
+    int r = v % n;
+    return r + (n & (r >> 31));
+}
+
+// XCD-aware block order: the dispatcher hands consecutive block ids to the 8
+// XCDs round-robin; remapping so that each XCD receives a contiguous range of
+// tiles keeps the halo rows of neighbouring tiles in the same L2.
+__device__ __forceinline__ int tt_xcd_tile(int b, int nb)
+{
+    const int per = nb >> 3;
+    if (per == 0 || (nb & 7)) return b;
+    return (b & 7) * per + (b >> 3);
+}
+
+// asthenosphere vector at cell (x, y): computeCurl of the 256x256 noise table
+// (plate_tectonics_kernel.cu:2788-2822), evaluated on the fly.
+__device__ __forceinline__ float tt_noise_at(const uint8_t *noise, int x, int y)
+{
+    return (float)noise[tt_wrapg(y, 256) * 256 + tt_wrapg(x, 256)] / 255.0f;
+}
+__device__ __forceinline__ void tt_asth_at(const uint8_t *noise, int x, int y, double xs, double ys, float &ax, float &ay)
+{
+    const int sx = (int)((double)x * xs), sy = (int)((double)y * ys);
+    const float left = tt_noise_at(noise, sx - 1, sy), right = tt_noise_at(noise, sx + 1, sy);
+    const float bottom = tt_noise_at(noise, sx, sy - 1), top = tt_noise_at(noise, sx, sy + 1);
+    ay = (float)((double)(right - left) * 0.5);
+    ax = (float)(-((double)(top - bottom) * 0.5));
+}
+
+// ---------------------------------------------------------------------------
+// device world view passed to kernels by value
+// ---------------------------------------------------------------------------
+struct tt_dev {
+    int W, H, N;
+    uint32_t iter;
+    int seed;
+    double axs, ays;        // 256 / W, 256 / H (noise scale)
+    tt_kernel_settings ks;
+};

@@ -1,0 +1,101 @@
+// tt_kernels.h -- host-side launch wrappers of the gfx950 kernels.  Every
+// wrapper enqueues on the given stream and returns the launch status.
+#pragma once
+
+#include "tt_device.h"
+
+// ---- scan / compaction (k_misc.hip) ----------------------------------------
+// exclusive prefix sum of n uint32 values; tmp must hold scan_tmp_words(n) words
+size_t tt_scan_tmp_words(long long n);
+hipError_t tt_scan_u32(hipStream_t s, const uint32_t *in, uint32_t *out, long long n, uint32_t *tmp, uint32_t *total);
+// ordered compaction of the indices i < n with flags[i] != 0 (u8 flags)
+hipError_t tt_compact_u8(hipStream_t s, const uint8_t *flags, long long n, uint32_t *out_idx, uint32_t *count,
+                         uint32_t *tmp);
+hipError_t tt_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, long long n);
+
+// ---- initialisation (k_misc.hip) ------------------------------------------
+hipError_t tt_launch_init_synthetic_heights(hipStream_t s, const tt_dev &d, uint64_t hseed, double *raw, double *tmp,
+                                            const double *wts, float *height);
+hipError_t tt_launch_init_reference_heights(hipStream_t s, const tt_dev &d, float *height);
+hipError_t tt_launch_init_plate_ids(hipStream_t s, const tt_dev &d, const float *sx, const float *sy, const uint8_t *sid,
+                                    int nseeds, uint8_t *ids);
+hipError_t tt_launch_asthenosphere(hipStream_t s, const tt_dev &d, const uint8_t *noise, float2 *out);
+hipError_t tt_launch_math_eval(hipStream_t s, int fn, const float *in, float *out, long long n);
+
+// ---- stage A/B (k_collide.hip) --------------------------------------------
+hipError_t tt_launch_plate_prep(hipStream_t s, const tt_plate_data *pd, tt_plate_step *ps, tt_step_ctl *ctl,
+                                long long *fric);
+hipError_t tt_launch_advect(hipStream_t s, const tt_dev &d, const uint8_t *ids, const float *height,
+                            const tt_plate_step *ps, const tt_step_ctl *ctl_dev, int max_r_host, const uint8_t *typeBitmap,
+                            uint32_t *word, uint8_t *cnt, uint32_t *segCount, int *typeCounts, tt_step_ctl *ctl);
+hipError_t tt_launch_keys(hipStream_t s, const tt_dev &d, const uint32_t *word, const uint8_t *cnt,
+                          const uint32_t *segBase, const uint8_t *ids, const tt_plate_step *ps, const tt_step_ctl *ctl,
+                          unsigned long long *keys);
+hipError_t tt_launch_type_matrix(hipStream_t s, const tt_dev &d, const int *typeCounts, uint8_t *typeBitmap);
+hipError_t tt_launch_resolve(hipStream_t s, const tt_dev &d, const uint32_t *word, const uint8_t *ids,
+                             const float *height, const tt_plate_step *ps, const uint32_t *divBitmap,
+                             const uint8_t *typeBitmap, float *hW, uint8_t *idW, tt_uplift *uplift, uint32_t *accw,
+                             uint32_t *hasDiv, long long *fric, uint8_t *srcFlag, uint8_t *contested, int do_uplift);
+hipError_t tt_launch_flip(hipStream_t s, uint32_t *divBitmap, const uint32_t *hasDiv);
+
+// ---- stage C (k_uplift.hip) -------------------------------------------------
+hipError_t tt_launch_propagate_local(hipStream_t s, const tt_dev &d, const uint32_t *word, const tt_uplift *uplift,
+                                     const uint8_t *srcFlag, const uint8_t *contested, tt_prop *prop, int *jfa,
+                                     uint8_t *serialFlag);
+hipError_t tt_launch_propagate_serial(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *count,
+                                      const uint32_t *word, const tt_uplift *uplift, const float *height,
+                                      const tt_plate_step *ps, tt_prop *prop);
+hipError_t tt_launch_combine_contested(hipStream_t s, const tt_dev &d, const uint32_t *word, const tt_uplift *uplift,
+                                       const uint8_t *contested, const uint8_t *srcFlag, tt_prop *prop, int *jfa,
+                                       tt_step_ctl *ctl);
+hipError_t tt_launch_jfa_pass(hipStream_t s, const tt_dev &d, const int *in, int *out, int step);
+hipError_t tt_launch_uplift(hipStream_t s, const tt_dev &d, const int *jfa, const tt_prop *prop, const uint8_t *idW,
+                            const tt_plate_step *ps, float *hW, tt_dist *dist);
+
+// ---- stage D/E (k_erosion.hip) -----------------------------------------------
+hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, const float *hin, float *hout);
+hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int first, int rseed, const float *hin, float *hout,
+                               const float *win, float *wout, const float4 *fin, float4 *fout, float2 *vel,
+                               const float *sprev, const float *sedState, float *sout);
+hipError_t tt_launch_transport(hipStream_t s, const tt_dev &d, const float *sprev, const float2 *vel, float *sout);
+
+// ---- stages F/G (k_plates.hip) ----------------------------------------------
+hipError_t tt_launch_plate_movement(hipStream_t s, const tt_dev &d, tt_plate_data *pd, const long long *fric,
+                                    const uint8_t *noise);
+hipError_t tt_launch_accretion_apply(hipStream_t s, const tt_dev &d, const uint32_t *accw, const float *height,
+                                     const uint8_t *idsIn, uint8_t *idsOut, uint8_t *accretion);
+hipError_t tt_launch_ccl_local(hipStream_t s, const tt_dev &d, const uint8_t *ids, uint32_t *labels);
+hipError_t tt_launch_ccl_merge(hipStream_t s, const tt_dev &d, const uint8_t *ids, uint32_t *labels);
+hipError_t tt_launch_ccl_flatten(hipStream_t s, const tt_dev &d, uint32_t *labels, uint32_t *rootCount,
+                                 uint8_t *rootFlag);
+hipError_t tt_launch_frag_build(hipStream_t s, const tt_dev &d, const uint32_t *rootList, const uint32_t *nroots,
+                                uint32_t nroots_host, const uint32_t *rootCount, const uint8_t *ids, unsigned long long *fragKey,
+                                int *maxCount, uint8_t *newId);
+hipError_t tt_launch_frag_sort(hipStream_t s, unsigned long long *fragKey, const uint32_t *nroots, int capacity);
+hipError_t tt_launch_frag_assign(hipStream_t s, const tt_dev &d, const unsigned long long *fragKey,
+                                 const uint32_t *nroots, const int *maxCount, const uint8_t *ids, tt_plate_data *pd,
+                                 uint8_t *newId);
+hipError_t tt_launch_relabel(hipStream_t s, const tt_dev &d, const uint32_t *labels, const uint8_t *newId,
+                             uint8_t *ids, uint8_t *unFlag);
+hipError_t tt_launch_fill_vote(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n,
+                               const uint8_t *ids, uint8_t *vote);
+hipError_t tt_launch_fill_apply(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n,
+                                const uint8_t *vote, uint8_t *ids, uint32_t *nextList, uint32_t *nextN);
+hipError_t tt_launch_neighbor_matrix(hipStream_t s, const tt_dev &d, const uint8_t *ids, uint8_t *nbm);
+hipError_t tt_launch_merge_decide(hipStream_t s, const tt_dev &d, const uint8_t *nbm, const tt_plate_data *pd,
+                                  int *mergeIds, uint8_t *typeBitmap, tt_step_ctl *ctl);
+hipError_t tt_launch_count(hipStream_t s, const tt_dev &d, const int *mergeIds, const uint8_t *idsIn, uint8_t *idsOut,
+                           const float *height, const uint8_t *noise, long long *acc, int split_mode);
+hipError_t tt_launch_finalize(hipStream_t s, const tt_dev &d, tt_plate_data *pd, const long long *acc, int mode,
+                              tt_step_ctl *ctl);
+// split
+hipError_t tt_launch_split_prepare(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, float *eff,
+                                   uint8_t *flag);
+hipError_t tt_launch_relax(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n, const float *height,
+                           float *eff, tt_step_ctl *ctl, int sweeps);
+hipError_t tt_launch_backtrack(hipStream_t s, const tt_dev &d, const float *eff, uint8_t *ids, int px, int py,
+                               int newId, tt_step_ctl *ctl);
+hipError_t tt_launch_flood(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n, uint8_t *ids,
+                           int oldId, int newId, tt_step_ctl *ctl, int sweeps);
+hipError_t tt_launch_split_finalize(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n,
+                                    uint8_t *ids, int oldId, int newId, tt_plate_data *pd);

@@ -1,0 +1,97 @@
+// tt_world.h -- the world object behind the C ABI (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "tt_device.h"
+
+struct tt_alloc {
+    void **ptr;
+    size_t bytes;
+};
+
+// One simulated world: device state of PlateTectonicSim
+// (plate_tectonic_sim.h:62-108) in one device allocation, plus the world's
+// stream and pinned read-back buffers.
+struct tt_world {
+    tt_sim_config cfg{};
+    tt_kernel_settings ks{};
+    int W = 0, H = 0, N = 0;
+    uint32_t iter = 0;
+    bool initialized = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char *base = nullptr;
+    std::vector<tt_alloc> allocs;
+
+    // state (ping-pong pairs; the index names the current buffer)
+    float *h[2] = {nullptr, nullptr};
+    uint8_t *ids[2] = {nullptr, nullptr};
+    float *water[2] = {nullptr, nullptr};
+    float4 *flux[2] = {nullptr, nullptr};
+    float *sed[2] = {nullptr, nullptr};
+    float2 *vel = nullptr;
+    int hc = 0, ic = 0, wc = 0, fc = 0, sc = 0;
+    tt_plate_data *pd = nullptr;
+    uint32_t *divBitmap = nullptr;
+    uint8_t *typeBitmap = nullptr;
+    tt_uplift *uplift = nullptr;
+    uint8_t *noise = nullptr;
+
+    // per-step products
+    uint32_t *word = nullptr;
+    uint8_t *cnt = nullptr;
+    uint32_t *segCount = nullptr, *segBase = nullptr;
+    unsigned long long *keys = nullptr;
+    int *typeCounts = nullptr;
+    uint32_t *hasDiv = nullptr;
+    uint32_t *accw = nullptr;
+    uint8_t *accretion = nullptr;
+    tt_prop *prop = nullptr;
+    uint8_t *srcFlag = nullptr, *contested = nullptr, *serialFlag = nullptr;
+    uint32_t *serialList = nullptr, *serialCount = nullptr;
+    int *jfa[2] = {nullptr, nullptr};
+    tt_dist *dist = nullptr;
+    tt_plate_step *ps = nullptr;
+    tt_step_ctl *ctl = nullptr;
+    long long *fric = nullptr;
+
+    // CCL / relabel
+    uint32_t *labels = nullptr, *rootCount = nullptr, *rootList = nullptr, *nroots = nullptr;
+    uint8_t *rootFlag = nullptr, *newId = nullptr, *unFlag = nullptr, *vote = nullptr;
+    unsigned long long *fragKey = nullptr;
+    int *maxCount = nullptr;
+    uint32_t *list[2] = {nullptr, nullptr};
+    uint32_t *listN[2] = {nullptr, nullptr};
+
+    // merge / split
+    uint8_t *nbm = nullptr;
+    int *mergeIds = nullptr;
+    long long *acc = nullptr;
+    float *eff = nullptr;
+
+    // scratch
+    uint32_t *scanTmp = nullptr;
+    uint8_t *initSeeds = nullptr;
+    double *initWts = nullptr;
+
+    // host mirrors
+    tt_step_ctl *hctl = nullptr;
+    uint32_t *hu32 = nullptr;
+    tt_step_stats stats{};
+    bool timing = false;
+    hipEvent_t ev[8] = {};
+    tt_stage_times times{};
+};
+
+// byte layout of one world; with alloc = true the pointer slots of w are
+// registered for the single device allocation
+size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w);
+int tt_world_do_init(tt_world *w);
+int tt_validate_config(const tt_sim_config *c);
+
+// host.cpp
+void tt_settings_fill_defaults(tt_kernel_settings *k);
+const std::vector<uint8_t> &tt_noise_table();

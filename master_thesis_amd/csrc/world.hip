@@ -1,0 +1,899 @@
+// world.hip -- host orchestration of one world (PlateTectonicSim,
+// src/cuda_compute/plate_tectonic_sim.{h,cu}) and the C ABI of libtectonic.
+//
+// Step schedule = executeIteration (plate_tectonic_sim.cu:54-131):
+//   collisions -> uplift -> thermal -> hydraulic -> copy-back ->
+//   plate movement -> accretion -> CCL (every iterations_for_ccl steps) ->
+//   merge + per-plate sums + break score + centres + statistics -> split.
+// All device work of a step goes to the world's own stream; the host reads
+// back small control values where the reference's control flow depends on
+// device data (CCL fragment count, neighbour-fill rounds, split decision and
+// relaxation convergence).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "tt_kernels.h"
+#include "tt_world.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const char *what, hipError_t e)
+{
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+}
+
+void set_error(const std::string &what) { g_last_error = what; }
+
+#define CK(expr)                                        \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) {                         \
+            set_error(#expr, e_);                       \
+            return TT_ERR_DEVICE;                       \
+        }                                               \
+    } while (0)
+
+#define CKS(expr)                   \
+    do {                            \
+        int st_ = (expr);           \
+        if (st_ != TT_OK) return st_; \
+    } while (0)
+
+constexpr int MAXP = TT_MAXP;
+
+struct SplitMix {
+    uint64_t s;
+    uint64_t next()
+    {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    float u01() { return (float)((double)(next() >> 40) * (1.0 / 16777216.0)); }
+};
+
+uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+tt_plate_data default_plate()
+{
+    tt_plate_data p;
+    memset(&p, 0, sizeof(p));
+    p.breakScore = 1.0f;
+    return p;
+}
+
+size_t rup(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// allocation
+// ---------------------------------------------------------------------------
+size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w)
+{
+    const size_t N = (size_t)c.width * (size_t)c.height;
+    size_t total = 0;
+    auto take = [&](void **p, size_t bytes) {
+        bytes = rup(bytes);
+        total += bytes;
+        if (alloc) w->allocs.push_back({p, bytes});
+    };
+    if (alloc) w->allocs.clear();
+    for (int i = 0; i < 2; ++i) {
+        take((void **)&w->h[i], N * 4);
+        take((void **)&w->ids[i], N);
+        take((void **)&w->water[i], N * 4);
+        take((void **)&w->flux[i], N * 16);
+        take((void **)&w->sed[i], N * 4);
+        take((void **)&w->jfa[i], N * 4);
+        take((void **)&w->list[i], N * 4);
+        take((void **)&w->listN[i], 16);
+    }
+    take((void **)&w->vel, N * 8);
+    take((void **)&w->word, N * 4);
+    take((void **)&w->cnt, N);
+    take((void **)&w->segCount, (N / 64 + 1) * 4);
+    take((void **)&w->segBase, (N / 64 + 1) * 4);
+    take((void **)&w->keys, N * 8);
+    take((void **)&w->typeCounts, (size_t)TT_NUM_TRI * 5 * 4);
+    take((void **)&w->typeBitmap, TT_NUM_TRI);
+    take((void **)&w->divBitmap, TT_NUM_TRI_WORDS * 4);
+    take((void **)&w->hasDiv, TT_NUM_TRI_WORDS * 4);
+    take((void **)&w->accw, N * 4);
+    take((void **)&w->accretion, N);
+    take((void **)&w->uplift, N * 8);
+    take((void **)&w->prop, N * 12);
+    take((void **)&w->srcFlag, N);
+    take((void **)&w->contested, N);
+    take((void **)&w->serialFlag, N);
+    take((void **)&w->serialList, N * 4);
+    take((void **)&w->serialCount, 16);
+    take((void **)&w->dist, N * 8);
+    take((void **)&w->pd, 256 * sizeof(tt_plate_data));
+    take((void **)&w->ps, 256 * sizeof(tt_plate_step));
+    take((void **)&w->ctl, sizeof(tt_step_ctl));
+    take((void **)&w->fric, 256 * 8);
+    take((void **)&w->noise, 65536);
+    take((void **)&w->labels, N * 4);
+    take((void **)&w->rootCount, N * 4);
+    take((void **)&w->rootFlag, N);
+    take((void **)&w->rootList, N * 4);
+    take((void **)&w->nroots, 16);
+    take((void **)&w->fragKey, N * 8);
+    take((void **)&w->maxCount, 256 * 4);
+    take((void **)&w->newId, N);
+    take((void **)&w->unFlag, N);
+    take((void **)&w->vote, N);
+    take((void **)&w->nbm, (size_t)MAXP * MAXP + 64);
+    take((void **)&w->mergeIds, 256 * 4);
+    take((void **)&w->acc, 256 * 10 * 8);
+    take((void **)&w->eff, N * 4);
+    take((void **)&w->scanTmp, (N / 1024 + tt_scan_tmp_words((long long)N) + 4096) * 4);
+    take((void **)&w->initSeeds, 4096 * 12);
+    take((void **)&w->initWts, 128 * 8);
+    return total;
+}
+
+static int zero(tt_world *w, void *p, size_t bytes)
+{
+    CK(tt_fill_u32(w->stream, (uint32_t *)p, 0u, (long long)((bytes + 3) / 4)));
+    return TT_OK;
+}
+
+static tt_dev make_dev(const tt_world *w)
+{
+    tt_dev d;
+    d.W = w->W;
+    d.H = w->H;
+    d.N = w->N;
+    d.iter = w->iter;
+    d.seed = (int)w->cfg.seed;
+    d.axs = 256.0 / (double)w->W;
+    d.ays = 256.0 / (double)w->H;
+    d.ks = w->ks;
+    return d;
+}
+
+static int sync_ctl(tt_world *w)
+{
+    CK(hipMemcpyAsync(w->hctl, w->ctl, sizeof(tt_step_ctl), hipMemcpyDeviceToHost, w->stream));
+    CK(hipStreamSynchronize(w->stream));
+    return TT_OK;
+}
+
+static int read_u32(tt_world *w, const uint32_t *dev, uint32_t *out)
+{
+    CK(hipMemcpyAsync(w->hu32, dev, 4, hipMemcpyDeviceToHost, w->stream));
+    CK(hipStreamSynchronize(w->stream));
+    *out = *w->hu32;
+    return TT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CCL + relabel (applyCCL, plate_tectonic_sim.cu:375-468)
+// ---------------------------------------------------------------------------
+static int run_ccl(tt_world *w)
+{
+    hipStream_t s = w->stream;
+    const tt_dev d = make_dev(w);
+    uint8_t *ids = w->ids[w->ic];
+    const size_t N = (size_t)w->N;
+    CKS(zero(w, w->rootCount, N * 4));
+    CK(tt_launch_ccl_local(s, d, ids, w->labels));
+    CK(tt_launch_ccl_merge(s, d, ids, w->labels));
+    CK(tt_launch_ccl_flatten(s, d, w->labels, w->rootCount, w->rootFlag));
+    CK(tt_compact_u8(s, w->rootFlag, (long long)N, w->rootList, w->nroots, w->scanTmp));
+    uint32_t nroots = 0;
+    CKS(read_u32(w, w->nroots, &nroots));
+    w->stats.num_labels = (int)nroots;
+    CKS(zero(w, w->maxCount, 256 * 4));
+    CK(tt_launch_frag_build(s, d, w->rootList, w->nroots, nroots, w->rootCount, ids, w->fragKey, w->maxCount,
+                            w->newId));
+    if (nroots <= 4096) {
+        int cap = 2;
+        while ((uint32_t)cap < nroots) cap <<= 1;
+        CK(tt_launch_frag_sort(s, w->fragKey, w->nroots, cap));
+    } else {
+        std::vector<unsigned long long> hk(nroots);
+        CK(hipMemcpyAsync(hk.data(), w->fragKey, (size_t)nroots * 8, hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+        std::sort(hk.begin(), hk.end());
+        CK(hipMemcpyAsync(w->fragKey, hk.data(), (size_t)nroots * 8, hipMemcpyHostToDevice, s));
+        CK(hipStreamSynchronize(s));
+    }
+    CK(tt_launch_frag_assign(s, d, w->fragKey, w->nroots, w->maxCount, ids, w->pd, w->newId));
+    CK(tt_launch_relabel(s, d, w->labels, w->newId, ids, w->unFlag));
+    CK(tt_compact_u8(s, w->unFlag, (long long)N, w->list[0], w->listN[0], w->scanTmp));
+    uint32_t n = 0;
+    CKS(read_u32(w, w->listN[0], &n));
+    w->stats.unassigned_cells = (int)n;
+    int rounds = 0, cur = 0;
+    while (n > 0) {
+        ++rounds;
+        CK(tt_launch_fill_vote(s, d, w->list[cur], w->listN[cur], ids, w->vote));
+        CKS(zero(w, w->listN[1 - cur], 4));
+        CK(tt_launch_fill_apply(s, d, w->list[cur], w->listN[cur], w->vote, ids, w->list[1 - cur], w->listN[1 - cur]));
+        uint32_t n2 = 0;
+        CKS(read_u32(w, w->listN[1 - cur], &n2));
+        cur = 1 - cur;
+        if (n2 == n) break;  // no cell had an assigned neighbour
+        n = n2;
+    }
+    w->stats.fill_rounds = rounds;
+    return TT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// split (SplitPlateV2, plate_tectonic_sim.cu:933-1040)
+// ---------------------------------------------------------------------------
+static int run_split(tt_world *w)
+{
+    hipStream_t s = w->stream;
+    const tt_dev d = make_dev(w);
+    const int plate = w->hctl->split_plate, nid = w->hctl->split_new;
+    w->stats.split_plate = plate;
+    w->stats.split_new_plate = nid;
+    uint8_t *ids = w->ids[w->ic];
+    const size_t N = (size_t)w->N;
+    tt_plate_data hp;
+    CK(hipMemcpyAsync(&hp, w->pd + plate, sizeof(hp), hipMemcpyDeviceToHost, s));
+    CK(tt_launch_split_prepare(s, d, ids, plate, w->eff, w->unFlag));
+    CK(tt_compact_u8(s, w->unFlag, (long long)N, w->list[0], w->listN[0], w->scanTmp));
+    CK(hipStreamSynchronize(s));
+    // relaxation to the fixed point
+    int sweeps = 0;
+    for (;;) {
+        CKS(zero(w, &w->ctl->changed, 4));
+        CK(tt_launch_relax(s, d, w->list[0], w->listN[0], w->h[w->hc], w->eff, w->ctl, 8));
+        sweeps += 8;
+        CKS(sync_ctl(w));
+        if (w->hctl->changed == 0) break;
+        if (sweeps > 200000) { set_error("split relaxation did not converge"); return TT_ERR_STATE; }
+    }
+    w->stats.split_iterations = sweeps;
+    const int px = (int)(hp.geometricCenter[0] + 0.5f), py = (int)(hp.geometricCenter[1] + 0.5f);
+    CK(tt_launch_backtrack(s, d, w->eff, ids, px, py, nid, w->ctl));
+    CKS(sync_ctl(w));
+    if (w->hctl->dead_end) {
+        CK(tt_launch_split_finalize(s, d, w->list[0], w->listN[0], ids, plate, plate, w->pd));
+        return TT_OK;
+    }
+    for (;;) {
+        CKS(zero(w, &w->ctl->changed, 4));
+        CK(tt_launch_flood(s, d, w->list[0], w->listN[0], ids, plate, nid, w->ctl, 8));
+        CKS(sync_ctl(w));
+        if (w->hctl->changed == 0) break;
+    }
+    CK(tt_launch_split_finalize(s, d, w->list[0], w->listN[0], ids, plate, nid, w->pd));
+    CKS(zero(w, w->acc, 256 * 10 * 8));
+    CK(tt_launch_count(s, d, w->mergeIds, ids, w->ids[1 - w->ic], w->h[w->hc], w->noise, w->acc, 1));
+    w->ic = 1 - w->ic;
+    CK(tt_launch_finalize(s, d, w->pd, w->acc, 1, w->ctl));
+    return TT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// one executeIteration
+// ---------------------------------------------------------------------------
+static int record(tt_world *w, int slot)
+{
+    if (w->timing) CK(hipEventRecord(w->ev[slot], w->stream));
+    return TT_OK;
+}
+
+static int step_once(tt_world *w)
+{
+    hipStream_t s = w->stream;
+    const tt_dev d = make_dev(w);
+    const int mask = w->cfg.stage_mask;
+    const size_t N = (size_t)w->N;
+    w->stats.overflow_cells = 0;
+    w->stats.complex_sources = 0;
+    w->stats.num_labels = 0;
+    w->stats.unassigned_cells = 0;
+    w->stats.fill_rounds = 0;
+    w->stats.merges = 0;
+    w->stats.split_plate = -1;
+    w->stats.split_new_plate = -1;
+    w->stats.split_iterations = 0;
+    CKS(record(w, 0));
+    float *hOld = w->h[w->hc], *hNew = w->h[1 - w->hc];
+    uint8_t *idOld = w->ids[w->ic], *idNew = w->ids[1 - w->ic];
+    CK(tt_launch_plate_prep(s, w->pd, w->ps, w->ctl, w->fric));
+    const bool upl = (mask & TT_STAGE_UPLIFT) != 0;
+    if (mask & TT_STAGE_COLLIDE) {
+        CKS(zero(w, w->typeCounts, (size_t)TT_NUM_TRI * 5 * 4));
+        CKS(zero(w, w->hasDiv, TT_NUM_TRI_WORDS * 4));
+        CKS(zero(w, w->accw, N * 4));
+        if (upl) CKS(zero(w, w->contested, N));
+        CK(tt_launch_advect(s, d, idOld, hOld, w->ps, w->ctl, 0, w->typeBitmap, w->word, w->cnt, w->segCount,
+                            w->typeCounts, w->ctl));
+        CK(tt_scan_u32(s, w->segCount, w->segBase, (long long)(N / 64), w->scanTmp, nullptr));
+        CK(tt_launch_keys(s, d, w->word, w->cnt, w->segBase, idOld, w->ps, w->ctl, w->keys));
+        CK(tt_launch_type_matrix(s, d, w->typeCounts, w->typeBitmap));
+        CK(tt_launch_resolve(s, d, w->word, idOld, hOld, w->ps, w->divBitmap, w->typeBitmap, hNew, idNew, w->uplift,
+                             w->accw, w->hasDiv, w->fric, w->srcFlag, w->contested, upl ? 1 : 0));
+        CK(tt_launch_flip(s, w->divBitmap, w->hasDiv));
+        CKS(record(w, 1));
+        if (upl) {
+            CK(tt_launch_propagate_local(s, d, w->word, w->uplift, w->srcFlag, w->contested, w->prop, w->jfa[0],
+                                         w->serialFlag));
+            CK(tt_compact_u8(s, w->serialFlag, (long long)N, w->serialList, w->serialCount, w->scanTmp));
+            CK(tt_launch_propagate_serial(s, d, w->serialList, w->serialCount, w->word, w->uplift, hOld, w->ps,
+                                          w->prop));
+            CK(tt_launch_combine_contested(s, d, w->word, w->uplift, w->contested, w->srcFlag, w->prop, w->jfa[0],
+                                           w->ctl));
+            const int maxDim = std::max(w->W, w->H);
+            int step = 1;
+            while (step < maxDim) step <<= 1;
+            step >>= 1;
+            int jc = 0;
+            while (step > 0) {
+                CK(tt_launch_jfa_pass(s, d, w->jfa[jc], w->jfa[1 - jc], step));
+                jc = 1 - jc;
+                step >>= 1;
+            }
+            CK(tt_launch_uplift(s, d, w->jfa[jc], w->prop, idNew, w->ps, hNew, w->dist));
+        }
+    } else {
+        CK(hipMemcpyAsync(hNew, hOld, N * 4, hipMemcpyDeviceToDevice, s));
+        CK(hipMemcpyAsync(idNew, idOld, N, hipMemcpyDeviceToDevice, s));
+        CKS(zero(w, w->accw, N * 4));
+        CKS(zero(w, w->fric, 256 * 8));
+        CKS(record(w, 1));
+    }
+    CKS(record(w, 2));
+    // thermal and hydraulic erosion on the written height field
+    float *hc = hNew, *ho = hOld;
+    if (mask & TT_STAGE_THERMAL)
+        for (int it = 0; it < w->cfg.thermal_iterations; ++it) {
+            CK(tt_launch_thermal(s, d, hc, ho));
+            std::swap(hc, ho);
+        }
+    CKS(record(w, 3));
+    if ((mask & TT_STAGE_HYDRAULIC) && w->cfg.hydraulic_iterations > 0) {
+        const int rseed = (int)(w->cfg.seed + w->iter);
+        int buf = w->sc;
+        for (int it = 0; it < w->cfg.hydraulic_iterations; ++it) {
+            float *sprev, *sout;
+            if (it == 0) { sprev = w->sed[buf]; sout = w->sed[buf]; }
+            else { sprev = w->sed[buf]; sout = w->sed[1 - buf]; buf = 1 - buf; }
+            CK(tt_launch_hydraulic(s, d, it == 0 ? 1 : 0, rseed, hc, ho, w->water[w->wc], w->water[1 - w->wc],
+                                   w->flux[w->fc], w->flux[1 - w->fc], w->vel, sprev, w->sed[w->sc], sout));
+            std::swap(hc, ho);
+            w->wc = 1 - w->wc;
+            w->fc = 1 - w->fc;
+        }
+        CK(tt_launch_transport(s, d, w->sed[buf], w->vel, w->sed[1 - buf]));
+        w->sc = 1 - buf;
+    }
+    CKS(record(w, 4));
+    // copy-back (plate_tectonic_sim.cu:77-78) is a change of buffer index
+    w->hc = (hc == w->h[0]) ? 0 : 1;
+    w->ic = 1 - w->ic;
+    if (mask & TT_STAGE_KINEMATICS) {
+        CK(tt_launch_plate_movement(s, d, w->pd, w->fric, w->noise));
+        CK(tt_launch_accretion_apply(s, d, w->accw, w->h[w->hc], w->ids[w->ic], w->ids[1 - w->ic], w->accretion));
+        w->ic = 1 - w->ic;
+    }
+    if ((mask & TT_STAGE_CCL) && (w->iter % (uint32_t)w->cfg.iterations_for_ccl) == 0) CKS(run_ccl(w));
+    if (mask & TT_STAGE_MERGE) {
+        CKS(zero(w, w->nbm, (size_t)MAXP * MAXP + 4));
+        CK(tt_launch_neighbor_matrix(s, d, w->ids[w->ic], w->nbm));
+        CK(tt_launch_merge_decide(s, d, w->nbm, w->pd, w->mergeIds, w->typeBitmap, w->ctl));
+        CKS(zero(w, w->acc, 256 * 10 * 8));
+        CK(tt_launch_count(s, d, w->mergeIds, w->ids[w->ic], w->ids[1 - w->ic], w->h[w->hc], w->noise, w->acc, 0));
+        w->ic = 1 - w->ic;
+        CK(tt_launch_finalize(s, d, w->pd, w->acc, (mask & TT_STAGE_SPLIT) ? 0 : 2, w->ctl));
+    }
+    CKS(record(w, 5));
+    CKS(sync_ctl(w));
+    if ((mask & TT_STAGE_MERGE) && (mask & TT_STAGE_SPLIT) && w->hctl->split_plate >= 0) CKS(run_split(w));
+    CKS(record(w, 6));
+    CKS(sync_ctl(w));
+    const tt_step_ctl &c = *w->hctl;
+    w->stats.overflow_cells = (mask & TT_STAGE_COLLIDE) ? c.overflow_cells : 0;
+    w->stats.complex_sources = ((mask & TT_STAGE_COLLIDE) && upl) ? c.complex_sources : 0;
+    if (mask & TT_STAGE_MERGE) {
+        w->stats.merges = c.merges;
+        w->stats.largest_value = c.largest_value;
+        w->stats.largest_plate_id = c.largest_id;
+        w->stats.heaviest_value = c.heaviest_value;
+        w->stats.heaviest_plate_id = c.heaviest_id;
+        w->stats.number_of_plates = c.n_plates;
+        w->stats.mass_ledger = tt_unfix_d(c.ledger, TT_MASS_BITS);
+    }
+    w->iter++;
+    w->stats.iteration = w->iter;
+    if (w->timing) {
+        float t[7] = {0};
+        for (int i = 1; i <= 6; ++i) CK(hipEventElapsedTime(&t[i], w->ev[i - 1], w->ev[i]));
+        w->times.ms_collide += t[1];
+        w->times.ms_uplift += t[2];
+        w->times.ms_thermal += t[3];
+        w->times.ms_hydraulic += t[4];
+        w->times.ms_post += t[5];
+        w->times.ms_split += t[6];
+        float tot = 0;
+        CK(hipEventElapsedTime(&tot, w->ev[0], w->ev[6]));
+        w->times.ms_total += tot;
+    }
+    return TT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// initialisation (initializeTectonics, plate_tectonic_sim.cu:705-773)
+// ---------------------------------------------------------------------------
+static int upload_seeds(tt_world *w, const std::vector<float> &sx, const std::vector<float> &sy,
+                        const std::vector<uint8_t> &sid)
+{
+    const size_t n = sx.size();
+    if (n > 4096) { set_error("too many Voronoi seeds (max 4096)"); return TT_ERR_UNSUPPORTED; }
+    std::vector<uint8_t> blob(4096 * 12, 0);
+    memcpy(blob.data(), sx.data(), n * 4);
+    memcpy(blob.data() + 4096 * 4, sy.data(), n * 4);
+    memcpy(blob.data() + 4096 * 8, sid.data(), n);
+    CK(hipMemcpyAsync(w->initSeeds, blob.data(), blob.size(), hipMemcpyHostToDevice, w->stream));
+    CK(hipStreamSynchronize(w->stream));
+    const tt_dev d = make_dev(w);
+    const float *dsx = (const float *)w->initSeeds;
+    const float *dsy = (const float *)(w->initSeeds + 4096 * 4);
+    const uint8_t *dsid = (const uint8_t *)(w->initSeeds + 4096 * 8);
+    CK(tt_launch_init_plate_ids(w->stream, d, dsx, dsy, dsid, (int)n, w->ids[w->ic]));
+    return TT_OK;
+}
+
+static void gen_plate_data(tt_world *w, SplitMix &rng, const std::vector<float> &cx, const std::vector<float> &cy,
+                           bool reference_speed, std::vector<tt_plate_data> &pd)
+{
+    const int P = w->cfg.num_plates;
+    pd.assign(256, default_plate());
+    for (int i = 0; i < P; ++i) {
+        tt_plate_data &p = pd[i];
+        p.pixelCenter[0] = cx[i] - tt_floorf(cx[i]);
+        p.pixelCenter[1] = cy[i] - tt_floorf(cy[i]);
+        if (reference_speed) p.velocity = ((rng.u01() * 2.0f - 1.0f) + 1.0f) / 2.0f;
+        else p.velocity = w->cfg.speed_min + (w->cfg.speed_max - w->cfg.speed_min) * rng.u01();
+        p.velocitySmoothed = p.velocity;
+        const float xd = rng.u01() * 2.0f - 1.0f;
+        const float yd = rng.u01() * 2.0f - 1.0f;
+        const float m = sqrtf(xd * xd + yd * yd);
+        if (m > 0.0f) { p.direction[0] = xd / m; p.direction[1] = yd / m; }
+        else { p.direction[0] = 1.0f; p.direction[1] = 0.0f; }
+        p.directionSmoothed[0] = p.direction[0];
+        p.directionSmoothed[1] = p.direction[1];
+    }
+}
+
+static int init_common_zero(tt_world *w)
+{
+    const size_t N = (size_t)w->N;
+    for (int i = 0; i < 2; ++i) {
+        CKS(zero(w, w->water[i], N * 4));
+        CKS(zero(w, w->flux[i], N * 16));
+        CKS(zero(w, w->sed[i], N * 4));
+    }
+    CKS(zero(w, w->vel, N * 8));
+    CKS(zero(w, w->word, N * 4));
+    CKS(zero(w, w->accretion, N));
+    CKS(zero(w, w->accw, N * 4));
+    CKS(zero(w, w->uplift, N * 8));
+    CKS(zero(w, w->prop, N * 12));
+    CKS(zero(w, w->divBitmap, TT_NUM_TRI_WORDS * 4));
+    CKS(zero(w, w->typeBitmap, TT_NUM_TRI));
+    CKS(zero(w, w->keys, N * 8));
+    CKS(zero(w, w->labels, N * 4));
+    CKS(zero(w, w->dist, N * 8));
+    return TT_OK;
+}
+
+static int init_counts(tt_world *w)
+{
+    const tt_dev d = make_dev(w);
+    CKS(zero(w, w->acc, 256 * 10 * 8));
+    CK(tt_launch_count(w->stream, d, w->mergeIds, w->ids[w->ic], w->ids[1 - w->ic], w->h[w->hc], w->noise, w->acc, 1));
+    w->ic = 1 - w->ic;
+    CK(tt_launch_finalize(w->stream, d, w->pd, w->acc, 3, w->ctl));
+    return TT_OK;
+}
+
+int tt_world_do_init(tt_world *w)
+{
+    hipStream_t s = w->stream;
+    w->iter = 0;
+    w->hc = w->ic = w->wc = w->fc = w->sc = 0;
+    CKS(init_common_zero(w));
+    const tt_dev d = make_dev(w);
+    const int mode = w->cfg.init_mode;
+    if (mode == TT_INIT_EMPTY) {
+        std::vector<tt_plate_data> pd(256, default_plate());
+        CK(hipMemcpyAsync(w->pd, pd.data(), 256 * sizeof(tt_plate_data), hipMemcpyHostToDevice, s));
+        CKS(zero(w, w->h[0], (size_t)w->N * 4));
+        CKS(zero(w, w->ids[0], (size_t)w->N));
+        CK(hipStreamSynchronize(s));
+        w->initialized = true;
+        return TT_OK;
+    }
+    SplitMix rng{(uint64_t)w->cfg.seed};
+    const int P = w->cfg.num_plates;
+    std::vector<float> cx(P), cy(P);
+    std::vector<uint8_t> cid(P);
+    for (int i = 0; i < P; ++i) {
+        cx[i] = rng.u01() * (float)w->H;
+        cy[i] = rng.u01() * (float)w->W;
+        cid[i] = (uint8_t)i;
+    }
+    std::vector<float> sx = cx, sy = cy;
+    std::vector<uint8_t> sid = cid;
+    if (mode == TT_INIT_REFERENCE) {
+        for (int g = 0; g < w->cfg.num_voronoi_gens; ++g) {
+            const int n = w->cfg.voronoi_gens[g];
+            std::vector<float> nx(n), ny(n);
+            std::vector<uint8_t> ni(n);
+            for (int i = 0; i < n; ++i) {
+                const float px = rng.u01() * (float)w->H;
+                const float py = rng.u01() * (float)w->W;
+                float best = 3.402823466e38f;
+                uint8_t cl = 0;
+                for (size_t j = 0; j < sx.size(); ++j) {
+                    const float dx = tt_fabsf(px - sx[j]), dy = tt_fabsf(py - sy[j]);
+                    const float wx = tt_fminf(dx, (float)w->W - dx), wy = tt_fminf(dy, (float)w->H - dy);
+                    const float d2 = wx * wx + wy * wy;
+                    if (d2 < best) { cl = sid[j]; best = d2; }
+                }
+                nx[i] = px;
+                ny[i] = py;
+                ni[i] = cl;
+            }
+            sx.swap(nx);
+            sy.swap(ny);
+            sid.swap(ni);
+        }
+        if (sx.size() < cx.size()) { sx = cx; sy = cy; sid = cid; }
+    }
+    std::vector<tt_plate_data> pd;
+    gen_plate_data(w, rng, cx, cy, mode == TT_INIT_REFERENCE, pd);
+    CK(hipMemcpyAsync(w->pd, pd.data(), 256 * sizeof(tt_plate_data), hipMemcpyHostToDevice, s));
+    if (mode == TT_INIT_SYNTHETIC) {
+        double wts[65], tot = 0.0;
+        for (int k = -32; k <= 32; ++k) { wts[k + 32] = tt_exp_core(-(double)(k * k) / 128.0); tot = tot + wts[k + 32]; }
+        for (int k = 0; k < 65; ++k) wts[k] = wts[k] / tot;
+        CK(hipMemcpyAsync(w->initWts, wts, sizeof(wts), hipMemcpyHostToDevice, s));
+        const uint64_t hseed = mix64((uint64_t)w->cfg.seed ^ 0x6A09E667F3BCC909ull);
+        // the two fp64 blur buffers borrow the flux arrays (16 B/cell each)
+        CK(tt_launch_init_synthetic_heights(s, d, hseed, (double *)w->flux[0], (double *)w->flux[1],
+                                            (const double *)w->initWts, w->h[w->hc]));
+    } else {
+        CK(tt_launch_init_reference_heights(s, d, w->h[w->hc]));
+    }
+    CKS(upload_seeds(w, sx, sy, sid));
+    CKS(init_counts(w));
+    for (int i = 0; i < 2; ++i) CKS(zero(w, w->flux[i], (size_t)w->N * 16));
+    CKS(run_ccl(w));
+    CK(hipStreamSynchronize(s));
+    w->initialized = true;
+    return TT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// field access
+// ---------------------------------------------------------------------------
+static bool field_ptr(tt_world *w, int f, void **p, size_t *bytes, bool write)
+{
+    const size_t N = (size_t)w->N;
+    switch (f) {
+        case TT_FIELD_HEIGHT: *p = w->h[w->hc]; *bytes = N * 4; return true;
+        case TT_FIELD_PLATE_IDS: *p = w->ids[w->ic]; *bytes = N; return true;
+        case TT_FIELD_WATER: *p = w->water[w->wc]; *bytes = N * 4; return true;
+        case TT_FIELD_FLUX: *p = w->flux[w->fc]; *bytes = N * 16; return true;
+        case TT_FIELD_VELOCITY: *p = w->vel; *bytes = N * 8; return true;
+        case TT_FIELD_SEDIMENT: *p = w->sed[w->sc]; *bytes = N * 4; return true;
+        case TT_FIELD_ASTHENOSPHERE: if (write) return false; *p = nullptr; *bytes = N * 8; return true;
+        case TT_FIELD_COLLISIONS: *p = w->word; *bytes = N * 4; return true;
+        case TT_FIELD_ACCRETION: *p = w->accretion; *bytes = N; return true;
+        case TT_FIELD_UPLIFT: *p = w->uplift; *bytes = N * 8; return true;
+        case TT_FIELD_PROPAGATED: *p = w->prop; *bytes = N * 12; return true;
+        case TT_FIELD_PLATE_DATA: *p = w->pd; *bytes = sizeof(tt_plate_data) * MAXP; return true;
+        case TT_FIELD_DIVERGENCE_BITMAP: *p = w->divBitmap; *bytes = TT_NUM_TRI_WORDS * 4; return true;
+        case TT_FIELD_COLLISION_TYPES: *p = w->typeBitmap; *bytes = TT_NUM_TRI; return true;
+        case TT_FIELD_SORTED_KEYS: *p = w->keys; *bytes = N * 8; return true;
+        case TT_FIELD_LABELS: *p = w->labels; *bytes = N * 4; return true;
+        case TT_FIELD_DISTANCE: *p = w->dist; *bytes = N * 8; return true;
+        default: return false;
+    }
+}
+
+int tt_world_geometry(tt_world *w, int32_t *width, int32_t *height, tt_kernel_settings *ks)
+{
+    if (!w || !width || !height || !ks) return TT_ERR_INVALID;
+    *width = w->W;
+    *height = w->H;
+    *ks = w->ks;
+    return TT_OK;
+}
+
+int tt_validate_config(const tt_sim_config *c)
+{
+    if (c->width < 64 || c->height < 64 || c->width % TT_TILE_W || c->height % TT_TILE_H) {
+        set_error("width must be a multiple of 64 and height a multiple of 16, both >= 64");
+        return TT_ERR_INVALID;
+    }
+    if (c->width > 16384 || c->height > 16384) { set_error("sides above 16384 are not supported"); return TT_ERR_UNSUPPORTED; }
+    if (c->num_plates < 1 || c->num_plates > 254) { set_error("num_plates must be in 1..254"); return TT_ERR_INVALID; }
+    if (c->iterations_for_ccl < 1) { set_error("iterations_for_ccl must be >= 1"); return TT_ERR_INVALID; }
+    if (c->hydraulic_iterations < 0 || c->thermal_iterations < 0) { set_error("negative iteration count"); return TT_ERR_INVALID; }
+    if (c->num_voronoi_gens < 0 || c->num_voronoi_gens > 4) { set_error("num_voronoi_gens must be in 0..4"); return TT_ERR_INVALID; }
+    if (c->init_mode < 0 || c->init_mode > 2) { set_error("bad init_mode"); return TT_ERR_INVALID; }
+    return TT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *tt_status_string(int status)
+{
+    switch (status) {
+        case TT_OK: return "ok";
+        case TT_ERR_INVALID: return "invalid argument";
+        case TT_ERR_DEVICE: return "device error";
+        case TT_ERR_NOMEM: return "out of device memory";
+        case TT_ERR_STATE: return "call out of order";
+        case TT_ERR_UNSUPPORTED: return "unsupported";
+        case TT_ERR_BUDGET: return "request exceeds the HBM budget";
+        default: return "unknown status";
+    }
+}
+
+const char *tt_last_error(void) { return g_last_error.c_str(); }
+
+int tt_device_count(int *count)
+{
+    if (!count) return TT_ERR_INVALID;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return TT_OK;
+}
+
+int tt_settings_default(tt_kernel_settings *k)
+{
+    if (!k) return TT_ERR_INVALID;
+    tt_settings_fill_defaults(k);
+    return TT_OK;
+}
+
+int tt_config_default(tt_sim_config *c)
+{
+    if (!c) return TT_ERR_INVALID;
+    memset(c, 0, sizeof(*c));
+    c->width = 1024;
+    c->height = 1024;
+    c->seed = 1000;
+    c->num_plates = 16;
+    c->num_voronoi_gens = 2;
+    c->voronoi_gens[0] = 200;
+    c->voronoi_gens[1] = 500;
+    c->iterations_for_ccl = 10;
+    c->hydraulic_iterations = 10;
+    c->thermal_iterations = 1;
+    c->init_mode = TT_INIT_REFERENCE;
+    c->speed_min = 0.0f;
+    c->speed_max = 1.0f;
+    c->stage_mask = TT_STAGE_ALL;
+    return TT_OK;
+}
+
+int tt_world_bytes(const tt_sim_config *cfg, size_t *bytes)
+{
+    if (!cfg || !bytes) return TT_ERR_INVALID;
+    tt_world tmp;
+    *bytes = tt_world_layout(*cfg, false, &tmp);
+    return TT_OK;
+}
+
+int tt_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, tt_world **out)
+{
+    if (!cfg || !ks || !out) return TT_ERR_INVALID;
+    *out = nullptr;
+    CKS(tt_validate_config(cfg));
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_error("no HIP device"); return TT_ERR_DEVICE; }
+    tt_world *w = new (std::nothrow) tt_world();
+    if (!w) return TT_ERR_NOMEM;
+    w->cfg = *cfg;
+    w->ks = *ks;
+    w->W = cfg->width;
+    w->H = cfg->height;
+    w->N = cfg->width * cfg->height;
+    CK(hipGetDevice(&w->device));
+    tt_world_layout(*cfg, true, w);
+    size_t total = 0;
+    for (auto &a : w->allocs) total += a.bytes;
+    char *base = nullptr;
+    if (hipMalloc((void **)&base, total) != hipSuccess) {
+        delete w;
+        set_error("hipMalloc of " + std::to_string(total) + " bytes failed");
+        return TT_ERR_NOMEM;
+    }
+    w->base = base;
+    size_t off = 0;
+    for (auto &a : w->allocs) { *a.ptr = base + off; off += a.bytes; }
+    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&w->hctl, sizeof(tt_step_ctl), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&w->hu32, 64, hipHostMallocDefault) != hipSuccess) {
+        tt_world_destroy(w);
+        set_error("stream / pinned memory allocation failed");
+        return TT_ERR_DEVICE;
+    }
+    for (int i = 0; i < 8; ++i)
+        if (hipEventCreate(&w->ev[i]) != hipSuccess) { tt_world_destroy(w); return TT_ERR_DEVICE; }
+    memset(&w->stats, 0, sizeof(w->stats));
+    memset(&w->times, 0, sizeof(w->times));
+    w->stats.split_plate = w->stats.split_new_plate = -1;
+    const std::vector<uint8_t> &noise = tt_noise_table();
+    if (noise.size() != 65536) {
+        tt_world_destroy(w);
+        set_error("noise table missing (master_thesis_amd/data/noise_r8.bin)");
+        return TT_ERR_STATE;
+    }
+    if (hipMemcpy(w->noise, noise.data(), 65536, hipMemcpyHostToDevice) != hipSuccess) {
+        tt_world_destroy(w);
+        return TT_ERR_DEVICE;
+    }
+    *out = w;
+    return TT_OK;
+}
+
+int tt_world_init(tt_world *w)
+{
+    if (!w) return TT_ERR_INVALID;
+    return tt_world_do_init(w);
+}
+
+int tt_world_reset(tt_world *w, uint32_t seed, int32_t num_plates)
+{
+    if (!w) return TT_ERR_INVALID;
+    tt_sim_config c = w->cfg;
+    c.seed = seed;
+    c.num_plates = num_plates;
+    CKS(tt_validate_config(&c));
+    w->cfg = c;
+    return tt_world_do_init(w);
+}
+
+int tt_world_step(tt_world *w, int32_t n)
+{
+    if (!w || n < 0) return TT_ERR_INVALID;
+    if (!w->initialized) { set_error("tt_world_step before tt_world_init"); return TT_ERR_STATE; }
+    for (int i = 0; i < n; ++i) CKS(step_once(w));
+    CK(hipStreamSynchronize(w->stream));
+    return TT_OK;
+}
+
+int tt_world_set_settings(tt_world *w, const tt_kernel_settings *ks)
+{
+    if (!w || !ks) return TT_ERR_INVALID;
+    w->ks = *ks;
+    return TT_OK;
+}
+
+int tt_world_download(tt_world *w, int32_t field, void *host, size_t bytes)
+{
+    if (!w || !host) return TT_ERR_INVALID;
+    void *p;
+    size_t b;
+    if (!field_ptr(w, field, &p, &b, false) || b != bytes) { set_error("bad field or size"); return TT_ERR_INVALID; }
+    if (field == TT_FIELD_ASTHENOSPHERE) {
+        float2 *tmp = (float2 *)w->fragKey;  // N * 8 bytes of CCL scratch
+        const tt_dev d = make_dev(w);
+        CK(tt_launch_asthenosphere(w->stream, d, w->noise, tmp));
+        CK(hipMemcpyAsync(host, tmp, b, hipMemcpyDeviceToHost, w->stream));
+        CK(hipStreamSynchronize(w->stream));
+        return TT_OK;
+    }
+    CK(hipMemcpyAsync(host, p, b, hipMemcpyDeviceToHost, w->stream));
+    CK(hipStreamSynchronize(w->stream));
+    return TT_OK;
+}
+
+int tt_world_upload(tt_world *w, int32_t field, const void *host, size_t bytes)
+{
+    if (!w || !host) return TT_ERR_INVALID;
+    void *p;
+    size_t b;
+    if (!field_ptr(w, field, &p, &b, true) || b != bytes) { set_error("bad field, read-only field or size"); return TT_ERR_INVALID; }
+    CK(hipMemcpyAsync(p, host, b, hipMemcpyHostToDevice, w->stream));
+    CK(hipStreamSynchronize(w->stream));
+    w->initialized = true;
+    return TT_OK;
+}
+
+int tt_world_stats(tt_world *w, tt_step_stats *out)
+{
+    if (!w || !out) return TT_ERR_INVALID;
+    *out = w->stats;
+    return TT_OK;
+}
+
+int tt_world_iteration(tt_world *w, uint32_t *it)
+{
+    if (!w || !it) return TT_ERR_INVALID;
+    *it = w->iter;
+    return TT_OK;
+}
+
+int tt_world_set_iteration(tt_world *w, uint32_t it)
+{
+    if (!w) return TT_ERR_INVALID;
+    w->iter = it;
+    return TT_OK;
+}
+
+int tt_world_set_timing(tt_world *w, int32_t enabled)
+{
+    if (!w) return TT_ERR_INVALID;
+    w->timing = enabled != 0;
+    memset(&w->times, 0, sizeof(w->times));
+    return TT_OK;
+}
+
+int tt_world_stage_times(tt_world *w, tt_stage_times *out)
+{
+    if (!w || !out) return TT_ERR_INVALID;
+    *out = w->times;
+    return TT_OK;
+}
+
+int tt_world_destroy(tt_world *w)
+{
+    if (!w) return TT_ERR_INVALID;
+    if (w->stream) (void)hipStreamSynchronize(w->stream);
+    for (int i = 0; i < 8; ++i)
+        if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
+    if (w->stream) (void)hipStreamDestroy(w->stream);
+    if (w->hctl) (void)hipHostFree(w->hctl);
+    if (w->hu32) (void)hipHostFree(w->hu32);
+    if (w->base) (void)hipFree(w->base);
+    delete w;
+    return TT_OK;
+}
+
+int tt_math_eval_device(int32_t fn, const float *in, float *out, int64_t n)
+{
+    if (!in || !out || n < 0) return TT_ERR_INVALID;
+    if (n == 0) return TT_OK;
+    const int per = (fn == TT_MATH_SIMPLEX || fn == TT_MATH_DISCRETE) ? 3 : (fn == TT_MATH_FMODF || fn == TT_MATH_ATAN2) ? 2 : 1;
+    float *din = nullptr, *dout = nullptr;
+    CK(hipMalloc((void **)&din, (size_t)n * per * 4));
+    if (hipMalloc((void **)&dout, (size_t)n * 4) != hipSuccess) { (void)hipFree(din); return TT_ERR_NOMEM; }
+    int st = TT_OK;
+    if (hipMemcpy(din, in, (size_t)n * per * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        tt_launch_math_eval(0, fn, din, dout, n) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("device math evaluation failed");
+        st = TT_ERR_DEVICE;
+    }
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return st;
+}
+
+}  // extern "C"

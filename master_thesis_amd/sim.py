@@ -1,0 +1,175 @@
+"""Python host mirror of the reference's step interface over the C ABI.
+
+`World` is a thin handle on a libtectonic world (tectonic.h).
+`PlateTectonicSim` keeps the public surface of the reference class
+(src/cuda_compute/plate_tectonic_sim.h:111-130): constructor
+(width, height, seed), initialize(numStartingPlates, numVoronoiSeeds),
+executeIteration(), resetSim(seed, numStartingPlates, numVoronoiSeeds).
+Every call goes to the gfx950 kernels; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, lib
+
+
+class World:
+    def __init__(self, cfg: _abi.SimConfig, ks: _abi.KernelSettings):
+        self.cfg = cfg.copy()
+        self.ks = ks.copy()
+        self.n = cfg.width * cfg.height
+        h = C.c_void_p()
+        check(lib().tt_world_create(C.byref(self.cfg), C.byref(self.ks), C.byref(h)), "tt_world_create")
+        self.h = h
+
+    # lifecycle -----------------------------------------------------------
+    def init(self) -> "World":
+        check(lib().tt_world_init(self.h), "tt_world_init")
+        return self
+
+    def reset(self, seed: int, plates: int) -> None:
+        check(lib().tt_world_reset(self.h, seed, plates), "tt_world_reset")
+        self.cfg.seed, self.cfg.num_plates = seed, plates
+
+    def step(self, n: int = 1) -> None:
+        check(lib().tt_world_step(self.h, n), "tt_world_step")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().tt_world_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # state ----------------------------------------------------------------
+    def download(self, field: int) -> np.ndarray:
+        dt, shape = _abi.field_spec(field, self.n)
+        a = np.empty(shape, dtype=dt)
+        check(lib().tt_world_download(self.h, field, a.ctypes.data, a.nbytes), f"download {field}")
+        return a
+
+    def upload(self, field: int, a: np.ndarray) -> None:
+        dt, shape = _abi.field_spec(field, self.n)
+        a = np.ascontiguousarray(a, dtype=dt).reshape(shape)
+        check(lib().tt_world_upload(self.h, field, a.ctypes.data, a.nbytes), f"upload {field}")
+
+    def set_settings(self, ks: _abi.KernelSettings) -> None:
+        check(lib().tt_world_set_settings(self.h, C.byref(ks)), "set_settings")
+        self.ks = ks.copy()
+
+    def stats(self) -> _abi.StepStats:
+        s = _abi.StepStats()
+        check(lib().tt_world_stats(self.h, C.byref(s)), "stats")
+        return s
+
+    @property
+    def iteration(self) -> int:
+        it = C.c_uint32()
+        check(lib().tt_world_iteration(self.h, C.byref(it)), "iteration")
+        return it.value
+
+    def set_iteration(self, it: int) -> None:
+        check(lib().tt_world_set_iteration(self.h, it), "set_iteration")
+
+    def set_timing(self, on: bool) -> None:
+        check(lib().tt_world_set_timing(self.h, 1 if on else 0), "set_timing")
+
+    def stage_times(self) -> _abi.StageTimes:
+        t = _abi.StageTimes()
+        check(lib().tt_world_stage_times(self.h, C.byref(t)), "stage_times")
+        return t
+
+    def export_png(self, what: int, path: str) -> None:
+        check(lib().tt_world_export_png(self.h, what, path.encode()), "export_png")
+
+
+STATE_FIELDS = (
+    _abi.FIELD_HEIGHT, _abi.FIELD_PLATE_IDS, _abi.FIELD_WATER, _abi.FIELD_FLUX, _abi.FIELD_VELOCITY,
+    _abi.FIELD_SEDIMENT, _abi.FIELD_UPLIFT, _abi.FIELD_PLATE_DATA, _abi.FIELD_DIVERGENCE_BITMAP,
+    _abi.FIELD_COLLISION_TYPES,
+)
+
+
+def world_bytes(cfg: _abi.SimConfig) -> int:
+    b = C.c_size_t()
+    check(lib().tt_world_bytes(C.byref(cfg), C.byref(b)), "tt_world_bytes")
+    return b.value
+
+
+def device_count() -> int:
+    n = C.c_int()
+    check(lib().tt_device_count(C.byref(n)), "tt_device_count")
+    return n.value
+
+
+def math_eval_device(fn: int, x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    per = {_abi.TT_MATH_SIMPLEX: 3, _abi.TT_MATH_DISCRETE: 3, _abi.TT_MATH_FMODF: 2, _abi.TT_MATH_ATAN2: 2}.get(fn, 1)
+    n = x.size // per
+    out = np.empty(n, dtype=np.float32)
+    check(lib().tt_math_eval_device(fn, x.ctypes.data, out.ctypes.data, n), "tt_math_eval_device")
+    return out
+
+
+def parse_json_config(text: str, cfg: _abi.SimConfig | None = None, ks: _abi.KernelSettings | None = None):
+    cfg = cfg.copy() if cfg is not None else _abi.SimConfig.defaults()
+    ks = ks.copy() if ks is not None else _abi.KernelSettings.defaults()
+    check(lib().tt_config_parse_json(text.encode(), C.byref(cfg), C.byref(ks)), "tt_config_parse_json")
+    return cfg, ks
+
+
+class PlateTectonicSim:
+    """Mirror of PlateTectonicSim (plate_tectonic_sim.h:56-179) without the GL
+    interop manager (headless; plate_tectonic_sim.cu:49,121 accept nullptr)."""
+
+    def __init__(self, width: int, height: int, seed: int, interop_manager=None,
+                 settings: _abi.KernelSettings | None = None, config: _abi.SimConfig | None = None):
+        if interop_manager is not None:
+            raise ValueError("the OpenGL interop path is not part of this library (headless only)")
+        self._cfg = config.copy() if config is not None else _abi.SimConfig.defaults()
+        self._cfg.width, self._cfg.height, self._cfg.seed = width, height, seed
+        self._ks = settings.copy() if settings is not None else _abi.KernelSettings.defaults()
+        self._world: World | None = None
+
+    def initialize(self, numStartingPlates: int, numVoronoiSeeds: Sequence[int]) -> None:
+        self._configure(numStartingPlates, numVoronoiSeeds)
+        if self._world is not None:
+            self._world.close()
+        self._world = World(self._cfg, self._ks).init()
+
+    def executeIteration(self) -> None:
+        if self._world is None:
+            raise RuntimeError("initialize() first")
+        self._world.step(1)
+
+    def resetSim(self, seed: int, numStartingPlates: int, numVoronoiSeeds: Sequence[int]) -> None:
+        self._cfg.seed = seed
+        self.initialize(numStartingPlates, numVoronoiSeeds)
+
+    def _configure(self, plates: int, gens: Sequence[int]) -> None:
+        gens = list(gens)[:4]
+        self._cfg.num_plates = plates
+        self._cfg.num_voronoi_gens = len(gens)
+        for i, g in enumerate(gens):
+            self._cfg.voronoi_gens[i] = int(g)
+
+    @property
+    def world(self) -> World:
+        if self._world is None:
+            raise RuntimeError("initialize() first")
+        return self._world

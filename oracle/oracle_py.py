@@ -35,6 +35,9 @@ def lib():
         L.oracle_world_set.argtypes = [vp, C.c_int, vp, C.c_size_t]
         L.oracle_world_set_iteration.argtypes = [vp, C.c_uint32]
         L.oracle_world_stats.argtypes = [vp, C.POINTER(_abi.StepStats)]
+        L.oracle_world_iteration.argtypes = [vp, C.POINTER(C.c_uint32)]
+        L.oracle_hash_eval.argtypes = [vp, vp, C.c_longlong]
+        L.oracle_jfa.argtypes = [C.c_int, C.c_int, vp, vp, vp]
         L.oracle_world_ccl.argtypes = [vp]
         L.oracle_world_destroy.argtypes = [vp]
         L.oracle_math_eval.argtypes = [C.c_int, vp, vp, C.c_longlong]
@@ -68,6 +71,12 @@ class OracleWorld:
         dt, shape = _abi.field_spec(field, self.n)
         a = np.ascontiguousarray(a, dtype=dt).reshape(shape)
         self._ok(lib().oracle_world_set(self.h, field, a.ctypes.data, a.nbytes), f"set {field}")
+
+    @property
+    def iteration(self) -> int:
+        it = C.c_uint32()
+        self._ok(lib().oracle_world_iteration(self.h, C.byref(it)), "iteration")
+        return it.value
 
     def set_iteration(self, it: int):
         self._ok(lib().oracle_world_set_iteration(self.h, it), "set_iteration")
@@ -106,3 +115,22 @@ def math_eval(fn: int, x: np.ndarray) -> np.ndarray:
     if st != 0:
         raise RuntimeError(f"oracle math {fn} failed")
     return out
+
+
+def hash_eval(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    out = np.empty_like(x)
+    lib().oracle_hash_eval(x.ctypes.data, out.ctypes.data, x.size)
+    return out
+
+
+def jfa(seed_mask: np.ndarray):
+    """Jump flood of the step on a boolean seed mask (H, W); returns (dist, orig)."""
+    H, W = seed_mask.shape
+    seed = np.where(seed_mask.ravel(), np.arange(W * H), -1).astype(np.int32)
+    dist = np.empty(W * H, np.int32)
+    orig = np.empty(W * H, np.uint32)
+    st = lib().oracle_jfa(W, H, seed.ctypes.data, dist.ctypes.data, orig.ctypes.data)
+    if st != 0:
+        raise RuntimeError("oracle_jfa failed")
+    return dist.reshape(H, W), orig.reshape(H, W)

@@ -417,7 +417,7 @@ void stage_collision_types(World &w, const std::vector<int> &offx, const std::ve
     for (int y = 0; y < w.H; ++y)
         for (int x = 0; x < w.W; ++x) {
             const uint32_t cp = w.collisions[(size_t)y * w.W + x];
-            if (cp <= 0xFF) continue;
+            if (cp <= 0xFF || ((cp >> 8) & 0xFF) == 0) continue;
             int p[4];
             for (int s = 0; s < 4; ++s) p[s] = (uint8_t)(MAXP - ((cp >> (8 * s)) & 0xFF));
             auto hgt = [&](int q) { return w.height[w.idx(x - offx[q], y - offy[q])]; };
@@ -496,7 +496,6 @@ void stage_resolve(World &w, const std::vector<int> &offx, const std::vector<int
                    std::vector<float> &hW, std::vector<uint8_t> &idW, std::vector<long long> &fric)
 {
     std::vector<uint32_t> hasDiv(NUM_TRI_WORDS, 0u);
-    std::fill(w.accretion.begin(), w.accretion.end(), 0);
     fric.assign(MAXP, 0);
     const int seed = (int)w.cfg.seed;
     static const int pairL[6][2] = {{0, 1}, {0, 2}, {1, 2}, {0, 3}, {1, 3}, {2, 3}};
@@ -510,7 +509,7 @@ void stage_resolve(World &w, const std::vector<int> &offx, const std::vector<int
                 const uint32_t ov = w.collisions[w.idx(x - offx[prev], y - offy[prev])];
                 const uint8_t opp = (uint8_t)((MAXP - ov) & 0xFF);
                 int nid;
-                if (ov != 0 && ov <= 0xFF && opp != prev) {
+                if (ov != 0 && ov <= 0xFF && opp != prev && prev != MAXP) {
                     const int mn = prev < opp ? prev : opp, mx = prev < opp ? opp : prev;
                     const unsigned bit = tri((unsigned)mn, (unsigned)mx);
                     nid = (w.divBitmap[bit / 32] & (1u << (bit % 32))) ? mn : mx;
@@ -1561,6 +1560,15 @@ void step(World &w)
     std::vector<long long> fric(MAXP, 0);
     w.stats.overflow_cells = 0;
     w.stats.complex_sources = 0;
+    w.stats.num_labels = 0;
+    w.stats.unassigned_cells = 0;
+    w.stats.fill_rounds = 0;
+    w.stats.merges = 0;
+    w.stats.split_plate = -1;
+    w.stats.split_new_plate = -1;
+    w.stats.split_iterations = 0;
+    // the accretion marks of a step start from zero (plate_tectonic_sim.cu:252)
+    std::fill(w.accretion.begin(), w.accretion.end(), 0);
     if (mask & TT_STAGE_COLLIDE) {
         stage_advect(w, offx, offy);
         stage_collision_types(w, offx, offy);
@@ -1646,12 +1654,12 @@ void *oracle_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks
     w.accretion.assign(N, 0);
     w.uplift.assign(N, UpliftData{0.0f, 0, {0, 0, 0}});
     w.prop.assign(N, PropData{0.0f, 0u, 0, {0, 0, 0}});
-    w.pd.assign(MAXP, default_plate());
+    w.pd.assign(MAXP + 1, default_plate());
     w.divBitmap.assign(NUM_TRI_WORDS, 0u);
     w.typeBitmap.assign(NUM_TRI, 0);
     w.keys.assign(N, 0ull);
     w.labels.assign(N, 0u);
-    w.dist.assign(N, DistBuf{-1, 0u});
+    w.dist.assign(N, DistBuf{0, 0u});
     memset(&w.stats, 0, sizeof(w.stats));
     w.stats.split_plate = -1;
     w.stats.split_new_plate = -1;
@@ -1705,6 +1713,13 @@ int oracle_world_set_iteration(void *hh, uint32_t it)
     return TT_OK;
 }
 
+int oracle_world_iteration(void *hh, uint32_t *it)
+{
+    if (!hh || !it) return TT_ERR_INVALID;
+    *it = ((ow_handle *)hh)->w.iter;
+    return TT_OK;
+}
+
 int oracle_world_stats(void *hh, tt_step_stats *out)
 {
     if (!hh || !out) return TT_ERR_INVALID;
@@ -1720,6 +1735,40 @@ int oracle_world_ccl(void *hh)
 }
 
 void oracle_world_destroy(void *hh) { delete (ow_handle *)hh; }
+
+int oracle_hash_eval(const uint32_t *in, uint32_t *out, long long n)
+{
+    for (long long i = 0; i < n; ++i) out[i] = tt_hash(in[i]);
+    return TT_OK;
+}
+
+// jump flood of stage C3 on an explicit seed map (seed[c] = c or -1), for tests
+int oracle_jfa(int W, int H, const int32_t *seed, int32_t *dist_out, uint32_t *orig_out)
+{
+    if (W <= 0 || H <= 0 || !seed || !dist_out || !orig_out) return TT_ERR_INVALID;
+    tt_sim_config c;
+    memset(&c, 0, sizeof(c));
+    c.width = W;
+    c.height = H;
+    tt_kernel_settings k;
+    memset(&k, 0, sizeof(k));
+    World w;
+    w.cfg = c;
+    w.ks = k;
+    w.W = W;
+    w.H = H;
+    w.N = W * H;
+    w.prop.assign((size_t)w.N, PropData{0.0f, 0u, 0, {0, 0, 0}});
+    for (int i = 0; i < w.N; ++i)
+        if (seed[i] >= 0) w.prop[i].inv = 0x0201u;  // two plates involved: a seed
+    w.dist.assign((size_t)w.N, DistBuf{-1, 0u});
+    std::vector<float> hW((size_t)w.N, 0.0f);
+    std::vector<uint8_t> idW((size_t)w.N, 0);
+    w.pd.assign(MAXP + 1, default_plate());
+    stage_jfa_uplift(w, hW, idW);
+    for (int i = 0; i < w.N; ++i) { dist_out[i] = w.dist[i].dist; orig_out[i] = w.dist[i].orig; }
+    return TT_OK;
+}
 
 int oracle_math_eval(int fn, const float *in, float *out, long long n)
 {
