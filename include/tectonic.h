@@ -96,7 +96,7 @@ typedef struct tt_kernel_settings {
 
 /* How the initial state is produced (untimed). */
 enum {
-    /* BASELINE-style synthetic world: heights U[0,1) smoothed by a toroidal
+    /* synthetic benchmark world: heights U[0,1) smoothed by a toroidal
      * Gaussian (sigma 8, radius 32, fp64), plates = nearest of num_plates
      * uniform sites, speeds U[speed_min, speed_max]. */
     TT_INIT_SYNTHETIC = 0,
@@ -271,10 +271,14 @@ typedef struct tt_stage_times {
     float ms_collide;
     float ms_uplift;
     float ms_thermal;
-    float ms_hydraulic;
-    float ms_post;     /* kinematics, accretion, CCL, merge, stats */
+    float ms_hydraulic;    /* the fused hydraulic sub-iteration kernels only */
+    float ms_transport;    /* the closing sediment transport of each step */
+    float ms_post;         /* kinematics, accretion, CCL, merge, stats */
     float ms_split;
-    float reserved[9];
+    float hydraulic_launches;
+    float thermal_launches;
+    float steps;
+    float reserved[5];
 } tt_stage_times;
 int tt_world_set_timing(tt_world *w, int32_t enabled);
 int tt_world_stage_times(tt_world *w, tt_stage_times *out);

@@ -217,9 +217,13 @@ class StageTimes(C.Structure):
         ("ms_uplift", C.c_float),
         ("ms_thermal", C.c_float),
         ("ms_hydraulic", C.c_float),
+        ("ms_transport", C.c_float),
         ("ms_post", C.c_float),
         ("ms_split", C.c_float),
-        ("reserved", C.c_float * 9),
+        ("hydraulic_launches", C.c_float),
+        ("thermal_launches", C.c_float),
+        ("steps", C.c_float),
+        ("reserved", C.c_float * 5),
     ]
 
 

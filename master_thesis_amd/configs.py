@@ -1,7 +1,7 @@
-"""Named workloads: the reference defaults and the BASELINE configurations.
+"""Named workloads: the reference defaults and the benchmark worlds.
 
 Each preset returns (SimConfig, KernelSettings).  Parameter mapping from the
-BASELINE knobs to reference parameters (kernel_settings.cuh:6-71,
+benchmark knobs to reference parameters (kernel_settings.cuh:6-71,
 generation_settings.h:67-76):
 
 * rain -> hydrationRainfall, evaporation -> hydrationEvaporation,
@@ -49,19 +49,19 @@ def synthetic(width: int, height: int, plates: int, speed: tuple, seed: int = 1,
     return cfg, _erosion(KernelSettings.defaults(), uplift_range)
 
 
-def config_a(seed: int = 1):
+def world_512(seed: int = 1):
     """512x512, 8 plates, speed U[0.5, 2], K=50, T=20, blur radius 3, CCL every 10 (100 steps)."""
     return synthetic(512, 512, 8, (0.5, 2.0), seed, ccl_every=10, uplift_range=3)
 
 
-def config_b(seed: int = 1):
+def world_4096(seed: int = 1):
     """4096x4096, 32 plates, speed U[0.5, 3], K=50, T=20, blur radius 5, CCL every step (500 steps)."""
     return synthetic(4096, 4096, 32, (0.5, 3.0), seed, ccl_every=1, uplift_range=5)
 
 
-def config_d(seed: int = 1):
-    """8192x8192, 64 plates (precision study), erosion as config B (300 steps)."""
+def world_8192(seed: int = 1):
+    """8192x8192, 64 plates (precision study), erosion as world_4096 (300 steps)."""
     return synthetic(8192, 8192, 64, (0.5, 3.0), seed, ccl_every=1, uplift_range=5)
 
 
-STEPS = {"A": 100, "B": 500, "D": 300}
+STEPS = {"world_512": 100, "world_4096": 500, "world_8192": 300}

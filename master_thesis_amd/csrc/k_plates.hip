@@ -822,9 +822,11 @@ __global__ void k_backtrack(tt_dev d, const float *eff, uint8_t *ids, int px, in
     auto at = [&](int x, int y) { return (size_t)tt_wrapg(y, H) * W + tt_wrapg(x, W); };
     const uint8_t oldAtPoint = ids[at(px, py)];
     int cx = px, cy = py;
-    long long guard = 0;
     bool dead = false;
+    // the first walk is memoryless: reaching a cell it already marked means an
+    // endless cycle (zero-height plateau); canonical outcome: dead end
     while (eff[at(cx, cy)] > 0.0f) {
+        if (ids[at(cx, cy)] == TT_MAXP) { dead = true; break; }
         ids[at(cx, cy)] = (uint8_t)TT_MAXP;
         float best = INF;
         int nx = cx, ny = cy;
@@ -832,7 +834,7 @@ __global__ void k_backtrack(tt_dev d, const float *eff, uint8_t *ids, int px, in
             const float v = eff[at(cx + OFF[i][0], cy + OFF[i][1])];
             if (v < best) { best = v; nx = cx + OFF[i][0]; ny = cy + OFF[i][1]; }
         }
-        if (best == INF || ++guard > (long long)d.N) { dead = true; break; }
+        if (best == INF) { dead = true; break; }
         cx = nx;
         cy = ny;
     }

@@ -379,10 +379,13 @@ static int step_once(tt_world *w)
             w->wc = 1 - w->wc;
             w->fc = 1 - w->fc;
         }
+        CKS(record(w, 4));
         CK(tt_launch_transport(s, d, w->sed[buf], w->vel, w->sed[1 - buf]));
         w->sc = 1 - buf;
+    } else {
+        CKS(record(w, 4));
     }
-    CKS(record(w, 4));
+    CKS(record(w, 5));
     // copy-back (plate_tectonic_sim.cu:77-78) is a change of buffer index
     w->hc = (hc == w->h[0]) ? 0 : 1;
     w->ic = 1 - w->ic;
@@ -401,10 +404,10 @@ static int step_once(tt_world *w)
         w->ic = 1 - w->ic;
         CK(tt_launch_finalize(s, d, w->pd, w->acc, (mask & TT_STAGE_SPLIT) ? 0 : 2, w->ctl));
     }
-    CKS(record(w, 5));
+    CKS(record(w, 6));
     CKS(sync_ctl(w));
     if ((mask & TT_STAGE_MERGE) && (mask & TT_STAGE_SPLIT) && w->hctl->split_plate >= 0) CKS(run_split(w));
-    CKS(record(w, 6));
+    CKS(record(w, 7));
     CKS(sync_ctl(w));
     const tt_step_ctl &c = *w->hctl;
     w->stats.overflow_cells = (mask & TT_STAGE_COLLIDE) ? c.overflow_cells : 0;
@@ -421,17 +424,21 @@ static int step_once(tt_world *w)
     w->iter++;
     w->stats.iteration = w->iter;
     if (w->timing) {
-        float t[7] = {0};
-        for (int i = 1; i <= 6; ++i) CK(hipEventElapsedTime(&t[i], w->ev[i - 1], w->ev[i]));
+        float t[8] = {0};
+        for (int i = 1; i <= 7; ++i) CK(hipEventElapsedTime(&t[i], w->ev[i - 1], w->ev[i]));
         w->times.ms_collide += t[1];
         w->times.ms_uplift += t[2];
         w->times.ms_thermal += t[3];
         w->times.ms_hydraulic += t[4];
-        w->times.ms_post += t[5];
-        w->times.ms_split += t[6];
+        w->times.ms_transport += t[5];
+        w->times.ms_post += t[6];
+        w->times.ms_split += t[7];
         float tot = 0;
-        CK(hipEventElapsedTime(&tot, w->ev[0], w->ev[6]));
+        CK(hipEventElapsedTime(&tot, w->ev[0], w->ev[7]));
         w->times.ms_total += tot;
+        if (mask & TT_STAGE_HYDRAULIC) w->times.hydraulic_launches += (float)w->cfg.hydraulic_iterations;
+        if (mask & TT_STAGE_THERMAL) w->times.thermal_launches += (float)w->cfg.thermal_iterations;
+        w->times.steps += 1.0f;
     }
     return TT_OK;
 }

@@ -16,8 +16,8 @@ LIB = HERE / "liboracle.so"
 _lib = None
 
 
-def build() -> None:
-    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+def build(force: bool = False) -> None:
+    subprocess.run(["make", "-s", *(["-B"] if force else []), "-C", str(HERE)], check=True)
 
 
 def lib():

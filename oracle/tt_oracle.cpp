@@ -13,8 +13,6 @@
 // / sin / atan2 / fmod written from published methods) and tt_noise.h (the
 // cudaNoise hash and simplex formulas).  Both are pinned separately by tests
 // (host libm error bounds; an independent Python restatement of the noise).
-//
-// Parity status per function is listed in DESIGN.md.
 
 #include <algorithm>
 #include <cmath>
@@ -1431,8 +1429,12 @@ void stage_split(World &w)
     bool deadEnd = false;
     const uint8_t oldAtPoint = ID(ptx, pty);
     int cx = ptx, cy = pty;
-    long long guard = 0;
+    // The first walk is memoryless (its next cell depends only on the effort
+    // field), so arriving at a cell it already marked means it would cycle for
+    // ever (the reference kernel never returns; a zero-height plateau does
+    // this).  Canonical outcome: dead end.
     while (E(cx, cy) > 0.0f) {
+        if (ID(cx, cy) == MAXP) { deadEnd = true; break; }
         ID(cx, cy) = (uint8_t)MAXP;
         float best = INF;
         int nx = cx, ny = cy;
@@ -1440,7 +1442,7 @@ void stage_split(World &w)
             const float v = E(cx + TOFF[i][0], cy + TOFF[i][1]);
             if (v < best) { best = v; nx = cx + TOFF[i][0]; ny = cy + TOFF[i][1]; }
         }
-        if (best == INF || ++guard > (long long)N) { deadEnd = true; break; }
+        if (best == INF) { deadEnd = true; break; }
         cx = nx; cy = ny;
     }
     float px = 0.0f, py = 0.0f;

@@ -1,0 +1,278 @@
+"""Parity of the gfx950 step path with the oracle, through the C ABI.
+
+Every case builds the same world on the GPU and in the oracle, steps both and
+compares every state field and every per-step output byte for byte (integer,
+index and float fields alike: the device arithmetic is the host arithmetic,
+IEEE basic operations only).  Run on an MI355X:
+
+    python -u -m pytest tests/test_parity_gpu.py -m gpu -x -v --timeout 120 --timeout-method thread
+"""
+from __future__ import annotations
+
+import sys
+import zlib
+
+import numpy as np
+import pytest
+
+from master_thesis_amd import _abi, configs
+from tests.helpers import STATE_FIELDS, STEP_OUTPUTS, diff_fields, get, put, transfer_state
+
+pytestmark = pytest.mark.gpu
+ALL_FIELDS = tuple(STATE_FIELDS) + tuple(STEP_OUTPUTS)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def sim(tectonic_lib):
+    from master_thesis_amd import sim as s
+    return s
+
+
+@pytest.fixture(scope="module")
+def O(oracle_lib):
+    return oracle_lib
+
+
+def pair(sim, O, cfg, ks):
+    g = sim.World(cfg, ks).init()
+    o = O.OracleWorld(cfg, ks)
+    o.init()
+    return g, o
+
+
+def stats_tuple(s):
+    return (s.iteration, s.largest_value, s.largest_plate_id, s.heaviest_plate_id, s.number_of_plates,
+            s.overflow_cells, s.num_labels, s.unassigned_cells, s.merges, s.split_plate, s.split_new_plate,
+            s.complex_sources)
+
+
+def run_parity(sim, O, cfg, ks, steps, tag, fields=ALL_FIELDS):
+    say(f"[{tag}] init {cfg.width}x{cfg.height} plates={cfg.num_plates} seed={cfg.seed} mode={cfg.init_mode}")
+    g, o = pair(sim, O, cfg, ks)
+    try:
+        bad = diff_fields(g, o, STATE_FIELDS)
+        assert not bad, f"{tag} init: {bad}"
+        seen = []
+        for s in range(steps):
+            say(f"[{tag}] step {s}")
+            g.step(1)
+            o.step(1)
+            bad = diff_fields(g, o, fields)
+            assert not bad, f"{tag} step {s}: {bad}"
+            gs, os_ = g.stats(), o.stats()
+            assert stats_tuple(gs) == stats_tuple(os_), f"{tag} step {s} stats"
+            assert gs.mass_ledger == os_.mass_ledger
+            seen.append(gs)
+        return seen
+    finally:
+        g.close()
+        o.close()
+
+
+# --------------------------------------------------------------------------
+def test_device_math_matches_host(sim, O):
+    rng = np.random.default_rng(0)
+    special = np.array([0.0, -0.0, 1.0, -1.0, 0.5, 1e-30, -1e-30, 1e-40, 88.7, -87.3, 89.0, -104.0,
+                        3.14159265, 1.5707964, 1e4, -1e4, 12345.678, 65504.0], np.float32)
+    cases = [
+        (_abi.TT_MATH_EXPF, np.concatenate([rng.uniform(-100, 89, 100000), special])),
+        (_abi.TT_MATH_LOG10F, np.concatenate([np.exp(rng.uniform(-80, 80, 100000)), np.abs(special[special != 0])])),
+        (_abi.TT_MATH_SINF, np.concatenate([rng.uniform(-2e4, 2e4, 100000), special])),
+        (_abi.TT_MATH_COSF, np.concatenate([rng.uniform(-2e4, 2e4, 100000), special])),
+        (_abi.TT_MATH_SIMPLEX, rng.uniform(0, 5000, 300000)),
+        (_abi.TT_MATH_DISCRETE, rng.uniform(0, 5000, 300000)),
+        (_abi.TT_MATH_FMODF, rng.uniform(-1e4, 1e4, 200000)),
+        (_abi.TT_MATH_ATAN2, np.concatenate([rng.uniform(-10, 10, 200000), [0, 0, 0, -0.0, 1, 0, -1, 0, 0, -1]])),
+    ]
+    for fn, x in cases:
+        x = np.asarray(x, np.float32)
+        say(f"[math] fn {fn} n={x.size}")
+        a = sim.math_eval_device(fn, x)
+        b = O.math_eval(fn, x)
+        assert a.tobytes() == b.tobytes(), f"function {fn}: {int(np.sum(a.view(np.uint32) != b.view(np.uint32)))}"
+
+
+@pytest.mark.parametrize("W,H,P,seed", [(128, 128, 8, 3), (320, 48, 5, 11), (64, 16, 2, 1)])
+def test_synthetic_worlds(sim, O, W, H, P, seed):
+    cfg, ks = configs.synthetic(W, H, P, (0.5, 3.0), seed, hydraulic=10, thermal=5, ccl_every=1)
+    run_parity(sim, O, cfg, ks, 5, f"synthetic {W}x{H}")
+
+
+@pytest.mark.parametrize("W,H,P,seed,gens", [(256, 256, 12, 7, (4, 8)), (192, 128, 20, 5, ())])
+def test_reference_init_worlds(sim, O, W, H, P, seed, gens):
+    """Simplex fBm heights and Voronoi generators as in the reference's
+    initialize(); continental collisions, accretion and complex sources."""
+    cfg, ks = configs.reference_defaults()
+    cfg.width, cfg.height, cfg.num_plates, cfg.seed, cfg.iterations_for_ccl = W, H, P, seed, 2
+    cfg.num_voronoi_gens = len(gens)
+    for i, v in enumerate(gens):
+        cfg.voronoi_gens[i] = v
+    seen = run_parity(sim, O, cfg, ks, 6, f"reference {W}x{H}")
+    assert sum(s.complex_sources for s in seen) >= 0
+
+
+def test_splits(sim, O):
+    cfg, ks = configs.reference_defaults()
+    cfg.width, cfg.height, cfg.num_plates, cfg.seed = 128, 128, 4, 11
+    cfg.num_voronoi_gens = 0
+    ks.targetMaximumPlateArea = 600
+    seen = run_parity(sim, O, cfg, ks, 8, "splits")
+    assert any(s.split_plate >= 0 for s in seen)
+
+
+def test_merges(sim, O):
+    cfg, ks = configs.synthetic(128, 64, 10, (0.2, 0.6), 4, hydraulic=3, thermal=2, ccl_every=1)
+    ks.mergeMinVelocity = 1.0
+    ks.mergeWaitAfterSplit = 0
+    seen = run_parity(sim, O, cfg, ks, 4, "merges")
+    assert sum(s.merges for s in seen) > 0
+
+
+def test_many_plates(sim, O):
+    """250 plates on a small grid: more CCL fragments than free ids, so
+    relabelling hands out every free id and leaves the rest on their old id."""
+    cfg, ks = configs.synthetic(256, 128, 250, (0.5, 3.0), 9, hydraulic=4, thermal=2, ccl_every=1)
+    ks.targetMaximumPlateArea = 50
+    run_parity(sim, O, cfg, ks, 3, "many plates")
+
+
+@pytest.mark.parametrize("mask", [
+    _abi.TT_STAGE_COLLIDE, _abi.TT_STAGE_COLLIDE | _abi.TT_STAGE_UPLIFT, _abi.TT_STAGE_THERMAL,
+    _abi.TT_STAGE_HYDRAULIC, _abi.TT_STAGE_KINEMATICS | _abi.TT_STAGE_CCL,
+    _abi.TT_STAGE_CCL | _abi.TT_STAGE_MERGE | _abi.TT_STAGE_SPLIT, 0,
+])
+def test_stage_masks(sim, O, mask):
+    cfg, ks = configs.synthetic(128, 64, 6, (0.5, 3.0), 2, hydraulic=6, thermal=4, ccl_every=1)
+    cfg.stage_mask = mask
+    run_parity(sim, O, cfg, ks, 3, f"mask {mask:#x}")
+
+
+def test_zero_sub_iterations_and_ccl_period(sim, O):
+    cfg, ks = configs.synthetic(128, 128, 7, (0.5, 3.0), 5, hydraulic=0, thermal=0, ccl_every=3)
+    run_parity(sim, O, cfg, ks, 4, "K=0 T=0")
+
+
+def test_empty_world(sim, O):
+    cfg, ks = configs.synthetic(64, 32, 1, (0.5, 1.0), 1, hydraulic=2, thermal=2)
+    cfg.init_mode = _abi.TT_INIT_EMPTY
+    run_parity(sim, O, cfg, ks, 2, "empty")
+
+
+def test_resume_from_transferred_state(sim, O):
+    """A GPU world's state handed to the oracle mid-run continues identically
+    (snapshot/resume through the field upload/download interface)."""
+    cfg, ks = configs.reference_defaults()
+    cfg.width, cfg.height, cfg.num_plates, cfg.seed, cfg.iterations_for_ccl = 128, 128, 10, 21, 1
+    say("[resume] gpu 3 steps")
+    g = sim.World(cfg, ks).init()
+    g.step(3)
+    o = O.OracleWorld(cfg, ks)
+    o.init()
+    transfer_state(g, o)
+    g2 = sim.World(cfg, ks).init()
+    transfer_state(g, g2)
+    try:
+        for s in range(3):
+            say(f"[resume] step {s}")
+            g.step(1)
+            o.step(1)
+            g2.step(1)
+            assert not diff_fields(g, o, ALL_FIELDS)
+            assert not diff_fields(g, g2, STATE_FIELDS)
+    finally:
+        g.close()
+        g2.close()
+        o.close()
+
+
+def test_gpu_runs_are_deterministic(sim):
+    cfg, ks = configs.reference_defaults()
+    cfg.width, cfg.height, cfg.num_plates, cfg.seed = 256, 128, 16, 8
+    out = []
+    for r in range(2):
+        say(f"[determinism] run {r}")
+        with sim.World(cfg, ks).init() as g:
+            g.step(4)
+            out.append([g.download(f).tobytes() for f in ALL_FIELDS])
+    assert out[0] == out[1]
+
+
+@pytest.mark.timeout(600)
+def test_full_size_step_matches_oracle(sim, O):
+    """Two steps of the 4096x4096, 32-plate world (50 hydraulic, 20 thermal
+    sub-iterations, CCL every step) against the oracle, plus invariants that do
+    not depend on the size."""
+    cfg, ks = configs.world_4096(seed=1)
+    say("[4096] init")
+    g, o = pair(sim, O, cfg, ks)
+    try:
+        assert not diff_fields(g, o, STATE_FIELDS)
+        for s in range(2):
+            say(f"[4096] step {s}")
+            g.step(1)
+            o.step(1)
+            assert not diff_fields(g, o, ALL_FIELDS), f"step {s}"
+        ids = g.download(_abi.FIELD_PLATE_IDS)
+        pd = g.download(_abi.FIELD_PLATE_DATA)
+        assert np.array_equal(pd["size"], np.bincount(ids, minlength=256)[:255])
+        keys = g.download(_abi.FIELD_SORTED_KEYS)
+        assert (np.diff(keys.astype(np.int64)) >= 0).all()
+        assert np.isfinite(g.download(_abi.FIELD_HEIGHT)).all()
+        assert (g.download(_abi.FIELD_WATER) >= 0).all()
+    finally:
+        g.close()
+        o.close()
+
+
+def _png_rows(path):
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, ihdr = 8, b"", None
+    while pos < len(data):
+        n = int.from_bytes(data[pos:pos + 4], "big")
+        typ = data[pos + 4:pos + 8]
+        body = data[pos + 8:pos + 8 + n]
+        assert zlib.crc32(typ + body) == int.from_bytes(data[pos + 8 + n:pos + 12 + n], "big")
+        if typ == b"IHDR":
+            ihdr = body
+        elif typ == b"IDAT":
+            idat += body
+        pos += 12 + n
+    w, h = int.from_bytes(ihdr[0:4], "big"), int.from_bytes(ihdr[4:8], "big")
+    raw = zlib.decompress(idat)
+    return w, h, ihdr[8], ihdr[9], raw
+
+
+def test_png_export(sim, tmp_path):
+    cfg, ks = configs.synthetic(128, 64, 5, (0.5, 3.0), 3, hydraulic=2, thermal=2)
+    say("[png] export")
+    with sim.World(cfg, ks).init() as g:
+        g.step(1)
+        p16 = str(tmp_path / "h.png")
+        prgb = str(tmp_path / "p.png")
+        g.export_png(0, p16)
+        g.export_png(1, prgb)
+        h = g.download(_abi.FIELD_HEIGHT)
+        ids = g.download(_abi.FIELD_PLATE_IDS)
+    w, hh, depth, ctype, raw = _png_rows(p16)
+    assert (w, hh, depth, ctype) == (128, 64, 16, 0)
+    rows = np.frombuffer(raw, np.uint8).reshape(64, 1 + 2 * 128)
+    assert (rows[:, 0] == 0).all()
+    gray = rows[:, 1:].reshape(64, 128, 2).astype(np.uint16)
+    gray = (gray[..., 0] << 8) | gray[..., 1]
+    mn, mx = np.float32(ks.minUpliftClamp), np.float32(ks.maxUpliftClamp)
+    norm = np.clip((h.reshape(64, 128) - mn) / np.float32(mx - mn), 0, 1).astype(np.float32)
+    assert np.array_equal(gray, (norm * np.float32(65535.0)).astype(np.uint16))
+    w, hh, depth, ctype, raw = _png_rows(prgb)
+    assert (w, hh, depth, ctype) == (128, 64, 8, 2)
+    rgb = np.frombuffer(raw, np.uint8).reshape(64, 1 + 3 * 128)[:, 1:].reshape(64 * 128, 3)
+    # one colour per plate id, distinct for distinct ids
+    col = {}
+    for i, c in zip(ids, map(tuple, rgb)):
+        assert col.setdefault(int(i), c) == c
+    assert len(set(col.values())) == len(col)
