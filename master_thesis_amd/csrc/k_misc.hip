@@ -174,6 +174,31 @@ hipError_t tt_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, long long n)
     return hipGetLastError();
 }
 
+// Global mass ledger: sum over all cells of the height in 44.20 fixed point.
+// Integer partial sums make the total independent of the reduction order.
+__global__ __launch_bounds__(256) void k_ledger(const float *__restrict__ h, long long n, long long *out)
+{
+    __shared__ long long part[4];
+    long long acc = 0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        acc += tt_fix(h[i], TT_MASS_BITS);
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicAdd((unsigned long long *)out, (unsigned long long)(part[0] + part[1] + part[2] + part[3]));
+}
+
+hipError_t tt_launch_ledger(hipStream_t s, const float *h, long long n, long long *out)
+{
+    hipError_t e = tt_fill_u32(s, (uint32_t *)out, 0u, 2);
+    if (e != hipSuccess) return e;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_ledger, dim3((unsigned)blocks), dim3(256), 0, s, h, n, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // initialisation
 // ---------------------------------------------------------------------------

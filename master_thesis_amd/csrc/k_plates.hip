@@ -647,7 +647,6 @@ __global__ __launch_bounds__(256) void k_finalize(tt_dev d, tt_plate_data *pd, c
 {
     const int p = threadIdx.x;
     const tt_kernel_settings &K = d.ks;
-    __shared__ long long ledger[256];
     if (p < TT_MAXP && mode == 3) {
         // initPixelDependantPlateData (:199-230): sizes and masses added
         tt_plate_data c = pd[p];
@@ -657,7 +656,6 @@ __global__ __launch_bounds__(256) void k_finalize(tt_dev d, tt_plate_data *pd, c
             c.mass = c.mass + tt_unfix(a[ACC_MASS], TT_MASS_BITS);
         }
         pd[p] = c;
-        ledger[p] = a[ACC_MASS];
     } else if (p < TT_MAXP) {
         tt_plate_data c = pd[p];
         const long long *a = &acc[p * ACC_N];
@@ -704,15 +702,9 @@ __global__ __launch_bounds__(256) void k_finalize(tt_dev d, tt_plate_data *pd, c
             }
         }
         pd[p] = c;
-        ledger[p] = a[ACC_MASS];
-    } else {
-        ledger[p] = 0;
     }
     __syncthreads();
     if (p != 0) return;
-    long long led = 0;
-    for (int i = 0; i < 256; ++i) led += ledger[i];
-    ctl->ledger = led;
     if (mode == 1 || mode == 3) return;
     // statisticsPass
     float heavy = 0.0f;

@@ -12,6 +12,7 @@ hipError_t tt_scan_u32(hipStream_t s, const uint32_t *in, uint32_t *out, long lo
 hipError_t tt_compact_u8(hipStream_t s, const uint8_t *flags, long long n, uint32_t *out_idx, uint32_t *count,
                          uint32_t *tmp);
 hipError_t tt_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, long long n);
+hipError_t tt_launch_ledger(hipStream_t s, const float *h, long long n, long long *out);
 
 // ---- initialisation (k_misc.hip) ------------------------------------------
 hipError_t tt_launch_init_synthetic_heights(hipStream_t s, const tt_dev &d, uint64_t hseed, double *raw, double *tmp,

@@ -407,6 +407,7 @@ static int step_once(tt_world *w)
     CKS(record(w, 6));
     CKS(sync_ctl(w));
     if ((mask & TT_STAGE_MERGE) && (mask & TT_STAGE_SPLIT) && w->hctl->split_plate >= 0) CKS(run_split(w));
+    CK(tt_launch_ledger(s, w->h[w->hc], (long long)N, &w->ctl->ledger));
     CKS(record(w, 7));
     CKS(sync_ctl(w));
     const tt_step_ctl &c = *w->hctl;
@@ -419,8 +420,8 @@ static int step_once(tt_world *w)
         w->stats.heaviest_value = c.heaviest_value;
         w->stats.heaviest_plate_id = c.heaviest_id;
         w->stats.number_of_plates = c.n_plates;
-        w->stats.mass_ledger = tt_unfix_d(c.ledger, TT_MASS_BITS);
     }
+    w->stats.mass_ledger = tt_unfix_d(c.ledger, TT_MASS_BITS);
     w->iter++;
     w->stats.iteration = w->iter;
     if (w->timing) {

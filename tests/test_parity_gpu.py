@@ -97,7 +97,7 @@ def test_device_math_matches_host(sim, O):
         assert a.tobytes() == b.tobytes(), f"function {fn}: {int(np.sum(a.view(np.uint32) != b.view(np.uint32)))}"
 
 
-@pytest.mark.parametrize("W,H,P,seed", [(128, 128, 8, 3), (320, 48, 5, 11), (64, 16, 2, 1)])
+@pytest.mark.parametrize("W,H,P,seed", [(128, 128, 8, 3), (320, 64, 5, 11), (64, 64, 2, 1)])
 def test_synthetic_worlds(sim, O, W, H, P, seed):
     cfg, ks = configs.synthetic(W, H, P, (0.5, 3.0), seed, hydraulic=10, thermal=5, ccl_every=1)
     run_parity(sim, O, cfg, ks, 5, f"synthetic {W}x{H}")
@@ -158,7 +158,7 @@ def test_zero_sub_iterations_and_ccl_period(sim, O):
 
 
 def test_empty_world(sim, O):
-    cfg, ks = configs.synthetic(64, 32, 1, (0.5, 1.0), 1, hydraulic=2, thermal=2)
+    cfg, ks = configs.synthetic(64, 64, 1, (0.5, 1.0), 1, hydraulic=2, thermal=2)
     cfg.init_mode = _abi.TT_INIT_EMPTY
     run_parity(sim, O, cfg, ks, 2, "empty")
 
