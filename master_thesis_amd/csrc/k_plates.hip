@@ -767,39 +767,176 @@ hipError_t tt_launch_split_prepare(hipStream_t s, const tt_dev &d, const uint8_t
     return hipGetLastError();
 }
 
-// propagateEffortToBoundary, relaxed in place (any order reaches the same
-// fixed point: the update is monotone and starts from an upper bound)
-__global__ __launch_bounds__(256) void k_relax(tt_dev d, const uint32_t *list, const uint32_t *n, const float *height,
-                                               float *eff, tt_step_ctl *ctl)
+// Tiles of the split stage: 64 columns by TH rows (TH = 64 when the height
+// allows it, else 16); 256 threads, thread t owns column t % 64 and rows
+// t / 64 + 4 k.  k_split_tiles flags the tiles that hold cells of the plate.
+template <int TH>
+__global__ __launch_bounds__(256) void k_split_tiles(tt_dev d, const uint8_t *ids, int plate, uint8_t *flag)
 {
-    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    int changed = 0;
-    if (k < *n) {
-        const uint32_t c = list[k];
-        const int x = (int)(c % (uint32_t)d.W), y = (int)(c / (uint32_t)d.W);
-        const float old = eff[c];
-        float md = old;
-        const float cost = height[c];
-        const float SQRT2 = 1.41421356237f;
-        const int OFF[8][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}, {-1, -1}, {-1, 1}, {1, -1}, {1, 1}};
+    const int tpr = d.W / 64;
+    const int tx0 = (int)(blockIdx.x % (unsigned)tpr) * 64, ty0 = (int)(blockIdx.x / (unsigned)tpr) * TH;
+    const int lx = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int any = 0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const float mul = i >= 4 ? SQRT2 : 1.0f;
-            const float t = eff[(size_t)tt_wrap1(y + OFF[i][1], d.H) * d.W + tt_wrap1(x + OFF[i][0], d.W)] + cost * mul;
-            if (t < md) md = t;
-        }
-        if (md != old) { eff[c] = md; changed = 1; }
-    }
-    const unsigned long long b = __ballot(changed);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&ctl->changed, __popcll(b));
+    for (int j = 0; j < TH / 4; ++j) any |= ids[(size_t)(ty0 + wv + 4 * j) * d.W + tx0 + lx] == plate;
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) flag[blockIdx.x] = any ? 1 : 0;
 }
 
-hipError_t tt_launch_relax(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n, const float *height,
-                           float *eff, tt_step_ctl *ctl, int sweeps)
+// propagateEffortToBoundary (plate_tectonics_kernel.cu:2838-2881) as rounds of
+// tile-local relaxation: each tile relaxes its cells in LDS until nothing
+// changes (halo values read at the start of the round), then writes back the
+// cells it lowered and counts itself in rchg[round].  A round in which no tile
+// changes anything is a global fixed point.  The update is monotone and starts
+// from an upper bound, so the fixed point does not depend on the order of the
+// updates: it equals the oracle's worklist result.
+template <int TH>
+__global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *tiles, const uint8_t *ids, int plate,
+                                                     const float *height, float *eff, uint32_t *rchg, int round)
 {
-    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
-    for (int i = 0; i < sweeps; ++i) {
-        hipLaunchKernelGGL(k_relax, dim3(blocks), dim3(256), 0, s, d, list, n, height, eff, ctl);
+    if (round > 0 && rchg[round - 1] == 0u) return;
+    constexpr int PR = TH / 4;
+    __shared__ float e[TH + 2][67];
+    const int tpr = d.W / 64;
+    const uint32_t tile = tiles[blockIdx.x];
+    const int tx0 = (int)(tile % (uint32_t)tpr) * 64, ty0 = (int)(tile / (uint32_t)tpr) * TH;
+    const int W = d.W, H = d.H;
+    for (int i = threadIdx.x; i < (TH + 2) * 66; i += 256) {
+        const int r = i / 66, q = i - r * 66;
+        e[r][q] = eff[(size_t)tt_wrap1(ty0 + r - 1, H) * W + tt_wrap1(tx0 + q - 1, W)];
+    }
+    const int lx = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float c1[PR], c2[PR], init[PR];
+    bool in[PR];
+    const float SQRT2 = 1.41421356237f;
+#pragma unroll
+    for (int j = 0; j < PR; ++j) {
+        const size_t c = (size_t)(ty0 + wv + 4 * j) * W + tx0 + lx;
+        in[j] = ids[c] == plate;
+        c1[j] = height[c] * 1.0f;
+        c2[j] = height[c] * SQRT2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PR; ++j) init[j] = e[wv + 4 * j + 1][lx + 1];
+    for (int it = 0; it < 4096; ++it) {
+        int ch = 0;
+#pragma unroll
+        for (int j = 0; j < PR; ++j) {
+            if (!in[j]) continue;
+            const int r = wv + 4 * j + 1, q = lx + 1;
+            const float old = e[r][q];
+            float md = old;
+            // neighbour order (-1,0),(1,0),(0,-1),(0,1),(-1,-1),(-1,1),(1,-1),(1,1)
+            float t;
+            t = e[r][q - 1] + c1[j]; if (t < md) md = t;
+            t = e[r][q + 1] + c1[j]; if (t < md) md = t;
+            t = e[r - 1][q] + c1[j]; if (t < md) md = t;
+            t = e[r + 1][q] + c1[j]; if (t < md) md = t;
+            t = e[r - 1][q - 1] + c2[j]; if (t < md) md = t;
+            t = e[r + 1][q - 1] + c2[j]; if (t < md) md = t;
+            t = e[r - 1][q + 1] + c2[j]; if (t < md) md = t;
+            t = e[r + 1][q + 1] + c2[j]; if (t < md) md = t;
+            if (md != old) { e[r][q] = md; ch = 1; }
+        }
+        if (!__syncthreads_or(ch)) break;
+    }
+    int wrote = 0;
+#pragma unroll
+    for (int j = 0; j < PR; ++j) {
+        const float v = e[wv + 4 * j + 1][lx + 1];
+        if (in[j] && v != init[j]) {
+            eff[(size_t)(ty0 + wv + 4 * j) * W + tx0 + lx] = v;
+            wrote = 1;
+        }
+    }
+    wrote = __syncthreads_or(wrote);
+    if (threadIdx.x == 0 && wrote) atomicAdd(&rchg[round], 1u);
+}
+
+// floodFillPlate (plate_tectonics_kernel.cu:2985-3024) in the same round
+// scheme: the new id spreads through 4-connected cells of the old plate.
+template <int TH>
+__global__ __launch_bounds__(256) void k_flood_tiles(tt_dev d, const uint32_t *tiles, uint8_t *ids, int oldId,
+                                                     int newId, uint32_t *rchg, int round)
+{
+    if (round > 0 && rchg[round - 1] == 0u) return;
+    constexpr int PR = TH / 4;
+    __shared__ uint8_t m[TH + 2][68];
+    const int tpr = d.W / 64;
+    const uint32_t tile = tiles[blockIdx.x];
+    const int tx0 = (int)(tile % (uint32_t)tpr) * 64, ty0 = (int)(tile / (uint32_t)tpr) * TH;
+    const int W = d.W, H = d.H;
+    for (int i = threadIdx.x; i < (TH + 2) * 66; i += 256) {
+        const int r = i / 66, q = i - r * 66;
+        m[r][q] = ids[(size_t)tt_wrap1(ty0 + r - 1, H) * W + tt_wrap1(tx0 + q - 1, W)];
+    }
+    __syncthreads();
+    const int lx = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    bool cand[PR];
+#pragma unroll
+    for (int j = 0; j < PR; ++j) cand[j] = m[wv + 4 * j + 1][lx + 1] == oldId;
+    for (int it = 0; it < 4096; ++it) {
+        int ch = 0;
+#pragma unroll
+        for (int j = 0; j < PR; ++j) {
+            const int r = wv + 4 * j + 1, q = lx + 1;
+            if (!cand[j] || m[r][q] != oldId) continue;
+            if (m[r][q - 1] == newId || m[r][q + 1] == newId || m[r - 1][q] == newId || m[r + 1][q] == newId) {
+                m[r][q] = (uint8_t)newId;
+                ch = 1;
+            }
+        }
+        if (!__syncthreads_or(ch)) break;
+    }
+    int wrote = 0;
+#pragma unroll
+    for (int j = 0; j < PR; ++j) {
+        if (cand[j] && m[wv + 4 * j + 1][lx + 1] == newId) {
+            ids[(size_t)(ty0 + wv + 4 * j) * W + tx0 + lx] = (uint8_t)newId;
+            wrote = 1;
+        }
+    }
+    wrote = __syncthreads_or(wrote);
+    if (threadIdx.x == 0 && wrote) atomicAdd(&rchg[round], 1u);
+}
+
+int tt_split_tile_h(const tt_dev &d) { return (d.H % 64) == 0 ? 64 : 16; }
+
+hipError_t tt_launch_split_tiles(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, uint8_t *flag)
+{
+    const int th = tt_split_tile_h(d);
+    const unsigned nt = (unsigned)((d.W / 64) * (d.H / th));
+    if (th == 64) hipLaunchKernelGGL(k_split_tiles<64>, dim3(nt), dim3(256), 0, s, d, ids, plate, flag);
+    else hipLaunchKernelGGL(k_split_tiles<16>, dim3(nt), dim3(256), 0, s, d, ids, plate, flag);
+    return hipGetLastError();
+}
+
+hipError_t tt_launch_relax_tiles(hipStream_t s, const tt_dev &d, const uint32_t *tiles, unsigned ntiles,
+                                 const uint8_t *ids, int plate, const float *height, float *eff, uint32_t *rchg,
+                                 int round0, int rounds)
+{
+    if (ntiles == 0) return hipSuccess;
+    for (int r = round0; r < round0 + rounds; ++r) {
+        if (tt_split_tile_h(d) == 64)
+            hipLaunchKernelGGL(k_relax_tiles<64>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, plate, height, eff,
+                               rchg, r);
+        else
+            hipLaunchKernelGGL(k_relax_tiles<16>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, plate, height, eff,
+                               rchg, r);
+    }
+    return hipGetLastError();
+}
+
+hipError_t tt_launch_flood_tiles(hipStream_t s, const tt_dev &d, const uint32_t *tiles, unsigned ntiles, uint8_t *ids,
+                                 int oldId, int newId, uint32_t *rchg, int round0, int rounds)
+{
+    if (ntiles == 0) return hipSuccess;
+    for (int r = round0; r < round0 + rounds; ++r) {
+        if (tt_split_tile_h(d) == 64)
+            hipLaunchKernelGGL(k_flood_tiles<64>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, oldId, newId, rchg, r);
+        else
+            hipLaunchKernelGGL(k_flood_tiles<16>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, oldId, newId, rchg, r);
     }
     return hipGetLastError();
 }
@@ -881,37 +1018,6 @@ hipError_t tt_launch_backtrack(hipStream_t s, const tt_dev &d, const float *eff,
                                int newId, tt_step_ctl *ctl)
 {
     hipLaunchKernelGGL(k_backtrack, dim3(1), dim3(64), 0, s, d, eff, ids, px, py, newId, ctl);
-    return hipGetLastError();
-}
-
-// floodFillPlate, in place (monotone; the fixed point is the 4-connected
-// component of the seed through old-plate cells)
-__global__ __launch_bounds__(256) void k_flood(tt_dev d, const uint32_t *list, const uint32_t *n, uint8_t *ids,
-                                               int oldId, int newId, tt_step_ctl *ctl)
-{
-    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    int changed = 0;
-    if (k < *n) {
-        const uint32_t c = list[k];
-        if (ids[c] == oldId) {
-            const int x = (int)(c % (uint32_t)d.W), y = (int)(c / (uint32_t)d.W);
-            const bool nb = ids[(size_t)y * d.W + tt_wrap1(x - 1, d.W)] == newId ||
-                            ids[(size_t)y * d.W + tt_wrap1(x + 1, d.W)] == newId ||
-                            ids[(size_t)tt_wrap1(y - 1, d.H) * d.W + x] == newId ||
-                            ids[(size_t)tt_wrap1(y + 1, d.H) * d.W + x] == newId;
-            if (nb) { ids[c] = (uint8_t)newId; changed = 1; }
-        }
-    }
-    const unsigned long long b = __ballot(changed);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&ctl->changed, __popcll(b));
-}
-
-hipError_t tt_launch_flood(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n, uint8_t *ids,
-                           int oldId, int newId, tt_step_ctl *ctl, int sweeps)
-{
-    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
-    for (int i = 0; i < sweeps; ++i)
-        hipLaunchKernelGGL(k_flood, dim3(blocks), dim3(256), 0, s, d, list, n, ids, oldId, newId, ctl);
     return hipGetLastError();
 }
 

@@ -92,11 +92,14 @@ hipError_t tt_launch_finalize(hipStream_t s, const tt_dev &d, tt_plate_data *pd,
 // split
 hipError_t tt_launch_split_prepare(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, float *eff,
                                    uint8_t *flag);
-hipError_t tt_launch_relax(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n, const float *height,
-                           float *eff, tt_step_ctl *ctl, int sweeps);
+int tt_split_tile_h(const tt_dev &d);
+hipError_t tt_launch_split_tiles(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, uint8_t *flag);
+hipError_t tt_launch_relax_tiles(hipStream_t s, const tt_dev &d, const uint32_t *tiles, unsigned ntiles,
+                                 const uint8_t *ids, int plate, const float *height, float *eff, uint32_t *rchg,
+                                 int round0, int rounds);
 hipError_t tt_launch_backtrack(hipStream_t s, const tt_dev &d, const float *eff, uint8_t *ids, int px, int py,
                                int newId, tt_step_ctl *ctl);
-hipError_t tt_launch_flood(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n, uint8_t *ids,
-                           int oldId, int newId, tt_step_ctl *ctl, int sweeps);
+hipError_t tt_launch_flood_tiles(hipStream_t s, const tt_dev &d, const uint32_t *tiles, unsigned ntiles, uint8_t *ids,
+                                 int oldId, int newId, uint32_t *rchg, int round0, int rounds);
 hipError_t tt_launch_split_finalize(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n,
                                     uint8_t *ids, int oldId, int newId, tt_plate_data *pd);

@@ -7,6 +7,9 @@
 
 #include "tt_device.h"
 
+// rounds of a tiled fixed-point kernel enqueued per host check
+#define TT_ROUND_BATCH 32
+
 struct tt_alloc {
     void **ptr;
     size_t bytes;
@@ -71,6 +74,7 @@ struct tt_world {
     int *mergeIds = nullptr;
     long long *acc = nullptr;
     float *eff = nullptr;
+    uint32_t *rchg = nullptr;   // per-round change counters of the split fixed points
 
     // scratch
     uint32_t *scanTmp = nullptr;

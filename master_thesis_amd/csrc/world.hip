@@ -143,6 +143,7 @@ size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w)
     take((void **)&w->mergeIds, 256 * 4);
     take((void **)&w->acc, 256 * 10 * 8);
     take((void **)&w->eff, N * 4);
+    take((void **)&w->rchg, TT_ROUND_BATCH * 4);
     take((void **)&w->scanTmp, (N / 1024 + tt_scan_tmp_words((long long)N) + 4096) * 4);
     take((void **)&w->initSeeds, 4096 * 12);
     take((void **)&w->initWts, 128 * 8);
@@ -241,6 +242,28 @@ static int run_ccl(tt_world *w)
 // ---------------------------------------------------------------------------
 // split (SplitPlateV2, plate_tectonic_sim.cu:933-1040)
 // ---------------------------------------------------------------------------
+// Runs rounds of a tiled fixed-point kernel in batches, with one host read per
+// batch: a round that changed nothing ends the loop (later rounds of the batch
+// exit at once).  Returns the number of rounds that did work.
+template <typename Launch>
+static int run_rounds(tt_world *w, Launch launch, int *rounds_out)
+{
+    const int B = TT_ROUND_BATCH;
+    int total = 0;
+    for (int batch = 0; batch < 100000; ++batch) {
+        CKS(zero(w, w->rchg, B * 4));
+        CK(launch(B));
+        CK(hipMemcpyAsync(w->hu32, w->rchg, B * 4, hipMemcpyDeviceToHost, w->stream));
+        CK(hipStreamSynchronize(w->stream));
+        for (int r = 0; r < B; ++r) {
+            ++total;
+            if (w->hu32[r] == 0u) { *rounds_out = total; return TT_OK; }
+        }
+    }
+    set_error("split fixed point did not converge");
+    return TT_ERR_STATE;
+}
+
 static int run_split(tt_world *w)
 {
     hipStream_t s = w->stream;
@@ -254,18 +277,20 @@ static int run_split(tt_world *w)
     CK(hipMemcpyAsync(&hp, w->pd + plate, sizeof(hp), hipMemcpyDeviceToHost, s));
     CK(tt_launch_split_prepare(s, d, ids, plate, w->eff, w->unFlag));
     CK(tt_compact_u8(s, w->unFlag, (long long)N, w->list[0], w->listN[0], w->scanTmp));
-    CK(hipStreamSynchronize(s));
-    // relaxation to the fixed point
-    int sweeps = 0;
-    for (;;) {
-        CKS(zero(w, &w->ctl->changed, 4));
-        CK(tt_launch_relax(s, d, w->list[0], w->listN[0], w->h[w->hc], w->eff, w->ctl, 8));
-        sweeps += 8;
-        CKS(sync_ctl(w));
-        if (w->hctl->changed == 0) break;
-        if (sweeps > 200000) { set_error("split relaxation did not converge"); return TT_ERR_STATE; }
-    }
-    w->stats.split_iterations = sweeps;
+    // tiles holding cells of the plate
+    const int th = tt_split_tile_h(d);
+    const long long ntiles_all = (long long)(w->W / 64) * (w->H / th);
+    CK(tt_launch_split_tiles(s, d, ids, plate, w->vote));
+    CK(tt_compact_u8(s, w->vote, ntiles_all, w->list[1], w->listN[1], w->scanTmp));
+    uint32_t ntiles = 0;
+    CKS(read_u32(w, w->listN[1], &ntiles));
+    const uint32_t *tiles = w->list[1];
+    const float *height = w->h[w->hc];
+    int rounds = 0;
+    CKS(run_rounds(w, [&](int B) {
+        return tt_launch_relax_tiles(s, d, tiles, ntiles, ids, plate, height, w->eff, w->rchg, 0, B);
+    }, &rounds));
+    w->stats.split_iterations = rounds;
     const int px = (int)(hp.geometricCenter[0] + 0.5f), py = (int)(hp.geometricCenter[1] + 0.5f);
     CK(tt_launch_backtrack(s, d, w->eff, ids, px, py, nid, w->ctl));
     CKS(sync_ctl(w));
@@ -273,12 +298,10 @@ static int run_split(tt_world *w)
         CK(tt_launch_split_finalize(s, d, w->list[0], w->listN[0], ids, plate, plate, w->pd));
         return TT_OK;
     }
-    for (;;) {
-        CKS(zero(w, &w->ctl->changed, 4));
-        CK(tt_launch_flood(s, d, w->list[0], w->listN[0], ids, plate, nid, w->ctl, 8));
-        CKS(sync_ctl(w));
-        if (w->hctl->changed == 0) break;
-    }
+    int frounds = 0;
+    CKS(run_rounds(w, [&](int B) {
+        return tt_launch_flood_tiles(s, d, tiles, ntiles, ids, plate, nid, w->rchg, 0, B);
+    }, &frounds));
     CK(tt_launch_split_finalize(s, d, w->list[0], w->listN[0], ids, plate, nid, w->pd));
     CKS(zero(w, w->acc, 256 * 10 * 8));
     CK(tt_launch_count(s, d, w->mergeIds, ids, w->ids[1 - w->ic], w->h[w->hc], w->noise, w->acc, 1));
@@ -746,7 +769,7 @@ int tt_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, tt_w
     for (auto &a : w->allocs) { *a.ptr = base + off; off += a.bytes; }
     if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void **)&w->hctl, sizeof(tt_step_ctl), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&w->hu32, 64, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&w->hu32, TT_ROUND_BATCH * 4, hipHostMallocDefault) != hipSuccess) {
         tt_world_destroy(w);
         set_error("stream / pinned memory allocation failed");
         return TT_ERR_DEVICE;
