@@ -84,24 +84,46 @@ hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, const float *hin, f
     return hipGetLastError();
 }
 
-// bilinear sample with toroidal wrap (interpolate, cuda_helper.cuh:49-65);
-// |v| <= 50 keeps the sample within one period of the grid.
-__device__ __forceinline__ float tt_bilinear(const float *s, float tx, float ty, int W, int H)
+// bilinear sample with toroidal wrap (interpolate, cuda_helper.cuh:49-65).
+// |v| <= 50 keeps the sample column within [-51, W + 50], where a single
+// conditional wrap is exact; other columns (non-finite velocities) take the
+// general modulo.  IEEE floor is exact, so the hardware floor equals the
+// oracle's bit-level floor.
+__device__ __forceinline__ int tt_wrap_near(int v, int n)
 {
-    const int x0 = (int)tt_floorf(tx), y0 = (int)tt_floorf(ty);
+    return ((unsigned)(v + n) < (unsigned)(3 * n)) ? tt_wrap1(v, n) : tt_wrapg(v, n);
+}
+
+__device__ __forceinline__ float tt_bilinear(const float *__restrict__ s, float tx, float ty, int W, int H)
+{
+    const float fx = __builtin_floorf(tx), fy = __builtin_floorf(ty);
+    const int x0 = (int)fx, y0 = (int)fy;
     const float wx = tx - (float)x0, wy = ty - (float)y0;
-    const int xa = tt_wrapg(x0, W), xb = tt_wrapg(x0 + 1, W);
-    const size_t ra = (size_t)tt_wrapg(y0, H) * W, rb = (size_t)tt_wrapg(y0 + 1, H) * W;
+    const int xa = tt_wrap_near(x0, W), xb = tt_wrap_near(x0 + 1, W);
+    const int ra = tt_wrap_near(y0, H) * W, rb = tt_wrap_near(y0 + 1, H) * W;
     const float I0 = (1.0f - wx) * s[ra + xa] + wx * s[ra + xb];
     const float I1 = (1.0f - wx) * s[rb + xa] + wx * s[rb + xb];
     return (1.0f - wy) * I0 + wy * I1;
 }
 
-__global__ __launch_bounds__(256) void k_hydraulic(tt_dev d, int first, int rseed, const float *__restrict__ hin,
+// rain amount of a cell: max(0, discreteNoise((x, 0, y), 1, seed + iter)) * dt * rain
+__device__ __forceinline__ float tt_rain_at(int gx, int gy, int rseed, float rain)
+{
+    const float n = tt_discrete_noise((float)gx, 0.0f, (float)gy, 1.0f, rseed);
+    return (tt_fmaxf(0.0f, n) * 1.0f) * rain;
+}
+
+// One sub-iteration.  FIRST: the water input has no rain yet (rain is added
+// while loading) and the sediment is the step's state; otherwise the water
+// input already holds this sub-iteration's rain (added by the previous one on
+// its final write) and the sediment is the previous sub-iteration's output
+// transported along its velocity.  LAST: the water output gets no rain.
+template <bool FIRST, bool LAST>
+__global__ __launch_bounds__(256) void k_hydraulic(tt_dev d, int rseed, const float *__restrict__ hin,
                                                    float *__restrict__ hout, const float *__restrict__ win,
                                                    float *__restrict__ wout, const float4 *__restrict__ fin,
-                                                   float4 *__restrict__ fout, float2 *vel, const float *sprev,
-                                                   const float *sedState, float *sout)
+                                                   float4 *__restrict__ fout, float2 *__restrict__ vel,
+                                                   const float *__restrict__ sprev, float *__restrict__ sout)
 {
     __shared__ float M[TH_H][TH_W];
     __shared__ float Wt[TH_H][TH_W];
@@ -112,14 +134,26 @@ __global__ __launch_bounds__(256) void k_hydraulic(tt_dev d, int first, int rsee
     const int b = tt_xcd_tile(blockIdx.x, nbx * nby);
     const int x0 = (b % nbx) * TT_TILE_W, y0 = (b / nbx) * TT_TILE_H;
     const float rain = d.ks.hydrationRainfall;
-    // heights and water (after rain) on the tile plus a two-cell halo
-    for (int i = tid; i < TH_H * TH_W; i += 256) {
-        const int ly = i / TH_W, lx = i - ly * TH_W;
-        const int gx = tt_wrap1(x0 + lx - 2, W), gy = tt_wrap1(y0 + ly - 2, H);
-        const size_t g = (size_t)gy * W + gx;
-        M[ly][lx] = hin[g];
-        const float n = tt_discrete_noise((float)gx, 0.0f, (float)gy, 1.0f, rseed);
-        Wt[ly][lx] = win[g] + (tt_fmaxf(0.0f, n) * 1.0f) * rain;
+    const int lane = tid & 63, wv = tid >> 6;
+    // heights and water on the tile plus a two-cell halo: rows wv, wv+4, ...;
+    // lanes 0..63 take columns 0..63, lanes 0..3 also columns 64..67
+    {
+        const int gxa = tt_wrap1(x0 + lane - 2, W);
+        const int gxb = tt_wrap1(x0 + 64 + lane - 2, W);
+        for (int ly = wv; ly < TH_H; ly += 4) {
+            const int gy = tt_wrap1(y0 + ly - 2, H);
+            const int row = gy * W;
+            float m = hin[row + gxa], w = win[row + gxa];
+            if (FIRST) w = w + tt_rain_at(gxa, gy, rseed, rain);
+            M[ly][lane] = m;
+            Wt[ly][lane] = w;
+            if (lane < TH_W - 64) {
+                float m2 = hin[row + gxb], w2 = win[row + gxb];
+                if (FIRST) w2 = w2 + tt_rain_at(gxb, gy, rseed, rain);
+                M[ly][64 + lane] = m2;
+                Wt[ly][64 + lane] = w2;
+            }
+        }
     }
     __syncthreads();
     // outflow flux on the tile plus a one-cell halo
@@ -129,15 +163,16 @@ __global__ __launch_bounds__(256) void k_hydraulic(tt_dev d, int first, int rsee
         const int fy = i / TT_W, fx = i - fy * TT_W;
         const int ly = fy + 1, lx = fx + 1;
         const int gx = tt_wrap1(x0 + fx - 1, W), gy = tt_wrap1(y0 + fy - 1, H);
-        const float4 cf = fin[(size_t)gy * W + gx];
-        const float sc = M[ly][lx] + Wt[ly][lx];
+        const float4 cf = fin[gy * W + gx];
+        const float mc = M[ly][lx], wc = Wt[ly][lx];
+        const float sc = mc + wc;
         float4 o;
         o.x = tt_fmaxf(0.0f, cf.x + fk * ((g * ((sc - M[ly][lx - 1]) - Wt[ly][lx - 1])) / L));
         o.y = tt_fmaxf(0.0f, cf.y + fk * ((g * ((sc - M[ly + 1][lx]) - Wt[ly + 1][lx])) / L));
         o.z = tt_fmaxf(0.0f, cf.z + fk * ((g * ((sc - M[ly][lx + 1]) - Wt[ly][lx + 1])) / L));
         o.w = tt_fmaxf(0.0f, cf.w + fk * ((g * ((sc - M[ly - 1][lx]) - Wt[ly - 1][lx])) / L));
         const float tot = (((o.x + o.y) + o.z) + o.w) + 1e-6f;
-        float K = tt_fminf(1.0f, Wt[ly][lx] / (tot * 1.0f + 1e-6f));
+        float K = tt_fminf(1.0f, wc / (tot * 1.0f + 1e-6f));
         K = tt_fmaxf(K, 0.01f);
         o.x = o.x * K;
         o.y = o.y * K;
@@ -146,13 +181,14 @@ __global__ __launch_bounds__(256) void k_hydraulic(tt_dev d, int first, int rsee
         F[fy][fx] = o;
     }
     __syncthreads();
-    const int lane = tid & 63, wv = tid >> 6;
     const float evap = d.ks.hydrationEvaporation;
     const float cap = d.ks.sedimentCapacity, ke = d.ks.sedimentErosionRate, kd = d.ks.sedimentDepositionRate;
+    const int gx = x0 + lane;
+#pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int ry = wv * 4 + r;
-        const int gx = x0 + lane, gy = y0 + ry;
-        const size_t c = (size_t)gy * W + gx;
+        const int gy = y0 + ry;
+        const int c = gy * W + gx;
         const int fy = ry + 1, fx = lane + 1;
         const float4 S = F[fy][fx], Lf = F[fy][fx - 1], Uf = F[fy + 1][fx], Rf = F[fy][fx + 1], Df = F[fy - 1][fx];
         float in = 0.0f;
@@ -169,16 +205,20 @@ __global__ __launch_bounds__(256) void k_hydraulic(tt_dev d, int first, int rsee
         const float hh = tt_fmaxf(wn, 0.01f);
         float vx = nx / hh, vy = ny / hh;
         const float vm = sqrtf(vx * vx + vy * vy);
+        float vmag = vm;
         if (vm > 50.0f) {
             const float sc = 50.0f / vm;
             vx = vx * sc;
             vy = vy * sc;
+            vmag = sqrtf(vx * vx + vy * vy);
         }
-        wout[c] = tt_fmaxf(0.0f, wn - (tt_fminf(wn, 20.0f) * evap) * 1.0f);
+        float wo = tt_fmaxf(0.0f, wn - (tt_fminf(wn, 20.0f) * evap) * 1.0f);
+        if (!LAST) wo = wo + tt_rain_at(gx, gy, rseed, rain);
+        wout[c] = wo;
         fout[c] = S;
         float sed;
-        if (first) {
-            sed = sedState[c];
+        if (FIRST) {
+            sed = sprev[c];
         } else {
             const float2 pv = vel[c];
             sed = tt_bilinear(sprev, (float)gx - pv.x * 1.0f, (float)gy - pv.y * 1.0f, W, H);
@@ -189,7 +229,6 @@ __global__ __launch_bounds__(256) void k_hydraulic(tt_dev d, int first, int rsee
         const float dzdx = (M[ly][lx + 1] - M[ly][lx - 1]) * 0.5f;
         const float dzdy = (M[ly + 1][lx] - M[ly - 1][lx]) * 0.5f;
         const float slope = sqrtf(dzdx * dzdx + dzdy * dzdy);
-        const float vmag = sqrtf(vx * vx + vy * vy);
         float C = (cap * tt_sinf(tt_fmaxf(slope, 0.03f))) * vmag;
         C = tt_fminf(C, 5.0f);
         float hnew;
@@ -209,13 +248,19 @@ __global__ __launch_bounds__(256) void k_hydraulic(tt_dev d, int first, int rsee
     }
 }
 
-hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int first, int rseed, const float *hin, float *hout,
-                               const float *win, float *wout, const float4 *fin, float4 *fout, float2 *vel,
-                               const float *sprev, const float *sedState, float *sout)
+hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int first, int last, int rseed, const float *hin,
+                               float *hout, const float *win, float *wout, const float4 *fin, float4 *fout,
+                               float2 *vel, const float *sprev, float *sout)
 {
     const unsigned blocks = (unsigned)((d.W / TT_TILE_W) * (d.H / TT_TILE_H));
-    hipLaunchKernelGGL(k_hydraulic, dim3(blocks), dim3(256), 0, s, d, first, rseed, hin, hout, win, wout, fin, fout,
-                       vel, sprev, sedState, sout);
+#define TT_HYD(F_, L_) \
+    hipLaunchKernelGGL((k_hydraulic<F_, L_>), dim3(blocks), dim3(256), 0, s, d, rseed, hin, hout, win, wout, fin, \
+                       fout, vel, sprev, sout)
+    if (first && last) TT_HYD(true, true);
+    else if (first) TT_HYD(true, false);
+    else if (last) TT_HYD(false, true);
+    else TT_HYD(false, false);
+#undef TT_HYD
     return hipGetLastError();
 }
 

@@ -2,7 +2,7 @@
 // CPU restatement of the terrain step.
 //
 // Every function here is built from IEEE-754 basic operations only (+, -, *, /,
-// sqrt, float<->double conversion, bit manipulation).  Both compilers are run
+// sqrt, fused multiply-add, float<->double conversion, bit manipulation).  Both compilers are run
 // with -ffp-contract=off and without fast-math, so a call produces the same bits
 // on the host (g++, SSE2) and on the device (hipcc, gfx950).  That is what lets
 // float heights stay bit-identical between the GPU path and the CPU oracle over
@@ -17,7 +17,8 @@
 //   * log   : x = m * 2^e, m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s) with
 //             s = (m-1)/(m+1), odd series to s^23, evaluated in double.
 //   * sin   : Cody-Waite reduction by pi/2 (three 33-bit fdlibm pieces, exact
-//             for |x| < 2^19 * pi/2), Taylor polynomials on |r| <= pi/4.
+//             for |x| < 2^19 * pi/2), Taylor polynomials on |r| <= pi/4
+//             (sine to r^13, cosine to r^14) evaluated with explicit FMAs.
 //   * atan2 : octant reduction, atan(u) = pi/4 + atan((u-1)/(u+1)) for
 //             u > tan(pi/8), odd Taylor series to u^41.
 //   * fmod  : exact repeated subtraction of b*2^k (every subtraction is exact
@@ -184,34 +185,36 @@ TT_HD float tt_log10f(float x)
 // ---------------------------------------------------------------------------
 // sin / cos
 // ---------------------------------------------------------------------------
-TT_HD double tt_sin_poly(double r)  // |r| <= pi/4
+// Fused multiply-add, correctly rounded on both sides (v_fma_f64 on the
+// device; the FMA instruction or libm's fma on the host).  Contraction is off,
+// so an FMA appears only where it is written.
+TT_HD double tt_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+TT_HD double tt_sin_poly(double r)  // |r| <= pi/4; truncation error < 2^-45 relative
 {
-    // sin r = r + r z P(z), P(z) = -1/3! + z/5! - z^2/7! + ... + z^7/17!
+    // sin r = r + r z P(z), P(z) = -1/3! + z/5! - z^2/7! + z^3/9! - z^4/11! + z^5/13!
     const double z = r * r;
-    double p = 1.0 / 355687428096000.0;      // +1/17!
-    p = p * z - 1.0 / 1307674368000.0;       // -1/15!
-    p = p * z + 1.0 / 6227020800.0;          // +1/13!
-    p = p * z - 1.0 / 39916800.0;            // -1/11!
-    p = p * z + 1.0 / 362880.0;              // +1/9!
-    p = p * z - 1.0 / 5040.0;                // -1/7!
-    p = p * z + 1.0 / 120.0;                 // +1/5!
-    p = p * z - 1.0 / 6.0;                   // -1/3!
-    return r + r * (z * p);
+    double p = 1.0 / 6227020800.0;           // +1/13!
+    p = tt_fma(p, z, -1.0 / 39916800.0);     // -1/11!
+    p = tt_fma(p, z, 1.0 / 362880.0);        // +1/9!
+    p = tt_fma(p, z, -1.0 / 5040.0);         // -1/7!
+    p = tt_fma(p, z, 1.0 / 120.0);           // +1/5!
+    p = tt_fma(p, z, -1.0 / 6.0);            // -1/3!
+    return tt_fma(r * z, p, r);
 }
 
-TT_HD double tt_cos_poly(double r)  // |r| <= pi/4
+TT_HD double tt_cos_poly(double r)  // |r| <= pi/4; truncation error < 2^-49
 {
+    // cos r = 1 - z (1/2 - z/4! + z^2/6! - ... + z^6/14!)
     const double z = r * r;
-    double p = 1.0 / 6402373705728000.0;     // 1/18!
-    p = p * z - 1.0 / 20922789888000.0;      // -1/16!
-    p = p * z + 1.0 / 87178291200.0;         // 1/14!
-    p = p * z - 1.0 / 479001600.0;           // -1/12!
-    p = p * z + 1.0 / 3628800.0;             // 1/10!
-    p = p * z - 1.0 / 40320.0;               // -1/8!
-    p = p * z + 1.0 / 720.0;                 // 1/6!
-    p = p * z - 1.0 / 24.0;                  // -1/4!
-    p = p * z + 0.5;                         // +1/2 (subtracted below)
-    return 1.0 - z * p;
+    double p = -1.0 / 87178291200.0;         // -1/14!
+    p = tt_fma(p, z, 1.0 / 479001600.0);     // +1/12!
+    p = tt_fma(p, z, -1.0 / 3628800.0);      // -1/10!
+    p = tt_fma(p, z, 1.0 / 40320.0);         // +1/8!
+    p = tt_fma(p, z, -1.0 / 720.0);          // -1/6!
+    p = tt_fma(p, z, 1.0 / 24.0);            // +1/4!
+    p = tt_fma(p, z, -0.5);                  // -1/2
+    return tt_fma(z, p, 1.0);
 }
 
 // returns quadrant in *q and the reduced argument
@@ -228,28 +231,22 @@ TT_HD double tt_reduce_pio2(double x, int* q)
     return r;
 }
 
+// one polynomial per call: the sine polynomial for even quadrants, the
+// cosine polynomial for odd ones, then the sign of the quadrant
 TT_HD double tt_sin_d(double x)
 {
     int q;
     const double r = tt_reduce_pio2(x, &q);
-    switch (q) {
-        case 0: return tt_sin_poly(r);
-        case 1: return tt_cos_poly(r);
-        case 2: return -tt_sin_poly(r);
-        default: return -tt_cos_poly(r);
-    }
+    const double v = (q & 1) ? tt_cos_poly(r) : tt_sin_poly(r);
+    return (q & 2) ? -v : v;
 }
 
 TT_HD double tt_cos_d(double x)
 {
     int q;
     const double r = tt_reduce_pio2(x, &q);
-    switch (q) {
-        case 0: return tt_cos_poly(r);
-        case 1: return -tt_sin_poly(r);
-        case 2: return -tt_cos_poly(r);
-        default: return tt_sin_poly(r);
-    }
+    const double v = (q & 1) ? tt_sin_poly(r) : tt_cos_poly(r);
+    return ((q + 1) & 2) ? -v : v;
 }
 
 TT_HD float tt_sinf(float x)

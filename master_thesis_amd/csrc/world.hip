@@ -392,12 +392,13 @@ static int step_once(tt_world *w)
     if ((mask & TT_STAGE_HYDRAULIC) && w->cfg.hydraulic_iterations > 0) {
         const int rseed = (int)(w->cfg.seed + w->iter);
         int buf = w->sc;
-        for (int it = 0; it < w->cfg.hydraulic_iterations; ++it) {
+        const int K = w->cfg.hydraulic_iterations;
+        for (int it = 0; it < K; ++it) {
             float *sprev, *sout;
             if (it == 0) { sprev = w->sed[buf]; sout = w->sed[buf]; }
             else { sprev = w->sed[buf]; sout = w->sed[1 - buf]; buf = 1 - buf; }
-            CK(tt_launch_hydraulic(s, d, it == 0 ? 1 : 0, rseed, hc, ho, w->water[w->wc], w->water[1 - w->wc],
-                                   w->flux[w->fc], w->flux[1 - w->fc], w->vel, sprev, w->sed[w->sc], sout));
+            CK(tt_launch_hydraulic(s, d, it == 0, it == K - 1, rseed, hc, ho, w->water[w->wc],
+                                   w->water[1 - w->wc], w->flux[w->fc], w->flux[1 - w->fc], w->vel, sprev, sout));
             std::swap(hc, ho);
             w->wc = 1 - w->wc;
             w->fc = 1 - w->fc;
