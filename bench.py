@@ -70,6 +70,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one plain device copy of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (worlds are independent; "
@@ -78,6 +80,18 @@ def main() -> int:
 
     from master_thesis_amd import _abi, configs
     from master_thesis_amd.sim import World, STATE_FIELDS
+
+    if args.copy_probe_mib > 0:
+        # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE counters
+        # give the scale factor of the hardware byte counters
+        import torch
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cfg, ks = configs.world_4096(seed=args.seed)
     if args.side != 4096:

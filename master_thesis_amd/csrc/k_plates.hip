@@ -767,9 +767,9 @@ hipError_t tt_launch_split_prepare(hipStream_t s, const tt_dev &d, const uint8_t
     return hipGetLastError();
 }
 
-// Tiles of the split stage: 64 columns by TH rows (TH = 64 when the height
-// allows it, else 16); 256 threads, thread t owns column t % 64 and rows
-// t / 64 + 4 k.  k_split_tiles flags the tiles that hold cells of the plate.
+// Tiles of the split stage: 64 columns by TH rows, 256 threads; thread t owns
+// column t % 64 and rows t / 64 + 4 k.  k_split_tiles flags the tiles that hold
+// cells of the plate.
 template <int TH>
 __global__ __launch_bounds__(256) void k_split_tiles(tt_dev d, const uint8_t *ids, int plate, uint8_t *flag)
 {
@@ -783,23 +783,63 @@ __global__ __launch_bounds__(256) void k_split_tiles(tt_dev d, const uint8_t *id
     if (threadIdx.x == 0) flag[blockIdx.x] = any ? 1 : 0;
 }
 
+// Round bookkeeping shared by the relaxation and the flood fill.  act[p][t]
+// (p = round parity) says that tile t must run again: it wrote cells a
+// neighbour reads, or it stopped at the sweep cap.  A tile runs in round r > 0
+// only if it or one of its 8 neighbouring tiles (toroidal) is marked in
+// act[(r - 1) & 1]; it always rewrites its own act[r & 1] entry.  rchg[r]
+// counts the tiles that wrote in round r; a round with rchg == 0 is a fixed
+// point and later rounds exit at once.  rchg is a ring of TT_RCHG counters
+// (two host batches), so a round can read its predecessor across batches.
+struct tt_round {
+    uint32_t *rchg;
+    uint8_t *act;   // 2 x tiles
+    int ntiles;     // tiles in the whole grid
+    int tpr;        // tiles per row
+    int round;
+};
+
+__device__ __forceinline__ bool tt_round_skip(const tt_round &R, uint32_t tile)
+{
+    if (R.round == 0) return false;
+    if (R.rchg[(R.round - 1) % TT_RCHG] == 0u) return true;
+    const uint8_t *prev = R.act + (size_t)((R.round - 1) & 1) * R.ntiles;
+    const int tpc = R.ntiles / R.tpr;
+    const int tx = (int)(tile % (uint32_t)R.tpr), ty = (int)(tile / (uint32_t)R.tpr);
+    int any = 0;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx)
+            any |= prev[tt_wrap1(ty + dy, tpc) * R.tpr + tt_wrap1(tx + dx, R.tpr)];
+    return any == 0;
+}
+
+__device__ __forceinline__ void tt_round_mark(const tt_round &R, uint32_t tile, int wrote, int capped)
+{
+    uint8_t *cur = R.act + (size_t)(R.round & 1) * R.ntiles;
+    cur[tile] = (uint8_t)((wrote || capped) ? 1 : 0);
+    if (wrote) atomicAdd(&R.rchg[R.round % TT_RCHG], 1u);
+}
+
+#define TT_SWEEP_CAP 2048
+
 // propagateEffortToBoundary (plate_tectonics_kernel.cu:2838-2881) as rounds of
-// tile-local relaxation: each tile relaxes its cells in LDS until nothing
-// changes (halo values read at the start of the round), then writes back the
-// cells it lowered and counts itself in rchg[round].  A round in which no tile
-// changes anything is a global fixed point.  The update is monotone and starts
-// from an upper bound, so the fixed point does not depend on the order of the
-// updates: it equals the oracle's worklist result.
+// tile-local relaxation: each active tile relaxes its cells in LDS until
+// nothing changes (halo values read at the start of the round), then writes
+// back the cells it lowered.  The update is monotone and starts from an upper
+// bound, so the fixed point does not depend on the order of the updates: it
+// equals the oracle's worklist result.
 template <int TH>
 __global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *tiles, const uint8_t *ids, int plate,
-                                                     const float *height, float *eff, uint32_t *rchg, int round)
+                                                     const float *height, float *eff, tt_round R)
 {
-    if (round > 0 && rchg[round - 1] == 0u) return;
+    const uint32_t tile = tiles[blockIdx.x];
+    if (tt_round_skip(R, tile)) {
+        if (threadIdx.x == 0 && R.round > 0) R.act[(size_t)(R.round & 1) * R.ntiles + tile] = 0;
+        return;
+    }
     constexpr int PR = TH / 4;
     __shared__ float e[TH + 2][67];
-    const int tpr = d.W / 64;
-    const uint32_t tile = tiles[blockIdx.x];
-    const int tx0 = (int)(tile % (uint32_t)tpr) * 64, ty0 = (int)(tile / (uint32_t)tpr) * TH;
+    const int tx0 = (int)(tile % (uint32_t)R.tpr) * 64, ty0 = (int)(tile / (uint32_t)R.tpr) * TH;
     const int W = d.W, H = d.H;
     for (int i = threadIdx.x; i < (TH + 2) * 66; i += 256) {
         const int r = i / 66, q = i - r * 66;
@@ -819,7 +859,8 @@ __global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *t
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < PR; ++j) init[j] = e[wv + 4 * j + 1][lx + 1];
-    for (int it = 0; it < 4096; ++it) {
+    int capped = 1;
+    for (int it = 0; it < TT_SWEEP_CAP; ++it) {
         int ch = 0;
 #pragma unroll
         for (int j = 0; j < PR; ++j) {
@@ -839,7 +880,7 @@ __global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *t
             t = e[r + 1][q + 1] + c2[j]; if (t < md) md = t;
             if (md != old) { e[r][q] = md; ch = 1; }
         }
-        if (!__syncthreads_or(ch)) break;
+        if (!__syncthreads_or(ch)) { capped = 0; break; }
     }
     int wrote = 0;
 #pragma unroll
@@ -851,21 +892,23 @@ __global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *t
         }
     }
     wrote = __syncthreads_or(wrote);
-    if (threadIdx.x == 0 && wrote) atomicAdd(&rchg[round], 1u);
+    if (threadIdx.x == 0) tt_round_mark(R, tile, wrote, capped);
 }
 
 // floodFillPlate (plate_tectonics_kernel.cu:2985-3024) in the same round
 // scheme: the new id spreads through 4-connected cells of the old plate.
 template <int TH>
 __global__ __launch_bounds__(256) void k_flood_tiles(tt_dev d, const uint32_t *tiles, uint8_t *ids, int oldId,
-                                                     int newId, uint32_t *rchg, int round)
+                                                     int newId, tt_round R)
 {
-    if (round > 0 && rchg[round - 1] == 0u) return;
+    const uint32_t tile = tiles[blockIdx.x];
+    if (tt_round_skip(R, tile)) {
+        if (threadIdx.x == 0 && R.round > 0) R.act[(size_t)(R.round & 1) * R.ntiles + tile] = 0;
+        return;
+    }
     constexpr int PR = TH / 4;
     __shared__ uint8_t m[TH + 2][68];
-    const int tpr = d.W / 64;
-    const uint32_t tile = tiles[blockIdx.x];
-    const int tx0 = (int)(tile % (uint32_t)tpr) * 64, ty0 = (int)(tile / (uint32_t)tpr) * TH;
+    const int tx0 = (int)(tile % (uint32_t)R.tpr) * 64, ty0 = (int)(tile / (uint32_t)R.tpr) * TH;
     const int W = d.W, H = d.H;
     for (int i = threadIdx.x; i < (TH + 2) * 66; i += 256) {
         const int r = i / 66, q = i - r * 66;
@@ -876,7 +919,8 @@ __global__ __launch_bounds__(256) void k_flood_tiles(tt_dev d, const uint32_t *t
     bool cand[PR];
 #pragma unroll
     for (int j = 0; j < PR; ++j) cand[j] = m[wv + 4 * j + 1][lx + 1] == oldId;
-    for (int it = 0; it < 4096; ++it) {
+    int capped = 1;
+    for (int it = 0; it < TT_SWEEP_CAP; ++it) {
         int ch = 0;
 #pragma unroll
         for (int j = 0; j < PR; ++j) {
@@ -887,7 +931,7 @@ __global__ __launch_bounds__(256) void k_flood_tiles(tt_dev d, const uint32_t *t
                 ch = 1;
             }
         }
-        if (!__syncthreads_or(ch)) break;
+        if (!__syncthreads_or(ch)) { capped = 0; break; }
     }
     int wrote = 0;
 #pragma unroll
@@ -898,46 +942,47 @@ __global__ __launch_bounds__(256) void k_flood_tiles(tt_dev d, const uint32_t *t
         }
     }
     wrote = __syncthreads_or(wrote);
-    if (threadIdx.x == 0 && wrote) atomicAdd(&rchg[round], 1u);
+    if (threadIdx.x == 0) tt_round_mark(R, tile, wrote, capped);
 }
 
-int tt_split_tile_h(const tt_dev &d) { return (d.H % 64) == 0 ? 64 : 16; }
+int tt_split_tile_h(const tt_dev &d) { return 16; }
 
 hipError_t tt_launch_split_tiles(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, uint8_t *flag)
 {
-    const int th = tt_split_tile_h(d);
-    const unsigned nt = (unsigned)((d.W / 64) * (d.H / th));
-    if (th == 64) hipLaunchKernelGGL(k_split_tiles<64>, dim3(nt), dim3(256), 0, s, d, ids, plate, flag);
-    else hipLaunchKernelGGL(k_split_tiles<16>, dim3(nt), dim3(256), 0, s, d, ids, plate, flag);
+    const unsigned nt = (unsigned)((d.W / 64) * (d.H / 16));
+    hipLaunchKernelGGL(k_split_tiles<16>, dim3(nt), dim3(256), 0, s, d, ids, plate, flag);
     return hipGetLastError();
+}
+
+static tt_round make_round(const tt_dev &d, uint32_t *rchg, uint8_t *act, int r)
+{
+    tt_round R;
+    R.rchg = rchg;
+    R.act = act;
+    R.tpr = d.W / 64;
+    R.ntiles = R.tpr * (d.H / 16);
+    R.round = r;
+    return R;
 }
 
 hipError_t tt_launch_relax_tiles(hipStream_t s, const tt_dev &d, const uint32_t *tiles, unsigned ntiles,
                                  const uint8_t *ids, int plate, const float *height, float *eff, uint32_t *rchg,
-                                 int round0, int rounds)
+                                 uint8_t *act, int round0, int rounds)
 {
     if (ntiles == 0) return hipSuccess;
-    for (int r = round0; r < round0 + rounds; ++r) {
-        if (tt_split_tile_h(d) == 64)
-            hipLaunchKernelGGL(k_relax_tiles<64>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, plate, height, eff,
-                               rchg, r);
-        else
-            hipLaunchKernelGGL(k_relax_tiles<16>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, plate, height, eff,
-                               rchg, r);
-    }
+    for (int r = round0; r < round0 + rounds; ++r)
+        hipLaunchKernelGGL(k_relax_tiles<16>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, plate, height, eff,
+                           make_round(d, rchg, act, r));
     return hipGetLastError();
 }
 
 hipError_t tt_launch_flood_tiles(hipStream_t s, const tt_dev &d, const uint32_t *tiles, unsigned ntiles, uint8_t *ids,
-                                 int oldId, int newId, uint32_t *rchg, int round0, int rounds)
+                                 int oldId, int newId, uint32_t *rchg, uint8_t *act, int round0, int rounds)
 {
     if (ntiles == 0) return hipSuccess;
-    for (int r = round0; r < round0 + rounds; ++r) {
-        if (tt_split_tile_h(d) == 64)
-            hipLaunchKernelGGL(k_flood_tiles<64>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, oldId, newId, rchg, r);
-        else
-            hipLaunchKernelGGL(k_flood_tiles<16>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, oldId, newId, rchg, r);
-    }
+    for (int r = round0; r < round0 + rounds; ++r)
+        hipLaunchKernelGGL(k_flood_tiles<16>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, oldId, newId,
+                           make_round(d, rchg, act, r));
     return hipGetLastError();
 }
 

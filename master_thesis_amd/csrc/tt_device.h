@@ -26,6 +26,11 @@
 #define TT_TILE_H 16
 #define TT_BLOCK 256
 
+// rounds of a tiled fixed-point kernel enqueued per host check, and the ring of
+// per-round change counters (two batches)
+#define TT_ROUND_BATCH 32
+#define TT_RCHG (2 * TT_ROUND_BATCH)
+
 struct tt_uplift { float cum; uint8_t types; uint8_t pad[3]; };
 struct tt_prop { float cum; uint32_t inv; uint8_t types; uint8_t pad[3]; };
 struct tt_dist { int32_t dist; uint32_t orig; };

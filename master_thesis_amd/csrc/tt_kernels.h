@@ -96,10 +96,10 @@ int tt_split_tile_h(const tt_dev &d);
 hipError_t tt_launch_split_tiles(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, uint8_t *flag);
 hipError_t tt_launch_relax_tiles(hipStream_t s, const tt_dev &d, const uint32_t *tiles, unsigned ntiles,
                                  const uint8_t *ids, int plate, const float *height, float *eff, uint32_t *rchg,
-                                 int round0, int rounds);
+                                 uint8_t *act, int round0, int rounds);
 hipError_t tt_launch_backtrack(hipStream_t s, const tt_dev &d, const float *eff, uint8_t *ids, int px, int py,
                                int newId, tt_step_ctl *ctl);
 hipError_t tt_launch_flood_tiles(hipStream_t s, const tt_dev &d, const uint32_t *tiles, unsigned ntiles, uint8_t *ids,
-                                 int oldId, int newId, uint32_t *rchg, int round0, int rounds);
+                                 int oldId, int newId, uint32_t *rchg, uint8_t *act, int round0, int rounds);
 hipError_t tt_launch_split_finalize(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n,
                                     uint8_t *ids, int oldId, int newId, tt_plate_data *pd);

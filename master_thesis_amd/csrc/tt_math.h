@@ -49,17 +49,17 @@ TT_HD int tt_isnanf(float x) { return (tt_f2u(x) & 0x7fffffffu) > 0x7f800000u; }
 
 // fminf / fmaxf with C99 NaN semantics (a NaN operand yields the other one),
 // which is what CUDA's min/max/fminf/fmaxf do (U4: min(NaN, 0) == 0).
+// Written with float comparisons only (a NaN compares false), so that a
+// constant operand folds to one compare and one select:
+//   a NaN, b not  -> b;   b NaN, a not -> a;   both NaN -> b;
+//   otherwise the smaller (larger); equal operands (+0 / -0) give b.
 TT_HD float tt_fminf(float a, float b)
 {
-    if (tt_isnanf(a)) return b;
-    if (tt_isnanf(b)) return a;
-    return a < b ? a : b;
+    return (a < b || (b != b && a == a)) ? a : b;
 }
 TT_HD float tt_fmaxf(float a, float b)
 {
-    if (tt_isnanf(a)) return b;
-    if (tt_isnanf(b)) return a;
-    return a > b ? a : b;
+    return (a > b || (b != b && a == a)) ? a : b;
 }
 
 // round-to-nearest-even of a double to an integer-valued double, |x| < 2^51.

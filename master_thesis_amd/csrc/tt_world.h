@@ -7,8 +7,6 @@
 
 #include "tt_device.h"
 
-// rounds of a tiled fixed-point kernel enqueued per host check
-#define TT_ROUND_BATCH 32
 
 struct tt_alloc {
     void **ptr;
@@ -74,7 +72,8 @@ struct tt_world {
     int *mergeIds = nullptr;
     long long *acc = nullptr;
     float *eff = nullptr;
-    uint32_t *rchg = nullptr;   // per-round change counters of the split fixed points
+    uint32_t *rchg = nullptr;   // per-round change counters of the split fixed points (TT_RCHG)
+    uint8_t *act = nullptr;     // per-tile activity flags of the split rounds (2 x tiles)
 
     // scratch
     uint32_t *scanTmp = nullptr;

@@ -143,7 +143,8 @@ size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w)
     take((void **)&w->mergeIds, 256 * 4);
     take((void **)&w->acc, 256 * 10 * 8);
     take((void **)&w->eff, N * 4);
-    take((void **)&w->rchg, TT_ROUND_BATCH * 4);
+    take((void **)&w->rchg, TT_RCHG * 4);
+    take((void **)&w->act, 2 * (N / 1024 + 64));
     take((void **)&w->scanTmp, (N / 1024 + tt_scan_tmp_words((long long)N) + 4096) * 4);
     take((void **)&w->initSeeds, 4096 * 12);
     take((void **)&w->initWts, 128 * 8);
@@ -244,21 +245,22 @@ static int run_ccl(tt_world *w)
 // ---------------------------------------------------------------------------
 // Runs rounds of a tiled fixed-point kernel in batches, with one host read per
 // batch: a round that changed nothing ends the loop (later rounds of the batch
-// exit at once).  Returns the number of rounds that did work.
+// exit at once).  Round numbers continue across batches; the counters of a
+// batch live in one half of the TT_RCHG ring and are cleared before it runs.
 template <typename Launch>
 static int run_rounds(tt_world *w, Launch launch, int *rounds_out)
 {
     const int B = TT_ROUND_BATCH;
-    int total = 0;
+    const long long ntiles_all = (long long)(w->W / 64) * (w->H / 16);
+    CKS(zero(w, w->act, (size_t)(2 * ntiles_all)));
     for (int batch = 0; batch < 100000; ++batch) {
-        CKS(zero(w, w->rchg, B * 4));
-        CK(launch(B));
-        CK(hipMemcpyAsync(w->hu32, w->rchg, B * 4, hipMemcpyDeviceToHost, w->stream));
+        uint32_t *half = w->rchg + (batch & 1) * B;
+        CKS(zero(w, half, B * 4));
+        CK(launch(batch * B, B));
+        CK(hipMemcpyAsync(w->hu32, half, B * 4, hipMemcpyDeviceToHost, w->stream));
         CK(hipStreamSynchronize(w->stream));
-        for (int r = 0; r < B; ++r) {
-            ++total;
-            if (w->hu32[r] == 0u) { *rounds_out = total; return TT_OK; }
-        }
+        for (int r = 0; r < B; ++r)
+            if (w->hu32[r] == 0u) { *rounds_out = batch * B + r + 1; return TT_OK; }
     }
     set_error("split fixed point did not converge");
     return TT_ERR_STATE;
@@ -287,8 +289,8 @@ static int run_split(tt_world *w)
     const uint32_t *tiles = w->list[1];
     const float *height = w->h[w->hc];
     int rounds = 0;
-    CKS(run_rounds(w, [&](int B) {
-        return tt_launch_relax_tiles(s, d, tiles, ntiles, ids, plate, height, w->eff, w->rchg, 0, B);
+    CKS(run_rounds(w, [&](int r0, int B) {
+        return tt_launch_relax_tiles(s, d, tiles, ntiles, ids, plate, height, w->eff, w->rchg, w->act, r0, B);
     }, &rounds));
     w->stats.split_iterations = rounds;
     const int px = (int)(hp.geometricCenter[0] + 0.5f), py = (int)(hp.geometricCenter[1] + 0.5f);
@@ -299,8 +301,8 @@ static int run_split(tt_world *w)
         return TT_OK;
     }
     int frounds = 0;
-    CKS(run_rounds(w, [&](int B) {
-        return tt_launch_flood_tiles(s, d, tiles, ntiles, ids, plate, nid, w->rchg, 0, B);
+    CKS(run_rounds(w, [&](int r0, int B) {
+        return tt_launch_flood_tiles(s, d, tiles, ntiles, ids, plate, nid, w->rchg, w->act, r0, B);
     }, &frounds));
     CK(tt_launch_split_finalize(s, d, w->list[0], w->listN[0], ids, plate, nid, w->pd));
     CKS(zero(w, w->acc, 256 * 10 * 8));
