@@ -239,7 +239,8 @@ __global__ __launch_bounds__(256) void k_keys(tt_dev d, const uint32_t *word, co
         return;
     }
     // more than four plates: enumerate all arrivals in ascending order
-    const int gx = (int)(c % d.W), gy = (int)(c / d.W);
+    int gx, gy;
+    tt_cell_xy((uint32_t)c, d, gx, gy);
     const int nd = ctl->n_distinct;
     int last = -1;
     for (int s = 0; s < n; ++s) {
@@ -335,7 +336,8 @@ __global__ __launch_bounds__(256) void k_resolve(tt_dev d, const uint32_t *word,
     for (int k = 0; k < 4; ++k) {
         const long long c = (long long)blockIdx.x * 1024 + k * 256 + tid;
         if (c >= d.N) break;
-        const int x = (int)(c % W), y = (int)(c / W);
+        int x, y;
+        tt_cell_xy((uint32_t)c, d, x, y);
         const uint32_t cv = word[c];
         if (cv == 0) {
             // divergence (processDivergence, plate_tectonics_kernel.cu:301-362)

@@ -23,57 +23,70 @@ static_assert(TT_TILE_W == 64 && TT_TILE_H == 16, "tiles are 64 x 16");
 
 __constant__ int c_toff[8][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}, {-1, -1}, {-1, 1}, {1, -1}, {1, 1}};
 
-__global__ __launch_bounds__(256) void k_thermal(tt_dev d, const float *hin, float *hout)
+__global__ __launch_bounds__(256) void k_thermal(tt_dev d, const float *__restrict__ hin, float *__restrict__ hout)
 {
     __shared__ float hs[TH_H][TH_W];
     __shared__ float tot[TT_H][TT_W];
     const int tid = threadIdx.x;
-    const int nbx = d.W / TT_TILE_W, nby = d.H / TT_TILE_H;
+    const int W = d.W, H = d.H;
+    const int nbx = W / TT_TILE_W, nby = H / TT_TILE_H;
     const int b = tt_xcd_tile(blockIdx.x, nbx * nby);
     const int x0 = (b % nbx) * TT_TILE_W, y0 = (b / nbx) * TT_TILE_H;
-    for (int i = tid; i < TH_H * TH_W; i += 256) {
-        const int ly = i / TH_W, lx = i - ly * TH_W;
-        const int gx = tt_wrap1(x0 + lx - 2, d.W), gy = tt_wrap1(y0 + ly - 2, d.H);
-        hs[ly][lx] = hin[(size_t)gy * d.W + gx];
+    const int lane = tid & 63, wv = tid >> 6;
+    {
+        const int gxa = tt_wrap1(x0 + lane - 2, W), gxb = tt_wrap1(x0 + 64 + lane - 2, W);
+        for (int ly = wv; ly < TH_H; ly += 4) {
+            const int row = tt_wrap1(y0 + ly - 2, H) * W;
+            hs[ly][lane] = hin[row + gxa];
+            if (lane < TH_W - 64) hs[ly][64 + lane] = hin[row + gxb];
+        }
     }
     __syncthreads();
     const float talus = d.ks.thermalThresholdAngle * d.ks.thermalCellSize;
     const float amp = d.ks.thermalErosionAmplitude, str = d.ks.thermalErosionStrength;
+    // sum of the downhill differences above the talus, on the tile + 1 halo
     for (int i = tid; i < TT_H * TT_W; i += 256) {
         const int ty = i / TT_W, tx = i - ty * TT_W;
         const int ly = ty + 1, lx = tx + 1;
         const float hc = hs[ly][lx];
+        float nb[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) nb[k] = hs[ly + c_toff[k][1]][lx + c_toff[k][0]];
         float t = 0.0f;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const float diff = hc - hs[ly + c_toff[k][1]][lx + c_toff[k][0]];
-            if (diff > talus) t = t + diff;
+            const float diff = hc - nb[k];
+            t = (diff > talus) ? t + diff : t;
         }
         tot[ty][tx] = t;
     }
     __syncthreads();
-    const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int ry = wv * 4 + r;
         const int ly = ry + 2, lx = lane + 2;
         const float hc = hs[ly][lx];
         const float tc = tot[ry + 1][lane + 1];
+        float dOut[8], dIn[8], tIn[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            dOut[k] = hc - hs[ly + c_toff[k][1]][lx + c_toff[k][0]];
+            const int sy = ly - c_toff[k][1], sx = lx - c_toff[k][0];
+            dIn[k] = hs[sy][sx] - hc;
+            tIn[k] = tot[sy - 1][sx - 1];
+        }
+        // own outflows (directions 0..7), then inflows (directions 0..7); the
+        // divisions sit in branches that a wavefront skips when no lane needs them
         float v = hc;
         if (tc > 0.0f) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const float diff = hc - hs[ly + c_toff[k][1]][lx + c_toff[k][0]];
-                if (diff > talus) v = v - (amp * (diff / tc)) * str;
-            }
+            for (int k = 0; k < 8; ++k)
+                if (dOut[k] > talus) v = v - (amp * (dOut[k] / tc)) * str;
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int sy = ly - c_toff[k][1], sx = lx - c_toff[k][0];
-            const float diff = hs[sy][sx] - hc;
-            const float ts = tot[sy - 1][sx - 1];
-            if (diff > talus && ts > 0.0f) v = v + (amp * (diff / ts)) * str;
-        }
-        hout[(size_t)(y0 + ry) * d.W + x0 + lane] = v;
+        for (int k = 0; k < 8; ++k)
+            if (dIn[k] > talus && tIn[k] > 0.0f) v = v + (amp * (dIn[k] / tIn[k])) * str;
+        hout[(y0 + ry) * W + x0 + lane] = v;
     }
 }
 
@@ -268,7 +281,8 @@ __global__ __launch_bounds__(256) void k_transport(tt_dev d, const float *sprev,
 {
     const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
     if (c >= d.N) return;
-    const int x = (int)(c % d.W), y = (int)(c / d.W);
+    int x, y;
+    tt_cell_xy((uint32_t)c, d, x, y);
     const float2 v = vel[c];
     sout[c] = tt_bilinear(sprev, (float)x - v.x * 1.0f, (float)y - v.y * 1.0f, d.W, d.H);
 }

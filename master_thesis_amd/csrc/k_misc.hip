@@ -221,7 +221,8 @@ __global__ __launch_bounds__(256) void k_blur_x(tt_dev d, const double *in, doub
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= d.N) return;
-    const int x = (int)(i % d.W), y = (int)(i / d.W);
+    int x, y;
+    tt_cell_xy((uint32_t)i, d, x, y);
     double s = 0.0;
     for (int k = -32; k <= 32; ++k) s = s + wts[k + 32] * in[(size_t)y * d.W + tt_wrapg(x + k, d.W)];
     out[i] = s;
@@ -231,7 +232,8 @@ __global__ __launch_bounds__(256) void k_blur_y(tt_dev d, const double *in, floa
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= d.N) return;
-    const int x = (int)(i % d.W), y = (int)(i / d.W);
+    int x, y;
+    tt_cell_xy((uint32_t)i, d, x, y);
     double s = 0.0;
     for (int k = -32; k <= 32; ++k) s = s + wts[k + 32] * in[(size_t)tt_wrapg(y + k, d.H) * d.W + x];
     out[i] = (float)s;
@@ -252,7 +254,8 @@ __global__ __launch_bounds__(256) void k_init_ref_heights(tt_dev d, float *heigh
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= d.N) return;
-    const int x = (int)(i % d.W), y = (int)(i / d.W);
+    int x, y;
+    tt_cell_xy((uint32_t)i, d, x, y);
     const float dlt = 0.001f;
     float value = 0.0f, amp = 0.9f, freq = 0.01f;
     for (int o = 0; o < 4; ++o) {
@@ -281,7 +284,9 @@ __global__ __launch_bounds__(256) void k_init_plate_ids(tt_dev d, const float *s
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= d.N) return;
-    const float x = (float)(int)(i % d.W), y = (float)(int)(i / d.W);
+    int xi, yi;
+    tt_cell_xy((uint32_t)i, d, xi, yi);
+    const float x = (float)xi, y = (float)yi;
     const float fw = (float)d.W, fh = (float)d.H;
     float best = 3.402823466e38f;
     int id = 0;
@@ -307,7 +312,9 @@ __global__ __launch_bounds__(256) void k_asthenosphere(tt_dev d, const uint8_t *
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= d.N) return;
     float ax, ay;
-    tt_asth_at(noise, (int)(i % d.W), (int)(i / d.W), d.axs, d.ays, ax, ay);
+    int xi, yi;
+    tt_cell_xy((uint32_t)i, d, xi, yi);
+    tt_asth_at(noise, xi, yi, d.axs, d.ays, ax, ay);
     out[i] = make_float2(ax, ay);
 }
 

@@ -150,19 +150,53 @@ __device__ __forceinline__ void tt_union32(unsigned *par, unsigned a, unsigned b
     }
 }
 
-__global__ __launch_bounds__(256) void k_ccl_local(tt_dev d, const uint8_t *ids, uint32_t *labels)
+// Local labelling of one 64x16 tile.  labels[c] = global index of the
+// smallest cell of c's component inside the tile (its local root);
+// rootFlag[c] = 1 at local roots; rootCount[r] = cells of local root r.  A tile
+// holding one plate id is one component (every row is chained by l edges and
+// rows are joined by tl edges), rooted at its first cell; other tiles run an
+// LDS union-find over the reference's edge set.
+__global__ __launch_bounds__(256) void k_ccl_local(tt_dev d, const uint8_t *ids, uint32_t *labels, uint8_t *rootFlag,
+                                                   uint32_t *rootCount)
 {
     __shared__ unsigned par[TT_TILE_H * TT_TILE_W];
+    __shared__ unsigned cnt[TT_TILE_H * TT_TILE_W];
     __shared__ uint8_t tid8[(TT_TILE_H + 1) * (TT_TILE_W + 2)];
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
     const int x0 = blockIdx.x * TT_TILE_W, y0 = blockIdx.y * TT_TILE_H;
+    const int W = d.W;
+    // fast path: one plate id over the whole tile
+    uint8_t own[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) own[k] = ids[(size_t)(y0 + wv * 4 + k) * W + x0 + lane];
+    __shared__ uint8_t first;
+    if (tid == 0) first = own[0];
+    __syncthreads();
+    int same = 1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) same &= own[k] == first;
+    if (__syncthreads_and(same)) {
+        const uint32_t root = (uint32_t)((size_t)y0 * W + x0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const size_t c = (size_t)(y0 + wv * 4 + k) * W + x0 + lane;
+            labels[c] = root;
+            rootFlag[c] = (uint8_t)(c == root ? 1 : 0);
+        }
+        if (tid == 0) rootCount[root] = TT_TILE_W * TT_TILE_H;
+        return;
+    }
     const int TW2 = TT_TILE_W + 2;
     for (int i = tid; i < (TT_TILE_H + 1) * TW2; i += 256) {
         const int ly = i / TW2, lx = i - ly * TW2;
         const int gx = tt_wrap1(x0 + lx - 1, d.W), gy = tt_wrap1(y0 + ly - 1, d.H);
         tid8[i] = ids[(size_t)gy * d.W + gx];
     }
-    for (int i = tid; i < TT_TILE_H * TT_TILE_W; i += 256) par[i] = (unsigned)i;
+    for (int i = tid; i < TT_TILE_H * TT_TILE_W; i += 256) {
+        par[i] = (unsigned)i;
+        cnt[i] = 0u;
+    }
     __syncthreads();
     for (int i = tid; i < TT_TILE_H * TT_TILE_W; i += 256) {
         const int ly = i / TT_TILE_W, lx = i - ly * TT_TILE_W;
@@ -178,18 +212,32 @@ __global__ __launch_bounds__(256) void k_ccl_local(tt_dev d, const uint8_t *ids,
     __syncthreads();
     for (int i = tid; i < TT_TILE_H * TT_TILE_W; i += 256) {
         const unsigned r = tt_find32(par, (unsigned)i);
+        atomicAdd(&cnt[r], 1u);
         const int ry = (int)(r / TT_TILE_W), rx = (int)(r % TT_TILE_W);
         const int ly = i / TT_TILE_W, lx = i - ly * TT_TILE_W;
-        labels[(size_t)(y0 + ly) * d.W + x0 + lx] = (uint32_t)((size_t)(y0 + ry) * d.W + x0 + rx);
+        const size_t c = (size_t)(y0 + ly) * d.W + x0 + lx;
+        labels[c] = (uint32_t)((size_t)(y0 + ry) * d.W + x0 + rx);
+        rootFlag[c] = (uint8_t)(r == (unsigned)i ? 1 : 0);
     }
+    __syncthreads();
+    for (int i = tid; i < TT_TILE_H * TT_TILE_W; i += 256)
+        if (cnt[i] != 0u) {
+            const int ly = i / TT_TILE_W, lx = i - ly * TT_TILE_W;
+            rootCount[(size_t)(y0 + ly) * d.W + x0 + lx] = cnt[i];
+        }
 }
 
-hipError_t tt_launch_ccl_local(hipStream_t s, const tt_dev &d, const uint8_t *ids, uint32_t *labels)
+hipError_t tt_launch_ccl_local(hipStream_t s, const tt_dev &d, const uint8_t *ids, uint32_t *labels, uint8_t *rootFlag,
+                               uint32_t *rootCount)
 {
-    hipLaunchKernelGGL(k_ccl_local, dim3(d.W / TT_TILE_W, d.H / TT_TILE_H), dim3(256), 0, s, d, ids, labels);
+    hipLaunchKernelGGL(k_ccl_local, dim3(d.W / TT_TILE_W, d.H / TT_TILE_H), dim3(256), 0, s, d, ids, labels, rootFlag,
+                       rootCount);
     return hipGetLastError();
 }
 
+// global union-find over local roots (labels[] of a local root is its parent;
+// the parent of a root is itself; atomicMin links the larger root under the
+// smaller as in unionStep, ccl.cu:60-85)
 __device__ __forceinline__ uint32_t tt_gfind(const uint32_t *lab, uint32_t a)
 {
     uint32_t p = lab[a];
@@ -217,65 +265,78 @@ __device__ __forceinline__ void tt_gunion(uint32_t *lab, uint32_t a, uint32_t b)
     }
 }
 
-// edges that cross tile borders: top row (tl, t, tr), left column (tl, l),
-// right column (tr)
-__global__ __launch_bounds__(256) void k_ccl_merge(tt_dev d, const uint8_t *ids, uint32_t *labels)
+// Edges that cross tile borders join local roots: for each tile its top row
+// (tl, t, tr), left column (tl, l) and right column (tr).  128 threads per tile
+// cover the 94 border cells; a lane skips a pair of roots its left neighbour
+// lane already submitted.
+__global__ __launch_bounds__(128) void k_ccl_merge(tt_dev d, const uint8_t *ids, uint32_t *labels)
 {
-    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (c >= d.N) return;
-    const int x = (int)(c % d.W), y = (int)(c / d.W);
-    const int lx = x % TT_TILE_W, ly = y % TT_TILE_H;
-    if (ly != 0 && lx != 0 && lx != TT_TILE_W - 1) return;
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TT_TILE_W, y0 = blockIdx.y * TT_TILE_H;
+    int lx, ly;
+    if (tid < 64) { lx = tid; ly = 0; }
+    else if (tid < 79) { lx = 0; ly = tid - 63; }
+    else if (tid < 94) { lx = TT_TILE_W - 1; ly = tid - 78; }
+    else { lx = -1; ly = -1; }
     const int W = d.W, H = d.H;
-    const int xl = tt_wrap1(x - 1, W), xr = tt_wrap1(x + 1, W), yu = tt_wrap1(y - 1, H);
-    const uint8_t id = ids[c];
-    const uint32_t ntl = (uint32_t)yu * W + xl, nt = (uint32_t)yu * W + x, ntr = (uint32_t)yu * W + xr;
-    const uint32_t nl = (uint32_t)y * W + xl;
-    const unsigned m = tt_ccl_edges(id, ids[ntl], ids[nt], ids[ntr], ids[nl], y == 0);
-    const uint32_t cc = (uint32_t)c;
-    if ((m & 1u) && (ly == 0 || lx == 0)) tt_gunion(labels, cc, ntl);
-    if ((m & 2u) && ly == 0) tt_gunion(labels, cc, nt);
-    if ((m & 4u) && (ly == 0 || lx == TT_TILE_W - 1)) tt_gunion(labels, cc, ntr);
-    if ((m & 8u) && lx == 0) tt_gunion(labels, cc, nl);
+    uint32_t pa[4], pb[4];
+    int np = 0;
+    if (lx >= 0) {
+        const int x = x0 + lx, y = y0 + ly;
+        const int xl = tt_wrap1(x - 1, W), xr = tt_wrap1(x + 1, W), yu = tt_wrap1(y - 1, H);
+        const uint32_t c = (uint32_t)y * W + x;
+        const uint8_t id = ids[c];
+        const uint32_t ntl = (uint32_t)yu * W + xl, nt = (uint32_t)yu * W + x, ntr = (uint32_t)yu * W + xr;
+        const uint32_t nl = (uint32_t)y * W + xl;
+        const unsigned m = tt_ccl_edges(id, ids[ntl], ids[nt], ids[ntr], ids[nl], y == 0);
+        const uint32_t lc = labels[c];
+        if ((m & 1u) && (ly == 0 || lx == 0)) { pa[np] = lc; pb[np] = labels[ntl]; ++np; }
+        if ((m & 2u) && ly == 0) { pa[np] = lc; pb[np] = labels[nt]; ++np; }
+        if ((m & 4u) && (ly == 0 || lx == TT_TILE_W - 1)) { pa[np] = lc; pb[np] = labels[ntr]; ++np; }
+        if ((m & 8u) && lx == 0) { pa[np] = lc; pb[np] = labels[nl]; ++np; }
+    }
+    // first pair of the left lane, for de-duplication along the top row
+    const uint32_t la = __shfl_up(np > 0 ? pa[0] : 0xFFFFFFFFu, 1, 64);
+    const uint32_t lb = __shfl_up(np > 0 ? pb[0] : 0xFFFFFFFFu, 1, 64);
+    for (int k = 0; k < np; ++k) {
+        if ((tid & 63) != 0 && pa[k] == la && pb[k] == lb) continue;
+        if (pa[k] != pb[k]) tt_gunion(labels, pa[k], pb[k]);
+    }
 }
 
 hipError_t tt_launch_ccl_merge(hipStream_t s, const tt_dev &d, const uint8_t *ids, uint32_t *labels)
 {
-    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
-    hipLaunchKernelGGL(k_ccl_merge, dim3(blocks), dim3(256), 0, s, d, ids, labels);
+    hipLaunchKernelGGL(k_ccl_merge, dim3(d.W / TT_TILE_W, d.H / TT_TILE_H), dim3(128), 0, s, d, ids, labels);
     return hipGetLastError();
 }
 
-// final labels, component sizes (wavefront-aggregated atomics) and root flags
-__global__ __launch_bounds__(256) void k_ccl_flatten(tt_dev d, uint32_t *labels, uint32_t *rootCount,
-                                                     uint8_t *rootFlag)
+// Final roots of the local roots by pointer jumping; a local root that is not
+// final adds its cell count to its final root and clears its root flag.
+// Grid-stride over the local-root list (its length is read on the device).
+__global__ __launch_bounds__(256) void k_ccl_flatten(const uint32_t *locList, const uint32_t *nloc, uint32_t *labels,
+                                                     uint32_t *rootCount, uint8_t *rootFlag)
 {
-    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
-    const bool valid = c < d.N;
-    uint32_t lab = 0;
-    if (valid) {
-        lab = tt_gfind(labels, (uint32_t)c);
-        rootFlag[c] = lab == (uint32_t)c ? 1 : 0;
+    const uint32_t n = *nloc;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const uint32_t r = locList[i];
+        uint32_t p = labels[r];
+        if (p == r) continue;
+        for (;;) {
+            const uint32_t gp = labels[p];
+            if (gp == p) break;
+            labels[r] = gp;
+            p = gp;
+        }
+        labels[r] = p;
+        atomicAdd(&rootCount[p], rootCount[r]);
+        rootFlag[r] = 0;
     }
-    // the forest is static here (all unions are done); writing a root into
-    // labels[c] only shortens paths other threads may be following
-    unsigned long long pending = __ballot(valid);
-    const int lane = threadIdx.x & 63;
-    while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const uint32_t ll = __shfl(lab, leader, 64);
-        const unsigned long long same = __ballot(valid && lab == ll) & pending;
-        if (lane == leader) atomicAdd(&rootCount[ll], (uint32_t)__popcll(same));
-        pending &= ~same;
-    }
-    if (valid) labels[c] = lab;
 }
 
-hipError_t tt_launch_ccl_flatten(hipStream_t s, const tt_dev &d, uint32_t *labels, uint32_t *rootCount,
-                                 uint8_t *rootFlag)
+hipError_t tt_launch_ccl_flatten(hipStream_t s, const uint32_t *locList, const uint32_t *nloc, uint32_t *labels,
+                                 uint32_t *rootCount, uint8_t *rootFlag)
 {
-    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
-    hipLaunchKernelGGL(k_ccl_flatten, dim3(blocks), dim3(256), 0, s, d, labels, rootCount, rootFlag);
+    hipLaunchKernelGGL(k_ccl_flatten, dim3(1024), dim3(256), 0, s, locList, nloc, labels, rootCount, rootFlag);
     return hipGetLastError();
 }
 
@@ -387,17 +448,22 @@ hipError_t tt_launch_frag_assign(hipStream_t s, const tt_dev &d, const unsigned 
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(256) void k_relabel(tt_dev d, const uint32_t *labels, const uint8_t *newId, uint8_t *ids,
+// postCClIdReassign; also completes the labels: labels[c] held c's local
+// root, whose entry now holds the final root (the minimum cell index of the
+// component)
+__global__ __launch_bounds__(256) void k_relabel(tt_dev d, uint32_t *labels, const uint8_t *newId, uint8_t *ids,
                                                  uint8_t *unFlag)
 {
     const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
     if (c >= d.N) return;
-    const uint8_t id = newId[labels[c]];
+    const uint32_t f = labels[labels[c]];
+    labels[c] = f;
+    const uint8_t id = newId[f];
     ids[c] = id;
     unFlag[c] = id == TT_MAXP ? 1 : 0;
 }
 
-hipError_t tt_launch_relabel(hipStream_t s, const tt_dev &d, const uint32_t *labels, const uint8_t *newId,
+hipError_t tt_launch_relabel(hipStream_t s, const tt_dev &d, uint32_t *labels, const uint8_t *newId,
                              uint8_t *ids, uint8_t *unFlag)
 {
     const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
@@ -412,7 +478,8 @@ __global__ __launch_bounds__(256) void k_fill_vote(tt_dev d, const uint32_t *lis
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
     if (k >= *n) return;
     const uint32_t c = list[k];
-    const int x = (int)(c % (uint32_t)d.W), y = (int)(c / (uint32_t)d.W);
+    int x, y;
+    tt_cell_xy((uint32_t)c, d, x, y);
     const int NO[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
     uint8_t nb[8];
     int cnt = 0;
@@ -472,7 +539,8 @@ __global__ __launch_bounds__(256) void k_neighbor_matrix(tt_dev d, const uint8_t
 {
     const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
     if (c >= d.N) return;
-    const int x = (int)(c % d.W), y = (int)(c / d.W);
+    int x, y;
+    tt_cell_xy((uint32_t)c, d, x, y);
     const uint8_t cur = ids[c];
     const uint8_t r = ids[(size_t)y * d.W + tt_wrap1(x + 1, d.W)];
     const uint8_t t = ids[(size_t)tt_wrap1(y + 1, d.H) * d.W + x];
@@ -569,8 +637,43 @@ hipError_t tt_launch_merge_decide(hipStream_t s, const tt_dev &d, const uint8_t 
 #define ACC_N 10
 
 // mergeAndCountSizeMass (mode 0) / countSizePostSplit (mode 1), plus the
-// angular samples of accumulatePlateAngularCoords in mode 0.  4096 cells per
-// workgroup, per-plate partial sums in LDS.
+// angular samples of accumulatePlateAngularCoords in mode 0.  Each thread
+// takes 16 consecutive cells of one row (vector loads), sums runs of equal
+// plate id in registers and adds each run to the per-plate LDS totals; 4096
+// cells per workgroup.  All sums are integers (fixed point), so the order of
+// the additions does not matter.
+struct tt_count_run {
+    long long mass, ax, ay, s0, s1, s2, s3;
+    unsigned size, per, acnt;
+};
+
+__device__ __forceinline__ void tt_count_flush(unsigned long long *sa, int pid, tt_count_run &R, int mode)
+{
+    if (R.size == 0u) return;
+    unsigned long long *a = &sa[pid * ACC_N];
+    atomicAdd(&a[ACC_MASS], (unsigned long long)R.mass);
+    atomicAdd(&a[ACC_SIZE], (unsigned long long)R.size);
+    if (R.per) atomicAdd(&a[ACC_PER], (unsigned long long)R.per);
+    if (mode == 0) {
+        atomicAdd(&a[ACC_AX], (unsigned long long)R.ax);
+        atomicAdd(&a[ACC_AY], (unsigned long long)R.ay);
+        if (R.acnt) {
+            atomicAdd(&a[ACC_S0], (unsigned long long)R.s0);
+            atomicAdd(&a[ACC_S1], (unsigned long long)R.s1);
+            atomicAdd(&a[ACC_S2], (unsigned long long)R.s2);
+            atomicAdd(&a[ACC_S3], (unsigned long long)R.s3);
+            atomicAdd(&a[ACC_ACNT], (unsigned long long)R.acnt);
+        }
+    }
+    R = tt_count_run{0, 0, 0, 0, 0, 0, 0, 0u, 0u, 0u};
+}
+
+__device__ __forceinline__ uint8_t tt_byte(const uint4 &v, int j)
+{
+    const uint32_t w = j < 4 ? v.x : j < 8 ? v.y : j < 12 ? v.z : v.w;
+    return (uint8_t)(w >> (8 * (j & 3)));
+}
+
 __global__ __launch_bounds__(256) void k_count(tt_dev d, const int *mergeIds, const uint8_t *idsIn, uint8_t *idsOut,
                                                const float *height, const uint8_t *noise, long long *acc, int mode)
 {
@@ -581,48 +684,73 @@ __global__ __launch_bounds__(256) void k_count(tt_dev d, const int *mergeIds, co
     mmap[tid] = (mode == 0 && tid < TT_MAXP) ? mergeIds[tid] : 0;
     __syncthreads();
     const int W = d.W, H = d.H;
-    const double twopi = 6.283185307179586;
-    for (int k = 0; k < 16; ++k) {
-        const long long c = (long long)blockIdx.x * 4096 + k * 256 + tid;
-        if (c >= d.N) break;
-        const int x = (int)(c % W), y = (int)(c / W);
-        int pid = idsIn[c];
-        if (mode == 0) {
-            const int mg = mmap[pid];
-            if (mg > 0) pid = TT_MAXP - mg;
-        }
-        idsOut[c] = (uint8_t)pid;
-        atomicAdd(&sa[pid * ACC_N + ACC_MASS], (unsigned long long)tt_fix(height[c], TT_MASS_BITS));
-        atomicAdd(&sa[pid * ACC_N + ACC_SIZE], 1ull);
-        const int NX[4] = {x, tt_wrap1(x + 1, W), x, tt_wrap1(x - 1, W)};
-        const int NY[4] = {tt_wrap1(y - 1, H), y, tt_wrap1(y + 1, H), y};
-        int cnt = 0;
+    const long long c0 = (long long)blockIdx.x * 4096 + (long long)tid * 16;
+    if (c0 < d.N) {
+        int x0, y;
+        tt_cell_xy((uint32_t)c0, d, x0, y);
+        // merged id of a raw id (uint8 arithmetic of the reference)
+        auto mid = [&](int raw) -> int {
+            if (mode != 0) return raw;
+            const int mg = mmap[raw];
+            return mg > 0 ? TT_MAXP - mg : raw;
+        };
+        auto nid = [&](int raw) -> int {
+            if (mode != 0) return raw;
+            const uint8_t mm = (uint8_t)mmap[raw];
+            return mm != 0 ? (int)(uint8_t)(TT_MAXP - mm) : raw;
+        };
+        const uint4 own = *(const uint4 *)(idsIn + c0);
+        const uint4 up = *(const uint4 *)(idsIn + (size_t)tt_wrap1(y - 1, H) * W + x0);
+        const uint4 dn = *(const uint4 *)(idsIn + (size_t)tt_wrap1(y + 1, H) * W + x0);
+        const int lft = idsIn[(size_t)y * W + tt_wrap1(x0 - 1, W)];
+        const int rgt = idsIn[(size_t)y * W + tt_wrap1(x0 + 16, W)];
+        float hv[16];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            int nid = idsIn[(size_t)NY[q] * W + NX[q]];
+            const float4 f = *(const float4 *)(height + c0 + 4 * q);
+            hv[4 * q] = f.x; hv[4 * q + 1] = f.y; hv[4 * q + 2] = f.z; hv[4 * q + 3] = f.w;
+        }
+        uint32_t ow[4] = {0u, 0u, 0u, 0u};
+        tt_count_run R = tt_count_run{0, 0, 0, 0, 0, 0, 0, 0u, 0u, 0u};
+        int cur = -1;
+        const double twopi = 6.283185307179586;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int x = x0 + j;
+            const int pid = mid(tt_byte(own, j));
+            ow[j >> 2] |= (uint32_t)pid << (8 * (j & 3));
+            if (pid != cur) {
+                if (cur >= 0) tt_count_flush(sa, cur, R, mode);
+                cur = pid;
+            }
+            R.mass += tt_fix(hv[j], TT_MASS_BITS);
+            R.size += 1u;
+            const int nb0 = nid(tt_byte(up, j));
+            const int nb1 = nid(j < 15 ? (int)tt_byte(own, j + 1) : rgt);
+            const int nb2 = nid(tt_byte(dn, j));
+            const int nb3 = nid(j > 0 ? (int)tt_byte(own, j - 1) : lft);
+            const unsigned cnt = (unsigned)(nb0 != pid) + (unsigned)(nb1 != pid) + (unsigned)(nb2 != pid) +
+                                 (unsigned)(nb3 != pid);
             if (mode == 0) {
-                const uint8_t mm = (uint8_t)mmap[nid];
-                if (mm != 0) nid = (uint8_t)(TT_MAXP - mm);
+                R.per += cnt;
+                float ax, ay;
+                tt_asth_at(noise, x, y, d.axs, d.ays, ax, ay);
+                R.ax += tt_fix(ax, TT_ASTH_BITS);
+                R.ay += tt_fix(ay, TT_ASTH_BITS);
+                if (x % 10 == 0 && y % 10 == 0) {
+                    const double agx = (twopi * (double)x) / (double)W, agy = (twopi * (double)y) / (double)H;
+                    R.s0 += tt_fix_d(tt_sin_d(agx), TT_ANG_BITS);
+                    R.s1 += tt_fix_d(tt_cos_d(agx), TT_ANG_BITS);
+                    R.s2 += tt_fix_d(tt_sin_d(agy), TT_ANG_BITS);
+                    R.s3 += tt_fix_d(tt_cos_d(agy), TT_ANG_BITS);
+                    R.acnt += 1u;
+                }
+            } else if (cnt) {
+                R.per += 1u;
             }
-            cnt += nid != pid;
         }
-        if (mode == 0) {
-            if (cnt) atomicAdd(&sa[pid * ACC_N + ACC_PER], (unsigned long long)cnt);
-            float ax, ay;
-            tt_asth_at(noise, x, y, d.axs, d.ays, ax, ay);
-            atomicAdd(&sa[pid * ACC_N + ACC_AX], (unsigned long long)tt_fix(ax, TT_ASTH_BITS));
-            atomicAdd(&sa[pid * ACC_N + ACC_AY], (unsigned long long)tt_fix(ay, TT_ASTH_BITS));
-            if (x % 10 == 0 && y % 10 == 0) {
-                const double agx = (twopi * (double)x) / (double)W, agy = (twopi * (double)y) / (double)H;
-                atomicAdd(&sa[pid * ACC_N + ACC_S0], (unsigned long long)tt_fix_d(tt_sin_d(agx), TT_ANG_BITS));
-                atomicAdd(&sa[pid * ACC_N + ACC_S1], (unsigned long long)tt_fix_d(tt_cos_d(agx), TT_ANG_BITS));
-                atomicAdd(&sa[pid * ACC_N + ACC_S2], (unsigned long long)tt_fix_d(tt_sin_d(agy), TT_ANG_BITS));
-                atomicAdd(&sa[pid * ACC_N + ACC_S3], (unsigned long long)tt_fix_d(tt_cos_d(agy), TT_ANG_BITS));
-                atomicAdd(&sa[pid * ACC_N + ACC_ACNT], 1ull);
-            }
-        } else if (cnt) {
-            atomicAdd(&sa[pid * ACC_N + ACC_PER], 1ull);
-        }
+        tt_count_flush(sa, cur, R, mode);
+        *(uint4 *)(idsOut + c0) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
     }
     __syncthreads();
     for (int i = tid; i < 256 * ACC_N; i += 256)

@@ -38,7 +38,8 @@ __global__ __launch_bounds__(256) void k_propagate_local(tt_dev d, const uint32_
 {
     const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
     if (c >= d.N) return;
-    const int x = (int)(c % d.W), y = (int)(c / d.W);
+    int x, y;
+    tt_cell_xy((uint32_t)c, d, x, y);
     const uint8_t sf = srcFlag[c];
     const uint8_t ct = contested[c];
     tt_prop P;
@@ -93,7 +94,8 @@ __global__ void k_propagate_serial(tt_dev d, const uint32_t *list, const uint32_
     const uint32_t n = *count;
     for (uint32_t li = 0; li < n; ++li) {
         const uint32_t c = list[li];
-        const int x = (int)(c % (uint32_t)d.W), y = (int)(c / (uint32_t)d.W);
+        int x, y;
+    tt_cell_xy((uint32_t)c, d, x, y);
         const uint32_t cv = word[c];
         const tt_uplift U = uplift[c];
         const int prim = tt_slot_plate(cv, (int)((U.types >> 1) & 3u));
@@ -153,7 +155,8 @@ __global__ __launch_bounds__(256) void k_combine_contested(tt_dev d, const uint3
     if (c < d.N) {
         remote = srcFlag[c] == 2;
         if (contested[c]) {
-            const int x = (int)(c % d.W), y = (int)(c / d.W);
+            int x, y;
+    tt_cell_xy((uint32_t)c, d, x, y);
             const tt_prop P = tt_combine(prop[c], uplift[c], word[c]);
             prop[c] = P;
             jfa[c] = tt_seed_of(P, x, y);
@@ -185,26 +188,47 @@ __device__ __forceinline__ int tt_tdist2(int ax, int ay, int bx, int by, int W, 
     return dx * dx + dy * dy;
 }
 
-__global__ __launch_bounds__(256) void k_jfa_pass(tt_dev d, const int *in, int *out, int step)
+// Squared toroidal distance between two cells packed as (y << 16) | x, with
+// packed 16-bit arithmetic: |a - b| per component, min(d, size - d), then
+// dx * dx + dy * dy by one 2-way dot product (exact: components < 2^14).
+typedef short tt_s2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ int tt_tdist2_pk(int p, int s, int wh)
 {
-    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (c >= d.N) return;
-    const int x = (int)(c % d.W), y = (int)(c / d.W);
+    const tt_s2 a = __builtin_bit_cast(tt_s2, p), b = __builtin_bit_cast(tt_s2, s);
+    const tt_s2 n = __builtin_bit_cast(tt_s2, wh);
+    tt_s2 dd = a - b;
+    dd = __builtin_elementwise_max(dd, -dd);
+    dd = __builtin_elementwise_min(dd, n - dd);
+    return __builtin_amdgcn_sdot2(dd, dd, 0, false);
+}
+
+// JFAPass (plate_tectonics_kernel.cu:1512-1571): candidates at (ox, oy) * step
+// in scan order oy = -1..1, ox = -1..1; a strictly smaller squared distance
+// replaces the current seed.
+__global__ __launch_bounds__(256) void k_jfa_pass(tt_dev d, const int *__restrict__ in, int *__restrict__ out, int step)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= (uint32_t)d.N) return;
+    int x, y;
+    tt_cell_xy(c, d, x, y);
+    const int W = d.W, H = d.H;
+    const int me = (y << 16) | x, wh = (H << 16) | W;
     int best = in[c];
     int bd = 0x7FFFFFFF;
-    if (best >= 0) bd = tt_tdist2(x, y, best & 0xFFFF, best >> 16, d.W, d.H);
+    if (best >= 0) bd = tt_tdist2_pk(me, best, wh);
     // (x + k * step) mod W == (x + k * (step mod W)) mod W, and the reduced
     // offset keeps the single add / subtract wrap in range
-    const int sx = step % d.W, sy = step % d.H;
+    const int sx = step % W, sy = step % H;
+    const int rows[3] = {tt_wrap1(y - sy, H) * W, y * W, tt_wrap1(y + sy, H) * W};
+    const int cols[3] = {tt_wrap1(x - sx, W), x, tt_wrap1(x + sx, W)};
 #pragma unroll
-    for (int oy = -1; oy <= 1; ++oy) {
-        const size_t row = (size_t)tt_wrap1(y + oy * sy, d.H) * d.W;
+    for (int oy = 0; oy < 3; ++oy) {
 #pragma unroll
-        for (int ox = -1; ox <= 1; ++ox) {
-            if (ox == 0 && oy == 0) continue;
-            const int cand = in[row + tt_wrap1(x + ox * sx, d.W)];
+        for (int ox = 0; ox < 3; ++ox) {
+            if (ox == 1 && oy == 1) continue;
+            const int cand = in[rows[oy] + cols[ox]];
             if (cand < 0) continue;
-            const int dd = tt_tdist2(x, y, cand & 0xFFFF, cand >> 16, d.W, d.H);
+            const int dd = tt_tdist2_pk(me, cand, wh);
             if (dd < bd) { bd = dd; best = cand; }
         }
     }
@@ -238,7 +262,8 @@ __global__ __launch_bounds__(256) void k_uplift(tt_dev d, const int *jfa, const 
 {
     const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
     if (c >= d.N) return;
-    const int x = (int)(c % d.W), y = (int)(c / d.W);
+    int x, y;
+    tt_cell_xy((uint32_t)c, d, x, y);
     const int s = jfa[c];
     if (s < 0) {
         tt_dist z;

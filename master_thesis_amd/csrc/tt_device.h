@@ -119,6 +119,30 @@ struct tt_dev {
     int W, H, N;
     uint32_t iter;
     int seed;
+    uint32_t wmul;          // ceil(2^(31 + l) / W), l = ceil(log2 W)
+    int wsh;                // l - 1
     double axs, ays;        // 256 / W, 256 / H (noise scale)
     tt_kernel_settings ks;
 };
+
+// Row and column of a cell index c < 2^31 without a division: with
+// m = ceil(2^(31+l) / W) and 2^(l-1) < W <= 2^l, m W - 2^(31+l) < W <= 2^l,
+// so floor(c / W) = floor(c m / 2^(31+l)) (Granlund-Montgomery) for all
+// c < 2^31.
+__device__ __forceinline__ void tt_cell_xy(uint32_t c, const tt_dev &d, int &x, int &y)
+{
+    const uint32_t q = __umulhi(c, d.wmul) >> d.wsh;
+    y = (int)q;
+    x = (int)(c - q * (uint32_t)d.W);
+}
+
+// host side: the constants of tt_cell_xy
+static inline void tt_dev_set_divisor(tt_dev &d)
+{
+    int l = 0;
+    while ((1 << l) < d.W) ++l;
+    if (l == 0) l = 1;
+    const unsigned long long two = 1ull << (31 + l);
+    d.wmul = (uint32_t)((two + (unsigned long long)d.W - 1ull) / (unsigned long long)d.W);
+    d.wsh = l - 1;
+}

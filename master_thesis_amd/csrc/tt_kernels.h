@@ -65,10 +65,11 @@ hipError_t tt_launch_plate_movement(hipStream_t s, const tt_dev &d, tt_plate_dat
                                     const uint8_t *noise);
 hipError_t tt_launch_accretion_apply(hipStream_t s, const tt_dev &d, const uint32_t *accw, const float *height,
                                      const uint8_t *idsIn, uint8_t *idsOut, uint8_t *accretion);
-hipError_t tt_launch_ccl_local(hipStream_t s, const tt_dev &d, const uint8_t *ids, uint32_t *labels);
+hipError_t tt_launch_ccl_local(hipStream_t s, const tt_dev &d, const uint8_t *ids, uint32_t *labels, uint8_t *rootFlag,
+                               uint32_t *rootCount);
 hipError_t tt_launch_ccl_merge(hipStream_t s, const tt_dev &d, const uint8_t *ids, uint32_t *labels);
-hipError_t tt_launch_ccl_flatten(hipStream_t s, const tt_dev &d, uint32_t *labels, uint32_t *rootCount,
-                                 uint8_t *rootFlag);
+hipError_t tt_launch_ccl_flatten(hipStream_t s, const uint32_t *locList, const uint32_t *nloc, uint32_t *labels,
+                                 uint32_t *rootCount, uint8_t *rootFlag);
 hipError_t tt_launch_frag_build(hipStream_t s, const tt_dev &d, const uint32_t *rootList, const uint32_t *nroots,
                                 uint32_t nroots_host, const uint32_t *rootCount, const uint8_t *ids, unsigned long long *fragKey,
                                 int *maxCount, uint8_t *newId);
@@ -76,7 +77,7 @@ hipError_t tt_launch_frag_sort(hipStream_t s, unsigned long long *fragKey, const
 hipError_t tt_launch_frag_assign(hipStream_t s, const tt_dev &d, const unsigned long long *fragKey,
                                  const uint32_t *nroots, const int *maxCount, const uint8_t *ids, tt_plate_data *pd,
                                  uint8_t *newId);
-hipError_t tt_launch_relabel(hipStream_t s, const tt_dev &d, const uint32_t *labels, const uint8_t *newId,
+hipError_t tt_launch_relabel(hipStream_t s, const tt_dev &d, uint32_t *labels, const uint8_t *newId,
                              uint8_t *ids, uint8_t *unFlag);
 hipError_t tt_launch_fill_vote(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n,
                                const uint8_t *ids, uint8_t *vote);

@@ -61,6 +61,7 @@ struct tt_world {
 
     // CCL / relabel
     uint32_t *labels = nullptr, *rootCount = nullptr, *rootList = nullptr, *nroots = nullptr;
+    uint32_t *locList = nullptr, *nloc = nullptr;  // local roots of the tiles
     uint8_t *rootFlag = nullptr, *newId = nullptr, *unFlag = nullptr, *vote = nullptr;
     unsigned long long *fragKey = nullptr;
     int *maxCount = nullptr;

@@ -133,6 +133,8 @@ size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w)
     take((void **)&w->rootCount, N * 4);
     take((void **)&w->rootFlag, N);
     take((void **)&w->rootList, N * 4);
+    take((void **)&w->locList, N * 4);
+    take((void **)&w->nloc, 16);
     take((void **)&w->nroots, 16);
     take((void **)&w->fragKey, N * 8);
     take((void **)&w->maxCount, 256 * 4);
@@ -168,6 +170,7 @@ static tt_dev make_dev(const tt_world *w)
     d.axs = 256.0 / (double)w->W;
     d.ays = 256.0 / (double)w->H;
     d.ks = w->ks;
+    tt_dev_set_divisor(d);
     return d;
 }
 
@@ -195,10 +198,12 @@ static int run_ccl(tt_world *w)
     const tt_dev d = make_dev(w);
     uint8_t *ids = w->ids[w->ic];
     const size_t N = (size_t)w->N;
-    CKS(zero(w, w->rootCount, N * 4));
-    CK(tt_launch_ccl_local(s, d, ids, w->labels));
+    // local roots per tile, unions across tile borders, final roots of the
+    // local roots; per-cell labels are completed by the relabel pass
+    CK(tt_launch_ccl_local(s, d, ids, w->labels, w->rootFlag, w->rootCount));
     CK(tt_launch_ccl_merge(s, d, ids, w->labels));
-    CK(tt_launch_ccl_flatten(s, d, w->labels, w->rootCount, w->rootFlag));
+    CK(tt_compact_u8(s, w->rootFlag, (long long)N, w->locList, w->nloc, w->scanTmp));
+    CK(tt_launch_ccl_flatten(s, w->locList, w->nloc, w->labels, w->rootCount, w->rootFlag));
     CK(tt_compact_u8(s, w->rootFlag, (long long)N, w->rootList, w->nroots, w->scanTmp));
     uint32_t nroots = 0;
     CKS(read_u32(w, w->nroots, &nroots));
