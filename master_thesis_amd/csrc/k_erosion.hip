@@ -33,17 +33,48 @@ __global__ __launch_bounds__(256) void k_thermal(tt_dev d, const float *__restri
     const int b = tt_xcd_tile(blockIdx.x, nbx * nby);
     const int x0 = (b % nbx) * TT_TILE_W, y0 = (b / nbx) * TT_TILE_H;
     const int lane = tid & 63, wv = tid >> 6;
+    __shared__ float rmin[4], rmax[4];
+    float mn = tt_u2f(0x7f800000u), mx = tt_u2f(0xff800000u);
     {
         const int gxa = tt_wrap1(x0 + lane - 2, W), gxb = tt_wrap1(x0 + 64 + lane - 2, W);
         for (int ly = wv; ly < TH_H; ly += 4) {
             const int row = tt_wrap1(y0 + ly - 2, H) * W;
-            hs[ly][lane] = hin[row + gxa];
-            if (lane < TH_W - 64) hs[ly][64 + lane] = hin[row + gxb];
+            const float a = hin[row + gxa];
+            hs[ly][lane] = a;
+            mn = tt_fminf(mn, a);
+            mx = tt_fmaxf(mx, a);
+            if (lane < TH_W - 64) {
+                const float b = hin[row + gxb];
+                hs[ly][64 + lane] = b;
+                mn = tt_fminf(mn, b);
+                mx = tt_fmaxf(mx, b);
+            }
         }
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = tt_fminf(mn, __shfl_xor(mn, o, 64));
+        mx = tt_fmaxf(mx, __shfl_xor(mx, o, 64));
+    }
+    if (lane == 0) { rmin[wv] = mn; rmax[wv] = mx; }
     __syncthreads();
     const float talus = d.ks.thermalThresholdAngle * d.ks.thermalCellSize;
     const float amp = d.ks.thermalErosionAmplitude, str = d.ks.thermalErosionStrength;
+    // No transfer anywhere in the tile when the tile plus its two-cell halo
+    // spans at most the talus: every difference h_a - h_b rounds to at most
+    // max - min (rounding is monotone), so no "diff > talus" test can pass and
+    // every cell keeps its height exactly.
+    {
+        const float tmn = tt_fminf(tt_fminf(rmin[0], rmin[1]), tt_fminf(rmin[2], rmin[3]));
+        const float tmx = tt_fmaxf(tt_fmaxf(rmax[0], rmax[1]), tt_fmaxf(rmax[2], rmax[3]));
+        if (tmx - tmn <= talus) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int ry = wv * 4 + r;
+                hout[(y0 + ry) * W + x0 + lane] = hs[ry + 2][lane + 2];
+            }
+            return;
+        }
+    }
     // sum of the downhill differences above the talus, on the tile + 1 halo
     for (int i = tid; i < TT_H * TT_W; i += 256) {
         const int ty = i / TT_W, tx = i - ty * TT_W;

@@ -152,6 +152,15 @@ def test_stage_masks(sim, O, mask):
     run_parity(sim, O, cfg, ks, 3, f"mask {mask:#x}")
 
 
+@pytest.mark.parametrize("talus", [1e-4, 4e-3])
+def test_thermal_transfers(sim, O, talus):
+    """Talus far below the height differences: transfers in every tile (1e-4)
+    or in some tiles only (4e-3), next to tiles that take the no-transfer path."""
+    cfg, ks = configs.synthetic(192, 128, 5, (0.5, 3.0), 6, hydraulic=3, thermal=6, ccl_every=1)
+    ks.thermalThresholdAngle = talus
+    run_parity(sim, O, cfg, ks, 3, f"thermal talus {talus}")
+
+
 def test_zero_sub_iterations_and_ccl_period(sim, O):
     cfg, ks = configs.synthetic(128, 128, 7, (0.5, 3.0), 5, hydraulic=0, thermal=0, ccl_every=3)
     run_parity(sim, O, cfg, ks, 4, "K=0 T=0")
