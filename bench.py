@@ -70,6 +70,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile-rows", type=int, default=0, help="erosion tile rows (0: autotuned)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -96,6 +97,7 @@ def main() -> int:
     cfg, ks = configs.world_4096(seed=args.seed)
     if args.side != 4096:
         cfg.width = cfg.height = args.side
+    cfg.tile_rows = args.tile_rows
     N = cfg.width * cfg.height
     world = World(cfg, ks).init()
     init_state = None
@@ -161,7 +163,7 @@ def main() -> int:
             "frac_of_hbm": model_b * N * args.steps / wall / (HBM_PEAK_GBS * 1e9),
         },
         "final_stats": {"plates": stats.number_of_plates, "labels": stats.num_labels,
-                        "iteration": stats.iteration},
+                        "iteration": stats.iteration, "tile_rows": stats.tile_rows},
     }
     world.close()
 

@@ -142,7 +142,9 @@ typedef struct tt_sim_config {
     float speed_min;               /* TT_INIT_SYNTHETIC plate speeds */
     float speed_max;
     int32_t stage_mask;            /* TT_STAGE_* */
-    int32_t reserved[7];
+    int32_t tile_rows;             /* rows of the hydraulic tiles: 8, 16 or 32 (dividing height);
+                                      0 = chosen by the tile autotuner (thermal tiles: 16 rows) */
+    int32_t reserved[6];
 } tt_sim_config;
 
 /* struct PlateData (src/cuda_compute/types/plate_data.h:5-38), 84 bytes, same
@@ -210,7 +212,8 @@ typedef struct tt_step_stats {
     int32_t split_new_plate;
     int32_t split_iterations;    /* relaxation sweeps until convergence */
     int32_t complex_sources;     /* continental sources handled in canonical serial order */
-    int32_t reserved[3];
+    int32_t tile_rows;           /* rows of the hydraulic tiles in use (tt_sim_config.tile_rows or tuned) */
+    int32_t reserved[2];
     double mass_ledger;          /* sum of height over the grid, exact fixed-point, fp64 */
 } tt_step_stats;
 

@@ -142,7 +142,8 @@ class SimConfig(C.Structure):
         ("speed_min", C.c_float),
         ("speed_max", C.c_float),
         ("stage_mask", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("tile_rows", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
     @classmethod
@@ -205,7 +206,8 @@ class StepStats(C.Structure):
         ("split_new_plate", C.c_int32),
         ("split_iterations", C.c_int32),
         ("complex_sources", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("tile_rows", C.c_int32),
+        ("reserved", C.c_int32 * 2),
         ("mass_ledger", C.c_double),
     ]
 

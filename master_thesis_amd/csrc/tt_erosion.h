@@ -1,0 +1,355 @@
+// tt_erosion.h -- tile bodies of stage D (thermal erosion) and stage E
+// (hydraulic erosion), shared by the one-world kernels (k_erosion.hip) and the
+// batched kernels over many worlds (k_batch.hip).
+//
+// Tiles are 64 columns by TH rows (TH = 8, 16 or 32), 256 threads: lane l of
+// wavefront w owns column l and rows w * TH/4 .. w * TH/4 + TH/4 - 1.  The
+// kernel owns the LDS and passes it in, so a kernel that can run several tile
+// variants still allocates it once.
+//
+// Thermal: thermalErosionKernel (plate_tectonics_kernel.cu:2735-2786) in the
+// canonical Jacobi form -- each pass reads the previous heights; a cell's new
+// height is its old height minus its own outflows (directions 0..7) plus the
+// inflows it receives (directions 0..7).  Two-cell halo.
+//
+// Hydraulic: one pass per sub-iteration fuses rain (:2430-2437), flux
+// (:2439-2485), flow (:2487-2533), evaporate (:2587-2593) and the
+// erosion/deposition exchange (:2535-2570 with CudaTexture::Slope), and begins
+// with the semi-Lagrangian transport (:2572-2585, cuda_helper.cuh:49-65) of the
+// previous sub-iteration, gathered from its sediment and velocity.  The order
+// of float operations is the reference's; dt = 1 (plate_tectonic_sim.cu:338-363).
+#pragma once
+
+#include "tt_device.h"
+
+#define TT_EW (TT_TILE_W + 4)   // tile width with a two-cell halo
+#define TT_FW (TT_TILE_W + 2)   // tile width with a one-cell halo
+
+__device__ __constant__ static const int c_toff[8][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1},
+                                                         {-1, -1}, {-1, 1}, {1, -1}, {1, 1}};
+
+// ---------------------------------------------------------------------------
+// thermal
+// ---------------------------------------------------------------------------
+struct tt_therm_args {
+    int W, H;
+    float talus, amp, str;
+    const float *hin;
+    float *hout;
+};
+
+template <int TH>
+struct tt_therm_lds {
+    float hs[TH + 4][TT_EW];
+    float tot[TH + 2][TT_FW];
+    float rmin[4], rmax[4];
+};
+
+__host__ __device__ inline tt_therm_args tt_therm_make(const tt_dev &d, const float *hin, float *hout)
+{
+    tt_therm_args A;
+    A.W = d.W;
+    A.H = d.H;
+    A.talus = d.ks.thermalThresholdAngle * d.ks.thermalCellSize;
+    A.amp = d.ks.thermalErosionAmplitude;
+    A.str = d.ks.thermalErosionStrength;
+    A.hin = hin;
+    A.hout = hout;
+    return A;
+}
+
+template <int TH>
+__device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm_lds<TH> &S, int x0, int y0)
+{
+    constexpr int RPW = TH / 4;
+    const int tid = threadIdx.x;
+    const int W = A.W, H = A.H;
+    const int lane = tid & 63, wv = tid >> 6;
+    float mn = tt_u2f(0x7f800000u), mx = tt_u2f(0xff800000u);
+    {
+        const int gxa = tt_wrap1(x0 + lane - 2, W), gxb = tt_wrap1(x0 + 64 + lane - 2, W);
+        for (int ly = wv; ly < TH + 4; ly += 4) {
+            const int row = tt_wrap1(y0 + ly - 2, H) * W;
+            const float a = A.hin[row + gxa];
+            S.hs[ly][lane] = a;
+            mn = tt_fminf(mn, a);
+            mx = tt_fmaxf(mx, a);
+            if (lane < TT_EW - 64) {
+                const float b = A.hin[row + gxb];
+                S.hs[ly][64 + lane] = b;
+                mn = tt_fminf(mn, b);
+                mx = tt_fmaxf(mx, b);
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = tt_fminf(mn, __shfl_xor(mn, o, 64));
+        mx = tt_fmaxf(mx, __shfl_xor(mx, o, 64));
+    }
+    if (lane == 0) { S.rmin[wv] = mn; S.rmax[wv] = mx; }
+    __syncthreads();
+    const float talus = A.talus, amp = A.amp, str = A.str;
+    // No transfer anywhere in the tile when the tile plus its two-cell halo
+    // spans at most the talus: every difference h_a - h_b rounds to at most
+    // max - min (rounding is monotone), so no "diff > talus" test can pass and
+    // every cell keeps its height exactly.
+    {
+        const float tmn = tt_fminf(tt_fminf(S.rmin[0], S.rmin[1]), tt_fminf(S.rmin[2], S.rmin[3]));
+        const float tmx = tt_fmaxf(tt_fmaxf(S.rmax[0], S.rmax[1]), tt_fmaxf(S.rmax[2], S.rmax[3]));
+        if (tmx - tmn <= talus) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int ry = wv * RPW + r;
+                A.hout[(y0 + ry) * W + x0 + lane] = S.hs[ry + 2][lane + 2];
+            }
+            return;
+        }
+    }
+    // sum of the downhill differences above the talus, on the tile + 1 halo
+    for (int i = tid; i < (TH + 2) * TT_FW; i += 256) {
+        const int ty = i / TT_FW, tx = i - ty * TT_FW;
+        const int ly = ty + 1, lx = tx + 1;
+        const float hc = S.hs[ly][lx];
+        float nb[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) nb[k] = S.hs[ly + c_toff[k][1]][lx + c_toff[k][0]];
+        float t = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float diff = hc - nb[k];
+            t = (diff > talus) ? t + diff : t;
+        }
+        S.tot[ty][tx] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int ry = wv * RPW + r;
+        const int ly = ry + 2, lx = lane + 2;
+        const float hc = S.hs[ly][lx];
+        const float tc = S.tot[ry + 1][lane + 1];
+        float dOut[8], dIn[8], tIn[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            dOut[k] = hc - S.hs[ly + c_toff[k][1]][lx + c_toff[k][0]];
+            const int sy = ly - c_toff[k][1], sx = lx - c_toff[k][0];
+            dIn[k] = S.hs[sy][sx] - hc;
+            tIn[k] = S.tot[sy - 1][sx - 1];
+        }
+        // own outflows (directions 0..7), then inflows (directions 0..7); the
+        // divisions sit in branches that a wavefront skips when no lane needs them
+        float v = hc;
+        if (tc > 0.0f) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (dOut[k] > talus) v = v - (amp * (dOut[k] / tc)) * str;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (dIn[k] > talus && tIn[k] > 0.0f) v = v + (amp * (dIn[k] / tIn[k])) * str;
+        A.hout[(y0 + ry) * W + x0 + lane] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// hydraulic
+// ---------------------------------------------------------------------------
+struct tt_hyd_args {
+    int W, H, rseed;
+    float rain, evap, cap, ke, kd, fk, g, L;
+    const float *hin;
+    float *hout;
+    const float *win;
+    float *wout;
+    const float4 *fin;
+    float4 *fout;
+    float2 *vel;
+    const float *sprev;
+    float *sout;
+};
+
+template <int TH>
+struct tt_hyd_lds {
+    float M[TH + 4][TT_EW];
+    float Wt[TH + 4][TT_EW];
+    float4 F[TH + 2][TT_FW];
+};
+
+__host__ __device__ inline void tt_hyd_settings(const tt_dev &d, int rseed, tt_hyd_args &A)
+{
+    A.W = d.W;
+    A.H = d.H;
+    A.rseed = rseed;
+    A.rain = d.ks.hydrationRainfall;
+    A.evap = d.ks.hydrationEvaporation;
+    A.cap = d.ks.sedimentCapacity;
+    A.ke = d.ks.sedimentErosionRate;
+    A.kd = d.ks.sedimentDepositionRate;
+    A.fk = 1.0f * d.ks.hydrationPipeCrossSection;
+    A.g = d.ks.gravity;
+    A.L = d.ks.hydrationPipeLength;
+}
+
+// bilinear sample with toroidal wrap (interpolate, cuda_helper.cuh:49-65).
+// |v| <= 50 keeps the sample column within [-51, W + 50], where a single
+// conditional wrap is exact; other columns (non-finite velocities) take the
+// general modulo.  IEEE floor is exact, so the hardware floor equals the
+// oracle's bit-level floor.
+__device__ __forceinline__ int tt_wrap_near(int v, int n)
+{
+    return ((unsigned)(v + n) < (unsigned)(3 * n)) ? tt_wrap1(v, n) : tt_wrapg(v, n);
+}
+
+__device__ __forceinline__ float tt_bilinear(const float *__restrict__ s, float tx, float ty, int W, int H)
+{
+    const float fx = __builtin_floorf(tx), fy = __builtin_floorf(ty);
+    const int x0 = (int)fx, y0 = (int)fy;
+    const float wx = tx - (float)x0, wy = ty - (float)y0;
+    const int xa = tt_wrap_near(x0, W), xb = tt_wrap_near(x0 + 1, W);
+    const int ra = tt_wrap_near(y0, H) * W, rb = tt_wrap_near(y0 + 1, H) * W;
+    const float I0 = (1.0f - wx) * s[ra + xa] + wx * s[ra + xb];
+    const float I1 = (1.0f - wx) * s[rb + xa] + wx * s[rb + xb];
+    return (1.0f - wy) * I0 + wy * I1;
+}
+
+// rain amount of a cell: max(0, discreteNoise((x, 0, y), 1, seed + iter)) * dt * rain
+__device__ __forceinline__ float tt_rain_at(int gx, int gy, int rseed, float rain)
+{
+    const float n = tt_discrete_noise((float)gx, 0.0f, (float)gy, 1.0f, rseed);
+    return (tt_fmaxf(0.0f, n) * 1.0f) * rain;
+}
+
+// One sub-iteration on one tile.  FIRST: the water input has no rain yet
+// (rain is added while loading) and the sediment is the step's state;
+// otherwise the water input already holds this sub-iteration's rain (added by
+// the previous one on its final write) and the sediment is the previous
+// sub-iteration's output transported along its velocity.  LAST: the water
+// output gets no rain.
+template <int TH, bool FIRST, bool LAST>
+__device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_lds<TH> &S, int x0, int y0)
+{
+    constexpr int RPW = TH / 4;
+    const int tid = threadIdx.x;
+    const int W = A.W, H = A.H;
+    const int lane = tid & 63, wv = tid >> 6;
+    const float rain = A.rain;
+    const int rseed = A.rseed;
+    // heights and water on the tile plus a two-cell halo: rows wv, wv+4, ...;
+    // lanes 0..63 take columns 0..63, lanes 0..3 also columns 64..67
+    {
+        const int gxa = tt_wrap1(x0 + lane - 2, W);
+        const int gxb = tt_wrap1(x0 + 64 + lane - 2, W);
+        for (int ly = wv; ly < TH + 4; ly += 4) {
+            const int gy = tt_wrap1(y0 + ly - 2, H);
+            const int row = gy * W;
+            float m = A.hin[row + gxa], w = A.win[row + gxa];
+            if (FIRST) w = w + tt_rain_at(gxa, gy, rseed, rain);
+            S.M[ly][lane] = m;
+            S.Wt[ly][lane] = w;
+            if (lane < TT_EW - 64) {
+                float m2 = A.hin[row + gxb], w2 = A.win[row + gxb];
+                if (FIRST) w2 = w2 + tt_rain_at(gxb, gy, rseed, rain);
+                S.M[ly][64 + lane] = m2;
+                S.Wt[ly][64 + lane] = w2;
+            }
+        }
+    }
+    __syncthreads();
+    // outflow flux on the tile plus a one-cell halo
+    const float fk = A.fk, g = A.g, L = A.L;
+    for (int i = tid; i < (TH + 2) * TT_FW; i += 256) {
+        const int fy = i / TT_FW, fx = i - fy * TT_FW;
+        const int ly = fy + 1, lx = fx + 1;
+        const int gx = tt_wrap1(x0 + fx - 1, W), gy = tt_wrap1(y0 + fy - 1, H);
+        const float4 cf = A.fin[gy * W + gx];
+        const float mc = S.M[ly][lx], wc = S.Wt[ly][lx];
+        const float sc = mc + wc;
+        float4 o;
+        o.x = tt_fmaxf(0.0f, cf.x + fk * ((g * ((sc - S.M[ly][lx - 1]) - S.Wt[ly][lx - 1])) / L));
+        o.y = tt_fmaxf(0.0f, cf.y + fk * ((g * ((sc - S.M[ly + 1][lx]) - S.Wt[ly + 1][lx])) / L));
+        o.z = tt_fmaxf(0.0f, cf.z + fk * ((g * ((sc - S.M[ly][lx + 1]) - S.Wt[ly][lx + 1])) / L));
+        o.w = tt_fmaxf(0.0f, cf.w + fk * ((g * ((sc - S.M[ly - 1][lx]) - S.Wt[ly - 1][lx])) / L));
+        const float tot = (((o.x + o.y) + o.z) + o.w) + 1e-6f;
+        float K = tt_fminf(1.0f, wc / (tot * 1.0f + 1e-6f));
+        K = tt_fmaxf(K, 0.01f);
+        o.x = o.x * K;
+        o.y = o.y * K;
+        o.z = o.z * K;
+        o.w = o.w * K;
+        S.F[fy][fx] = o;
+    }
+    __syncthreads();
+    const float evap = A.evap, cap = A.cap, ke = A.ke, kd = A.kd;
+    const int gx = x0 + lane;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int ry = wv * RPW + r;
+        const int gy = y0 + ry;
+        const int c = gy * W + gx;
+        const int fy = ry + 1, fx = lane + 1;
+        const float4 Sf = S.F[fy][fx], Lf = S.F[fy][fx - 1], Uf = S.F[fy + 1][fx], Rf = S.F[fy][fx + 1],
+                     Df = S.F[fy - 1][fx];
+        float in = 0.0f;
+        in = in + Lf.z;
+        in = in + Uf.w;
+        in = in + Rf.x;
+        in = in + Df.y;
+        const float out = ((Sf.x + Sf.y) + Sf.z) + Sf.w;
+        const float dv = 1.0f * (in - out);
+        const int ly = ry + 2, lx = lane + 2;
+        const float wn = tt_fmaxf(0.0f, S.Wt[ly][lx] + dv);
+        const float nx = (((Lf.z - Sf.x) + Sf.z) - Rf.x) * 0.5f;
+        const float ny = (((Uf.w - Sf.y) + Sf.y) - Df.w) * 0.5f;
+        const float hh = tt_fmaxf(wn, 0.01f);
+        float vx = nx / hh, vy = ny / hh;
+        const float vm = sqrtf(vx * vx + vy * vy);
+        float vmag = vm;
+        if (vm > 50.0f) {
+            const float sc = 50.0f / vm;
+            vx = vx * sc;
+            vy = vy * sc;
+            vmag = sqrtf(vx * vx + vy * vy);
+        }
+        float wo = tt_fmaxf(0.0f, wn - (tt_fminf(wn, 20.0f) * evap) * 1.0f);
+        if (!LAST) wo = wo + tt_rain_at(gx, gy, rseed, rain);
+        A.wout[c] = wo;
+        A.fout[c] = Sf;
+        float sed;
+        if (FIRST) {
+            sed = A.sprev[c];
+        } else {
+            const float2 pv = A.vel[c];
+            sed = tt_bilinear(A.sprev, (float)gx - pv.x * 1.0f, (float)gy - pv.y * 1.0f, W, H);
+        }
+        A.vel[c] = make_float2(vx, vy);
+        const float mc = S.M[ly][lx];
+        // Slope: (m[+1] - m[-1]) / 2.0 in the reference; halving is exact in float
+        const float dzdx = (S.M[ly][lx + 1] - S.M[ly][lx - 1]) * 0.5f;
+        const float dzdy = (S.M[ly + 1][lx] - S.M[ly - 1][lx]) * 0.5f;
+        const float slope = sqrtf(dzdx * dzdx + dzdy * dzdy);
+        float C = (cap * tt_sinf(tt_fmaxf(slope, 0.03f))) * vmag;
+        C = tt_fminf(C, 5.0f);
+        float hnew;
+        if (C > sed) {
+            float s = (ke * (C - sed)) * 1.0f;
+            s = tt_fminf(s, mc);
+            hnew = mc - s;
+            sed = sed + s;
+        } else {
+            float s = (kd * (sed - C)) * 1.0f;
+            s = tt_fminf(s, sed);
+            sed = sed - s;
+            hnew = mc + s;
+        }
+        A.hout[c] = hnew;
+        A.sout[c] = sed;
+    }
+}
+
+// closing transport of the step (the last sub-iteration's sediment along its velocity)
+__device__ __forceinline__ void tt_transport_cell(int W, int H, uint32_t c, int x, int y, const float *sprev,
+                                                  const float2 *vel, float *sout)
+{
+    const float2 v = vel[c];
+    sout[c] = tt_bilinear(sprev, (float)x - v.x * 1.0f, (float)y - v.y * 1.0f, W, H);
+}

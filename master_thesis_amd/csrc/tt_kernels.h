@@ -54,10 +54,11 @@ hipError_t tt_launch_uplift(hipStream_t s, const tt_dev &d, const int *jfa, cons
                             const tt_plate_step *ps, float *hW, tt_dist *dist);
 
 // ---- stage D/E (k_erosion.hip) -----------------------------------------------
-hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, const float *hin, float *hout);
-hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int first, int last, int rseed, const float *hin,
-                               float *hout, const float *win, float *wout, const float4 *fin, float4 *fout,
-                               float2 *vel, const float *sprev, float *sout);
+int tt_tile_rows_valid(int rows, int H);
+hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, int rows, const float *hin, float *hout);
+hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int rows, int first, int last, int rseed,
+                               const float *hin, float *hout, const float *win, float *wout, const float4 *fin,
+                               float4 *fout, float2 *vel, const float *sprev, float *sout);
 hipError_t tt_launch_transport(hipStream_t s, const tt_dev &d, const float *sprev, const float2 *vel, float *sout);
 
 // ---- stages F/G (k_plates.hip) ----------------------------------------------

@@ -23,6 +23,7 @@ struct tt_world {
     uint32_t iter = 0;
     bool initialized = false;
     int device = 0;
+    int rows = 0;            // rows of the erosion tiles (0 until resolved)
     hipStream_t stream = nullptr;
     char *base = nullptr;
     std::vector<tt_alloc> allocs;

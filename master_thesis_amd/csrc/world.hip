@@ -12,8 +12,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
 #include <mutex>
+#include <sstream>
 #include <new>
 #include <string>
 #include <vector>
@@ -392,7 +396,7 @@ static int step_once(tt_world *w)
     float *hc = hNew, *ho = hOld;
     if (mask & TT_STAGE_THERMAL)
         for (int it = 0; it < w->cfg.thermal_iterations; ++it) {
-            CK(tt_launch_thermal(s, d, hc, ho));
+            CK(tt_launch_thermal(s, d, TT_TILE_H, hc, ho));
             std::swap(hc, ho);
         }
     CKS(record(w, 3));
@@ -404,7 +408,7 @@ static int step_once(tt_world *w)
             float *sprev, *sout;
             if (it == 0) { sprev = w->sed[buf]; sout = w->sed[buf]; }
             else { sprev = w->sed[buf]; sout = w->sed[1 - buf]; buf = 1 - buf; }
-            CK(tt_launch_hydraulic(s, d, it == 0, it == K - 1, rseed, hc, ho, w->water[w->wc],
+            CK(tt_launch_hydraulic(s, d, w->rows, it == 0, it == K - 1, rseed, hc, ho, w->water[w->wc],
                                    w->water[1 - w->wc], w->flux[w->fc], w->flux[1 - w->fc], w->vel, sprev, sout));
             std::swap(hc, ho);
             w->wc = 1 - w->wc;
@@ -455,6 +459,7 @@ static int step_once(tt_world *w)
     w->stats.mass_ledger = tt_unfix_d(c.ledger, TT_MASS_BITS);
     w->iter++;
     w->stats.iteration = w->iter;
+    w->stats.tile_rows = w->rows;
     if (w->timing) {
         float t[8] = {0};
         for (int i = 1; i <= 7; ++i) CK(hipEventElapsedTime(&t[i], w->ev[i - 1], w->ev[i]));
@@ -666,6 +671,117 @@ int tt_world_geometry(tt_world *w, int32_t *width, int32_t *height, tt_kernel_se
     return TT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// tile autotuner: rows of the erosion tiles per size class
+// ---------------------------------------------------------------------------
+// The hydraulic sub-iteration is timed for each tile height that divides the
+// world's height (8, 16, 32 rows), on the world's own buffers before
+// initialisation overwrites them; the fastest is kept per (device, size class),
+// with size class = floor(log2(cells)).  Results are cached in the process and,
+// when TECTONIC_TUNE_FILE names a file, in a flat JSON object
+// {"<arch>/<class>": rows, ...} that later processes read.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tune;
+bool g_tune_loaded = false;
+
+std::string tune_key(const tt_world *w)
+{
+    hipDeviceProp_t prop;
+    std::string arch = "gpu";
+    if (hipGetDeviceProperties(&prop, w->device) == hipSuccess) arch = prop.gcnArchName;
+    const std::string::size_type colon = arch.find(':');
+    if (colon != std::string::npos) arch = arch.substr(0, colon);
+    int cls = 0;
+    const long long n = (long long)w->W * w->H;
+    while ((1ll << (cls + 1)) <= n) ++cls;
+    return arch + "/" + std::to_string(cls);
+}
+
+void tune_load()
+{
+    g_tune_loaded = true;
+    const char *path = std::getenv("TECTONIC_TUNE_FILE");
+    if (!path) return;
+    std::ifstream f(path);
+    if (!f) return;
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string t = ss.str();
+    size_t i = 0;
+    while ((i = t.find('"', i)) != std::string::npos) {
+        const size_t j = t.find('"', i + 1);
+        if (j == std::string::npos) break;
+        const std::string key = t.substr(i + 1, j - i - 1);
+        const size_t colon = t.find(':', j);
+        if (colon == std::string::npos) break;
+        const int v = std::atoi(t.c_str() + colon + 1);
+        if (v == 8 || v == 16 || v == 32) g_tune[key] = v;
+        i = t.find_first_of(",}", colon);
+        if (i == std::string::npos) break;
+    }
+}
+
+void tune_save()
+{
+    const char *path = std::getenv("TECTONIC_TUNE_FILE");
+    if (!path) return;
+    std::ofstream f(path);
+    if (!f) return;
+    f << "{";
+    bool first = true;
+    for (const auto &kv : g_tune) {
+        f << (first ? "" : ", ") << "\"" << kv.first << "\": " << kv.second;
+        first = false;
+    }
+    f << "}\n";
+}
+}  // namespace
+
+static int tune_rows(tt_world *w)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    if (!g_tune_loaded) tune_load();
+    const std::string key = tune_key(w);
+    const auto it = g_tune.find(key);
+    if (it != g_tune.end() && tt_tile_rows_valid(it->second, w->H)) {
+        w->rows = it->second;
+        return TT_OK;
+    }
+    // defined contents for the timing launches (initialisation overwrites them)
+    CKS(init_common_zero(w));
+    for (int i = 0; i < 2; ++i) CKS(zero(w, w->h[i], (size_t)w->N * 4));
+    const tt_dev d = make_dev(w);
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    int best = 16;
+    float bestMs = 1e30f;
+    const int cand[3] = {16, 8, 32};
+    for (int rows : cand) {
+        if (!tt_tile_rows_valid(rows, w->H)) continue;
+        auto launch = [&](int k) {
+            return tt_launch_hydraulic(w->stream, d, rows, 0, 0, 1, w->h[k & 1], w->h[1 - (k & 1)], w->water[k & 1],
+                                       w->water[1 - (k & 1)], w->flux[k & 1], w->flux[1 - (k & 1)], w->vel,
+                                       w->sed[k & 1], w->sed[1 - (k & 1)]);
+        };
+        CK(launch(0));
+        CK(hipEventRecord(e0, w->stream));
+        for (int k = 1; k <= 4; ++k) CK(launch(k));
+        CK(hipEventRecord(e1, w->stream));
+        CK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        CK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < bestMs) { bestMs = ms; best = rows; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    w->rows = best;
+    g_tune[key] = best;
+    tune_save();
+    return TT_OK;
+}
+
 int tt_validate_config(const tt_sim_config *c)
 {
     if (c->width < 64 || c->height < 64 || c->width % TT_TILE_W || c->height % TT_TILE_H) {
@@ -678,6 +794,10 @@ int tt_validate_config(const tt_sim_config *c)
     if (c->hydraulic_iterations < 0 || c->thermal_iterations < 0) { set_error("negative iteration count"); return TT_ERR_INVALID; }
     if (c->num_voronoi_gens < 0 || c->num_voronoi_gens > 4) { set_error("num_voronoi_gens must be in 0..4"); return TT_ERR_INVALID; }
     if (c->init_mode < 0 || c->init_mode > 2) { set_error("bad init_mode"); return TT_ERR_INVALID; }
+    if (c->tile_rows != 0 && !tt_tile_rows_valid(c->tile_rows, c->height)) {
+        set_error("tile_rows must be 0, or 8, 16 or 32 dividing the height");
+        return TT_ERR_INVALID;
+    }
     return TT_OK;
 }
 
@@ -762,6 +882,7 @@ int tt_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, tt_w
     w->W = cfg->width;
     w->H = cfg->height;
     w->N = cfg->width * cfg->height;
+    w->rows = cfg->tile_rows;
     CK(hipGetDevice(&w->device));
     tt_world_layout(*cfg, true, w);
     size_t total = 0;
@@ -804,7 +925,10 @@ int tt_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, tt_w
 int tt_world_init(tt_world *w)
 {
     if (!w) return TT_ERR_INVALID;
-    return tt_world_do_init(w);
+    if (w->rows == 0) CKS(tune_rows(w));
+    CKS(tt_world_do_init(w));
+    w->stats.tile_rows = w->rows;
+    return TT_OK;
 }
 
 int tt_world_reset(tt_world *w, uint32_t seed, int32_t num_plates)

@@ -161,6 +161,23 @@ def test_thermal_transfers(sim, O, talus):
     run_parity(sim, O, cfg, ks, 3, f"thermal talus {talus}")
 
 
+@pytest.mark.parametrize("rows", [8, 32])
+def test_tile_rows(sim, O, rows):
+    """Hydraulic tiles of 8 and 32 rows give the same bits as the oracle."""
+    cfg, ks = configs.synthetic(128, 128, 6, (0.5, 3.0), 12, hydraulic=5, thermal=4, ccl_every=1)
+    ks.thermalThresholdAngle = 4e-3
+    cfg.tile_rows = rows
+    seen = run_parity(sim, O, cfg, ks, 3, f"tile rows {rows}")
+    assert all(s.tile_rows == rows for s in seen)
+
+
+def test_tuned_tile_rows_are_valid(sim):
+    cfg, ks = configs.synthetic(256, 192, 4, (0.5, 3.0), 3, hydraulic=2, thermal=1)
+    with sim.World(cfg, ks).init() as g:
+        rows = g.stats().tile_rows
+        assert rows in (8, 16, 32) and 192 % rows == 0
+
+
 def test_zero_sub_iterations_and_ccl_period(sim, O):
     cfg, ks = configs.synthetic(128, 128, 7, (0.5, 3.0), 5, hydraulic=0, thermal=0, ccl_every=3)
     run_parity(sim, O, cfg, ks, 4, "K=0 T=0")
