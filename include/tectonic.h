@@ -268,6 +268,27 @@ int tt_world_destroy(tt_world *w);
 int tt_world_export_png(tt_world *w, int32_t what, const char *path);
 enum { TT_EXPORT_HEIGHT16 = 0, TT_EXPORT_PLATES_RGB = 1 };
 
+/* ---- batches: many worlds stepped together (no reference counterpart; the
+ * batched form of executeIteration for many independent worlds) ------------
+ * A step of a batch advances every member by one executeIteration.  The
+ * members' erosion passes run as one launch per pass over all members (replayed
+ * from a captured hipGraph); the other stages run per member on the member's
+ * stream from a pool of host threads.  Each member's result is bit-identical to
+ * stepping it alone.  Members must be initialised worlds of the batch's
+ * device; a member must not be stepped on its own while in a batch.  Removing
+ * a member does not destroy it. */
+typedef struct tt_batch tt_batch;
+int tt_batch_create(tt_batch **out);
+int tt_batch_add(tt_batch *b, tt_world *w);
+int tt_batch_remove(tt_batch *b, tt_world *w);
+int tt_batch_size(tt_batch *b, int32_t *n);
+int tt_batch_step(tt_batch *b, int32_t steps);
+/* host threads driving the per-member stages (default: min(16, cores)) */
+int tt_batch_set_threads(tt_batch *b, int32_t threads);
+/* 1 (default): replay the erosion passes from a captured hipGraph; 0: launch them directly */
+int tt_batch_set_graph(tt_batch *b, int32_t enabled);
+int tt_batch_destroy(tt_batch *b);
+
 /* ---- timing (hipEvent around the stages of the last tt_world_step) -------- */
 typedef struct tt_stage_times {
     float ms_total;

@@ -25,9 +25,11 @@ LIB = PKG / "libtectonic.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TT_OFFLOAD_ARCH", "gfx950")
-HIP_SOURCES = ["k_collide.hip", "k_uplift.hip", "k_erosion.hip", "k_plates.hip", "k_misc.hip", "world.hip"]
+HIP_SOURCES = ["k_collide.hip", "k_uplift.hip", "k_erosion.hip", "k_plates.hip", "k_misc.hip", "k_batch.hip",
+               "world.hip", "batch.hip"]
 HOST_SOURCES = ["host.cpp"]
-HEADERS = ["tt_math.h", "tt_noise.h", "tt_device.h", "tt_stage.h", "tt_kernels.h", "tt_world.h"]
+HEADERS = ["tt_math.h", "tt_noise.h", "tt_device.h", "tt_stage.h", "tt_kernels.h", "tt_world.h", "tt_erosion.h",
+           "tt_batch.h"]
 
 HIP_FLAGS = [
     "-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17",
@@ -81,7 +83,8 @@ def build(force: bool = False, jobs: int | None = None) -> Path:
             f.result()
     if force or tasks or not LIB.exists():
         tmp = LIB.with_suffix(".so.tmp")
-        _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp), "-lz", "-ldl"])
+        _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp), "-lz", "-ldl",
+              "-lpthread"])
         os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,99 @@
+// k_batch.hip -- erosion of many worlds in one launch per pass.
+//
+// A batch keeps, in device memory, one descriptor per member world (sizes,
+// settings, both buffers of every ping-pong field and the buffer indices at
+// the start of the step's erosion) and a tile table per tile height: entry
+// (world << 20) | tile.  One launch of k_thermal_batch runs thermal pass t of
+// every member, one launch of k_hydraulic_batch<TH> runs hydraulic
+// sub-iteration k of every member whose tiles have TH rows.  A member whose
+// pass count is smaller skips the later launches.  The tile bodies are the
+// one-world kernels' (tt_erosion.h), so a world's bits do not depend on the
+// batch it is in.
+#include "tt_kernels.h"
+#include "tt_erosion.h"
+#include "tt_batch.h"
+
+__global__ __launch_bounds__(256) void k_thermal_batch(const tt_bworld *__restrict__ worlds,
+                                                       const uint32_t *__restrict__ tiles, int t)
+{
+    __shared__ tt_therm_lds<TT_TILE_H> S;
+    const uint32_t e = tiles[tt_xcd_tile(blockIdx.x, gridDim.x)];
+    const tt_bworld &B = worlds[e >> 20];
+    if (t >= B.T) return;
+    const int tile = (int)(e & 0xFFFFFu);
+    const int nbx = B.W / TT_TILE_W;
+    tt_therm_args A;
+    A.W = B.W;
+    A.H = B.H;
+    A.talus = B.talus;
+    A.amp = B.amp;
+    A.str = B.str;
+    A.hin = B.h[B.hc0 ^ (t & 1)];
+    A.hout = B.h[B.hc0 ^ ((t + 1) & 1)];
+    tt_thermal_tile<TT_TILE_H>(A, S, (tile % nbx) * TT_TILE_W, (tile / nbx) * TT_TILE_H);
+}
+
+template <int TH>
+__global__ __launch_bounds__(256) void k_hydraulic_batch(const tt_bworld *__restrict__ worlds,
+                                                         const uint32_t *__restrict__ tiles, int k)
+{
+    __shared__ tt_hyd_lds<TH> S;
+    const uint32_t e = tiles[tt_xcd_tile(blockIdx.x, gridDim.x)];
+    const tt_bworld &B = worlds[e >> 20];
+    if (k >= B.K) return;
+    const int tile = (int)(e & 0xFFFFFu);
+    const int nbx = B.W / TT_TILE_W;
+    tt_hyd_args A;
+    A.W = B.W;
+    A.H = B.H;
+    A.rseed = B.rseed;
+    A.rain = B.rain;
+    A.evap = B.evap;
+    A.cap = B.cap;
+    A.ke = B.ke;
+    A.kd = B.kd;
+    A.fk = B.fk;
+    A.g = B.g;
+    A.L = B.L;
+    const int hc = B.hc1 ^ (k & 1), wc = B.wc ^ (k & 1), fc = B.fc ^ (k & 1);
+    A.hin = B.h[hc];
+    A.hout = B.h[hc ^ 1];
+    A.win = B.water[wc];
+    A.wout = B.water[wc ^ 1];
+    A.fin = B.flux[fc];
+    A.fout = B.flux[fc ^ 1];
+    A.vel = B.vel;
+    if (k == 0) {
+        A.sprev = B.sed[B.sc];
+        A.sout = B.sed[B.sc];
+    } else {
+        A.sprev = B.sed[B.sc ^ ((k - 1) & 1)];
+        A.sout = B.sed[B.sc ^ (k & 1)];
+    }
+    const int x0 = (tile % nbx) * TT_TILE_W, y0 = (tile / nbx) * TH;
+    const bool first = k == 0, last = k == B.K - 1;
+    if (first) {
+        if (last) tt_hydraulic_tile<TH, true, true>(A, S, x0, y0);
+        else tt_hydraulic_tile<TH, true, false>(A, S, x0, y0);
+    } else {
+        if (last) tt_hydraulic_tile<TH, false, true>(A, S, x0, y0);
+        else tt_hydraulic_tile<TH, false, false>(A, S, x0, y0);
+    }
+}
+
+hipError_t tt_launch_thermal_batch(hipStream_t s, const tt_bworld *worlds, const uint32_t *tiles, int ntiles, int t)
+{
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_thermal_batch, dim3((unsigned)ntiles), dim3(256), 0, s, worlds, tiles, t);
+    return hipGetLastError();
+}
+
+hipError_t tt_launch_hydraulic_batch(hipStream_t s, int rows, const tt_bworld *worlds, const uint32_t *tiles,
+                                     int ntiles, int k)
+{
+    if (ntiles <= 0) return hipSuccess;
+    if (rows == 8) hipLaunchKernelGGL(k_hydraulic_batch<8>, dim3((unsigned)ntiles), dim3(256), 0, s, worlds, tiles, k);
+    else if (rows == 32) hipLaunchKernelGGL(k_hydraulic_batch<32>, dim3((unsigned)ntiles), dim3(256), 0, s, worlds, tiles, k);
+    else hipLaunchKernelGGL(k_hydraulic_batch<16>, dim3((unsigned)ntiles), dim3(256), 0, s, worlds, tiles, k);
+    return hipGetLastError();
+}

@@ -1,0 +1,29 @@
+// tt_batch.h -- device descriptor of a batch member and the batched erosion
+// launchers (k_batch.hip), used by the batch host code (batch.hip).
+#pragma once
+
+#include "tt_device.h"
+
+// One member world as the batched erosion kernels see it for one step.
+struct tt_bworld {
+    int W, H;
+    int T, K;              // thermal passes, hydraulic sub-iterations this step
+    int hc0;               // height buffer index at the first thermal pass
+    int hc1;               // height buffer index at the first hydraulic sub-iteration
+    int wc, fc, sc;        // water, flux, sediment buffer indices at the first sub-iteration
+    int rseed;             // seed + iteration (rain noise)
+    float talus, amp, str; // thermal settings
+    float rain, evap, cap, ke, kd, fk, g, L;  // hydraulic settings
+    float *h[2];
+    float *water[2];
+    float4 *flux[2];
+    float *sed[2];
+    float2 *vel;
+};
+
+// tile table entries: (world index << 20) | tile index within the world
+#define TT_BATCH_MAX_WORLDS 4096
+
+hipError_t tt_launch_thermal_batch(hipStream_t s, const tt_bworld *worlds, const uint32_t *tiles, int ntiles, int t);
+hipError_t tt_launch_hydraulic_batch(hipStream_t s, int rows, const tt_bworld *worlds, const uint32_t *tiles,
+                                     int ntiles, int k);

@@ -23,7 +23,8 @@ struct tt_world {
     uint32_t iter = 0;
     bool initialized = false;
     int device = 0;
-    int rows = 0;            // rows of the erosion tiles (0 until resolved)
+    int rows = 0;            // rows of the hydraulic tiles (0 until resolved)
+    int ehc = 0;             // height buffer index during erosion
     hipStream_t stream = nullptr;
     char *base = nullptr;
     std::vector<tt_alloc> allocs;
@@ -94,6 +95,14 @@ struct tt_world {
 // byte layout of one world; with alloc = true the pointer slots of w are
 // registered for the single device allocation
 size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w);
+
+// step phases (world.hip), used by the batched step (batch.hip)
+int tt_step_pre(tt_world *w);
+int tt_step_post(tt_world *w);
+int tt_step_thermal_count(const tt_world *w);
+int tt_step_hydraulic_count(const tt_world *w);
+void tt_step_erode_indices(tt_world *w, int T, int K, int *sedLast);
+int tt_step_transport(tt_world *w, int sedLast);
 int tt_world_do_init(tt_world *w);
 int tt_validate_config(const tt_sim_config *c);
 

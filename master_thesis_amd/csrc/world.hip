@@ -36,6 +36,14 @@ void set_error(const char *what, hipError_t e)
 
 void set_error(const std::string &what) { g_last_error = what; }
 
+}  // namespace
+
+// error text of the calling thread, shared with batch.hip and server.cpp
+void tt_set_error(const std::string &msg) { g_last_error = msg; }
+std::string tt_get_error() { return g_last_error; }
+
+namespace {
+
 #define CK(expr)                                        \
     do {                                                \
         hipError_t e_ = (expr);                         \
@@ -330,7 +338,12 @@ static int record(tt_world *w, int slot)
     return TT_OK;
 }
 
-static int step_once(tt_world *w)
+// executeIteration in three phases: before erosion (plate movement,
+// collisions, uplift), erosion (thermal, hydraulic, closing transport) and
+// after erosion (copy-back, kinematics, accretion, CCL, merge, split,
+// ledger).  A batch of worlds runs the first and last phases per world and the
+// erosion of all its worlds together (batch.hip).
+int tt_step_pre(tt_world *w)
 {
     hipStream_t s = w->stream;
     const tt_dev d = make_dev(w);
@@ -392,37 +405,81 @@ static int step_once(tt_world *w)
         CKS(record(w, 1));
     }
     CKS(record(w, 2));
-    // thermal and hydraulic erosion on the written height field
-    float *hc = hNew, *ho = hOld;
-    if (mask & TT_STAGE_THERMAL)
-        for (int it = 0; it < w->cfg.thermal_iterations; ++it) {
-            CK(tt_launch_thermal(s, d, TT_TILE_H, hc, ho));
-            std::swap(hc, ho);
-        }
-    CKS(record(w, 3));
-    if ((mask & TT_STAGE_HYDRAULIC) && w->cfg.hydraulic_iterations > 0) {
-        const int rseed = (int)(w->cfg.seed + w->iter);
-        int buf = w->sc;
-        const int K = w->cfg.hydraulic_iterations;
-        for (int it = 0; it < K; ++it) {
-            float *sprev, *sout;
-            if (it == 0) { sprev = w->sed[buf]; sout = w->sed[buf]; }
-            else { sprev = w->sed[buf]; sout = w->sed[1 - buf]; buf = 1 - buf; }
-            CK(tt_launch_hydraulic(s, d, w->rows, it == 0, it == K - 1, rseed, hc, ho, w->water[w->wc],
-                                   w->water[1 - w->wc], w->flux[w->fc], w->flux[1 - w->fc], w->vel, sprev, sout));
-            std::swap(hc, ho);
-            w->wc = 1 - w->wc;
-            w->fc = 1 - w->fc;
-        }
-        CKS(record(w, 4));
-        CK(tt_launch_transport(s, d, w->sed[buf], w->vel, w->sed[1 - buf]));
-        w->sc = 1 - buf;
-    } else {
-        CKS(record(w, 4));
+    w->ehc = 1 - w->hc;   // erosion works on the written height field
+    return TT_OK;
+}
+
+int tt_step_thermal_count(const tt_world *w)
+{
+    return (w->cfg.stage_mask & TT_STAGE_THERMAL) ? w->cfg.thermal_iterations : 0;
+}
+
+int tt_step_hydraulic_count(const tt_world *w)
+{
+    return (w->cfg.stage_mask & TT_STAGE_HYDRAULIC) ? w->cfg.hydraulic_iterations : 0;
+}
+
+// buffer indices after T thermal and K hydraulic passes (the launches of
+// tt_step_erode); returns the sediment buffer holding the last sub-iteration's
+// output (input of the closing transport) in *sedLast
+void tt_step_erode_indices(tt_world *w, int T, int K, int *sedLast)
+{
+    w->ehc ^= (T + K) & 1;
+    w->wc ^= K & 1;
+    w->fc ^= K & 1;
+    *sedLast = K > 0 ? (w->sc ^ ((K - 1) & 1)) : w->sc;
+}
+
+// the closing sediment transport of a step whose hydraulic loop ran K > 0
+// sub-iterations (sediment output in sed[sedLast])
+int tt_step_transport(tt_world *w, int sedLast)
+{
+    const tt_dev d = make_dev(w);
+    CK(tt_launch_transport(w->stream, d, w->sed[sedLast], w->vel, w->sed[1 - sedLast]));
+    w->sc = 1 - sedLast;
+    return TT_OK;
+}
+
+static int step_erode(tt_world *w)
+{
+    hipStream_t s = w->stream;
+    const tt_dev d = make_dev(w);
+    const int T = tt_step_thermal_count(w), K = tt_step_hydraulic_count(w);
+    int hc = w->ehc;
+    for (int it = 0; it < T; ++it) {
+        CK(tt_launch_thermal(s, d, TT_TILE_H, w->h[hc], w->h[1 - hc]));
+        hc = 1 - hc;
     }
+    CKS(record(w, 3));
+    const int rseed = (int)(w->cfg.seed + w->iter);
+    int wc = w->wc, fc = w->fc;
+    for (int it = 0; it < K; ++it) {
+        float *sprev, *sout;
+        if (it == 0) { sprev = w->sed[w->sc]; sout = w->sed[w->sc]; }
+        else { sprev = w->sed[w->sc ^ ((it - 1) & 1)]; sout = w->sed[w->sc ^ (it & 1)]; }
+        CK(tt_launch_hydraulic(s, d, w->rows, it == 0, it == K - 1, rseed, w->h[hc], w->h[1 - hc], w->water[wc],
+                               w->water[1 - wc], w->flux[fc], w->flux[1 - fc], w->vel, sprev, sout));
+        hc = 1 - hc;
+        wc = 1 - wc;
+        fc = 1 - fc;
+    }
+    CKS(record(w, 4));
+    int sedLast = 0;
+    tt_step_erode_indices(w, T, K, &sedLast);
+    if (K > 0) CKS(tt_step_transport(w, sedLast));
     CKS(record(w, 5));
+    return TT_OK;
+}
+
+int tt_step_post(tt_world *w)
+{
+    hipStream_t s = w->stream;
+    const tt_dev d = make_dev(w);
+    const int mask = w->cfg.stage_mask;
+    const size_t N = (size_t)w->N;
+    const bool upl = (mask & TT_STAGE_UPLIFT) != 0;
     // copy-back (plate_tectonic_sim.cu:77-78) is a change of buffer index
-    w->hc = (hc == w->h[0]) ? 0 : 1;
+    w->hc = w->ehc;
     w->ic = 1 - w->ic;
     if (mask & TT_STAGE_KINEMATICS) {
         CK(tt_launch_plate_movement(s, d, w->pd, w->fric, w->noise));
@@ -473,11 +530,18 @@ static int step_once(tt_world *w)
         float tot = 0;
         CK(hipEventElapsedTime(&tot, w->ev[0], w->ev[7]));
         w->times.ms_total += tot;
-        if (mask & TT_STAGE_HYDRAULIC) w->times.hydraulic_launches += (float)w->cfg.hydraulic_iterations;
-        if (mask & TT_STAGE_THERMAL) w->times.thermal_launches += (float)w->cfg.thermal_iterations;
+        w->times.hydraulic_launches += (float)tt_step_hydraulic_count(w);
+        w->times.thermal_launches += (float)tt_step_thermal_count(w);
         w->times.steps += 1.0f;
     }
     return TT_OK;
+}
+
+static int step_once(tt_world *w)
+{
+    CKS(tt_step_pre(w));
+    CKS(step_erode(w));
+    return tt_step_post(w);
 }
 
 // ---------------------------------------------------------------------------

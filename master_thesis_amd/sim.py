@@ -98,6 +98,57 @@ class World:
         check(lib().tt_world_export_png(self.h, what, path.encode()), "export_png")
 
 
+class Batch:
+    """Many worlds stepped together (tt_batch_*): one launch per erosion pass
+    over all members, replayed from a captured hipGraph; the other stages per
+    member on its own stream.  Members keep their own results bit for bit."""
+
+    def __init__(self, worlds=(), threads: int | None = None, graph: bool = True):
+        h = C.c_void_p()
+        check(lib().tt_batch_create(C.byref(h)), "tt_batch_create")
+        self.h = h
+        self.members: list[World] = []
+        if threads is not None:
+            check(lib().tt_batch_set_threads(self.h, threads), "tt_batch_set_threads")
+        if not graph:
+            check(lib().tt_batch_set_graph(self.h, 0), "tt_batch_set_graph")
+        for w in worlds:
+            self.add(w)
+
+    def add(self, w: World) -> None:
+        check(lib().tt_batch_add(self.h, w.h), "tt_batch_add")
+        self.members.append(w)
+
+    def remove(self, w: World) -> None:
+        check(lib().tt_batch_remove(self.h, w.h), "tt_batch_remove")
+        self.members.remove(w)
+
+    def step(self, n: int = 1) -> None:
+        check(lib().tt_batch_step(self.h, n), "tt_batch_step")
+
+    def __len__(self) -> int:
+        n = C.c_int32()
+        check(lib().tt_batch_size(self.h, C.byref(n)), "tt_batch_size")
+        return n.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().tt_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 STATE_FIELDS = (
     _abi.FIELD_HEIGHT, _abi.FIELD_PLATE_IDS, _abi.FIELD_WATER, _abi.FIELD_FLUX, _abi.FIELD_VELOCITY,
     _abi.FIELD_SEDIMENT, _abi.FIELD_UPLIFT, _abi.FIELD_PLATE_DATA, _abi.FIELD_DIVERGENCE_BITMAP,

@@ -1,0 +1,99 @@
+"""Batched stepping (tt_batch_*): every member of a mixed batch ends with the
+same bits as the same world stepped alone (and as the oracle), with the
+erosion replayed from a captured hipGraph or launched directly."""
+from __future__ import annotations
+
+import sys
+
+import pytest
+
+from master_thesis_amd import _abi, configs
+from tests.helpers import STATE_FIELDS, diff_fields
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def mixed_configs():
+    out = []
+    for (W, H, P, seed, K, T, rows) in [(128, 128, 6, 1, 6, 3, 0), (256, 64, 9, 2, 4, 2, 8), (64, 64, 3, 3, 5, 1, 16),
+                                          (192, 96, 7, 4, 3, 4, 32), (320, 160, 12, 5, 6, 0, 0)]:
+        cfg, ks = configs.synthetic(W, H, P, (0.5, 3.0), seed, hydraulic=K, thermal=T, ccl_every=1)
+        ks.thermalThresholdAngle = 2e-3
+        cfg.tile_rows = rows if rows and H % rows == 0 else 0
+        out.append((cfg, ks))
+    cfg, ks = configs.reference_defaults()
+    cfg.width, cfg.height, cfg.num_plates, cfg.seed, cfg.iterations_for_ccl = 128, 128, 8, 9, 2
+    out.append((cfg, ks))
+    return out
+
+
+@pytest.fixture(scope="module")
+def sim(tectonic_lib):
+    from master_thesis_amd import sim as s
+    return s
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_members_match_solo_worlds(sim, graph):
+    cfgs = mixed_configs()
+    say(f"[batch graph={graph}] {len(cfgs)} worlds")
+    members = [sim.World(c, k).init() for c, k in cfgs]
+    solo = [sim.World(c, k).init() for c, k in cfgs]
+    with sim.Batch(members, graph=graph) as b:
+        assert len(b) == len(cfgs)
+        for s in range(4):
+            say(f"[batch graph={graph}] step {s}")
+            b.step(1)
+            for w in solo:
+                w.step(1)
+            for i, (m, w) in enumerate(zip(members, solo)):
+                bad = diff_fields(m, w, STATE_FIELDS)
+                assert not bad, f"member {i} step {s}: {bad}"
+                assert m.iteration == w.iteration == s + 1
+    for w in members + solo:
+        w.close()
+
+
+def test_batch_member_matches_oracle(sim, oracle_lib):
+    cfg, ks = configs.synthetic(128, 64, 5, (0.5, 3.0), 7, hydraulic=4, thermal=2, ccl_every=1)
+    other = configs.synthetic(256, 256, 9, (0.5, 3.0), 8, hydraulic=7, thermal=1, ccl_every=1)
+    say("[batch oracle]")
+    a = sim.World(cfg, ks).init()
+    b2 = sim.World(*other).init()
+    o = oracle_lib.OracleWorld(cfg, ks)
+    o.init()
+    with sim.Batch([a, b2]) as b:
+        b.step(3)
+    o.step(3)
+    assert not diff_fields(a, o, STATE_FIELDS)
+    a.close()
+    b2.close()
+    o.close()
+
+
+def test_batch_membership_changes(sim):
+    """Worlds join and leave between steps (continuous batching); a world that
+    spent steps in batches of different composition matches its solo twin."""
+    cfgs = mixed_configs()[:4]
+    say("[batch membership]")
+    ws = [sim.World(c, k).init() for c, k in cfgs]
+    solo = [sim.World(c, k).init() for c, k in cfgs]
+    with sim.Batch(ws[:2]) as b:
+        b.step(2)
+        b.add(ws[2])
+        b.step(1)
+        b.remove(ws[0])
+        b.add(ws[3])
+        b.step(2)
+    steps = [3, 5, 3, 2]
+    for w, n in zip(solo, steps):
+        w.step(n)
+    for i, (m, w) in enumerate(zip(ws, solo)):
+        assert not diff_fields(m, w, STATE_FIELDS), f"world {i}"
+    for w in ws + solo:
+        w.close()
