@@ -289,6 +289,52 @@ int tt_batch_set_threads(tt_batch *b, int32_t threads);
 int tt_batch_set_graph(tt_batch *b, int32_t enabled);
 int tt_batch_destroy(tt_batch *b);
 
+/* ---- serving: world-generation jobs under an HBM budget (no reference
+ * counterpart; a request loop around the step for many independent worlds) --
+ * Jobs are admitted in submission order while the summed tt_world_bytes() of
+ * the resident worlds stays within the budget and at most max_active worlds
+ * are resident.  Resident worlds are stepped together (a tt_batch) one step at
+ * a time; a world leaves at the step boundary where its steps are done, and
+ * waiting jobs are admitted into the freed space (continuous batching).  Times
+ * are seconds on the server's monotonic clock. */
+typedef struct tt_server tt_server;
+typedef struct tt_job_result {
+    int64_t id;
+    int32_t status;        /* 1 while queued or running, TT_OK when done, < 0 on failure */
+    uint32_t iteration;    /* iterations of the finished world */
+    double mass_ledger;    /* the world's final mass ledger (tt_step_stats) */
+    double t_submit, t_admit, t_done;
+    int64_t bytes;         /* device bytes of the world (admission size) */
+    int32_t width, height, steps;
+    int32_t reserved[3];
+} tt_job_result;
+typedef struct tt_server_report {
+    int64_t submitted, admitted, completed, failed, rejected;
+    int64_t active, pending;
+    int64_t bytes_in_use, bytes_peak, budget;
+    int64_t batch_steps;
+    double wall_s;         /* first submission to last completion */
+    double worlds_per_s;   /* completed / wall_s */
+    double latency_p50_s, latency_p99_s, latency_mean_s, latency_max_s;  /* completion - submission */
+    double cell_updates;   /* sum over completed jobs of cells x steps */
+    int64_t reserved[4];
+} tt_server_report;
+int tt_server_create(uint64_t hbm_budget_bytes, int32_t max_active, tt_server **out);
+/* 1: keep finished worlds for tt_server_take (default 0: released at once) */
+int tt_server_set_keep(tt_server *s, int32_t keep);
+/* Queue a job; TT_ERR_BUDGET if the world alone exceeds the budget. */
+int tt_server_submit(tt_server *s, const tt_sim_config *cfg, const tt_kernel_settings *ks, int32_t steps,
+                     int64_t *id);
+/* Result of a job; waits up to wait_ms (-1: until done).  TT_ERR_STATE if not done yet. */
+int tt_server_result(tt_server *s, int64_t id, int32_t wait_ms, tt_job_result *out);
+/* Ownership of a finished world (keep mode); the caller destroys it. */
+int tt_server_take(tt_server *s, int64_t id, tt_world **w);
+/* Wait until every submitted job has finished (timeout_ms -1: no limit). */
+int tt_server_drain(tt_server *s, int32_t timeout_ms);
+int tt_server_stats(tt_server *s, tt_server_report *out);
+/* Stops the scheduler; queued and running jobs end with TT_ERR_STATE. */
+int tt_server_destroy(tt_server *s);
+
 /* ---- timing (hipEvent around the stages of the last tt_world_step) -------- */
 typedef struct tt_stage_times {
     float ms_total;

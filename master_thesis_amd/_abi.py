@@ -15,6 +15,7 @@ LIB_PATH = PKG_DIR / "libtectonic.so"
 NOISE_PATH = PKG_DIR / "data" / "noise_r8.bin"
 
 TT_OK = 0
+TT_ERR_INVALID, TT_ERR_DEVICE, TT_ERR_NOMEM, TT_ERR_STATE, TT_ERR_UNSUPPORTED, TT_ERR_BUDGET = -1, -2, -3, -4, -5, -6
 STATUS_NAMES = {0: "ok", -1: "invalid argument", -2: "device error", -3: "out of device memory",
                 -4: "bad state", -5: "unsupported", -6: "over HBM budget"}
 
@@ -229,6 +230,30 @@ class StageTimes(C.Structure):
     ]
 
 
+class JobResult(C.Structure):
+    _fields_ = [
+        ("id", C.c_int64),
+        ("status", C.c_int32),
+        ("iteration", C.c_uint32),
+        ("mass_ledger", C.c_double),
+        ("t_submit", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_done", C.c_double),
+        ("bytes", C.c_int64),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("steps", C.c_int32),
+        ("reserved", C.c_int32 * 3),
+    ]
+
+
+class ServerReport(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("submitted", "admitted", "completed", "failed", "rejected", "active",
+                                           "pending", "bytes_in_use", "bytes_peak", "budget", "batch_steps")] + [
+        (n, C.c_double) for n in ("wall_s", "worlds_per_s", "latency_p50_s", "latency_p99_s", "latency_mean_s",
+                                  "latency_max_s", "cell_updates")] + [("reserved", C.c_int64 * 4)]
+
+
 # numpy-compatible record layouts of the per-cell structs
 import numpy as np  # noqa: E402
 
@@ -331,6 +356,15 @@ def lib() -> C.CDLL:
     L.tt_batch_set_threads.argtypes = [vp, C.c_int32]
     L.tt_batch_set_graph.argtypes = [vp, C.c_int32]
     L.tt_batch_destroy.argtypes = [vp]
+    L.tt_server_create.argtypes = [C.c_uint64, C.c_int32, C.POINTER(vp)]
+    L.tt_server_set_keep.argtypes = [vp, C.c_int32]
+    L.tt_server_submit.argtypes = [vp, C.POINTER(SimConfig), C.POINTER(KernelSettings), C.c_int32,
+                                   C.POINTER(C.c_int64)]
+    L.tt_server_result.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(JobResult)]
+    L.tt_server_take.argtypes = [vp, C.c_int64, C.POINTER(vp)]
+    L.tt_server_drain.argtypes = [vp, C.c_int32]
+    L.tt_server_stats.argtypes = [vp, C.POINTER(ServerReport)]
+    L.tt_server_destroy.argtypes = [vp]
     _lib = L
     return L
 

@@ -27,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TT_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["k_collide.hip", "k_uplift.hip", "k_erosion.hip", "k_plates.hip", "k_misc.hip", "k_batch.hip",
                "world.hip", "batch.hip"]
-HOST_SOURCES = ["host.cpp"]
+HOST_SOURCES = ["host.cpp", "server.cpp"]
 HEADERS = ["tt_math.h", "tt_noise.h", "tt_device.h", "tt_stage.h", "tt_kernels.h", "tt_world.h", "tt_erosion.h",
            "tt_batch.h"]
 
@@ -38,7 +38,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-result",
     f"-I{ROOT / 'include'}",
 ]
-HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-pthread", "-ffp-contract=off", "-fno-fast-math", "-Wall",
               f"-I{ROOT / 'include'}"]
 
 

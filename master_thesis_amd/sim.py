@@ -27,6 +27,16 @@ class World:
         check(lib().tt_world_create(C.byref(self.cfg), C.byref(self.ks), C.byref(h)), "tt_world_create")
         self.h = h
 
+    @classmethod
+    def _adopt(cls, handle: C.c_void_p, cfg: _abi.SimConfig) -> "World":
+        """Wrap a world handle created by the library (a finished server job)."""
+        w = cls.__new__(cls)
+        w.cfg = cfg.copy()
+        w.ks = _abi.KernelSettings.defaults()
+        w.n = cfg.width * cfg.height
+        w.h = handle
+        return w
+
     # lifecycle -----------------------------------------------------------
     def init(self) -> "World":
         check(lib().tt_world_init(self.h), "tt_world_init")
@@ -134,6 +144,64 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "h", None):
             lib().tt_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """World-generation jobs under an HBM budget (tt_server_*): continuous
+    batching of the resident worlds, FIFO admission by tt_world_bytes()."""
+
+    def __init__(self, budget_bytes: int, max_active: int = 256, keep: bool = False):
+        h = C.c_void_p()
+        check(lib().tt_server_create(int(budget_bytes), int(max_active), C.byref(h)), "tt_server_create")
+        self.h = h
+        if keep:
+            check(lib().tt_server_set_keep(self.h, 1), "tt_server_set_keep")
+
+    def submit(self, cfg: _abi.SimConfig, ks: _abi.KernelSettings, steps: int) -> int:
+        jid = C.c_int64()
+        check(lib().tt_server_submit(self.h, C.byref(cfg), C.byref(ks), int(steps), C.byref(jid)), "tt_server_submit")
+        return jid.value
+
+    def result(self, jid: int, wait_ms: int = -1) -> _abi.JobResult | None:
+        r = _abi.JobResult()
+        st = lib().tt_server_result(self.h, int(jid), int(wait_ms), C.byref(r))
+        if st == _abi.TT_ERR_STATE:
+            return None
+        check(st, "tt_server_result")
+        return r
+
+    def take(self, jid: int) -> World:
+        h = C.c_void_p()
+        check(lib().tt_server_take(self.h, int(jid), C.byref(h)), "tt_server_take")
+        r = self.result(jid, 0)
+        cfg = _abi.SimConfig()
+        cfg.width, cfg.height = r.width, r.height
+        return World._adopt(h, cfg)
+
+    def drain(self, timeout_ms: int = -1) -> None:
+        check(lib().tt_server_drain(self.h, int(timeout_ms)), "tt_server_drain")
+
+    def report(self) -> _abi.ServerReport:
+        r = _abi.ServerReport()
+        check(lib().tt_server_stats(self.h, C.byref(r)), "tt_server_stats")
+        return r
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().tt_server_destroy(self.h)
             self.h = None
 
     def __enter__(self):

@@ -24,6 +24,11 @@ def test_exports_every_declared_symbol(tectonic_lib):
         assert hasattr(tectonic_lib, n), n
 
 
+def test_server_struct_sizes():
+    assert C.sizeof(_abi.JobResult) == 8 + 4 + 4 + 8 * 4 + 8 + 4 * 6
+    assert C.sizeof(_abi.ServerReport) == 11 * 8 + 7 * 8 + 4 * 8
+
+
 def test_struct_sizes():
     assert C.sizeof(_abi.KernelSettings) == 200          # 50 x 4-byte fields (kernel_settings.cuh:4-81)
     assert C.sizeof(_abi.PlateData) == 84                # sizeof(PlateData) (plate_data.h:5-38)
