@@ -16,6 +16,12 @@ Prints one JSON line with the throughput (cell-updates/s), the roofline of the
 dominant kernel (the fused hydraulic sub-iteration, HBM-bound) timed with HIP
 events on the world's stream, and the CPU oracle timed on this host on one
 step of the same world from the same initial state.
+
+By default (--mode all) the line also carries two bounded samples of the
+other system forms: "batched" (256 mixed-size worlds stepped as one batch and
+one at a time) and "serving" (Poisson world-generation requests under a 200 GB
+HBM budget, worlds/s and p50/p99 latency).  --mode batch / serve print those
+alone; --mode plain prints the 4096^2 measurement only.
 """
 from __future__ import annotations
 
@@ -25,6 +31,8 @@ import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -62,6 +70,102 @@ def load_traffic():
     return None
 
 
+def round64(x: float) -> int:
+    return max(64, int(round(x / 64.0)) * 64)
+
+
+def mixed_world(rng, lo: int, hi: int, pmin: int, pmax: int, seed: int, cfgs):
+    side = round64(float(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+    plates = int(rng.integers(pmin, pmax + 1))
+    cfg, ks = cfgs.synthetic(side, side, plates, (0.5, 3.0), seed, hydraulic=50, thermal=20, ccl_every=1,
+                             uplift_range=5)
+    return cfg, ks
+
+
+def run_batch_mode(args) -> dict:
+    """Mixed-size worlds (sides log-uniform in [256, 2048], multiples of 64,
+    plates U{4..40}): the same worlds stepped as one batch and one at a time."""
+    from master_thesis_amd import configs
+    from master_thesis_amd.sim import Batch, World
+    rng = np.random.default_rng(args.seed)
+    specs = [mixed_world(rng, 256, 2048, 4, 40, args.seed + i, configs) for i in range(args.batch_worlds)]
+    cells = sum(c.width * c.height for c, _ in specs)
+    worlds = [World(c, k).init() for c, k in specs]
+    with Batch(worlds) as b:
+        b.step(args.warmup)
+        t0 = time.perf_counter()
+        b.step(args.batch_steps)
+        tb = time.perf_counter() - t0
+    for w in worlds:
+        w.close()
+    worlds = [World(c, k).init() for c, k in specs]
+    for w in worlds:
+        w.step(args.warmup)
+    t0 = time.perf_counter()
+    for w in worlds:
+        w.step(args.batch_steps)
+    tp = time.perf_counter() - t0
+    for w in worlds:
+        w.close()
+    n = len(specs)
+    return {
+        "workload": f"{n} worlds, sides log-uniform [256,2048] (multiples of 64), plates U{{4..40}}, "
+                    f"K=50 hydraulic, T=20 thermal, CCL every step; {args.batch_steps} timed steps each",
+        "worlds": n, "cells": cells, "steps": args.batch_steps,
+        "batched_s": tb, "plain_s": tp,
+        "batched_cell_updates_per_s": cells * args.batch_steps / tb,
+        "plain_cell_updates_per_s": cells * args.batch_steps / tp,
+        "batched_worlds_per_s": n * args.batch_steps / tb / args.batch_steps,
+        "plain_worlds_per_s": n * args.batch_steps / tp / args.batch_steps,
+        "speedup": tp / tb,
+    }
+
+
+def run_serve_mode(args) -> dict:
+    """Poisson arrivals of world-generation jobs (side log-uniform in
+    [256, 4096], plates U{4..48}, steps U[50, 1000]) served under an HBM
+    budget; a bounded sample of the request stream."""
+    import threading
+    from master_thesis_amd import configs
+    from master_thesis_amd.sim import Server
+    rng = np.random.default_rng(args.seed + 1000)
+    reqs = []
+    t = 0.0
+    for i in range(args.serve_requests):
+        t += float(rng.exponential(1.0 / args.serve_rate))
+        cfg, ks = mixed_world(rng, 256, 4096, 4, 48, args.seed + 10000 + i, configs)
+        steps = int(rng.integers(50, 1001))
+        reqs.append((t, cfg, ks, steps))
+    budget = int(args.serve_budget_gb * (1 << 30))
+    with Server(budget) as srv:
+        ids = []
+        t0 = time.perf_counter()
+
+        def submitter():
+            for (ta, cfg, ks, steps) in reqs:
+                dt = ta - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                ids.append(srv.submit(cfg, ks, steps))
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        srv.drain(-1)
+        rep = srv.report()
+        res = [srv.result(i, 0) for i in ids]
+    work = sum(r.width * r.height * r.steps for r in res)
+    return {
+        "workload": f"{len(reqs)} requests, Poisson lambda={args.serve_rate}/s, side log-uniform [256,4096] "
+                    f"(multiples of 64), plates U{{4..48}}, steps U[50,1000], K=50, T=20, budget "
+                    f"{args.serve_budget_gb:g} GB",
+        "requests": len(reqs), "completed": int(rep.completed), "failed": int(rep.failed),
+        "worlds_per_s": rep.worlds_per_s, "latency_p50_s": rep.latency_p50_s, "latency_p99_s": rep.latency_p99_s,
+        "latency_mean_s": rep.latency_mean_s, "wall_s": rep.wall_s, "cell_updates_per_s": work / rep.wall_s,
+        "bytes_peak_gb": rep.bytes_peak / (1 << 30), "batch_steps": int(rep.batch_steps),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +175,14 @@ def main() -> int:
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=0, help="erosion tile rows (0: autotuned)")
+    ap.add_argument("--mode", choices=["plain", "batch", "serve", "all"], default="all",
+                    help="plain: the 4096^2 step (the metric); batch / serve: the batched and served forms; "
+                         "all: the plain line with batched and served samples added")
+    ap.add_argument("--batch-worlds", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -81,6 +193,21 @@ def main() -> int:
 
     from master_thesis_amd import _abi, configs
     from master_thesis_amd.sim import World, STATE_FIELDS
+
+    if args.mode in ("batch", "serve"):
+        out = run_batch_mode(args) if args.mode == "batch" else run_serve_mode(args)
+        line = {
+            "metric": "batched worlds/s (mixed sizes, same worlds one at a time beside)" if args.mode == "batch"
+            else "served worlds/s with p50/p99 completion latency",
+            "value": out["batched_worlds_per_s"] if args.mode == "batch" else out["worlds_per_s"],
+            "unit": "worlds/s", "n_gpus": 1,
+            "steps": args.batch_steps if args.mode == "batch" else None, "warmup": args.warmup,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded worlds)", "config": {"workload": out["workload"]},
+        }
+        line.update({k: v for k, v in out.items() if k != "workload"})
+        print(json.dumps(line))
+        return 0
 
     if args.copy_probe_mib > 0:
         # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE counters
@@ -185,6 +312,12 @@ def main() -> int:
             "sample": f"1 step of the same {cfg.width}x{cfg.height} world from the same initial state "
                       f"(CPU restatement, OpenMP, {threads} threads)",
         }
+    if args.mode == "all":
+        for key, fn in (("batched", run_batch_mode), ("serving", run_serve_mode)):
+            try:
+                result[key] = fn(args)
+            except Exception as e:  # the plain measurement stands on its own
+                result[key] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(result))
     return 0
 
