@@ -144,8 +144,16 @@ typedef struct tt_sim_config {
     int32_t stage_mask;            /* TT_STAGE_* */
     int32_t tile_rows;             /* rows of the hydraulic tiles: 8, 16 or 32 (dividing height);
                                       0 = chosen by the tile autotuner (thermal tiles: 16 rows) */
-    int32_t reserved[6];
+    int32_t uplift_mode;           /* TT_UPLIFT_JFA (the reference's live path) or TT_UPLIFT_SEPARABLE */
+    int32_t reserved[5];
 } tt_sim_config;
+
+/* Uplift spread.  JFA: jump flood + applyUplift (plate_tectonic_sim.cu:283-317,
+ * the reference's live path).  SEPARABLE: the bounded-radius VerticalBlur +
+ * HorizontalBlur pair (plate_tectonics_kernel.cu:1603-1643, 1740-1843; its
+ * call is commented out at plate_tectonic_sim.cu:319-327), radius
+ * KernelSettings.upliftRange. */
+enum { TT_UPLIFT_JFA = 0, TT_UPLIFT_SEPARABLE = 1 };
 
 /* struct PlateData (src/cuda_compute/types/plate_data.h:5-38), 84 bytes, same
  * field order and layout; the plate table has 255 entries (MAX_PLATE_COUNT,

@@ -20,6 +20,7 @@ STATUS_NAMES = {0: "ok", -1: "invalid argument", -2: "device error", -3: "out of
                 -4: "bad state", -5: "unsupported", -6: "over HBM budget"}
 
 TT_INIT_SYNTHETIC, TT_INIT_REFERENCE, TT_INIT_EMPTY = 0, 1, 2
+TT_UPLIFT_JFA, TT_UPLIFT_SEPARABLE = 0, 1
 
 TT_STAGE_COLLIDE = 1 << 0
 TT_STAGE_UPLIFT = 1 << 1
@@ -144,7 +145,8 @@ class SimConfig(C.Structure):
         ("speed_max", C.c_float),
         ("stage_mask", C.c_int32),
         ("tile_rows", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("uplift_mode", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
     @classmethod

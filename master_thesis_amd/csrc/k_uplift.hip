@@ -316,3 +316,104 @@ hipError_t tt_launch_uplift(hipStream_t s, const tt_dev &d, const int *jfa, cons
     hipLaunchKernelGGL(k_uplift, dim3(blocks), dim3(256), 0, s, d, jfa, prop, idW, ps, hW, dist);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// bounded-radius separable uplift (uplift_mode 1): VerticalBlur
+// (plate_tectonics_kernel.cu:1603-1643) and HorizontalBlur (:1740-1843)
+// ---------------------------------------------------------------------------
+// nearest seed cell (>= 2 involved plates) within R rows of the column; first
+// in offset order -R..R on ties
+__global__ __launch_bounds__(256) void k_vblur(tt_dev d, const tt_prop *__restrict__ prop, tt_dist *__restrict__ dist)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= (uint32_t)d.N) return;
+    int x, y;
+    tt_cell_xy(c, d, x, y);
+    const int R = d.ks.upliftRange;
+    int best = -1, by = 0;
+    for (int i = -R; i <= R; ++i) {
+        const int sy = tt_wrapg(y + i, d.H);
+        if ((prop[(uint32_t)sy * (uint32_t)d.W + (uint32_t)x].inv >> 8) != 0u) {
+            const int sd = i < 0 ? -i : i;
+            if (best == -1 || sd < best) { best = sd; by = sy; }
+        }
+    }
+    tt_dist o;
+    o.dist = best;
+    o.orig = best != -1 ? (uint32_t)by * (uint32_t)d.W + (uint32_t)x : 0u;
+    dist[c] = o;
+}
+
+// the vertical results within R columns of the row whose collision involves
+// the cell's plate (isPartOfCollision, :1409-1416), plus the column offset;
+// first strictly smallest total; then the uplift formulas with
+// subductionOverFormula(dist)
+__global__ __launch_bounds__(256) void k_hblur(tt_dev d, const tt_dist *__restrict__ dist,
+                                               const tt_prop *__restrict__ prop, const uint8_t *__restrict__ idW,
+                                               const tt_plate_step *__restrict__ ps, float *__restrict__ hW)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= (uint32_t)d.N) return;
+    int x, y;
+    tt_cell_xy(c, d, x, y);
+    const int R = d.ks.upliftRange;
+    const int pid = idW[c];
+    int cd = -1;
+    uint32_t corig = 0u;
+    tt_prop U;
+    U.cum = 0.0f;
+    U.inv = 0u;
+    U.types = 0;
+    const uint32_t row = (uint32_t)y * (uint32_t)d.W;
+    for (int i = -R; i <= R; ++i) {
+        const tt_dist b = dist[row + (uint32_t)tt_wrapg(x + i, d.W)];
+        if (b.dist == -1) continue;
+        const tt_prop P = prop[b.orig];
+        bool part = false;
+#pragma unroll
+        for (int k = 0; k != 4; ++k)
+            if ((int)(((uint32_t)TT_MAXP - (P.inv >> (8 * k))) & 0xFFu) == pid) part = true;
+        if (!part) continue;
+        const int td = b.dist + (i < 0 ? -i : i);
+        if (cd == -1 || td < cd) { cd = td; corig = b.orig; U = P; }
+    }
+    if (cd <= -1) return;
+    const int type = U.types & 1;
+    const int pi = (U.types >> 1) & 3, si = (U.types >> 3) & 3;
+    const int prim = tt_slot_plate(U.inv, pi);
+    const int sec = tt_slot_plate(U.inv, si);
+    float h = hW[c];
+    if (type == 0 && sec == pid) {
+        const float v = tt_sub_under((float)cd);
+        const float m = 1.0f + v * d.ks.subductionUnderUpliftMultiplier;
+        h = tt_fmaxf(h * m, d.ks.minUpliftClamp);
+    } else if (type == 0 && prim == pid) {
+        int ox, oy;
+        tt_cell_xy(corig, d, ox, oy);
+        const float sizeX = (float)d.W, sizeY = (float)d.H;
+        const float dx = tt_fmodf((float)(x - ox) + sizeX * 0.5f, sizeX) - sizeX * 0.5f;
+        const float dy = tt_fmodf((float)(y - oy) + sizeY * 0.5f, sizeY) - sizeY * 0.5f;
+        const float mg = tt_fmaxf(sqrtf(dx * dx + dy * dy), 1e-20f);
+        const float nx = dx / mg, ny = dy / mg;
+        const float dot = tt_fmaxf(nx * ps[sec].dirx + ny * ps[sec].diry, 0.0f);
+        const float v = tt_cont_formula((float)cd - 10.0f);
+        const float m = 1.0f + (v * dot) * d.ks.subductionOverUpliftMultiplier;
+        h = tt_fmaxf(h * m, d.ks.minUpliftClamp);
+    } else if (type == 1) {
+        const float v = tt_cont_formula((float)cd);
+        const float hc = (v * U.cum) * d.ks.continentalUpliftMultiplier;
+        h = tt_fminf(h + hc, d.ks.maxUpliftClamp);
+    } else {
+        return;
+    }
+    hW[c] = h;
+}
+
+hipError_t tt_launch_separable_uplift(hipStream_t s, const tt_dev &d, const tt_prop *prop, const uint8_t *idW,
+                                      const tt_plate_step *ps, float *hW, tt_dist *dist)
+{
+    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    hipLaunchKernelGGL(k_vblur, dim3(blocks), dim3(256), 0, s, d, prop, dist);
+    hipLaunchKernelGGL(k_hblur, dim3(blocks), dim3(256), 0, s, d, dist, prop, idW, ps, hW);
+    return hipGetLastError();
+}

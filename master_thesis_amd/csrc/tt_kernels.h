@@ -52,6 +52,8 @@ hipError_t tt_launch_combine_contested(hipStream_t s, const tt_dev &d, const uin
 hipError_t tt_launch_jfa_pass(hipStream_t s, const tt_dev &d, const int *in, int *out, int step);
 hipError_t tt_launch_uplift(hipStream_t s, const tt_dev &d, const int *jfa, const tt_prop *prop, const uint8_t *idW,
                             const tt_plate_step *ps, float *hW, tt_dist *dist);
+hipError_t tt_launch_separable_uplift(hipStream_t s, const tt_dev &d, const tt_prop *prop, const uint8_t *idW,
+                                      const tt_plate_step *ps, float *hW, tt_dist *dist);
 
 // ---- stage D/E (k_erosion.hip) -----------------------------------------------
 int tt_tile_rows_valid(int rows, int H);

@@ -385,17 +385,21 @@ int tt_step_pre(tt_world *w)
                                           w->prop));
             CK(tt_launch_combine_contested(s, d, w->word, w->uplift, w->contested, w->srcFlag, w->prop, w->jfa[0],
                                            w->ctl));
-            const int maxDim = std::max(w->W, w->H);
-            int step = 1;
-            while (step < maxDim) step <<= 1;
-            step >>= 1;
-            int jc = 0;
-            while (step > 0) {
-                CK(tt_launch_jfa_pass(s, d, w->jfa[jc], w->jfa[1 - jc], step));
-                jc = 1 - jc;
+            if (w->cfg.uplift_mode == TT_UPLIFT_SEPARABLE) {
+                CK(tt_launch_separable_uplift(s, d, w->prop, idNew, w->ps, hNew, w->dist));
+            } else {
+                const int maxDim = std::max(w->W, w->H);
+                int step = 1;
+                while (step < maxDim) step <<= 1;
                 step >>= 1;
+                int jc = 0;
+                while (step > 0) {
+                    CK(tt_launch_jfa_pass(s, d, w->jfa[jc], w->jfa[1 - jc], step));
+                    jc = 1 - jc;
+                    step >>= 1;
+                }
+                CK(tt_launch_uplift(s, d, w->jfa[jc], w->prop, idNew, w->ps, hNew, w->dist));
             }
-            CK(tt_launch_uplift(s, d, w->jfa[jc], w->prop, idNew, w->ps, hNew, w->dist));
         }
     } else {
         CK(hipMemcpyAsync(hNew, hOld, N * 4, hipMemcpyDeviceToDevice, s));
@@ -858,6 +862,10 @@ int tt_validate_config(const tt_sim_config *c)
     if (c->hydraulic_iterations < 0 || c->thermal_iterations < 0) { set_error("negative iteration count"); return TT_ERR_INVALID; }
     if (c->num_voronoi_gens < 0 || c->num_voronoi_gens > 4) { set_error("num_voronoi_gens must be in 0..4"); return TT_ERR_INVALID; }
     if (c->init_mode < 0 || c->init_mode > 2) { set_error("bad init_mode"); return TT_ERR_INVALID; }
+    if (c->uplift_mode != TT_UPLIFT_JFA && c->uplift_mode != TT_UPLIFT_SEPARABLE) {
+        set_error("uplift_mode must be TT_UPLIFT_JFA or TT_UPLIFT_SEPARABLE");
+        return TT_ERR_INVALID;
+    }
     if (c->tile_rows != 0 && !tt_tile_rows_valid(c->tile_rows, c->height)) {
         set_error("tile_rows must be 0, or 8, 16 or 32 dividing the height");
         return TT_ERR_INVALID;

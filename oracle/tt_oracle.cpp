@@ -843,6 +843,78 @@ void stage_jfa_uplift(World &w, std::vector<float> &hW, const std::vector<uint8_
         }
 }
 
+// Bounded-radius separable uplift, selected by uplift_mode 1: VerticalBlur
+// (plate_tectonics_kernel.cu:1603-1643) writes, per cell, the nearest seed cell
+// (a cell with >= 2 involved plates) within upliftRange rows of its column,
+// first in offset order -R..R on ties; HorizontalBlur (:1740-1843) scans the
+// vertical results within upliftRange columns of its row, keeps those whose
+// collision involves the cell's plate (isPartOfCollision, :1409-1416), adds the
+// column offset and keeps the first strictly smallest total; the uplift
+// formulas follow, with subductionOverFormula(dist) (no subductionOverRange
+// shift, unlike applyUplift).  The distance field holds the vertical result,
+// as the reference writes it into the same DistanceFieldBuffer.
+void stage_separable_uplift(World &w, std::vector<float> &hW, const std::vector<uint8_t> &idW)
+{
+    const int W = w.W, H = w.H, R = w.ks.upliftRange;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            int best = -1, bx = 0, by = 0;
+            for (int i = -R; i <= R; ++i) {
+                const int sy = wrapc(y + i, H);
+                if ((w.prop[(size_t)sy * W + x].inv >> 8) != 0) {
+                    const int sd = i < 0 ? -i : i;
+                    if (best == -1 || sd < best) { best = sd; bx = x; by = sy; }
+                }
+            }
+            w.dist[(size_t)y * W + x] = DistBuf{best, best != -1 ? (uint32_t)(by * W + bx) : 0u};
+        }
+    const float sizeX = (float)W, sizeY = (float)H;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const size_t c = (size_t)y * W + x;
+            const int pid = idW[c];
+            DistBuf cb{-1, 0u};
+            PropData U{0.0f, 0u, 0, {0, 0, 0}};
+            for (int i = -R; i <= R; ++i) {
+                DistBuf b = w.dist[(size_t)y * W + wrapc(x + i, W)];
+                if (b.dist == -1) continue;
+                const PropData P = w.prop[b.orig];
+                bool part = false;
+                for (int k = 0; k != 4; ++k)
+                    if ((int)(((uint32_t)MAXP - (P.inv >> (8 * k))) & 0xFFu) == pid) part = true;
+                if (!part) continue;
+                b.dist += i < 0 ? -i : i;
+                if (cb.dist == -1 || b.dist < cb.dist) { cb = b; U = P; }
+            }
+            if (cb.dist <= -1) continue;
+            const int type = U.types & 1;
+            const int pi = (U.types >> 1) & 3, si = (U.types >> 3) & 3;
+            const int prim = (uint8_t)(MAXP - ((U.inv >> (8 * pi)) & 0xFF));
+            const int sec = (uint8_t)(MAXP - ((U.inv >> (8 * si)) & 0xFF));
+            if (type == 0 && sec == pid) {
+                const float v = sub_under((float)cb.dist);
+                const float m = 1.0f + v * w.ks.subductionUnderUpliftMultiplier;
+                hW[c] = tt_fmaxf(hW[c] * m, w.ks.minUpliftClamp);
+            } else if (type == 0 && prim == pid) {
+                const int ox = (int)(cb.orig % (uint32_t)W), oy = (int)(cb.orig / (uint32_t)W);
+                const float dx = tt_fmodf((float)(x - ox) + sizeX * 0.5f, sizeX) - sizeX * 0.5f;
+                const float dy = tt_fmodf((float)(y - oy) + sizeY * 0.5f, sizeY) - sizeY * 0.5f;
+                const V2 dir = norm_zs(V2{dx, dy});
+                const tt_plate_data &OP = w.pd[sec];
+                const float dot = tt_fmaxf(dir.x * OP.direction[0] + dir.y * OP.direction[1], 0.0f);
+                const float v = cont_formula((float)cb.dist - 10.0f);
+                const float m = 1.0f + (v * dot) * w.ks.subductionOverUpliftMultiplier;
+                hW[c] = tt_fmaxf(hW[c] * m, w.ks.minUpliftClamp);
+            } else if (type == 1) {
+                const float v = cont_formula((float)cb.dist);
+                const float hc = (v * U.cum) * w.ks.continentalUpliftMultiplier;
+                hW[c] = tt_fminf(hW[c] + hc, w.ks.maxUpliftClamp);
+            }
+        }
+}
+
 // ---------------------------------------------------------------------------
 // Stage D: thermal erosion (thermalErosionKernel :2735-2786), canonical Jacobi
 // form: every cell reads the heights of the previous pass; its new height is
@@ -1577,7 +1649,8 @@ void step(World &w)
         stage_resolve(w, offx, offy, hW, idW, fric);
         if (mask & TT_STAGE_UPLIFT) {
             stage_propagate(w);
-            stage_jfa_uplift(w, hW, idW);
+            if (w.cfg.uplift_mode == TT_UPLIFT_SEPARABLE) stage_separable_uplift(w, hW, idW);
+            else stage_jfa_uplift(w, hW, idW);
         }
     }
     if (mask & TT_STAGE_THERMAL) stage_thermal(w, hW);

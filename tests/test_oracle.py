@@ -519,3 +519,70 @@ def test_no_stage_leaves_the_state(O):
     w.step(2)
     assert [w.get(f).tobytes() for f in (_abi.FIELD_HEIGHT, _abi.FIELD_PLATE_IDS, _abi.FIELD_WATER)] == before
     assert w.iteration == 2
+
+
+def test_separable_uplift_vertical_pass_is_brute_force(O):
+    """uplift_mode 1: the distance field holds VerticalBlur's result, the
+    nearest seed (>= 2 involved plates) within upliftRange rows of the column,
+    first in offset order -R..R on ties (plate_tectonics_kernel.cu:1603-1643)."""
+    cfg, ks = configs.reference_defaults()
+    cfg.width, cfg.height, cfg.num_plates, cfg.seed = 128, 96, 10, 17
+    cfg.num_voronoi_gens = 0
+    cfg.uplift_mode = _abi.TT_UPLIFT_SEPARABLE
+    ks.upliftRange = 6
+    w = O.OracleWorld(cfg, ks)
+    w.init()
+    w.step(2)
+    prop = w.get(_abi.FIELD_PROPAGATED)
+    dist = w.get(_abi.FIELD_DISTANCE)
+    H, W = 96, 128
+    seed = ((prop["inv"] >> 8) != 0).reshape(H, W)
+    assert seed.any()
+    R = ks.upliftRange
+    exp_d = np.full((H, W), -1, np.int64)
+    exp_o = np.zeros((H, W), np.int64)
+    for y in range(H):
+        for x in range(W):
+            for i in range(-R, R + 1):
+                sy = (y + i) % H
+                if seed[sy, x] and (exp_d[y, x] == -1 or abs(i) < exp_d[y, x]):
+                    exp_d[y, x], exp_o[y, x] = abs(i), sy * W + x
+    assert np.array_equal(dist["dist"].reshape(H, W), exp_d)
+    assert np.array_equal(dist["orig"].reshape(H, W), exp_o)
+
+
+def test_separable_uplift_touches_only_cells_in_reach(O):
+    """Heights change by uplift only where the horizontal window holds a
+    vertical result whose collision involves the cell's plate."""
+    cfg, ks = configs.reference_defaults()
+    cfg.width, cfg.height, cfg.num_plates, cfg.seed = 128, 96, 10, 17
+    cfg.num_voronoi_gens = 0
+    cfg.uplift_mode = _abi.TT_UPLIFT_SEPARABLE
+    cfg.stage_mask = _abi.TT_STAGE_COLLIDE | _abi.TT_STAGE_UPLIFT
+    ks.upliftRange = 4
+    ref = O.OracleWorld(cfg, ks)
+    ref.init()
+    cfg2 = cfg.copy()
+    cfg2.stage_mask = _abi.TT_STAGE_COLLIDE
+    base = O.OracleWorld(cfg2, ks)
+    base.init()
+    ref.step(1)
+    base.step(1)
+    h_up, h_no = ref.get(_abi.FIELD_HEIGHT), base.get(_abi.FIELD_HEIGHT)
+    prop, dist = ref.get(_abi.FIELD_PROPAGATED), ref.get(_abi.FIELD_DISTANCE)
+    ids = ref.get(_abi.FIELD_PLATE_IDS)
+    H, W, R = 96, 128, 4
+    reach = np.zeros(H * W, bool)
+    for c in range(H * W):
+        y, x = divmod(c, W)
+        for i in range(-R, R + 1):
+            b = dist[y * W + (x + i) % W]
+            if b["dist"] == -1:
+                continue
+            inv = int(prop[b["orig"]]["inv"])
+            if any(((255 - (inv >> (8 * k))) & 0xFF) == ids[c] for k in range(4)):
+                reach[c] = True
+                break
+    changed = h_up != h_no
+    assert changed.any()
+    assert not (changed & ~reach).any()

@@ -178,6 +178,20 @@ def test_tuned_tile_rows_are_valid(sim):
         assert rows in (8, 16, 32) and 192 % rows == 0
 
 
+@pytest.mark.parametrize("R,init", [(3, "reference"), (12, "reference"), (5, "synthetic")])
+def test_separable_uplift(sim, O, R, init):
+    """uplift_mode 1: the bounded-radius VerticalBlur + HorizontalBlur pair."""
+    if init == "reference":
+        cfg, ks = configs.reference_defaults()
+        cfg.width, cfg.height, cfg.num_plates, cfg.seed, cfg.iterations_for_ccl = 192, 128, 10, 17, 2
+        cfg.num_voronoi_gens = 0
+    else:
+        cfg, ks = configs.synthetic(128, 128, 8, (0.5, 3.0), 19, hydraulic=4, thermal=2, ccl_every=1)
+    cfg.uplift_mode = _abi.TT_UPLIFT_SEPARABLE
+    ks.upliftRange = R
+    run_parity(sim, O, cfg, ks, 4, f"separable R={R} {init}")
+
+
 def test_zero_sub_iterations_and_ccl_period(sim, O):
     cfg, ks = configs.synthetic(128, 128, 7, (0.5, 3.0), 5, hydraulic=0, thermal=0, ccl_every=3)
     run_parity(sim, O, cfg, ks, 4, "K=0 T=0")
