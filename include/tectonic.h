@@ -274,7 +274,23 @@ int tt_world_destroy(tt_world *w);
  * grey PNG of height normalised to [minUpliftClamp, maxUpliftClamp], or the
  * RGB plate palette.  Untimed. */
 int tt_world_export_png(tt_world *w, int32_t what, const char *path);
-enum { TT_EXPORT_HEIGHT16 = 0, TT_EXPORT_PLATES_RGB = 1 };
+/* HEIGHT16 and PLATES_RGB as above; COLLISION_CLASSES and PROPAGATED_CLASSES:
+ * the createVisualizationCollisionTexture / createVisualizationPropagated-
+ * CollisionTexture views (plate_tectonics_kernel.cu:3250-3313), classes 0..5
+ * (none, divergence, subduction, continental with 2 / 3 / 4 plates) in RGB. */
+enum { TT_EXPORT_HEIGHT16 = 0, TT_EXPORT_PLATES_RGB = 1, TT_EXPORT_COLLISION_CLASSES = 2,
+       TT_EXPORT_PROPAGATED_CLASSES = 3 };
+/* Diagnostics as JSON text: step statistics, field summaries, counts of the
+ * collision classes, and one record per live plate with the fields of
+ * copyPlateDataGuiKernel (plate_tectonics_kernel.cu:1220-1281) including its
+ * continental partners and signed subduction partners.  With buf == NULL only
+ * *len (the text length) is set; otherwise cap must exceed *len. */
+int tt_world_diagnostics_json(tt_world *w, char *buf, size_t cap, size_t *len);
+/* Snapshot: every state field, the configuration, the settings and the
+ * iteration in one file (magic "TTSNAP", version 1, CRC32 per field).  A world
+ * loaded from a snapshot continues bit-identically. */
+int tt_world_save(tt_world *w, const char *path);
+int tt_world_load(const char *path, tt_world **out);
 
 /* ---- batches: many worlds stepped together (no reference counterpart; the
  * batched form of executeIteration for many independent worlds) ------------

@@ -21,6 +21,7 @@ STATUS_NAMES = {0: "ok", -1: "invalid argument", -2: "device error", -3: "out of
 
 TT_INIT_SYNTHETIC, TT_INIT_REFERENCE, TT_INIT_EMPTY = 0, 1, 2
 TT_UPLIFT_JFA, TT_UPLIFT_SEPARABLE = 0, 1
+TT_EXPORT_HEIGHT16, TT_EXPORT_PLATES_RGB, TT_EXPORT_COLLISION_CLASSES, TT_EXPORT_PROPAGATED_CLASSES = 0, 1, 2, 3
 
 TT_STAGE_COLLIDE = 1 << 0
 TT_STAGE_UPLIFT = 1 << 1
@@ -367,6 +368,9 @@ def lib() -> C.CDLL:
     L.tt_server_drain.argtypes = [vp, C.c_int32]
     L.tt_server_stats.argtypes = [vp, C.POINTER(ServerReport)]
     L.tt_server_destroy.argtypes = [vp]
+    L.tt_world_diagnostics_json.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.tt_world_save.argtypes = [vp, C.c_char_p]
+    L.tt_world_load.argtypes = [C.c_char_p, C.POINTER(vp)]
     _lib = L
     return L
 

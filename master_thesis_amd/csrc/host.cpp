@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <string>
@@ -15,6 +16,9 @@
 
 #include "../../include/tectonic.h"
 #include "tt_math.h"
+
+void tt_set_error(const std::string &msg);
+int collision_classes(tt_world *w, size_t n, bool propagated, std::vector<uint8_t> &cls);
 
 // KernelSettings defaults (src/cuda_compute/kernel_settings.cuh:6-71)
 void tt_settings_fill_defaults(tt_kernel_settings *k)
@@ -302,6 +306,11 @@ int write_png(const char *path, int w, int h, int bit_depth, int color_type, int
     return wr == out.size() ? TT_OK : TT_ERR_INVALID;
 }
 
+// colours of the collision classes 0..5: none, divergence, subduction,
+// continental with 2 / 3 / 4 plates
+const uint8_t kClassColor[6][3] = {{20, 20, 24}, {70, 130, 220}, {220, 80, 60}, {240, 200, 80}, {250, 150, 40},
+                                   {255, 255, 255}};
+
 void plate_color(uint8_t val, uint8_t *rgb)
 {
     if (val == 255) { rgb[0] = 230; rgb[1] = 235; rgb[2] = 240; return; }
@@ -366,6 +375,235 @@ extern "C" int tt_world_export_png(tt_world *w, int32_t what, const char *path)
             for (size_t i = 0; i < n; ++i) plate_color(ids[i], &px[3 * i]);
             return write_png(path, width, height, 8, 2, 3, px, false);
         }
+        if (what == TT_EXPORT_COLLISION_CLASSES || what == TT_EXPORT_PROPAGATED_CLASSES) {
+            std::vector<uint8_t> cls;
+            const int s2 = collision_classes(w, n, what == TT_EXPORT_PROPAGATED_CLASSES, cls);
+            if (s2 != TT_OK) return s2;
+            std::vector<uint8_t> px(n * 3);
+            for (size_t i = 0; i < n; ++i) memcpy(&px[3 * i], kClassColor[cls[i] < 6 ? cls[i] : 0], 3);
+            return write_png(path, width, height, 8, 2, 3, px, false);
+        }
     }
     return TT_ERR_INVALID;
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics: per-plate records (copyPlateDataGuiKernel,
+// plate_tectonics_kernel.cu:1220-1281), collision classes
+// (createVisualizationCollisionTexture / ...PropagatedCollisionTexture,
+// :3250-3313), step statistics and field summaries, as JSON
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Json {
+    std::string s;
+    bool first = true;
+    void raw(const std::string &t) { s += t; }
+    void key(const char *k)
+    {
+        if (!first) s += ", ";
+        first = false;
+        s += "\"";
+        s += k;
+        s += "\": ";
+    }
+    void num(const char *k, double v)
+    {
+        key(k);
+        char b[64];
+        if (std::isfinite(v)) snprintf(b, sizeof b, "%.9g", v);
+        else snprintf(b, sizeof b, "null");
+        s += b;
+    }
+    void integer(const char *k, long long v)
+    {
+        key(k);
+        s += std::to_string(v);
+    }
+    void open(const char *k, char c)
+    {
+        key(k);
+        s += c;
+        first = true;
+    }
+    void close(char c)
+    {
+        s += c;
+        first = false;
+    }
+};
+
+unsigned tri_index(unsigned i, unsigned j) { return i * (2u * 255u - i - 1u) / 2u + (j - i - 1u); }
+
+}  // namespace
+
+int collision_classes(tt_world *w, size_t n, bool propagated, std::vector<uint8_t> &cls)
+{
+    cls.assign(n, 0);
+    if (!propagated) {
+        std::vector<uint32_t> word(n);
+        std::vector<uint8_t> up(n * 8);
+        int st = tt_world_download(w, TT_FIELD_COLLISIONS, word.data(), n * 4);
+        if (st == TT_OK) st = tt_world_download(w, TT_FIELD_UPLIFT, up.data(), n * 8);
+        if (st != TT_OK) return st;
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t p = word[i];
+            const uint8_t types = up[8 * i + 4];
+            uint8_t v = 0;
+            if (p == 0) v = 1;
+            else if (p >> 8 != 0) v = (types & 1) == 0 ? 2 : (p >> 16 == 0 ? 3 : (p >> 24 == 0 ? 4 : 5));
+            cls[i] = v;
+        }
+    } else {
+        std::vector<uint8_t> pr(n * 12);
+        const int st = tt_world_download(w, TT_FIELD_PROPAGATED, pr.data(), n * 12);
+        if (st != TT_OK) return st;
+        for (size_t i = 0; i < n; ++i) {
+            uint32_t p;
+            memcpy(&p, &pr[12 * i + 4], 4);
+            const uint8_t types = pr[12 * i + 8];
+            uint8_t v = 0;
+            if (p == 0) v = 1;
+            else if (p >> 8 != 0) v = (types & 1) == 0 ? 2 : (p >> 16 == 0 ? 3 : (p >> 24 == 0 ? 4 : 5));
+            cls[i] = v;
+        }
+    }
+    return TT_OK;
+}
+
+extern "C" int tt_world_diagnostics_json(tt_world *w, char *buf, size_t cap, size_t *len)
+{
+    if (!w || !len) return TT_ERR_INVALID;
+    int32_t width = 0, height = 0;
+    tt_kernel_settings ks;
+    int st = tt_world_geometry(w, &width, &height, &ks);
+    if (st != TT_OK) return st;
+    const size_t n = (size_t)width * (size_t)height;
+    tt_step_stats ss;
+    std::vector<tt_plate_data> pd(255);
+    std::vector<uint8_t> types(32385);
+    std::vector<float> hgt(n), water(n), sed(n);
+    st = tt_world_stats(w, &ss);
+    if (st == TT_OK) st = tt_world_download(w, TT_FIELD_PLATE_DATA, pd.data(), pd.size() * sizeof(tt_plate_data));
+    if (st == TT_OK) st = tt_world_download(w, TT_FIELD_COLLISION_TYPES, types.data(), types.size());
+    if (st == TT_OK) st = tt_world_download(w, TT_FIELD_HEIGHT, hgt.data(), n * 4);
+    if (st == TT_OK) st = tt_world_download(w, TT_FIELD_WATER, water.data(), n * 4);
+    if (st == TT_OK) st = tt_world_download(w, TT_FIELD_SEDIMENT, sed.data(), n * 4);
+    if (st != TT_OK) return st;
+    std::vector<uint8_t> cls, pcls;
+    st = collision_classes(w, n, false, cls);
+    if (st == TT_OK) st = collision_classes(w, n, true, pcls);
+    if (st != TT_OK) return st;
+
+    Json j;
+    j.raw("{");
+    j.integer("width", width);
+    j.integer("height", height);
+    j.integer("iteration", ss.iteration);
+    j.open("stats", '{');
+    j.integer("largest_value", ss.largest_value);
+    j.integer("largest_plate_id", ss.largest_plate_id);
+    j.num("heaviest_value", ss.heaviest_value);
+    j.integer("heaviest_plate_id", ss.heaviest_plate_id);
+    j.integer("number_of_plates", ss.number_of_plates);
+    j.integer("overflow_cells", ss.overflow_cells);
+    j.integer("num_labels", ss.num_labels);
+    j.integer("unassigned_cells", ss.unassigned_cells);
+    j.integer("merges", ss.merges);
+    j.integer("split_plate", ss.split_plate);
+    j.integer("split_new_plate", ss.split_new_plate);
+    j.integer("complex_sources", ss.complex_sources);
+    j.num("mass_ledger", ss.mass_ledger);
+    j.close('}');
+    // field summaries (fp64 sums on the host)
+    double hmin = 0, hmax = 0, hsum = 0, wsum = 0, wmax = 0, ssum = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const double h = hgt[i];
+        if (i == 0 || h < hmin) hmin = h;
+        if (i == 0 || h > hmax) hmax = h;
+        hsum += h;
+        wsum += water[i];
+        wmax = std::max(wmax, (double)water[i]);
+        ssum += sed[i];
+    }
+    j.open("fields", '{');
+    j.num("height_min", hmin);
+    j.num("height_max", hmax);
+    j.num("height_mean", n ? hsum / (double)n : 0.0);
+    j.num("water_total", wsum);
+    j.num("water_max", wmax);
+    j.num("sediment_total", ssum);
+    j.close('}');
+    const char *names[6] = {"none", "divergence", "subduction", "continental_2", "continental_3", "continental_4"};
+    for (int pass = 0; pass < 2; ++pass) {
+        long long cnt[6] = {0, 0, 0, 0, 0, 0};
+        for (uint8_t v : pass ? pcls : cls) ++cnt[v < 6 ? v : 0];
+        j.open(pass ? "propagated_classes" : "collision_classes", '{');
+        for (int k = 0; k < 6; ++k) j.integer(names[k], cnt[k]);
+        j.close('}');
+    }
+    j.open("plates", '[');
+    for (int p = 0; p < 255; ++p) {
+        const tt_plate_data &d = pd[(size_t)p];
+        if (d.size <= 0) continue;
+        if (!j.first) j.raw(", ");
+        j.raw("{");
+        j.first = true;
+        j.integer("id", p);
+        j.integer("size", d.size);
+        j.num("mass", d.mass);
+        j.integer("perimeter", d.perimeter);
+        j.integer("has_moved", d.hasMoved);
+        j.num("velocity", d.velocity);
+        j.num("velocity_smoothed", d.velocitySmoothed);
+        j.open("direction", '[');
+        j.raw(std::to_string(d.direction[0]) + ", " + std::to_string(d.direction[1]));
+        j.close(']');
+        j.open("direction_smoothed", '[');
+        j.raw(std::to_string(d.directionSmoothed[0]) + ", " + std::to_string(d.directionSmoothed[1]));
+        j.close(']');
+        j.open("center", '[');
+        j.raw(std::to_string(d.geometricCenter[0]) + ", " + std::to_string(d.geometricCenter[1]));
+        j.close(']');
+        j.num("break_score", d.breakScore);
+        j.num("circularity", d.circularity);
+        j.integer("merge_wait_time", d.mergeWaitTime);
+        // partners from the collision-type matrix: continental partners, and
+        // subductions as +other (this plate over) or -other (this plate under)
+        std::string cont, subd;
+        for (int q = 0; q < 255; ++q) {
+            if (q == p) continue;
+            const unsigned lo = (unsigned)std::min(p, q), hi = (unsigned)std::max(p, q);
+            const uint8_t t = types[tri_index(lo, hi)];
+            if (!(t & 1)) continue;
+            const bool continental = (t >> 1) & 1, polarity = (t >> 3) & 1;
+            if (continental) {
+                cont += (cont.empty() ? "" : ", ") + std::to_string(q);
+            } else {
+                const int v = polarity ? ((int)lo == p ? (int)hi : -(int)lo) : ((int)lo == p ? -(int)hi : (int)lo);
+                subd += (subd.empty() ? "" : ", ") + std::to_string(v);
+            }
+        }
+        j.open("continental", '[');
+        j.raw(cont);
+        j.close(']');
+        j.open("subductions", '[');
+        j.raw(subd);
+        j.close(']');
+        j.raw("}");
+        j.first = false;
+    }
+    j.close(']');
+    j.raw("}");
+    *len = j.s.size();
+    if (buf && cap > j.s.size()) {
+        memcpy(buf, j.s.data(), j.s.size());
+        buf[j.s.size()] = '\0';
+        return TT_OK;
+    }
+    if (buf) {
+        tt_set_error("diagnostics buffer too small: " + std::to_string(j.s.size() + 1) + " bytes needed");
+        return TT_ERR_INVALID;
+    }
+    return TT_OK;
 }

@@ -9,6 +9,8 @@
 // back small control values where the reference's control flow depends on
 // device data (CCL fragment count, neighbour-fill rounds, split decision and
 // relaxation convergence).
+#include <zlib.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -1058,6 +1060,119 @@ int tt_world_upload(tt_world *w, int32_t field, const void *host, size_t bytes)
     CK(hipMemcpyAsync(p, host, b, hipMemcpyHostToDevice, w->stream));
     CK(hipStreamSynchronize(w->stream));
     w->initialized = true;
+    return TT_OK;
+}
+
+// ---- snapshot file ------------------------------------------------------
+// "TTSNAP1\n", u32 header bytes, tt_sim_config, tt_kernel_settings, u32
+// iteration, tt_step_stats, u32 field count; per field: i32 id, u64 bytes, u32
+// CRC32 of the bytes, the bytes.
+static const int kSnapFields[] = {TT_FIELD_HEIGHT, TT_FIELD_PLATE_IDS, TT_FIELD_WATER, TT_FIELD_FLUX,
+                                  TT_FIELD_VELOCITY, TT_FIELD_SEDIMENT, TT_FIELD_UPLIFT, TT_FIELD_PLATE_DATA,
+                                  TT_FIELD_DIVERGENCE_BITMAP, TT_FIELD_COLLISION_TYPES, TT_FIELD_LABELS};
+static const char kSnapMagic[8] = {'T', 'T', 'S', 'N', 'A', 'P', '1', '\n'};
+
+struct snap_header {
+    tt_sim_config cfg;
+    tt_kernel_settings ks;
+    uint32_t iteration;
+    tt_step_stats stats;
+    uint32_t nfields;
+};
+
+int tt_world_save(tt_world *w, const char *path)
+{
+    if (!w || !path) return TT_ERR_INVALID;
+    if (!w->initialized) { set_error("tt_world_save before tt_world_init"); return TT_ERR_STATE; }
+    FILE *f = fopen(path, "wb");
+    if (!f) { set_error(std::string("cannot open ") + path); return TT_ERR_INVALID; }
+    snap_header hd;
+    memset(&hd, 0, sizeof hd);
+    hd.cfg = w->cfg;
+    hd.ks = w->ks;
+    hd.iteration = w->iter;
+    hd.stats = w->stats;
+    hd.nfields = (uint32_t)(sizeof kSnapFields / sizeof kSnapFields[0]);
+    const uint32_t hb = (uint32_t)sizeof hd;
+    bool ok = fwrite(kSnapMagic, 8, 1, f) == 1 && fwrite(&hb, 4, 1, f) == 1 && fwrite(&hd, sizeof hd, 1, f) == 1;
+    std::vector<uint8_t> buf;
+    for (int fid : kSnapFields) {
+        if (!ok) break;
+        void *p;
+        size_t b;
+        if (!field_ptr(w, fid, &p, &b, false)) { ok = false; break; }
+        buf.resize(b);
+        const int st = tt_world_download(w, fid, buf.data(), b);
+        if (st != TT_OK) { fclose(f); return st; }
+        const int32_t id = fid;
+        const uint64_t bytes = b;
+        const uint32_t crc = (uint32_t)crc32(0L, buf.data(), (uInt)b);
+        ok = fwrite(&id, 4, 1, f) == 1 && fwrite(&bytes, 8, 1, f) == 1 && fwrite(&crc, 4, 1, f) == 1 &&
+             fwrite(buf.data(), 1, b, f) == b;
+    }
+    if (fclose(f) != 0) ok = false;
+    if (!ok) { set_error(std::string("writing ") + path + " failed"); return TT_ERR_INVALID; }
+    return TT_OK;
+}
+
+int tt_world_load(const char *path, tt_world **out)
+{
+    if (!path || !out) return TT_ERR_INVALID;
+    *out = nullptr;
+    FILE *f = fopen(path, "rb");
+    if (!f) { set_error(std::string("cannot open ") + path); return TT_ERR_INVALID; }
+    char magic[8];
+    uint32_t hb = 0;
+    snap_header hd;
+    if (fread(magic, 8, 1, f) != 1 || memcmp(magic, kSnapMagic, 8) != 0 || fread(&hb, 4, 1, f) != 1 ||
+        hb != sizeof hd || fread(&hd, sizeof hd, 1, f) != 1) {
+        fclose(f);
+        set_error(std::string(path) + " is not a version-1 world snapshot");
+        return TT_ERR_INVALID;
+    }
+    // read and check every field before any device work
+    struct blob { int32_t id; std::vector<uint8_t> data; };
+    std::vector<blob> blobs;
+    for (uint32_t k = 0; k < hd.nfields; ++k) {
+        int32_t id = 0;
+        uint64_t bytes = 0;
+        uint32_t crc = 0;
+        if (fread(&id, 4, 1, f) != 1 || fread(&bytes, 8, 1, f) != 1 || fread(&crc, 4, 1, f) != 1 ||
+            bytes > (uint64_t)16384 * 16384 * 16) {
+            fclose(f);
+            set_error(std::string(path) + ": truncated field header");
+            return TT_ERR_INVALID;
+        }
+        blob b;
+        b.id = id;
+        b.data.resize((size_t)bytes);
+        if (fread(b.data.data(), 1, (size_t)bytes, f) != (size_t)bytes ||
+            (uint32_t)crc32(0L, b.data.data(), (uInt)bytes) != crc) {
+            fclose(f);
+            set_error(std::string(path) + ": field " + std::to_string(id) + " is truncated or corrupt");
+            return TT_ERR_INVALID;
+        }
+        blobs.push_back(std::move(b));
+    }
+    fclose(f);
+    CKS(tt_validate_config(&hd.cfg));
+    tt_sim_config c = hd.cfg;
+    c.init_mode = TT_INIT_EMPTY;
+    tt_world *w = nullptr;
+    CKS(tt_world_create(&c, &hd.ks, &w));
+    int st = tt_world_init(w);
+    for (const blob &b : blobs) {
+        if (st != TT_OK) break;
+        st = tt_world_upload(w, b.id, b.data.data(), b.data.size());
+    }
+    if (st != TT_OK) {
+        tt_world_destroy(w);
+        return st;
+    }
+    w->cfg = hd.cfg;
+    w->iter = hd.iteration;
+    w->stats = hd.stats;
+    *out = w;
     return TT_OK;
 }
 

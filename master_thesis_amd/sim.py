@@ -107,6 +107,32 @@ class World:
     def export_png(self, what: int, path: str) -> None:
         check(lib().tt_world_export_png(self.h, what, path.encode()), "export_png")
 
+    def diagnostics(self) -> dict:
+        """Step statistics, field summaries, collision-class counts and one
+        record per live plate (tt_world_diagnostics_json)."""
+        import json
+        n = C.c_size_t()
+        check(lib().tt_world_diagnostics_json(self.h, None, 0, C.byref(n)), "diagnostics")
+        buf = C.create_string_buffer(n.value + 1)
+        check(lib().tt_world_diagnostics_json(self.h, buf, n.value + 1, C.byref(n)), "diagnostics")
+        return json.loads(buf.value.decode())
+
+    def save(self, path: str) -> None:
+        check(lib().tt_world_save(self.h, str(path).encode()), "tt_world_save")
+
+    @classmethod
+    def load(cls, path: str) -> "World":
+        h = C.c_void_p()
+        check(lib().tt_world_load(str(path).encode(), C.byref(h)), "tt_world_load")
+        w = cls._adopt(h, _abi.SimConfig())
+        return w._refresh_geometry()
+
+    def _refresh_geometry(self) -> "World":
+        d = self.diagnostics()
+        self.cfg.width, self.cfg.height = d["width"], d["height"]
+        self.n = self.cfg.width * self.cfg.height
+        return self
+
 
 class Batch:
     """Many worlds stepped together (tt_batch_*): one launch per erosion pass

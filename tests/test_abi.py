@@ -102,3 +102,14 @@ def test_no_device_is_an_error_not_a_crash(tectonic_lib):
         cfg, ks = configs.synthetic(64, 64, 2, (0.5, 1.0))
         h = C.c_void_p()
         assert tectonic_lib.tt_world_create(C.byref(cfg), C.byref(ks), C.byref(h)) == -2
+
+
+def test_snapshot_load_rejects_bad_files(tectonic_lib, tmp_path):
+    """tt_world_load checks the magic, the header and every field's CRC before
+    any device work, so bad files fail without a GPU."""
+    vp = C.c_void_p()
+    p = tmp_path / "junk.snap"
+    p.write_bytes(b"not a snapshot at all")
+    assert tectonic_lib.tt_world_load(str(p).encode(), C.byref(vp)) == _abi.TT_ERR_INVALID
+    assert b"snapshot" in tectonic_lib.tt_last_error()
+    assert tectonic_lib.tt_world_load(str(tmp_path / "missing.snap").encode(), C.byref(vp)) == _abi.TT_ERR_INVALID
