@@ -175,7 +175,7 @@ def main() -> int:
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=0, help="erosion tile rows (0: autotuned)")
-    ap.add_argument("--mode", choices=["plain", "batch", "serve", "all"], default="all",
+    ap.add_argument("--mode", choices=["plain", "batch", "serve", "precision", "all"], default="all",
                     help="plain: the 4096^2 step (the metric); batch / serve: the batched and served forms; "
                          "all: the plain line with batched and served samples added")
     ap.add_argument("--batch-worlds", type=int, default=256)
@@ -183,6 +183,9 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--precision-side", type=int, default=8192)
+    ap.add_argument("--precision-plates", type=int, default=64)
+    ap.add_argument("--precision-steps", type=int, default=300)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -193,6 +196,22 @@ def main() -> int:
 
     from master_thesis_amd import _abi, configs
     from master_thesis_amd.sim import World, STATE_FIELDS
+
+    if args.mode == "precision":
+        from master_thesis_amd.sim import precision_study
+        pc, pk = configs.synthetic(args.precision_side, args.precision_side, args.precision_plates, (0.5, 3.0),
+                                   args.seed, hydraulic=50, thermal=20, ccl_every=1, uplift_range=5)
+        rep = precision_study(pc, pk, args.precision_steps)
+        line = {"metric": "erosion error vs an fp64 twin (fp32 and compensated fp32), mass drift",
+                "value": rep["height"]["rms_comp"], "unit": "height RMS error (compensated fp32)",
+                "n_gpus": 1, "steps": args.precision_steps, "warmup": 0, "higher_is_better": False,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32+compensated / f64",
+                "data": "synthetic (seeded uniform heights, Gaussian sigma 8)",
+                "config": {"workload": f"{args.precision_side}x{args.precision_side}, {args.precision_plates} plates, "
+                                       f"{args.precision_steps} erosion-only steps (K=50, T=20)"}}
+        line.update(rep)
+        print(json.dumps(line))
+        return 0
 
     if args.mode in ("batch", "serve"):
         out = run_batch_mode(args) if args.mode == "batch" else run_serve_mode(args)

@@ -145,7 +145,9 @@ typedef struct tt_sim_config {
     int32_t tile_rows;             /* rows of the hydraulic tiles: 8, 16 or 32 (dividing height);
                                       0 = chosen by the tile autotuner (thermal tiles: 16 rows) */
     int32_t uplift_mode;           /* TT_UPLIFT_JFA (the reference's live path) or TT_UPLIFT_SEPARABLE */
-    int32_t reserved[5];
+    int32_t precision;             /* TT_PRECISION_FP32 (reference arithmetic, bit-exact with the oracle)
+                                      or TT_PRECISION_COMPENSATED (Kahan height / sediment updates) */
+    int32_t reserved[4];
 } tt_sim_config;
 
 /* Uplift spread.  JFA: jump flood + applyUplift (plate_tectonic_sim.cu:283-317,
@@ -154,6 +156,14 @@ typedef struct tt_sim_config {
  * call is commented out at plate_tectonic_sim.cu:319-327), radius
  * KernelSettings.upliftRange. */
 enum { TT_UPLIFT_JFA = 0, TT_UPLIFT_SEPARABLE = 1 };
+
+/* Arithmetic of the erosion stages.  FP32: the reference's float arithmetic
+ * (results bit-identical to the CPU oracle).  COMPENSATED: the height and
+ * sediment updates of thermal and hydraulic erosion carry per-cell Kahan
+ * compensation terms (the sediment term travels with the sediment transport);
+ * the height term restarts whenever collisions move heights between cells.
+ * Not bit-comparable with the oracle; see tt_precision_study. */
+enum { TT_PRECISION_FP32 = 0, TT_PRECISION_COMPENSATED = 1 };
 
 /* struct PlateData (src/cuda_compute/types/plate_data.h:5-38), 84 bytes, same
  * field order and layout; the plate table has 255 entries (MAX_PLATE_COUNT,
@@ -358,6 +368,29 @@ int tt_server_drain(tt_server *s, int32_t timeout_ms);
 int tt_server_stats(tt_server *s, tt_server_report *out);
 /* Stops the scheduler; queued and running jobs end with TT_ERR_STATE. */
 int tt_server_destroy(tt_server *s);
+
+/* ---- precision study (no reference counterpart) ----------------------------
+ * Runs `steps` erosion-only steps (thermal + hydraulic stages, so every
+ * variant takes the same discrete decisions) of the configured world three
+ * times from one initial state: TT_PRECISION_FP32, TT_PRECISION_COMPENSATED and
+ * an fp64 twin of the same formulas, and reports each float variant's error
+ * against the fp64 twin and the drift of sum(height + sediment). */
+typedef struct tt_precision_field {
+    double max_abs_fp32, rms_fp32;   /* TT_PRECISION_FP32 - fp64 twin */
+    double max_abs_comp, rms_comp;   /* TT_PRECISION_COMPENSATED - fp64 twin */
+    double rms_ref;                  /* RMS of the fp64 field (scale) */
+} tt_precision_field;
+typedef struct tt_precision_report {
+    int32_t steps, width, height_cells, reserved0;
+    tt_precision_field height, water, sediment;
+    double mass_start;               /* sum of height + sediment before the first step */
+    double mass_ref, mass_fp32, mass_comp;
+    double drift_ref, drift_fp32, drift_comp;   /* (end - start) / start */
+    double ms_fp32, ms_comp, ms_ref;            /* device time per step */
+    double reserved[4];
+} tt_precision_report;
+int tt_precision_study(const tt_sim_config *cfg, const tt_kernel_settings *ks, int32_t steps,
+                       tt_precision_report *out);
 
 /* ---- timing (hipEvent around the stages of the last tt_world_step) -------- */
 typedef struct tt_stage_times {

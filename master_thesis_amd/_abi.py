@@ -21,6 +21,7 @@ STATUS_NAMES = {0: "ok", -1: "invalid argument", -2: "device error", -3: "out of
 
 TT_INIT_SYNTHETIC, TT_INIT_REFERENCE, TT_INIT_EMPTY = 0, 1, 2
 TT_UPLIFT_JFA, TT_UPLIFT_SEPARABLE = 0, 1
+TT_PRECISION_FP32, TT_PRECISION_COMPENSATED = 0, 1
 TT_EXPORT_HEIGHT16, TT_EXPORT_PLATES_RGB, TT_EXPORT_COLLISION_CLASSES, TT_EXPORT_PROPAGATED_CLASSES = 0, 1, 2, 3
 
 TT_STAGE_COLLIDE = 1 << 0
@@ -147,7 +148,8 @@ class SimConfig(C.Structure):
         ("stage_mask", C.c_int32),
         ("tile_rows", C.c_int32),
         ("uplift_mode", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("precision", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
     @classmethod
@@ -255,6 +257,17 @@ class ServerReport(C.Structure):
                                            "pending", "bytes_in_use", "bytes_peak", "budget", "batch_steps")] + [
         (n, C.c_double) for n in ("wall_s", "worlds_per_s", "latency_p50_s", "latency_p99_s", "latency_mean_s",
                                   "latency_max_s", "cell_updates")] + [("reserved", C.c_int64 * 4)]
+
+
+class PrecisionField(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("max_abs_fp32", "rms_fp32", "max_abs_comp", "rms_comp", "rms_ref")]
+
+
+class PrecisionReport(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("width", C.c_int32), ("height_cells", C.c_int32), ("reserved0", C.c_int32),
+                ("height", PrecisionField), ("water", PrecisionField), ("sediment", PrecisionField)] + [
+        (n, C.c_double) for n in ("mass_start", "mass_ref", "mass_fp32", "mass_comp", "drift_ref", "drift_fp32",
+                                  "drift_comp", "ms_fp32", "ms_comp", "ms_ref")] + [("reserved", C.c_double * 4)]
 
 
 # numpy-compatible record layouts of the per-cell structs
@@ -371,6 +384,8 @@ def lib() -> C.CDLL:
     L.tt_world_diagnostics_json.argtypes = [vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.tt_world_save.argtypes = [vp, C.c_char_p]
     L.tt_world_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.tt_precision_study.argtypes = [C.POINTER(SimConfig), C.POINTER(KernelSettings), C.c_int32,
+                                     C.POINTER(PrecisionReport)]
     _lib = L
     return L
 

@@ -26,7 +26,7 @@ LIB = PKG / "libtectonic.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TT_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["k_collide.hip", "k_uplift.hip", "k_erosion.hip", "k_plates.hip", "k_misc.hip", "k_batch.hip",
-               "world.hip", "batch.hip"]
+               "world.hip", "batch.hip", "precision.hip"]
 HOST_SOURCES = ["host.cpp", "server.cpp"]
 HEADERS = ["tt_math.h", "tt_noise.h", "tt_device.h", "tt_stage.h", "tt_kernels.h", "tt_world.h", "tt_erosion.h",
            "tt_batch.h"]

@@ -310,6 +310,7 @@ int tt_batch_add(tt_batch *b, tt_world *w)
     if (!b || !w) return TT_ERR_INVALID;
     if (!w->initialized) { tt_set_error("tt_batch_add: world not initialised"); return TT_ERR_STATE; }
     if (w->device != b->device) { tt_set_error("tt_batch_add: world on another device"); return TT_ERR_INVALID; }
+    if (w->ch) { tt_set_error("tt_batch_add: compensated-precision worlds step alone"); return TT_ERR_UNSUPPORTED; }
     if (std::find(b->worlds.begin(), b->worlds.end(), w) != b->worlds.end()) return TT_OK;
     if ((int)b->worlds.size() >= TT_BATCH_MAX_WORLDS) { tt_set_error("batch is full"); return TT_ERR_UNSUPPORTED; }
     b->worlds.push_back(w);

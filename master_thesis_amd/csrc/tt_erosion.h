@@ -36,7 +36,17 @@ struct tt_therm_args {
     float talus, amp, str;
     const float *hin;
     float *hout;
+    float *ch;      // precision mode: Kahan compensation of the height (in place)
 };
+
+// Kahan-compensated sum += d: c carries the low-order part lost so far
+__device__ __forceinline__ float tt_kahan_add(float sum, float &c, float d)
+{
+    const float y = d - c;
+    const float t = sum + y;
+    c = (t - sum) - y;
+    return t;
+}
 
 template <int TH>
 struct tt_therm_lds {
@@ -55,10 +65,11 @@ __host__ __device__ inline tt_therm_args tt_therm_make(const tt_dev &d, const fl
     A.str = d.ks.thermalErosionStrength;
     A.hin = hin;
     A.hout = hout;
+    A.ch = nullptr;
     return A;
 }
 
-template <int TH>
+template <int TH, bool COMP = false>
 __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm_lds<TH> &S, int x0, int y0)
 {
     constexpr int RPW = TH / 4;
@@ -139,15 +150,25 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
         // own outflows (directions 0..7), then inflows (directions 0..7); the
         // divisions sit in branches that a wavefront skips when no lane needs them
         float v = hc;
+        const int cidx = (y0 + ry) * W + x0 + lane;
+        float cc = 0.0f;
+        if (COMP) cc = A.ch[cidx];
         if (tc > 0.0f) {
 #pragma unroll
             for (int k = 0; k < 8; ++k)
-                if (dOut[k] > talus) v = v - (amp * (dOut[k] / tc)) * str;
+                if (dOut[k] > talus) {
+                    const float q = (amp * (dOut[k] / tc)) * str;
+                    v = COMP ? tt_kahan_add(v, cc, -q) : v - q;
+                }
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            if (dIn[k] > talus && tIn[k] > 0.0f) v = v + (amp * (dIn[k] / tIn[k])) * str;
-        A.hout[(y0 + ry) * W + x0 + lane] = v;
+            if (dIn[k] > talus && tIn[k] > 0.0f) {
+                const float q = (amp * (dIn[k] / tIn[k])) * str;
+                v = COMP ? tt_kahan_add(v, cc, q) : v + q;
+            }
+        if (COMP) A.ch[cidx] = cc;
+        A.hout[cidx] = v;
     }
 }
 
@@ -166,6 +187,9 @@ struct tt_hyd_args {
     float2 *vel;
     const float *sprev;
     float *sout;
+    float *ch;            // precision mode: height compensation (in place)
+    const float *csprev;  // precision mode: sediment compensation, gathered like sprev
+    float *csout;
 };
 
 template <int TH>
@@ -225,7 +249,7 @@ __device__ __forceinline__ float tt_rain_at(int gx, int gy, int rseed, float rai
 // the previous one on its final write) and the sediment is the previous
 // sub-iteration's output transported along its velocity.  LAST: the water
 // output gets no rain.
-template <int TH, bool FIRST, bool LAST>
+template <int TH, bool FIRST, bool LAST, bool COMP = false>
 __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_lds<TH> &S, int x0, int y0)
 {
     constexpr int RPW = TH / 4;
@@ -314,12 +338,14 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         if (!LAST) wo = wo + tt_rain_at(gx, gy, rseed, rain);
         A.wout[c] = wo;
         A.fout[c] = Sf;
-        float sed;
+        float sed, cs = 0.0f;
         if (FIRST) {
             sed = A.sprev[c];
+            if (COMP) cs = A.csprev[c];
         } else {
             const float2 pv = A.vel[c];
             sed = tt_bilinear(A.sprev, (float)gx - pv.x * 1.0f, (float)gy - pv.y * 1.0f, W, H);
+            if (COMP) cs = tt_bilinear(A.csprev, (float)gx - pv.x * 1.0f, (float)gy - pv.y * 1.0f, W, H);
         }
         A.vel[c] = make_float2(vx, vy);
         const float mc = S.M[ly][lx];
@@ -330,16 +356,32 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         float C = (cap * tt_sinf(tt_fmaxf(slope, 0.03f))) * vmag;
         C = tt_fminf(C, 5.0f);
         float hnew;
+        float chv = 0.0f;
+        if (COMP) chv = A.ch[c];
         if (C > sed) {
             float s = (ke * (C - sed)) * 1.0f;
             s = tt_fminf(s, mc);
-            hnew = mc - s;
-            sed = sed + s;
+            if (COMP) {
+                hnew = tt_kahan_add(mc, chv, -s);
+                sed = tt_kahan_add(sed, cs, s);
+            } else {
+                hnew = mc - s;
+                sed = sed + s;
+            }
         } else {
             float s = (kd * (sed - C)) * 1.0f;
             s = tt_fminf(s, sed);
-            sed = sed - s;
-            hnew = mc + s;
+            if (COMP) {
+                sed = tt_kahan_add(sed, cs, -s);
+                hnew = tt_kahan_add(mc, chv, s);
+            } else {
+                sed = sed - s;
+                hnew = mc + s;
+            }
+        }
+        if (COMP) {
+            A.ch[c] = chv;
+            A.csout[c] = cs;
         }
         A.hout[c] = hnew;
         A.sout[c] = sed;

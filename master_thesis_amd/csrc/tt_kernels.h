@@ -57,11 +57,15 @@ hipError_t tt_launch_separable_uplift(hipStream_t s, const tt_dev &d, const tt_p
 
 // ---- stage D/E (k_erosion.hip) -----------------------------------------------
 int tt_tile_rows_valid(int rows, int H);
-hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, int rows, const float *hin, float *hout);
+// ch / csprev / csout: Kahan compensation arrays of the precision mode (null: plain)
+hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, int rows, const float *hin, float *hout,
+                             float *ch = nullptr);
 hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int rows, int first, int last, int rseed,
                                const float *hin, float *hout, const float *win, float *wout, const float4 *fin,
-                               float4 *fout, float2 *vel, const float *sprev, float *sout);
-hipError_t tt_launch_transport(hipStream_t s, const tt_dev &d, const float *sprev, const float2 *vel, float *sout);
+                               float4 *fout, float2 *vel, const float *sprev, float *sout, float *ch = nullptr,
+                               const float *csprev = nullptr, float *csout = nullptr);
+hipError_t tt_launch_transport(hipStream_t s, const tt_dev &d, const float *sprev, const float2 *vel, float *sout,
+                               const float *csprev = nullptr, float *csout = nullptr);
 
 // ---- stages F/G (k_plates.hip) ----------------------------------------------
 hipError_t tt_launch_plate_movement(hipStream_t s, const tt_dev &d, tt_plate_data *pd, const long long *fric,

@@ -77,6 +77,8 @@ struct tt_world {
     float *eff = nullptr;
     uint32_t *rchg = nullptr;   // per-round change counters of the split fixed points (TT_RCHG)
     uint8_t *act = nullptr;     // per-tile activity flags of the split rounds (2 x tiles)
+    float *ch = nullptr;        // precision mode: height compensation
+    float *cs[2] = {nullptr, nullptr};  // precision mode: sediment compensation (ping-pong like sed)
 
     // scratch
     uint32_t *scanTmp = nullptr;

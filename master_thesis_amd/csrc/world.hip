@@ -161,6 +161,11 @@ size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w)
     take((void **)&w->eff, N * 4);
     take((void **)&w->rchg, TT_RCHG * 4);
     take((void **)&w->act, 2 * (N / 1024 + 64));
+    if (c.precision == TT_PRECISION_COMPENSATED) {
+        take((void **)&w->ch, N * 4);
+        take((void **)&w->cs[0], N * 4);
+        take((void **)&w->cs[1], N * 4);
+    }
     take((void **)&w->scanTmp, (N / 1024 + tt_scan_tmp_words((long long)N) + 4096) * 4);
     take((void **)&w->initSeeds, 4096 * 12);
     take((void **)&w->initWts, 128 * 8);
@@ -412,6 +417,9 @@ int tt_step_pre(tt_world *w)
     }
     CKS(record(w, 2));
     w->ehc = 1 - w->hc;   // erosion works on the written height field
+    // the height compensation belongs to a cell's height: it restarts when the
+    // collision stage has moved heights between cells
+    if (w->ch && (mask & TT_STAGE_COLLIDE)) CKS(zero(w, w->ch, N * 4));
     return TT_OK;
 }
 
@@ -441,7 +449,8 @@ void tt_step_erode_indices(tt_world *w, int T, int K, int *sedLast)
 int tt_step_transport(tt_world *w, int sedLast)
 {
     const tt_dev d = make_dev(w);
-    CK(tt_launch_transport(w->stream, d, w->sed[sedLast], w->vel, w->sed[1 - sedLast]));
+    CK(tt_launch_transport(w->stream, d, w->sed[sedLast], w->vel, w->sed[1 - sedLast], w->ch ? w->cs[sedLast] : nullptr,
+                           w->ch ? w->cs[1 - sedLast] : nullptr));
     w->sc = 1 - sedLast;
     return TT_OK;
 }
@@ -453,18 +462,18 @@ static int step_erode(tt_world *w)
     const int T = tt_step_thermal_count(w), K = tt_step_hydraulic_count(w);
     int hc = w->ehc;
     for (int it = 0; it < T; ++it) {
-        CK(tt_launch_thermal(s, d, TT_TILE_H, w->h[hc], w->h[1 - hc]));
+        CK(tt_launch_thermal(s, d, TT_TILE_H, w->h[hc], w->h[1 - hc], w->ch));
         hc = 1 - hc;
     }
     CKS(record(w, 3));
     const int rseed = (int)(w->cfg.seed + w->iter);
     int wc = w->wc, fc = w->fc;
     for (int it = 0; it < K; ++it) {
-        float *sprev, *sout;
-        if (it == 0) { sprev = w->sed[w->sc]; sout = w->sed[w->sc]; }
-        else { sprev = w->sed[w->sc ^ ((it - 1) & 1)]; sout = w->sed[w->sc ^ (it & 1)]; }
+        const int si = it == 0 ? w->sc : (w->sc ^ ((it - 1) & 1)), so = it == 0 ? w->sc : (w->sc ^ (it & 1));
+        float *sprev = w->sed[si], *sout = w->sed[so];
         CK(tt_launch_hydraulic(s, d, w->rows, it == 0, it == K - 1, rseed, w->h[hc], w->h[1 - hc], w->water[wc],
-                               w->water[1 - wc], w->flux[fc], w->flux[1 - fc], w->vel, sprev, sout));
+                               w->water[1 - wc], w->flux[fc], w->flux[1 - fc], w->vel, sprev, sout, w->ch,
+                               w->ch ? w->cs[si] : nullptr, w->ch ? w->cs[so] : nullptr));
         hc = 1 - hc;
         wc = 1 - wc;
         fc = 1 - fc;
@@ -603,6 +612,11 @@ static int init_common_zero(tt_world *w)
         CKS(zero(w, w->sed[i], N * 4));
     }
     CKS(zero(w, w->vel, N * 8));
+    if (w->ch) {
+        CKS(zero(w, w->ch, N * 4));
+        CKS(zero(w, w->cs[0], N * 4));
+        CKS(zero(w, w->cs[1], N * 4));
+    }
     CKS(zero(w, w->word, N * 4));
     CKS(zero(w, w->accretion, N));
     CKS(zero(w, w->accw, N * 4));
@@ -866,6 +880,10 @@ int tt_validate_config(const tt_sim_config *c)
     if (c->init_mode < 0 || c->init_mode > 2) { set_error("bad init_mode"); return TT_ERR_INVALID; }
     if (c->uplift_mode != TT_UPLIFT_JFA && c->uplift_mode != TT_UPLIFT_SEPARABLE) {
         set_error("uplift_mode must be TT_UPLIFT_JFA or TT_UPLIFT_SEPARABLE");
+        return TT_ERR_INVALID;
+    }
+    if (c->precision != TT_PRECISION_FP32 && c->precision != TT_PRECISION_COMPENSATED) {
+        set_error("precision must be TT_PRECISION_FP32 or TT_PRECISION_COMPENSATED");
         return TT_ERR_INVALID;
     }
     if (c->tile_rows != 0 && !tt_tile_rows_valid(c->tile_rows, c->height)) {

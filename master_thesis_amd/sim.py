@@ -271,6 +271,21 @@ def math_eval_device(fn: int, x: np.ndarray) -> np.ndarray:
     return out
 
 
+def precision_study(cfg: _abi.SimConfig, ks: _abi.KernelSettings, steps: int) -> dict:
+    """Erosion-only steps in fp32, compensated fp32 and an fp64 twin from one
+    initial state (tt_precision_study): per-field errors and mass drift."""
+    r = _abi.PrecisionReport()
+    check(lib().tt_precision_study(C.byref(cfg), C.byref(ks), int(steps), C.byref(r)), "tt_precision_study")
+    out = {"steps": r.steps, "width": r.width, "height": r.height_cells}
+    for f in ("height", "water", "sediment"):
+        pf = getattr(r, f)
+        out[f] = {k: getattr(pf, k) for k, _ in _abi.PrecisionField._fields_}
+    for k in ("mass_start", "mass_ref", "mass_fp32", "mass_comp", "drift_ref", "drift_fp32", "drift_comp",
+              "ms_fp32", "ms_comp", "ms_ref"):
+        out[k] = getattr(r, k)
+    return out
+
+
 def parse_json_config(text: str, cfg: _abi.SimConfig | None = None, ks: _abi.KernelSettings | None = None):
     cfg = cfg.copy() if cfg is not None else _abi.SimConfig.defaults()
     ks = ks.copy() if ks is not None else _abi.KernelSettings.defaults()

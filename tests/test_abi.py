@@ -24,6 +24,10 @@ def test_exports_every_declared_symbol(tectonic_lib):
         assert hasattr(tectonic_lib, n), n
 
 
+def test_precision_struct_sizes():
+    assert C.sizeof(_abi.PrecisionReport) == 16 + 3 * 5 * 8 + 10 * 8 + 4 * 8
+
+
 def test_server_struct_sizes():
     assert C.sizeof(_abi.JobResult) == 8 + 4 + 4 + 8 * 4 + 8 + 4 * 6
     assert C.sizeof(_abi.ServerReport) == 11 * 8 + 7 * 8 + 4 * 8
