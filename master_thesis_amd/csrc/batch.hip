@@ -202,6 +202,7 @@ void fill_desc(tt_bworld &B, const tt_world *w, int T, int K)
         B.sed[i] = w->sed[i];
     }
     B.vel = w->vel;
+    B.frozen = w->tileFrozen;
 }
 
 int launch_erosion(tt_batch *b, int Tmax, int Kmax)
@@ -252,6 +253,7 @@ int batch_step_once(tt_batch *b)
         tt_world *w = b->worlds[i];
         w->timing = false;
         BCKS(tt_step_pre(w));
+        BCKS(tt_step_thermal_prep(w));
         BCK(hipEventRecord(b->preDone[i], w->stream));
         return TT_OK;
     }));

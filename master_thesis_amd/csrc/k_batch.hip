@@ -30,6 +30,9 @@ __global__ __launch_bounds__(256) void k_thermal_batch(const tt_bworld *__restri
     A.str = B.str;
     A.hin = B.h[B.hc0 ^ (t & 1)];
     A.hout = B.h[B.hc0 ^ ((t + 1) & 1)];
+    A.ch = nullptr;
+    A.frozen = B.frozen;
+    A.pass = t;
     tt_thermal_tile<TT_TILE_H>(A, S, (tile % nbx) * TT_TILE_W, (tile / nbx) * TT_TILE_H);
 }
 
@@ -63,6 +66,9 @@ __global__ __launch_bounds__(256) void k_hydraulic_batch(const tt_bworld *__rest
     A.fin = B.flux[fc];
     A.fout = B.flux[fc ^ 1];
     A.vel = B.vel;
+    A.ch = nullptr;
+    A.csprev = nullptr;
+    A.csout = nullptr;
     if (k == 0) {
         A.sprev = B.sed[B.sc];
         A.sout = B.sed[B.sc];

@@ -25,10 +25,71 @@ __global__ __launch_bounds__(256) void k_hydraulic(tt_hyd_args A)
 
 int tt_tile_rows_valid(int rows, int H) { return (rows == 8 || rows == 16 || rows == 32) && (H % rows) == 0; }
 
-hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, int rows, const float *hin, float *hout, float *ch)
+// Per 64x16 tile: minimum and maximum height (a NaN yields the other operand).
+__global__ __launch_bounds__(256) void k_tile_minmax(tt_dev d, const float *__restrict__ h, float2 *__restrict__ mm)
+{
+    __shared__ float rmin[4], rmax[4];
+    const int nbx = d.W / TT_TILE_W;
+    const int x0 = (int)(blockIdx.x % (unsigned)nbx) * TT_TILE_W, y0 = (int)(blockIdx.x / (unsigned)nbx) * TT_TILE_H;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float mn = tt_u2f(0x7f800000u), mx = tt_u2f(0xff800000u);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const float v = h[(y0 + wv * 4 + r) * d.W + x0 + lane];
+        mn = tt_fminf(mn, v);
+        mx = tt_fmaxf(mx, v);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = tt_fminf(mn, __shfl_xor(mn, o, 64));
+        mx = tt_fmaxf(mx, __shfl_xor(mx, o, 64));
+    }
+    if (lane == 0) { rmin[wv] = mn; rmax[wv] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        mm[blockIdx.x] = make_float2(tt_fminf(tt_fminf(rmin[0], rmin[1]), tt_fminf(rmin[2], rmin[3])),
+                                     tt_fmaxf(tt_fmaxf(rmax[0], rmax[1]), tt_fmaxf(rmax[2], rmax[3])));
+}
+
+// A tile is frozen for T passes when the tile grown by T cells on every side
+// spans at most the talus: a cell changes in a pass only through an adjacent
+// pair differing by more than the talus, so after pass k the cells within
+// T - k of the tile are unchanged and the region stays within the talus.
+// The grown region is covered by the neighbouring tiles within ceil(T / 16)
+// tile rows and ceil(T / 64) tile columns (toroidal).
+__global__ __launch_bounds__(256) void k_tile_frozen(tt_dev d, const float2 *__restrict__ mm, int T,
+                                                     uint8_t *__restrict__ frozen)
+{
+    const int nbx = d.W / TT_TILE_W, nby = d.H / TT_TILE_H;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nbx * nby) return;
+    const int tx = t % nbx, ty = t / nbx;
+    const int ry = (T + TT_TILE_H - 1) / TT_TILE_H, rx = (T + TT_TILE_W - 1) / TT_TILE_W;
+    float mn = tt_u2f(0x7f800000u), mx = tt_u2f(0xff800000u);
+    for (int dy = -ry; dy <= ry; ++dy)
+        for (int dx = -rx; dx <= rx; ++dx) {
+            const float2 v = mm[tt_wrapg(ty + dy, nby) * nbx + tt_wrapg(tx + dx, nbx)];
+            mn = tt_fminf(mn, v.x);
+            mx = tt_fmaxf(mx, v.y);
+        }
+    const float talus = d.ks.thermalThresholdAngle * d.ks.thermalCellSize;
+    frozen[t] = (mx - mn <= talus) ? 1 : 0;
+}
+
+hipError_t tt_launch_thermal_frozen(hipStream_t s, const tt_dev &d, int T, const float *h, float2 *mm, uint8_t *frozen)
+{
+    const unsigned nt = (unsigned)((d.W / TT_TILE_W) * (d.H / TT_TILE_H));
+    hipLaunchKernelGGL(k_tile_minmax, dim3(nt), dim3(256), 0, s, d, h, mm);
+    hipLaunchKernelGGL(k_tile_frozen, dim3((nt + 255) / 256), dim3(256), 0, s, d, mm, T, frozen);
+    return hipGetLastError();
+}
+
+hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, int rows, const float *hin, float *hout, float *ch,
+                             const uint8_t *frozen, int pass)
 {
     tt_therm_args A = tt_therm_make(d, hin, hout);
     A.ch = ch;
+    A.frozen = rows == TT_TILE_H ? frozen : nullptr;
+    A.pass = pass;
     const unsigned blocks = (unsigned)((d.W / TT_TILE_W) * (d.H / rows));
     if (ch) {
         if (rows == 8) hipLaunchKernelGGL((k_thermal<8, true>), dim3(blocks), dim3(256), 0, s, A);

@@ -19,6 +19,7 @@ struct tt_bworld {
     float4 *flux[2];
     float *sed[2];
     float2 *vel;
+    const uint8_t *frozen;  // per 64x16 tile: unchanged by this step's thermal passes
 };
 
 // tile table entries: (world index << 20) | tile index within the world

@@ -36,7 +36,9 @@ struct tt_therm_args {
     float talus, amp, str;
     const float *hin;
     float *hout;
-    float *ch;      // precision mode: Kahan compensation of the height (in place)
+    float *ch;              // precision mode: Kahan compensation of the height (in place)
+    const uint8_t *frozen;  // per 64x16 tile: unchanged by every pass of this step (null: unknown)
+    int pass;               // pass index within the step
 };
 
 // Kahan-compensated sum += d: c carries the low-order part lost so far
@@ -66,6 +68,8 @@ __host__ __device__ inline tt_therm_args tt_therm_make(const tt_dev &d, const fl
     A.hin = hin;
     A.hout = hout;
     A.ch = nullptr;
+    A.frozen = nullptr;
+    A.pass = 0;
     return A;
 }
 
@@ -76,6 +80,18 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
     const int tid = threadIdx.x;
     const int W = A.W, H = A.H;
     const int lane = tid & 63, wv = tid >> 6;
+    // A frozen tile keeps its heights through every pass of the step: the
+    // first pass copies it to the other buffer, later passes find both
+    // buffers equal and do nothing.
+    if (A.frozen && TH == TT_TILE_H && A.frozen[(y0 / TT_TILE_H) * (W / TT_TILE_W) + x0 / TT_TILE_W]) {
+        if (A.pass == 0)
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int c = (y0 + wv * RPW + r) * W + x0 + lane;
+                A.hout[c] = A.hin[c];
+            }
+        return;
+    }
     float mn = tt_u2f(0x7f800000u), mx = tt_u2f(0xff800000u);
     {
         const int gxa = tt_wrap1(x0 + lane - 2, W), gxb = tt_wrap1(x0 + 64 + lane - 2, W);

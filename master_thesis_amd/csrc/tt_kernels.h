@@ -58,8 +58,12 @@ hipError_t tt_launch_separable_uplift(hipStream_t s, const tt_dev &d, const tt_p
 // ---- stage D/E (k_erosion.hip) -----------------------------------------------
 int tt_tile_rows_valid(int rows, int H);
 // ch / csprev / csout: Kahan compensation arrays of the precision mode (null: plain)
+// frozen: per 64x16 tile, unchanged through all thermal passes of the step
+// (tt_launch_thermal_frozen); pass: index of the pass within the step
 hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, int rows, const float *hin, float *hout,
-                             float *ch = nullptr);
+                             float *ch = nullptr, const uint8_t *frozen = nullptr, int pass = 0);
+hipError_t tt_launch_thermal_frozen(hipStream_t s, const tt_dev &d, int T, const float *h, float2 *mm,
+                                    uint8_t *frozen);
 hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int rows, int first, int last, int rseed,
                                const float *hin, float *hout, const float *win, float *wout, const float4 *fin,
                                float4 *fout, float2 *vel, const float *sprev, float *sout, float *ch = nullptr,

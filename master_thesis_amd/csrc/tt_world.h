@@ -77,6 +77,8 @@ struct tt_world {
     float *eff = nullptr;
     uint32_t *rchg = nullptr;   // per-round change counters of the split fixed points (TT_RCHG)
     uint8_t *act = nullptr;     // per-tile activity flags of the split rounds (2 x tiles)
+    float2 *tileMM = nullptr;   // per 64x16 tile: min / max height (thermal)
+    uint8_t *tileFrozen = nullptr;  // per 64x16 tile: unchanged by this step's thermal passes
     float *ch = nullptr;        // precision mode: height compensation
     float *cs[2] = {nullptr, nullptr};  // precision mode: sediment compensation (ping-pong like sed)
 
@@ -105,6 +107,7 @@ int tt_step_thermal_count(const tt_world *w);
 int tt_step_hydraulic_count(const tt_world *w);
 void tt_step_erode_indices(tt_world *w, int T, int K, int *sedLast);
 int tt_step_transport(tt_world *w, int sedLast);
+int tt_step_thermal_prep(tt_world *w);
 int tt_world_do_init(tt_world *w);
 int tt_validate_config(const tt_sim_config *c);
 

@@ -161,6 +161,8 @@ size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w)
     take((void **)&w->eff, N * 4);
     take((void **)&w->rchg, TT_RCHG * 4);
     take((void **)&w->act, 2 * (N / 1024 + 64));
+    take((void **)&w->tileMM, (N / 1024 + 64) * 8);
+    take((void **)&w->tileFrozen, N / 1024 + 64);
     if (c.precision == TT_PRECISION_COMPENSATED) {
         take((void **)&w->ch, N * 4);
         take((void **)&w->cs[0], N * 4);
@@ -455,14 +457,25 @@ int tt_step_transport(tt_world *w, int sedLast)
     return TT_OK;
 }
 
+// per-tile frozen flags of this step's thermal passes (tiles that no pass changes)
+int tt_step_thermal_prep(tt_world *w)
+{
+    const int T = tt_step_thermal_count(w);
+    if (T <= 0) return TT_OK;
+    const tt_dev d = make_dev(w);
+    CK(tt_launch_thermal_frozen(w->stream, d, T, w->h[w->ehc], w->tileMM, w->tileFrozen));
+    return TT_OK;
+}
+
 static int step_erode(tt_world *w)
 {
     hipStream_t s = w->stream;
     const tt_dev d = make_dev(w);
     const int T = tt_step_thermal_count(w), K = tt_step_hydraulic_count(w);
     int hc = w->ehc;
+    CKS(tt_step_thermal_prep(w));
     for (int it = 0; it < T; ++it) {
-        CK(tt_launch_thermal(s, d, TT_TILE_H, w->h[hc], w->h[1 - hc], w->ch));
+        CK(tt_launch_thermal(s, d, TT_TILE_H, w->h[hc], w->h[1 - hc], w->ch, w->tileFrozen, it));
         hc = 1 - hc;
     }
     CKS(record(w, 3));
