@@ -911,7 +911,7 @@ __global__ __launch_bounds__(256) void k_split_tiles(tt_dev d, const uint8_t *id
     if (threadIdx.x == 0) flag[blockIdx.x] = any ? 1 : 0;
 }
 
-// Round bookkeeping shared by the relaxation and the flood fill.  act[p][t]
+// Round bookkeeping of the relaxation.  act[p][t]
 // (p = round parity) says that tile t must run again: it wrote cells a
 // neighbour reads, or it stopped at the sweep cap.  A tile runs in round r > 0
 // only if it or one of its 8 neighbouring tiles (toroidal) is marked in
@@ -1023,62 +1023,18 @@ __global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *t
     if (threadIdx.x == 0) tt_round_mark(R, tile, wrote, capped);
 }
 
-// floodFillPlate (plate_tectonics_kernel.cu:2985-3024) in the same round
-// scheme: the new id spreads through 4-connected cells of the old plate.
-template <int TH>
-__global__ __launch_bounds__(256) void k_flood_tiles(tt_dev d, const uint32_t *tiles, uint8_t *ids, int oldId,
-                                                     int newId, tt_round R)
-{
-    const uint32_t tile = tiles[blockIdx.x];
-    if (tt_round_skip(R, tile)) {
-        if (threadIdx.x == 0 && R.round > 0) R.act[(size_t)(R.round & 1) * R.ntiles + tile] = 0;
-        return;
-    }
-    constexpr int PR = TH / 4;
-    __shared__ uint8_t m[TH + 2][68];
-    const int tx0 = (int)(tile % (uint32_t)R.tpr) * 64, ty0 = (int)(tile / (uint32_t)R.tpr) * TH;
-    const int W = d.W, H = d.H;
-    for (int i = threadIdx.x; i < (TH + 2) * 66; i += 256) {
-        const int r = i / 66, q = i - r * 66;
-        m[r][q] = ids[(size_t)tt_wrap1(ty0 + r - 1, H) * W + tt_wrap1(tx0 + q - 1, W)];
-    }
-    __syncthreads();
-    const int lx = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    bool cand[PR];
-#pragma unroll
-    for (int j = 0; j < PR; ++j) cand[j] = m[wv + 4 * j + 1][lx + 1] == oldId;
-    int capped = 1;
-    for (int it = 0; it < TT_SWEEP_CAP; ++it) {
-        int ch = 0;
-#pragma unroll
-        for (int j = 0; j < PR; ++j) {
-            const int r = wv + 4 * j + 1, q = lx + 1;
-            if (!cand[j] || m[r][q] != oldId) continue;
-            if (m[r][q - 1] == newId || m[r][q + 1] == newId || m[r - 1][q] == newId || m[r + 1][q] == newId) {
-                m[r][q] = (uint8_t)newId;
-                ch = 1;
-            }
-        }
-        if (!__syncthreads_or(ch)) { capped = 0; break; }
-    }
-    int wrote = 0;
-#pragma unroll
-    for (int j = 0; j < PR; ++j) {
-        if (cand[j] && m[wv + 4 * j + 1][lx + 1] == newId) {
-            ids[(size_t)(ty0 + wv + 4 * j) * W + tx0 + lx] = (uint8_t)newId;
-            wrote = 1;
-        }
-    }
-    wrote = __syncthreads_or(wrote);
-    if (threadIdx.x == 0) tt_round_mark(R, tile, wrote, capped);
-}
-
+// Rows of the split tiles (16, 32 and 64 are instantiated).  16 measures
+// fastest: a taller tile needs fewer rounds, but its in-tile convergence costs
+// more than the rounds it saves.
 int tt_split_tile_h(const tt_dev &d) { return 16; }
 
 hipError_t tt_launch_split_tiles(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, uint8_t *flag)
 {
-    const unsigned nt = (unsigned)((d.W / 64) * (d.H / 16));
-    hipLaunchKernelGGL(k_split_tiles<16>, dim3(nt), dim3(256), 0, s, d, ids, plate, flag);
+    const int th = tt_split_tile_h(d);
+    const unsigned nt = (unsigned)((d.W / 64) * (d.H / th));
+    if (th == 64) hipLaunchKernelGGL(k_split_tiles<64>, dim3(nt), dim3(256), 0, s, d, ids, plate, flag);
+    else if (th == 32) hipLaunchKernelGGL(k_split_tiles<32>, dim3(nt), dim3(256), 0, s, d, ids, plate, flag);
+    else hipLaunchKernelGGL(k_split_tiles<16>, dim3(nt), dim3(256), 0, s, d, ids, plate, flag);
     return hipGetLastError();
 }
 
@@ -1088,7 +1044,7 @@ static tt_round make_round(const tt_dev &d, uint32_t *rchg, uint8_t *act, int r)
     R.rchg = rchg;
     R.act = act;
     R.tpr = d.W / 64;
-    R.ntiles = R.tpr * (d.H / 16);
+    R.ntiles = R.tpr * (d.H / tt_split_tile_h(d));
     R.round = r;
     return R;
 }
@@ -1098,19 +1054,150 @@ hipError_t tt_launch_relax_tiles(hipStream_t s, const tt_dev &d, const uint32_t 
                                  uint8_t *act, int round0, int rounds)
 {
     if (ntiles == 0) return hipSuccess;
-    for (int r = round0; r < round0 + rounds; ++r)
-        hipLaunchKernelGGL(k_relax_tiles<16>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, plate, height, eff,
-                           make_round(d, rchg, act, r));
+    const int th = tt_split_tile_h(d);
+    for (int r = round0; r < round0 + rounds; ++r) {
+        const tt_round R = make_round(d, rchg, act, r);
+        if (th == 64)
+            hipLaunchKernelGGL(k_relax_tiles<64>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, plate, height, eff, R);
+        else if (th == 32)
+            hipLaunchKernelGGL(k_relax_tiles<32>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, plate, height, eff, R);
+        else
+            hipLaunchKernelGGL(k_relax_tiles<16>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, plate, height, eff, R);
+    }
     return hipGetLastError();
 }
 
-hipError_t tt_launch_flood_tiles(hipStream_t s, const tt_dev &d, const uint32_t *tiles, unsigned ntiles, uint8_t *ids,
-                                 int oldId, int newId, uint32_t *rchg, uint8_t *act, int round0, int rounds)
+// ---------------------------------------------------------------------------
+// floodFillPlate (plate_tectonics_kernel.cu:2985-3024) as connected
+// components: the flood from the seed through 4-connected cells of the old
+// plate reaches exactly the components (4-connectivity, toroidal) of old-plate
+// cells that touch the seed cell, so they are labelled once and relabelled.
+// lab[c] = parent in a union-find over old-plate cells (0xFFFFFFFF elsewhere).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fl_local(tt_dev d, const uint8_t *ids, int oldId, uint32_t *lab,
+                                                  uint8_t *rootFlag)
 {
-    if (ntiles == 0) return hipSuccess;
-    for (int r = round0; r < round0 + rounds; ++r)
-        hipLaunchKernelGGL(k_flood_tiles<16>, dim3(ntiles), dim3(256), 0, s, d, tiles, ids, oldId, newId,
-                           make_round(d, rchg, act, r));
+    __shared__ unsigned par[TT_TILE_H * TT_TILE_W];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TT_TILE_W, y0 = blockIdx.y * TT_TILE_H;
+    const int W = d.W;
+    bool in[4];
+    int any = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = tid + 256 * k;
+        const int ly = i >> 6, lx = i & 63;
+        in[k] = ids[(size_t)(y0 + ly) * W + x0 + lx] == oldId;
+        any |= in[k];
+        par[i] = (unsigned)i;
+    }
+    if (!__syncthreads_or(any)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = tid + 256 * k;
+            const size_t c = (size_t)(y0 + (i >> 6)) * W + x0 + (i & 63);
+            lab[c] = 0xFFFFFFFFu;
+            rootFlag[c] = 0;
+        }
+        return;
+    }
+    // in-tile edges: left and up neighbours of the same plate
+    __shared__ uint8_t inL[TT_TILE_H * TT_TILE_W];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) inL[tid + 256 * k] = in[k] ? 1 : 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = tid + 256 * k;
+        if (!in[k]) continue;
+        const int ly = i >> 6, lx = i & 63;
+        if (lx > 0 && inL[i - 1]) tt_union32(par, (unsigned)i, (unsigned)(i - 1));
+        if (ly > 0 && inL[i - TT_TILE_W]) tt_union32(par, (unsigned)i, (unsigned)(i - TT_TILE_W));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i = tid + 256 * k;
+        const size_t c = (size_t)(y0 + (i >> 6)) * W + x0 + (i & 63);
+        if (!in[k]) {
+            lab[c] = 0xFFFFFFFFu;
+            rootFlag[c] = 0;
+            continue;
+        }
+        const unsigned r = tt_find32(par, (unsigned)i);
+        lab[c] = (uint32_t)((size_t)(y0 + (int)(r >> 6)) * W + x0 + (int)(r & 63));
+        rootFlag[c] = r == (unsigned)i ? 1 : 0;
+    }
+}
+
+// unions across tile borders: the left column joins its left neighbours, the
+// top row its upper neighbours (toroidal)
+__global__ __launch_bounds__(128) void k_fl_merge(tt_dev d, const uint8_t *ids, int oldId, uint32_t *lab)
+{
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TT_TILE_W, y0 = blockIdx.y * TT_TILE_H;
+    int x, y, nx, ny;
+    if (tid < 64) { x = x0 + tid; y = y0; nx = x; ny = tt_wrap1(y - 1, d.H); }
+    else if (tid < 64 + TT_TILE_H) { x = x0; y = y0 + tid - 64; nx = tt_wrap1(x - 1, d.W); ny = y; }
+    else return;
+    const uint32_t c = (uint32_t)y * d.W + x, n = (uint32_t)ny * d.W + nx;
+    if (ids[c] != oldId || ids[n] != oldId) return;
+    const uint32_t a = lab[c], b = lab[n];
+    if (a != b) tt_gunion(lab, a, b);
+}
+
+// final roots of the local roots (pointer jumping; grid-stride over the list)
+__global__ __launch_bounds__(256) void k_fl_flatten(const uint32_t *list, const uint32_t *n, uint32_t *lab)
+{
+    const uint32_t cnt = *n;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < cnt; i += gridDim.x * 256) {
+        const uint32_t r = list[i];
+        uint32_t p = lab[r];
+        if (p == r) continue;
+        for (;;) {
+            const uint32_t gp = lab[p];
+            if (gp == p) break;
+            lab[r] = gp;
+            p = gp;
+        }
+        lab[r] = p;
+    }
+}
+
+// cells whose component touches the seed take the new id
+__global__ __launch_bounds__(256) void k_fl_apply(tt_dev d, const uint32_t *lab, const tt_step_ctl *ctl,
+                                                  uint8_t *ids, int oldId, int newId)
+{
+    const int seed = ctl->split_seed;
+    if (seed < 0) return;
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c >= (uint32_t)d.N || ids[c] != oldId) return;
+    int sx, sy;
+    tt_cell_xy((uint32_t)seed, d, sx, sy);
+    const uint32_t nb[4] = {(uint32_t)sy * d.W + tt_wrap1(sx - 1, d.W), (uint32_t)sy * d.W + tt_wrap1(sx + 1, d.W),
+                            (uint32_t)tt_wrap1(sy - 1, d.H) * d.W + sx, (uint32_t)tt_wrap1(sy + 1, d.H) * d.W + sx};
+    const uint32_t r = lab[lab[c]];
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t l = lab[nb[k]];
+        if (l != 0xFFFFFFFFu && lab[l] == r) hit = true;
+    }
+    if (hit) ids[c] = (uint8_t)newId;
+}
+
+hipError_t tt_launch_flood_cc(hipStream_t s, const tt_dev &d, uint8_t *ids, int oldId, int newId, uint32_t *lab,
+                              uint8_t *rootFlag, uint32_t *list, uint32_t *nlist, uint32_t *scanTmp,
+                              const tt_step_ctl *ctl)
+{
+    const dim3 tiles(d.W / TT_TILE_W, d.H / TT_TILE_H);
+    hipLaunchKernelGGL(k_fl_local, tiles, dim3(256), 0, s, d, ids, oldId, lab, rootFlag);
+    hipLaunchKernelGGL(k_fl_merge, tiles, dim3(128), 0, s, d, ids, oldId, lab);
+    hipError_t e = tt_compact_u8(s, rootFlag, (long long)d.N, list, nlist, scanTmp);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fl_flatten, dim3(1024), dim3(256), 0, s, list, nlist, lab);
+    hipLaunchKernelGGL(k_fl_apply, dim3((unsigned)(((long long)d.N + 255) / 256)), dim3(256), 0, s, d, lab, ctl, ids,
+                       oldId, newId);
     return hipGetLastError();
 }
 
@@ -1125,6 +1212,7 @@ __global__ void k_backtrack(tt_dev d, const float *eff, uint8_t *ids, int px, in
     const uint8_t oldAtPoint = ids[at(px, py)];
     int cx = px, cy = py;
     bool dead = false;
+    int seed = -1;
     // the first walk is memoryless: reaching a cell it already marked means an
     // endless cycle (zero-height plateau); canonical outcome: dead end
     while (eff[at(cx, cy)] > 0.0f) {
@@ -1182,9 +1270,13 @@ __global__ void k_backtrack(tt_dev d, const float *eff, uint8_t *ids, int px, in
             sx = (int)(fx + 0.5f);
             sy = (int)(fy + 0.5f);
         }
-        if (ids[at(sx, sy)] == oldAtPoint) ids[at(sx, sy)] = (uint8_t)newId;
+        if (ids[at(sx, sy)] == oldAtPoint) {
+            ids[at(sx, sy)] = (uint8_t)newId;
+            seed = (int)at(sx, sy);
+        }
     }
     ctl->dead_end = dead ? 1 : 0;
+    ctl->split_seed = seed;
 }
 
 hipError_t tt_launch_backtrack(hipStream_t s, const tt_dev &d, const float *eff, uint8_t *ids, int px, int py,

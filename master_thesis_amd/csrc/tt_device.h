@@ -62,7 +62,7 @@ struct tt_step_ctl {
     int n_plate_cells;
     int largest_value, largest_id, heaviest_id, n_plates;
     float heaviest_value;
-    int pad0;
+    int split_seed;      // cell where the split's new id starts, -1 if none
     long long ledger;
 };
 

@@ -111,7 +111,8 @@ hipError_t tt_launch_relax_tiles(hipStream_t s, const tt_dev &d, const uint32_t 
                                  uint8_t *act, int round0, int rounds);
 hipError_t tt_launch_backtrack(hipStream_t s, const tt_dev &d, const float *eff, uint8_t *ids, int px, int py,
                                int newId, tt_step_ctl *ctl);
-hipError_t tt_launch_flood_tiles(hipStream_t s, const tt_dev &d, const uint32_t *tiles, unsigned ntiles, uint8_t *ids,
-                                 int oldId, int newId, uint32_t *rchg, uint8_t *act, int round0, int rounds);
+hipError_t tt_launch_flood_cc(hipStream_t s, const tt_dev &d, uint8_t *ids, int oldId, int newId, uint32_t *lab,
+                              uint8_t *rootFlag, uint32_t *list, uint32_t *nlist, uint32_t *scanTmp,
+                              const tt_step_ctl *ctl);
 hipError_t tt_launch_split_finalize(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n,
                                     uint8_t *ids, int oldId, int newId, tt_plate_data *pd);

@@ -277,7 +277,7 @@ template <typename Launch>
 static int run_rounds(tt_world *w, Launch launch, int *rounds_out)
 {
     const int B = TT_ROUND_BATCH;
-    const long long ntiles_all = (long long)(w->W / 64) * (w->H / 16);
+    const long long ntiles_all = (long long)(w->W / 64) * (w->H / tt_split_tile_h(make_dev(w)));
     CKS(zero(w, w->act, (size_t)(2 * ntiles_all)));
     for (int batch = 0; batch < 100000; ++batch) {
         uint32_t *half = w->rchg + (batch & 1) * B;
@@ -326,10 +326,9 @@ static int run_split(tt_world *w)
         CK(tt_launch_split_finalize(s, d, w->list[0], w->listN[0], ids, plate, plate, w->pd));
         return TT_OK;
     }
-    int frounds = 0;
-    CKS(run_rounds(w, [&](int r0, int B) {
-        return tt_launch_flood_tiles(s, d, tiles, ntiles, ids, plate, nid, w->rchg, w->act, r0, B);
-    }, &frounds));
+    // flood fill from the seed: components of the old plate touching the seed
+    // (CCL scratch buffers are free again at this point of the step)
+    CK(tt_launch_flood_cc(s, d, ids, plate, nid, w->rootCount, w->rootFlag, w->locList, w->nloc, w->scanTmp, w->ctl));
     CK(tt_launch_split_finalize(s, d, w->list[0], w->listN[0], ids, plate, nid, w->pd));
     CKS(zero(w, w->acc, 256 * 10 * 8));
     CK(tt_launch_count(s, d, w->mergeIds, ids, w->ids[1 - w->ic], w->h[w->hc], w->noise, w->acc, 1));
