@@ -240,16 +240,40 @@ __device__ __forceinline__ int tt_wrap_near(int v, int n)
     return ((unsigned)(v + n) < (unsigned)(3 * n)) ? tt_wrap1(v, n) : tt_wrapg(v, n);
 }
 
-__device__ __forceinline__ float tt_bilinear(const float *__restrict__ s, float tx, float ty, int W, int H)
+// the four taps and the weights of a sample, split from the mix so that a
+// caller can issue the tap loads early
+struct tt_taps {
+    int a, b, c, d;  // (x0, y0), (x0 + 1, y0), (x0, y0 + 1), (x0 + 1, y0 + 1)
+    float wx, wy;
+};
+
+__device__ __forceinline__ tt_taps tt_bilinear_taps(float tx, float ty, int W, int H)
 {
     const float fx = __builtin_floorf(tx), fy = __builtin_floorf(ty);
     const int x0 = (int)fx, y0 = (int)fy;
-    const float wx = tx - (float)x0, wy = ty - (float)y0;
+    tt_taps t;
+    t.wx = tx - (float)x0;
+    t.wy = ty - (float)y0;
     const int xa = tt_wrap_near(x0, W), xb = tt_wrap_near(x0 + 1, W);
     const int ra = tt_wrap_near(y0, H) * W, rb = tt_wrap_near(y0 + 1, H) * W;
-    const float I0 = (1.0f - wx) * s[ra + xa] + wx * s[ra + xb];
-    const float I1 = (1.0f - wx) * s[rb + xa] + wx * s[rb + xb];
+    t.a = ra + xa;
+    t.b = ra + xb;
+    t.c = rb + xa;
+    t.d = rb + xb;
+    return t;
+}
+
+__device__ __forceinline__ float tt_bilinear_mix(float s00, float s10, float s01, float s11, float wx, float wy)
+{
+    const float I0 = (1.0f - wx) * s00 + wx * s10;
+    const float I1 = (1.0f - wx) * s01 + wx * s11;
     return (1.0f - wy) * I0 + wy * I1;
+}
+
+__device__ __forceinline__ float tt_bilinear(const float *__restrict__ s, float tx, float ty, int W, int H)
+{
+    const tt_taps t = tt_bilinear_taps(tx, ty, W, H);
+    return tt_bilinear_mix(s[t.a], s[t.b], s[t.c], s[t.d], t.wx, t.wy);
 }
 
 // rain amount of a cell: max(0, discreteNoise((x, 0, y), 1, seed + iter)) * dt * rain
@@ -265,31 +289,99 @@ __device__ __forceinline__ float tt_rain_at(int gx, int gy, int rseed, float rai
 // the previous one on its final write) and the sediment is the previous
 // sub-iteration's output transported along its velocity.  LAST: the water
 // output gets no rain.
+//
+// Every global load of the tile is issued before the first barrier: the
+// velocities and then the sediment taps they address, the heights and water
+// with their halo, and the incoming fluxes of the flux ring.  The barriers
+// wait for LDS traffic only, so these loads stay in flight together instead
+// of forming four dependent memory round trips per wavefront.
 template <int TH, bool FIRST, bool LAST, bool COMP = false>
 __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_lds<TH> &S, int x0, int y0)
 {
     constexpr int RPW = TH / 4;
+    constexpr int NFI = (TH + 2) * TT_FW;        // flux items (tile plus a one-cell halo)
+    constexpr int NFR = (NFI + 255) / 256;       // flux items per thread
+    constexpr int NHR = (TH + 4 + 3) / 4;        // halo rows per wavefront
     const int tid = threadIdx.x;
     const int W = A.W, H = A.H;
     const int lane = tid & 63, wv = tid >> 6;
     const float rain = A.rain;
     const int rseed = A.rseed;
-    // heights and water on the tile plus a two-cell halo: rows wv, wv+4, ...;
-    // lanes 0..63 take columns 0..63, lanes 0..3 also columns 64..67
-    {
-        const int gxa = tt_wrap1(x0 + lane - 2, W);
-        const int gxb = tt_wrap1(x0 + 64 + lane - 2, W);
-        for (int ly = wv; ly < TH + 4; ly += 4) {
+    const int gx = x0 + lane;
+
+    // 1. velocities of the own cells, then the sediment they sample
+    float2 pv[RPW];
+    float s4[RPW][4], c4[RPW][4];
+    float wxs[RPW], wys[RPW];
+    if (!FIRST) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) pv[r] = A.vel[(y0 + wv * RPW + r) * W + gx];
+    }
+    // 2. heights and water on the tile plus a two-cell halo: rows wv, wv+4,
+    // ...; lanes 0..63 take columns 0..63, lanes 0..3 also columns 64..67
+    const int gxa = tt_wrap1(x0 + lane - 2, W);
+    const int gxb = tt_wrap1(x0 + 64 + lane - 2, W);
+    float hm[NHR], hw[NHR], hm2[NHR], hw2[NHR];
+#pragma unroll
+    for (int j = 0; j < NHR; ++j) {
+        const int ly = wv + 4 * j;
+        if (ly < TH + 4) {
+            const int row = tt_wrap1(y0 + ly - 2, H) * W;
+            hm[j] = A.hin[row + gxa];
+            hw[j] = A.win[row + gxa];
+            if (lane < TT_EW - 64) {
+                hm2[j] = A.hin[row + gxb];
+                hw2[j] = A.win[row + gxb];
+            }
+        }
+    }
+    // 3. incoming fluxes of the flux ring
+    float4 cf[NFR];
+#pragma unroll
+    for (int k = 0; k < NFR; ++k) {
+        const int i = tid + 256 * k;
+        if (i < NFI) {
+            const int fy = i / TT_FW, fx = i - fy * TT_FW;
+            cf[k] = A.fin[tt_wrap1(y0 + fy - 1, H) * W + tt_wrap1(x0 + fx - 1, W)];
+        }
+    }
+    // sediment taps (the state itself on the first sub-iteration)
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int gy = y0 + wv * RPW + r;
+        const int c = gy * W + gx;
+        if (FIRST) {
+            s4[r][0] = A.sprev[c];
+            if (COMP) c4[r][0] = A.csprev[c];
+        } else {
+            const tt_taps t = tt_bilinear_taps((float)gx - pv[r].x * 1.0f, (float)gy - pv[r].y * 1.0f, W, H);
+            wxs[r] = t.wx;
+            wys[r] = t.wy;
+            s4[r][0] = A.sprev[t.a];
+            s4[r][1] = A.sprev[t.b];
+            s4[r][2] = A.sprev[t.c];
+            s4[r][3] = A.sprev[t.d];
+            if (COMP) {
+                c4[r][0] = A.csprev[t.a];
+                c4[r][1] = A.csprev[t.b];
+                c4[r][2] = A.csprev[t.c];
+                c4[r][3] = A.csprev[t.d];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NHR; ++j) {
+        const int ly = wv + 4 * j;
+        if (ly < TH + 4) {
             const int gy = tt_wrap1(y0 + ly - 2, H);
-            const int row = gy * W;
-            float m = A.hin[row + gxa], w = A.win[row + gxa];
+            float m = hm[j], w = hw[j];
             if (FIRST) w = w + tt_rain_at(gxa, gy, rseed, rain);
             S.M[ly][lane] = m;
             S.Wt[ly][lane] = w;
             if (lane < TT_EW - 64) {
-                float m2 = A.hin[row + gxb], w2 = A.win[row + gxb];
+                float w2 = hw2[j];
                 if (FIRST) w2 = w2 + tt_rain_at(gxb, gy, rseed, rain);
-                S.M[ly][64 + lane] = m2;
+                S.M[ly][64 + lane] = hm2[j];
                 S.Wt[ly][64 + lane] = w2;
             }
         }
@@ -297,18 +389,19 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
     __syncthreads();
     // outflow flux on the tile plus a one-cell halo
     const float fk = A.fk, g = A.g, L = A.L;
-    for (int i = tid; i < (TH + 2) * TT_FW; i += 256) {
+#pragma unroll
+    for (int k = 0; k < NFR; ++k) {
+        const int i = tid + 256 * k;
+        if (i >= NFI) break;
         const int fy = i / TT_FW, fx = i - fy * TT_FW;
         const int ly = fy + 1, lx = fx + 1;
-        const int gx = tt_wrap1(x0 + fx - 1, W), gy = tt_wrap1(y0 + fy - 1, H);
-        const float4 cf = A.fin[gy * W + gx];
         const float mc = S.M[ly][lx], wc = S.Wt[ly][lx];
         const float sc = mc + wc;
         float4 o;
-        o.x = tt_fmaxf(0.0f, cf.x + fk * ((g * ((sc - S.M[ly][lx - 1]) - S.Wt[ly][lx - 1])) / L));
-        o.y = tt_fmaxf(0.0f, cf.y + fk * ((g * ((sc - S.M[ly + 1][lx]) - S.Wt[ly + 1][lx])) / L));
-        o.z = tt_fmaxf(0.0f, cf.z + fk * ((g * ((sc - S.M[ly][lx + 1]) - S.Wt[ly][lx + 1])) / L));
-        o.w = tt_fmaxf(0.0f, cf.w + fk * ((g * ((sc - S.M[ly - 1][lx]) - S.Wt[ly - 1][lx])) / L));
+        o.x = tt_fmaxf(0.0f, cf[k].x + fk * ((g * ((sc - S.M[ly][lx - 1]) - S.Wt[ly][lx - 1])) / L));
+        o.y = tt_fmaxf(0.0f, cf[k].y + fk * ((g * ((sc - S.M[ly + 1][lx]) - S.Wt[ly + 1][lx])) / L));
+        o.z = tt_fmaxf(0.0f, cf[k].z + fk * ((g * ((sc - S.M[ly][lx + 1]) - S.Wt[ly][lx + 1])) / L));
+        o.w = tt_fmaxf(0.0f, cf[k].w + fk * ((g * ((sc - S.M[ly - 1][lx]) - S.Wt[ly - 1][lx])) / L));
         const float tot = (((o.x + o.y) + o.z) + o.w) + 1e-6f;
         float K = tt_fminf(1.0f, wc / (tot * 1.0f + 1e-6f));
         K = tt_fmaxf(K, 0.01f);
@@ -320,7 +413,6 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
     }
     __syncthreads();
     const float evap = A.evap, cap = A.cap, ke = A.ke, kd = A.kd;
-    const int gx = x0 + lane;
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
         const int ry = wv * RPW + r;
@@ -356,12 +448,11 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         A.fout[c] = Sf;
         float sed, cs = 0.0f;
         if (FIRST) {
-            sed = A.sprev[c];
-            if (COMP) cs = A.csprev[c];
+            sed = s4[r][0];
+            if (COMP) cs = c4[r][0];
         } else {
-            const float2 pv = A.vel[c];
-            sed = tt_bilinear(A.sprev, (float)gx - pv.x * 1.0f, (float)gy - pv.y * 1.0f, W, H);
-            if (COMP) cs = tt_bilinear(A.csprev, (float)gx - pv.x * 1.0f, (float)gy - pv.y * 1.0f, W, H);
+            sed = tt_bilinear_mix(s4[r][0], s4[r][1], s4[r][2], s4[r][3], wxs[r], wys[r]);
+            if (COMP) cs = tt_bilinear_mix(c4[r][0], c4[r][1], c4[r][2], c4[r][3], wxs[r], wys[r]);
         }
         A.vel[c] = make_float2(vx, vy);
         const float mc = S.M[ly][lx];
