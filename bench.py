@@ -186,6 +186,10 @@ def main() -> int:
     ap.add_argument("--precision-side", type=int, default=8192)
     ap.add_argument("--precision-plates", type=int, default=64)
     ap.add_argument("--precision-steps", type=int, default=300)
+    ap.add_argument("--precision-hydraulic", type=int, default=50, help="hydraulic sub-iterations per step")
+    ap.add_argument("--precision-thermal", type=int, default=20, help="thermal passes per step")
+    ap.add_argument("--no-hbm-probe", action="store_true",
+                    help="skip the measured HBM copy/read/fill ceilings (2 GiB buffers)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -200,7 +204,8 @@ def main() -> int:
     if args.mode == "precision":
         from master_thesis_amd.sim import precision_study
         pc, pk = configs.synthetic(args.precision_side, args.precision_side, args.precision_plates, (0.5, 3.0),
-                                   args.seed, hydraulic=50, thermal=20, ccl_every=1, uplift_range=5)
+                                   args.seed, hydraulic=args.precision_hydraulic,
+                                   thermal=args.precision_thermal, ccl_every=1, uplift_range=5)
         rep = precision_study(pc, pk, args.precision_steps)
         line = {"metric": "erosion error vs an fp64 twin (fp32 and compensated fp32), mass drift",
                 "value": rep["height"]["rms_comp"], "unit": "height RMS error (compensated fp32)",
@@ -208,7 +213,8 @@ def main() -> int:
                 "scaling": "weak", "vs_baseline": None, "dtype": "f32+compensated / f64",
                 "data": "synthetic (seeded uniform heights, Gaussian sigma 8)",
                 "config": {"workload": f"{args.precision_side}x{args.precision_side}, {args.precision_plates} plates, "
-                                       f"{args.precision_steps} erosion-only steps (K=50, T=20)"}}
+                                       f"{args.precision_steps} erosion-only steps (K={args.precision_hydraulic}, "
+                                       f"T={args.precision_thermal})"}}
         line.update(rep)
         print(json.dumps(line))
         return 0
@@ -312,6 +318,13 @@ def main() -> int:
                         "iteration": stats.iteration, "tile_rows": stats.tile_rows},
     }
     world.close()
+    if not args.no_hbm_probe:
+        # measured ceilings beside the published peak: a float4 copy has the
+        # same 50/50 read/write mix as the hydraulic kernel
+        from master_thesis_amd.sim import probe_hbm
+        probe = probe_hbm(2 << 30, 5)
+        result["roofline"]["measured_ceiling"] = probe
+        result["roofline"]["frac_of_copy"] = achieved / probe["copy_gbps"] if probe["copy_gbps"] > 0 else None
 
     if init_state is not None:
         from oracle.oracle_py import OracleWorld

@@ -416,6 +416,22 @@ enum { TT_MATH_EXPF = 0, TT_MATH_LOG10F = 1, TT_MATH_SINF = 2, TT_MATH_COSF = 3,
        TT_MATH_SIMPLEX = 4, TT_MATH_DISCRETE = 5, TT_MATH_FMODF = 6, TT_MATH_ATAN2 = 7 };
 int tt_math_eval_device(int32_t fn, const float *in, float *out, int64_t n);
 
+/* ---- HBM ceiling probe (no reference counterpart: gives the roofline a
+ * measured ceiling beside the published peak).  Times `reps` launches each of
+ * a float4 copy (bytes read + bytes written), a float4 read-only sum and a
+ * float4 fill over two device buffers of `bytes` each (a multiple of 16, at
+ * most 4 GiB), and reports the best GB/s of each.  Allocates 2 x bytes of
+ * device memory for the duration of the call. */
+typedef struct tt_hbm_probe {
+    double copy_gbps;
+    double read_gbps;
+    double write_gbps;
+    int64_t bytes;
+    int32_t reps;
+    int32_t reserved;
+} tt_hbm_probe;
+int tt_probe_hbm(int64_t bytes, int32_t reps, tt_hbm_probe *out);
+
 #ifdef __cplusplus
 }
 #endif

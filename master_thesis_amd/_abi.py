@@ -270,6 +270,11 @@ class PrecisionReport(C.Structure):
                                   "drift_comp", "ms_fp32", "ms_comp", "ms_ref")] + [("reserved", C.c_double * 4)]
 
 
+class HbmProbe(C.Structure):
+    _fields_ = [("copy_gbps", C.c_double), ("read_gbps", C.c_double), ("write_gbps", C.c_double),
+                ("bytes", C.c_int64), ("reps", C.c_int32), ("reserved", C.c_int32)]
+
+
 # numpy-compatible record layouts of the per-cell structs
 import numpy as np  # noqa: E402
 
@@ -386,6 +391,7 @@ def lib() -> C.CDLL:
     L.tt_world_load.argtypes = [C.c_char_p, C.POINTER(vp)]
     L.tt_precision_study.argtypes = [C.POINTER(SimConfig), C.POINTER(KernelSettings), C.c_int32,
                                      C.POINTER(PrecisionReport)]
+    L.tt_probe_hbm.argtypes = [C.c_int64, C.c_int32, C.POINTER(HbmProbe)]
     _lib = L
     return L
 

@@ -352,3 +352,36 @@ hipError_t tt_launch_math_eval(hipStream_t s, int fn, const float *in, float *ou
     hipLaunchKernelGGL(k_math_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fn, in, out, n);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// HBM ceiling probe: float4 copy, read-only sum, fill (one float4 per thread)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_probe_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+__global__ __launch_bounds__(256) void k_probe_read(const float4 *__restrict__ a, float *__restrict__ sink, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 v = a[i];
+    // the sum of a zero buffer is never the sentinel; the branch keeps the load
+    if (((v.x + v.y) + v.z) + v.w == 1.0e30f) sink[0] = 1.0f;
+}
+
+__global__ __launch_bounds__(256) void k_probe_fill(float4 *__restrict__ b, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = make_float4(1.0f, 2.0f, 3.0f, 4.0f);
+}
+
+hipError_t tt_launch_probe(hipStream_t s, int which, const float4 *a, float4 *b, float *sink, long long n)
+{
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (which == 0) hipLaunchKernelGGL(k_probe_copy, dim3(blocks), dim3(256), 0, s, a, b, n);
+    else if (which == 1) hipLaunchKernelGGL(k_probe_read, dim3(blocks), dim3(256), 0, s, a, sink, n);
+    else hipLaunchKernelGGL(k_probe_fill, dim3(blocks), dim3(256), 0, s, b, n);
+    return hipGetLastError();
+}

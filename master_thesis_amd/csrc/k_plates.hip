@@ -565,28 +565,75 @@ hipError_t tt_launch_neighbor_matrix(hipStream_t s, const tt_dev &d, const uint8
 #define TT_NBM_WORDS ((TT_NBM + 31) / 32)
 
 // getPlateMerges + clearCollisionTypes: candidates evaluated in parallel,
-// pairing applied by one thread in ascending matrix index
+// pairing applied by one thread in ascending matrix index.  The plate fields
+// the test reads are staged in LDS and the adjacency matrix is read 16 bytes
+// per load, so the candidate pass costs a few memory round trips.
+struct tt_merge_pl {
+    float dsx, dsy, vs, v, mass;
+    int size, wait;
+};
+
+__device__ __forceinline__ void tt_merge_test(int i, const tt_merge_pl *P, const tt_kernel_settings &K, unsigned *cand)
+{
+    const int x = i % TT_MAXP, y = i / TT_MAXP;
+    if (x >= y) return;
+    const tt_merge_pl &X = P[x], &Y = P[y];
+    const float dot = X.dsx * Y.dsx + X.dsy * Y.dsy;
+    const float vd = tt_fabsf(X.vs - Y.vs);
+    if (!(X.v <= K.mergeMinVelocity && Y.v <= K.mergeMinVelocity) &&
+        !(dot >= K.mergeDotDirectionThreshold && vd <= K.mergeVelocityDiffThreshold))
+        return;
+    if (X.wait > 0 || Y.wait > 0) return;
+    if (X.size + Y.size > K.targetMaximumPlateArea) return;
+    atomicOr(&cand[i >> 5], 1u << (i & 31));
+}
+
+__device__ __forceinline__ void tt_merge_word(uint32_t wd, int i0, const tt_merge_pl *P, const tt_kernel_settings &K,
+                                              unsigned *cand)
+{
+    if (wd == 0u) return;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        if (((wd >> (8 * b)) & 0xFFu) && i0 + b < TT_NBM) tt_merge_test(i0 + b, P, K, cand);
+}
+
 __global__ __launch_bounds__(1024) void k_merge_decide(tt_dev d, const uint8_t *nbm, const tt_plate_data *pd,
                                                        int *mergeIds, uint8_t *typeBitmap, tt_step_ctl *ctl)
 {
     __shared__ unsigned cand[TT_NBM_WORDS];
     __shared__ int mids[256];
+    __shared__ tt_merge_pl P[TT_MAXP];
     const tt_kernel_settings &K = d.ks;
     for (int i = threadIdx.x; i < TT_NBM_WORDS; i += 1024) cand[i] = 0u;
     for (int i = threadIdx.x; i < 256; i += 1024) mids[i] = 0;
+    if (threadIdx.x < TT_MAXP) {
+        const tt_plate_data &Q = pd[threadIdx.x];
+        tt_merge_pl q;
+        q.dsx = Q.directionSmoothed[0];
+        q.dsy = Q.directionSmoothed[1];
+        q.vs = Q.velocitySmoothed;
+        q.v = Q.velocity;
+        q.size = Q.size;
+        q.mass = Q.mass;
+        q.wait = Q.mergeWaitTime;
+        P[threadIdx.x] = q;
+    }
+    // the matrix as 16-byte groups: 1024 threads x 4 groups = 65536 bytes; the
+    // buffer holds at least TT_NBM + 64 bytes, zero past TT_NBM + 4
+    uint4 g[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int q = threadIdx.x + 1024 * k;
+        g[k] = q * 16 < TT_NBM ? reinterpret_cast<const uint4 *>(nbm)[q] : make_uint4(0u, 0u, 0u, 0u);
+    }
     __syncthreads();
-    for (int i = threadIdx.x; i < TT_NBM; i += 1024) {
-        const int x = i % TT_MAXP, y = i / TT_MAXP;
-        if (x >= y || !nbm[i]) continue;
-        const tt_plate_data &X = pd[x], &Y = pd[y];
-        const float dot = X.directionSmoothed[0] * Y.directionSmoothed[0] + X.directionSmoothed[1] * Y.directionSmoothed[1];
-        const float vd = tt_fabsf(X.velocitySmoothed - Y.velocitySmoothed);
-        if (!(X.velocity <= K.mergeMinVelocity && Y.velocity <= K.mergeMinVelocity) &&
-            !(dot >= K.mergeDotDirectionThreshold && vd <= K.mergeVelocityDiffThreshold))
-            continue;
-        if (X.mergeWaitTime > 0 || Y.mergeWaitTime > 0) continue;
-        if (X.size + Y.size > K.targetMaximumPlateArea) continue;
-        atomicOr(&cand[i >> 5], 1u << (i & 31));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int i0 = (threadIdx.x + 1024 * k) * 16;
+        tt_merge_word(g[k].x, i0, P, K, cand);
+        tt_merge_word(g[k].y, i0 + 4, P, K, cand);
+        tt_merge_word(g[k].z, i0 + 8, P, K, cand);
+        tt_merge_word(g[k].w, i0 + 12, P, K, cand);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -598,7 +645,7 @@ __global__ __launch_bounds__(1024) void k_merge_decide(tt_dev d, const uint8_t *
                 bits &= bits - 1u;
                 const int i = wi * 32 + bit;
                 const int x = i % TT_MAXP, y = i / TT_MAXP;
-                const bool xIsParent = pd[y].mass <= pd[x].mass;
+                const bool xIsParent = P[y].mass <= P[x].mass;
                 if (mids[x] != 0) continue;
                 mids[x] = xIsParent ? -1 : TT_MAXP - y;
                 if (mids[y] != 0) { mids[x] = 0; continue; }

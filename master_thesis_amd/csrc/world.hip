@@ -1276,4 +1276,62 @@ int tt_math_eval_device(int32_t fn, const float *in, float *out, int64_t n)
     return st;
 }
 
+int tt_probe_hbm(int64_t bytes, int32_t reps, tt_hbm_probe *out)
+{
+    if (!out || bytes < 16 || (bytes % 16) != 0 || reps < 1 || bytes > (4LL << 30)) {
+        set_error("tt_probe_hbm: bytes must be a multiple of 16 in [16, 4 GiB] and reps >= 1");
+        return TT_ERR_INVALID;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { set_error("no HIP device"); return TT_ERR_DEVICE; }
+    memset(out, 0, sizeof *out);
+    out->bytes = bytes;
+    out->reps = reps;
+    float4 *a = nullptr, *b = nullptr;
+    float *sink = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int st = TT_OK;
+    const long long n = bytes / 16;
+    if (hipMalloc((void **)&a, (size_t)bytes) != hipSuccess || hipMalloc((void **)&b, (size_t)bytes) != hipSuccess ||
+        hipMalloc((void **)&sink, 256) != hipSuccess) {
+        set_error("tt_probe_hbm: device allocation failed");
+        st = TT_ERR_NOMEM;
+    }
+    if (st == TT_OK && (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                        hipEventCreate(&e1) != hipSuccess))
+        st = TT_ERR_DEVICE;
+    if (st == TT_OK && (tt_fill_u32(s, (uint32_t *)a, 0u, n * 4) != hipSuccess ||
+                        tt_fill_u32(s, (uint32_t *)b, 0u, n * 4) != hipSuccess))
+        st = TT_ERR_DEVICE;
+    double best[3] = {0.0, 0.0, 0.0};
+    for (int which = 0; which < 3 && st == TT_OK; ++which) {
+        // one untimed launch, then reps timed launches one at a time
+        if (tt_launch_probe(s, which, a, b, sink, n) != hipSuccess) { st = TT_ERR_DEVICE; break; }
+        for (int r = 0; r < reps; ++r) {
+            float ms = 0.0f;
+            if (hipEventRecord(e0, s) != hipSuccess || tt_launch_probe(s, which, a, b, sink, n) != hipSuccess ||
+                hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                st = TT_ERR_DEVICE;
+                break;
+            }
+            const double moved = (which == 0 ? 2.0 : 1.0) * (double)bytes;
+            if (ms > 0.0f) best[which] = std::max(best[which], moved / (ms * 1e-3) / 1e9);
+        }
+    }
+    if (st == TT_ERR_DEVICE) set_error("tt_probe_hbm: device error");
+    out->copy_gbps = best[0];
+    out->read_gbps = best[1];
+    out->write_gbps = best[2];
+    if (s) (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipFree(sink);
+    return st;
+}
+
 }  // extern "C"

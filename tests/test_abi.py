@@ -28,6 +28,10 @@ def test_precision_struct_sizes():
     assert C.sizeof(_abi.PrecisionReport) == 16 + 3 * 5 * 8 + 10 * 8 + 4 * 8
 
 
+def test_hbm_probe_struct_size():
+    assert C.sizeof(_abi.HbmProbe) == 40
+
+
 def test_server_struct_sizes():
     assert C.sizeof(_abi.JobResult) == 8 + 4 + 4 + 8 * 4 + 8 + 4 * 6
     assert C.sizeof(_abi.ServerReport) == 11 * 8 + 7 * 8 + 4 * 8
@@ -106,6 +110,15 @@ def test_no_device_is_an_error_not_a_crash(tectonic_lib):
         cfg, ks = configs.synthetic(64, 64, 2, (0.5, 1.0))
         h = C.c_void_p()
         assert tectonic_lib.tt_world_create(C.byref(cfg), C.byref(ks), C.byref(h)) == -2
+
+
+def test_hbm_probe_rejects_bad_sizes(tectonic_lib):
+    """Argument checks come before any device call."""
+    r = _abi.HbmProbe()
+    assert tectonic_lib.tt_probe_hbm(24, 1, C.byref(r)) == _abi.TT_ERR_INVALID
+    assert tectonic_lib.tt_probe_hbm(16, 0, C.byref(r)) == _abi.TT_ERR_INVALID
+    assert tectonic_lib.tt_probe_hbm((4 << 30) + 16, 1, C.byref(r)) == _abi.TT_ERR_INVALID
+    assert b"multiple of 16" in tectonic_lib.tt_last_error()
 
 
 def test_snapshot_load_rejects_bad_files(tectonic_lib, tmp_path):

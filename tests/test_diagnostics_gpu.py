@@ -84,3 +84,12 @@ def test_class_view_export(sim, tmp_path):
             data = p.read_bytes()
             assert data[:8] == b"\x89PNG\r\n\x1a\n"
             assert int.from_bytes(data[16:20], "big") == 192 and int.from_bytes(data[20:24], "big") == 128
+
+
+def test_hbm_probe(sim):
+    """The ceiling probe reports positive bandwidths of a plausible size."""
+    print("[hbm probe] 64 MiB", flush=True)
+    r = sim.probe_hbm(64 << 20, 2)
+    assert r["bytes"] == 64 << 20 and r["reps"] == 2
+    for k in ("copy_gbps", "read_gbps", "write_gbps"):
+        assert 10.0 < r[k] < 100000.0, (k, r[k])
