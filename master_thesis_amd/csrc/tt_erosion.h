@@ -444,8 +444,11 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         }
         float wo = tt_fmaxf(0.0f, wn - (tt_fminf(wn, 20.0f) * evap) * 1.0f);
         if (!LAST) wo = wo + tt_rain_at(gx, gy, rseed, rain);
-        A.wout[c] = wo;
-        A.fout[c] = Sf;
+        __builtin_nontemporal_store(wo, &A.wout[c]);
+        __builtin_nontemporal_store(Sf.x, &A.fout[c].x);
+        __builtin_nontemporal_store(Sf.y, &A.fout[c].y);
+        __builtin_nontemporal_store(Sf.z, &A.fout[c].z);
+        __builtin_nontemporal_store(Sf.w, &A.fout[c].w);
         float sed, cs = 0.0f;
         if (FIRST) {
             sed = s4[r][0];
@@ -454,7 +457,8 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             sed = tt_bilinear_mix(s4[r][0], s4[r][1], s4[r][2], s4[r][3], wxs[r], wys[r]);
             if (COMP) cs = tt_bilinear_mix(c4[r][0], c4[r][1], c4[r][2], c4[r][3], wxs[r], wys[r]);
         }
-        A.vel[c] = make_float2(vx, vy);
+        __builtin_nontemporal_store(vx, &A.vel[c].x);
+        __builtin_nontemporal_store(vy, &A.vel[c].y);
         const float mc = S.M[ly][lx];
         // Slope: (m[+1] - m[-1]) / 2.0 in the reference; halving is exact in float
         const float dzdx = (S.M[ly][lx + 1] - S.M[ly][lx - 1]) * 0.5f;
@@ -490,8 +494,8 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             A.ch[c] = chv;
             A.csout[c] = cs;
         }
-        A.hout[c] = hnew;
-        A.sout[c] = sed;
+        __builtin_nontemporal_store(hnew, &A.hout[c]);
+        __builtin_nontemporal_store(sed, &A.sout[c]);
     }
 }
 
