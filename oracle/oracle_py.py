@@ -41,6 +41,8 @@ def lib():
         L.oracle_world_ccl.argtypes = [vp]
         L.oracle_world_destroy.argtypes = [vp]
         L.oracle_math_eval.argtypes = [C.c_int, vp, vp, C.c_longlong]
+        L.oracle_precision_study.argtypes = [C.POINTER(_abi.SimConfig), C.POINTER(_abi.KernelSettings), C.c_char_p,
+                                             C.c_int, C.POINTER(_abi.PrecisionReport)]
         _lib = L
     return _lib
 
@@ -134,3 +136,22 @@ def jfa(seed_mask: np.ndarray):
     if st != 0:
         raise RuntimeError("oracle_jfa failed")
     return dist.reshape(H, W), orig.reshape(H, W)
+
+
+def precision_study(cfg: _abi.SimConfig, ks: _abi.KernelSettings, steps: int, noise: bytes | None = None) -> dict:
+    """The precision study (float, compensated float, fp64 twin) on the CPU;
+    same keys as master_thesis_amd.sim.precision_study, timings zero."""
+    cfg, ks = cfg.copy(), ks.copy()
+    noise = noise if noise is not None else _abi.load_noise()
+    r = _abi.PrecisionReport()
+    st = lib().oracle_precision_study(C.byref(cfg), C.byref(ks), noise, int(steps), C.byref(r))
+    if st != 0:
+        raise RuntimeError(f"oracle precision study failed: {st}")
+    out = {"steps": r.steps, "width": r.width, "height": r.height_cells}
+    for f in ("height", "water", "sediment"):
+        pf = getattr(r, f)
+        out[f] = {k: getattr(pf, k) for k, _ in _abi.PrecisionField._fields_}
+    for k in ("mass_start", "mass_ref", "mass_fp32", "mass_comp", "drift_ref", "drift_fp32", "drift_comp",
+              "ms_fp32", "ms_comp", "ms_ref"):
+        out[k] = getattr(r, k)
+    return out

@@ -88,6 +88,10 @@ struct World {
     uint32_t iter = 0;
 
     std::vector<float> height, water, sediment;
+    // precision mode (TT_PRECISION_COMPENSATED): Kahan terms of the height
+    // (reset when collisions rewrite heights) and of the sediment (carried
+    // across steps, transported with the sediment)
+    std::vector<float> ch, cs;
     std::vector<uint8_t> ids;
     std::vector<F4> flux;
     std::vector<V2> velocity, asth;
@@ -922,11 +926,22 @@ void stage_separable_uplift(World &w, std::vector<float> &hW, const std::vector<
 // ---------------------------------------------------------------------------
 static const int TOFF[8][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}, {-1, -1}, {-1, 1}, {1, -1}, {1, 1}};
 
+// Kahan-compensated sum += d; c carries the low-order part lost so far
+// (Kahan 1965; the compensated mode of the erosion updates)
+static inline float kahan_add(float sum, float &c, float d)
+{
+    const float y = d - c;
+    const float t = sum + y;
+    c = (t - sum) - y;
+    return t;
+}
+
 void stage_thermal(World &w, std::vector<float> &h)
 {
     const int W = w.W, H = w.H;
     const float talus = w.ks.thermalThresholdAngle * w.ks.thermalCellSize;
     const float amp = w.ks.thermalErosionAmplitude, str = w.ks.thermalErosionStrength;
+    const bool comp = w.cfg.precision == TT_PRECISION_COMPENSATED;
     std::vector<float> tot((size_t)w.N), nh((size_t)w.N);
     for (int it = 0; it < w.cfg.thermal_iterations; ++it) {
 #pragma omp parallel for schedule(static)
@@ -946,16 +961,24 @@ void stage_thermal(World &w, std::vector<float> &h)
                 const size_t c = (size_t)y * W + x;
                 const float hc = h[c];
                 float v = hc;
+                float cc = comp ? w.ch[c] : 0.0f;
                 if (tot[c] > 0.0f)
                     for (int d = 0; d < 8; ++d) {
                         const float diff = hc - h[w.idx(x + TOFF[d][0], y + TOFF[d][1])];
-                        if (diff > talus) v = v - (amp * (diff / tot[c])) * str;
+                        if (diff > talus) {
+                            const float q = (amp * (diff / tot[c])) * str;
+                            v = comp ? kahan_add(v, cc, -q) : v - q;
+                        }
                     }
                 for (int d = 0; d < 8; ++d) {
                     const int s = w.idx(x - TOFF[d][0], y - TOFF[d][1]);
                     const float diff = h[s] - hc;
-                    if (diff > talus && tot[s] > 0.0f) v = v + (amp * (diff / tot[s])) * str;
+                    if (diff > talus && tot[s] > 0.0f) {
+                        const float q = (amp * (diff / tot[s])) * str;
+                        v = comp ? kahan_add(v, cc, q) : v + q;
+                    }
                 }
+                if (comp) w.ch[c] = cc;
                 nh[c] = v;
             }
         h.swap(nh);
@@ -974,8 +997,9 @@ void stage_hydraulic(World &w, std::vector<float> &m)
     const tt_kernel_settings &K = w.ks;
     const float dt = 1.0f;
     const int rseed = (int)(w.cfg.seed + w.iter);
+    const bool comp = w.cfg.precision == TT_PRECISION_COMPENSATED;
     std::vector<F4> nf((size_t)w.N);
-    std::vector<float> snap((size_t)w.N), sb((size_t)w.N);
+    std::vector<float> snap((size_t)w.N), sb((size_t)w.N), csb(comp ? (size_t)w.N : 0);
     for (int it = 0; it < w.cfg.hydraulic_iterations; ++it) {
         // rain
 #pragma omp parallel for schedule(static)
@@ -1044,7 +1068,24 @@ void stage_hydraulic(World &w, std::vector<float> &m)
                 float C = (K.sedimentCapacity * tt_sinf(tt_fmaxf(slope, 0.03f))) * mag(w.velocity[c]);
                 C = tt_fminf(C, 5.0f);
                 const float sed = w.sediment[c];
-                if (C > sed) {
+                if (comp) {
+                    float chv = w.ch[c], csv = w.cs[c];
+                    float sv = sed;
+                    if (C > sed) {
+                        float s = (K.sedimentErosionRate * (C - sed)) * dt;
+                        s = tt_fminf(s, snap[c]);
+                        m[c] = kahan_add(snap[c], chv, -s);
+                        sv = kahan_add(sv, csv, s);
+                    } else {
+                        float s = (K.sedimentDepositionRate * (sed - C)) * dt;
+                        s = tt_fminf(s, sed);
+                        sv = kahan_add(sv, csv, -s);
+                        m[c] = kahan_add(snap[c], chv, s);
+                    }
+                    w.ch[c] = chv;
+                    w.cs[c] = csv;
+                    w.sediment[c] = sv;
+                } else if (C > sed) {
                     float s = (K.sedimentErosionRate * (C - sed)) * dt;
                     s = tt_fminf(s, snap[c]);
                     m[c] = snap[c] - s;
@@ -1069,6 +1110,12 @@ void stage_hydraulic(World &w, std::vector<float> &m)
                 const float I0 = (1.0f - wx) * w.sediment[w.idx(x0, y0)] + wx * w.sediment[w.idx(x1, y0)];
                 const float I1 = (1.0f - wx) * w.sediment[w.idx(x0, y1)] + wx * w.sediment[w.idx(x1, y1)];
                 sb[c] = (1.0f - wy) * I0 + wy * I1;
+                if (comp) {
+                    // the compensation travels with its sediment
+                    const float J0 = (1.0f - wx) * w.cs[w.idx(x0, y0)] + wx * w.cs[w.idx(x1, y0)];
+                    const float J1 = (1.0f - wx) * w.cs[w.idx(x0, y1)] + wx * w.cs[w.idx(x1, y1)];
+                    csb[c] = (1.0f - wy) * J0 + wy * J1;
+                }
             }
         // evaporate
 #pragma omp parallel for schedule(static)
@@ -1077,6 +1124,7 @@ void stage_hydraulic(World &w, std::vector<float> &m)
             w.water[c] = tt_fmaxf(0.0f, wv - (tt_fminf(wv, 20.0f) * K.hydrationEvaporation) * dt);
         }
         w.sediment.swap(sb);
+        if (comp) w.cs.swap(csb);
     }
 }
 
@@ -1643,6 +1691,8 @@ void step(World &w)
     w.stats.split_iterations = 0;
     // the accretion marks of a step start from zero (plate_tectonic_sim.cu:252)
     std::fill(w.accretion.begin(), w.accretion.end(), 0);
+    // collisions rewrite heights: their compensation starts again from zero
+    if (!w.ch.empty() && (mask & TT_STAGE_COLLIDE)) std::fill(w.ch.begin(), w.ch.end(), 0.0f);
     if (mask & TT_STAGE_COLLIDE) {
         stage_advect(w, offx, offy);
         stage_collision_types(w, offx, offy);
@@ -1701,6 +1751,207 @@ bool field_info(World &w, int f, void **ptr, size_t *bytes)
     }
 }
 
+// ---------------------------------------------------------------------------
+// fp64 erosion twin of the precision study: the thermal and hydraulic
+// formulas above in double, pass by pass in the reference's order (rain,
+// flux, flow, sediment, transport, evaporate; plate_tectonic_sim.cu:331-373),
+// the CPU counterpart of the device twin of tt_precision_study.
+// ---------------------------------------------------------------------------
+struct D4 { double x, y, z, w; };
+struct D2 { double x, y; };
+
+struct Twin {
+    int W, H, N;
+    double talus, amp, str, rain, evap, cap, ke, kd, fk, g, L;
+    std::vector<double> h, water, sed, tot, nh, nsed;
+    std::vector<D4> flux, nflux;
+    std::vector<D2> vel;
+    int idx(int x, int y) const { return wrapc(y, H) * W + wrapc(x, W); }
+};
+
+void twin_init(Twin &T, const World &w)
+{
+    const tt_kernel_settings &K = w.ks;
+    T.W = w.W;
+    T.H = w.H;
+    T.N = w.N;
+    T.talus = (double)(K.thermalThresholdAngle * K.thermalCellSize);
+    T.amp = K.thermalErosionAmplitude;
+    T.str = K.thermalErosionStrength;
+    T.rain = K.hydrationRainfall;
+    T.evap = K.hydrationEvaporation;
+    T.cap = K.sedimentCapacity;
+    T.ke = K.sedimentErosionRate;
+    T.kd = K.sedimentDepositionRate;
+    T.fk = K.hydrationPipeCrossSection;
+    T.g = K.gravity;
+    T.L = K.hydrationPipeLength;
+    const size_t N = (size_t)w.N;
+    T.h.assign(w.height.begin(), w.height.end());
+    T.water.assign(w.water.begin(), w.water.end());
+    T.sed.assign(w.sediment.begin(), w.sediment.end());
+    T.tot.assign(N, 0.0);
+    T.nh.assign(N, 0.0);
+    T.nsed.assign(N, 0.0);
+    T.flux.resize(N);
+    T.nflux.resize(N);
+    T.vel.resize(N);
+    for (size_t c = 0; c < N; ++c) {
+        T.flux[c] = D4{w.flux[c].x, w.flux[c].y, w.flux[c].z, w.flux[c].w};
+        T.vel[c] = D2{w.velocity[c].x, w.velocity[c].y};
+    }
+}
+
+void twin_step(Twin &T, int thermal, int hydraulic, int rseed)
+{
+    const int W = T.W, H = T.H;
+    for (int t = 0; t < thermal; ++t) {
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const double hc = T.h[(size_t)y * W + x];
+                double s = 0.0;
+                for (int k = 0; k < 8; ++k) {
+                    const double diff = hc - T.h[T.idx(x + TOFF[k][0], y + TOFF[k][1])];
+                    if (diff > T.talus) s += diff;
+                }
+                T.tot[(size_t)y * W + x] = s;
+            }
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                const double hc = T.h[c], tc = T.tot[c];
+                double v = hc;
+                if (tc > 0.0)
+                    for (int k = 0; k < 8; ++k) {
+                        const double diff = hc - T.h[T.idx(x + TOFF[k][0], y + TOFF[k][1])];
+                        if (diff > T.talus) v -= (T.amp * (diff / tc)) * T.str;
+                    }
+                for (int k = 0; k < 8; ++k) {
+                    const int sc = T.idx(x - TOFF[k][0], y - TOFF[k][1]);
+                    const double diff = T.h[sc] - hc;
+                    if (diff > T.talus && T.tot[sc] > 0.0) v += (T.amp * (diff / T.tot[sc])) * T.str;
+                }
+                T.nh[c] = v;
+            }
+        T.h.swap(T.nh);
+    }
+    for (int it = 0; it < hydraulic; ++it) {
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const float n = tt_discrete_noise((float)x, 0.0f, (float)y, 1.0f, rseed);
+                T.water[(size_t)y * W + x] += (double)tt_fmaxf(0.0f, n) * T.rain;
+            }
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                const double sc = T.h[c] + T.water[c];
+                auto sub = [&](double f, int nx, int ny) {
+                    const size_t n = (size_t)T.idx(nx, ny);
+                    return std::max(0.0, f + T.fk * ((T.g * ((sc - T.h[n]) - T.water[n])) / T.L));
+                };
+                const D4 cf = T.flux[c];
+                D4 o{sub(cf.x, x - 1, y), sub(cf.y, x, y + 1), sub(cf.z, x + 1, y), sub(cf.w, x, y - 1)};
+                const double tot = (((o.x + o.y) + o.z) + o.w) + 1e-6;
+                double K = std::min(1.0, T.water[c] / (tot + 1e-6));
+                K = std::max(K, 0.01);
+                T.nflux[c] = D4{o.x * K, o.y * K, o.z * K, o.w * K};
+            }
+        T.flux.swap(T.nflux);
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                const D4 S = T.flux[c], Lf = T.flux[T.idx(x - 1, y)], Uf = T.flux[T.idx(x, y + 1)],
+                         Rf = T.flux[T.idx(x + 1, y)], Df = T.flux[T.idx(x, y - 1)];
+                const double in = ((Lf.z + Uf.w) + Rf.x) + Df.y;
+                const double out = ((S.x + S.y) + S.z) + S.w;
+                const double wn = std::max(0.0, T.water[c] + (in - out));
+                T.water[c] = wn;
+                const double nx = (((Lf.z - S.x) + S.z) - Rf.x) * 0.5;
+                const double ny = (((Uf.w - S.y) + S.y) - Df.w) * 0.5;
+                const double hh = std::max(wn, 0.01);
+                double vx = nx / hh, vy = ny / hh;
+                const double vm = sqrt(vx * vx + vy * vy);
+                if (vm > 50.0) {
+                    vx *= 50.0 / vm;
+                    vy *= 50.0 / vm;
+                }
+                T.vel[c] = D2{vx, vy};
+            }
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                const double dzdx = (T.h[T.idx(x + 1, y)] - T.h[T.idx(x - 1, y)]) * 0.5;
+                const double dzdy = (T.h[T.idx(x, y + 1)] - T.h[T.idx(x, y - 1)]) * 0.5;
+                const double slope = sqrt(dzdx * dzdx + dzdy * dzdy);
+                const D2 v = T.vel[c];
+                double C = (T.cap * tt_sin_d(std::max(slope, 0.03))) * sqrt(v.x * v.x + v.y * v.y);
+                C = std::min(C, 5.0);
+                double sd = T.sed[c], hv = T.h[c];
+                if (C > sd) {
+                    const double q = std::min(T.ke * (C - sd), hv);
+                    hv -= q;
+                    sd += q;
+                } else {
+                    const double q = std::min(T.kd * (sd - C), sd);
+                    sd -= q;
+                    hv += q;
+                }
+                T.nh[c] = hv;
+                T.nsed[c] = sd;
+            }
+        T.h.swap(T.nh);
+        T.sed.swap(T.nsed);
+#pragma omp parallel for schedule(static)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const size_t c = (size_t)y * W + x;
+                const D2 v = T.vel[c];
+                const double tx = (double)x - v.x, ty = (double)y - v.y;
+                const double fx = floor(tx), fy = floor(ty);
+                const int x0 = (int)fx, y0 = (int)fy;
+                const double wx = tx - fx, wy = ty - fy;
+                const double I0 = (1.0 - wx) * T.sed[T.idx(x0, y0)] + wx * T.sed[T.idx(x0 + 1, y0)];
+                const double I1 = (1.0 - wx) * T.sed[T.idx(x0, y0 + 1)] + wx * T.sed[T.idx(x0 + 1, y0 + 1)];
+                T.nsed[c] = (1.0 - wy) * I0 + wy * I1;
+            }
+        T.sed.swap(T.nsed);
+#pragma omp parallel for schedule(static)
+        for (long long c = 0; c < (long long)T.N; ++c) {
+            const double wv = T.water[c];
+            T.water[c] = std::max(0.0, wv - std::min(wv, 20.0) * T.evap);
+        }
+    }
+}
+
+// max |a - r|, RMS of a - r and RMS of r over all cells
+void twin_compare(const std::vector<float> &a, const std::vector<double> &r, double &mx, double &rms, double *rref)
+{
+    double m = 0.0, s2 = 0.0, r2 = 0.0;
+    for (size_t c = 0; c < r.size(); ++c) {
+        const double d = (double)a[c] - r[c];
+        m = std::max(m, fabs(d));
+        s2 += d * d;
+        r2 += r[c] * r[c];
+    }
+    mx = m;
+    rms = sqrt(s2 / (double)r.size());
+    if (rref) *rref = sqrt(r2 / (double)r.size());
+}
+
+template <typename T>
+double twin_total(const std::vector<T> &a, const std::vector<T> &b)
+{
+    double s = 0.0;
+    for (size_t c = 0; c < a.size(); ++c) s += (double)a[c] + (double)b[c];
+    return s;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1721,6 +1972,10 @@ void *oracle_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks
     w.height.assign(N, 0.0f);
     w.water.assign(N, 0.0f);
     w.sediment.assign(N, 0.0f);
+    if (cfg->precision == TT_PRECISION_COMPENSATED) {
+        w.ch.assign(N, 0.0f);
+        w.cs.assign(N, 0.0f);
+    }
     w.ids.assign(N, 0);
     w.flux.assign(N, F4{0, 0, 0, 0});
     w.velocity.assign(N, V2{0, 0});
@@ -1842,6 +2097,61 @@ int oracle_jfa(int W, int H, const int32_t *seed, int32_t *dist_out, uint32_t *o
     w.pd.assign(MAXP + 1, default_plate());
     stage_jfa_uplift(w, hW, idW);
     for (int i = 0; i < w.N; ++i) { dist_out[i] = w.dist[i].dist; orig_out[i] = w.dist[i].orig; }
+    return TT_OK;
+}
+
+// the precision study of tt_precision_study on the CPU: erosion-only steps of
+// a float world, a compensated float world and the fp64 twin from one
+// initial state (timings are left at zero)
+int oracle_precision_study(const tt_sim_config *cfg_in, const tt_kernel_settings *ks, const uint8_t *noise256,
+                           int steps, tt_precision_report *out)
+{
+    if (!cfg_in || !ks || !out || steps < 0) return TT_ERR_INVALID;
+    tt_sim_config cfg = *cfg_in;
+    cfg.stage_mask = TT_STAGE_THERMAL | TT_STAGE_HYDRAULIC;
+    tt_sim_config ca = cfg, cb = cfg;
+    ca.precision = TT_PRECISION_FP32;
+    cb.precision = TT_PRECISION_COMPENSATED;
+    auto *ha = (ow_handle *)oracle_world_create(&ca, ks, noise256);
+    auto *hb = (ow_handle *)oracle_world_create(&cb, ks, noise256);
+    if (!ha || !hb) {
+        delete ha;
+        delete hb;
+        return TT_ERR_INVALID;
+    }
+    oracle_world_init(ha);
+    oracle_world_init(hb);
+    World &A = ha->w, &B = hb->w;
+    Twin T;
+    twin_init(T, A);
+    tt_precision_report R;
+    memset(&R, 0, sizeof R);
+    const double t0[3] = {twin_total(T.h, T.sed), twin_total(A.height, A.sediment), twin_total(B.height, B.sediment)};
+    for (int k = 0; k < steps; ++k) {
+        twin_step(T, cfg.thermal_iterations, cfg.hydraulic_iterations, (int)(cfg.seed + A.iter));
+        step(A);
+        step(B);
+    }
+    twin_compare(A.height, T.h, R.height.max_abs_fp32, R.height.rms_fp32, &R.height.rms_ref);
+    twin_compare(B.height, T.h, R.height.max_abs_comp, R.height.rms_comp, nullptr);
+    twin_compare(A.water, T.water, R.water.max_abs_fp32, R.water.rms_fp32, &R.water.rms_ref);
+    twin_compare(B.water, T.water, R.water.max_abs_comp, R.water.rms_comp, nullptr);
+    twin_compare(A.sediment, T.sed, R.sediment.max_abs_fp32, R.sediment.rms_fp32, &R.sediment.rms_ref);
+    twin_compare(B.sediment, T.sed, R.sediment.max_abs_comp, R.sediment.rms_comp, nullptr);
+    const double t1[3] = {twin_total(T.h, T.sed), twin_total(A.height, A.sediment), twin_total(B.height, B.sediment)};
+    R.steps = steps;
+    R.width = T.W;
+    R.height_cells = T.H;
+    R.mass_start = t0[0];
+    R.mass_ref = t1[0];
+    R.mass_fp32 = t1[1];
+    R.mass_comp = t1[2];
+    R.drift_ref = (t1[0] - t0[0]) / t0[0];
+    R.drift_fp32 = (t1[1] - t0[1]) / t0[1];
+    R.drift_comp = (t1[2] - t0[2]) / t0[2];
+    *out = R;
+    delete ha;
+    delete hb;
     return TT_OK;
 }
 

@@ -7,6 +7,7 @@ import sys
 import pytest
 
 from master_thesis_amd import _abi, configs
+from tests.helpers import STATE_FIELDS, STEP_OUTPUTS, diff_fields
 
 pytestmark = pytest.mark.gpu
 
@@ -47,3 +48,63 @@ def test_precision_mode_runs_and_rejects_batches(sim):
         with sim.Batch() as b:
             with pytest.raises(RuntimeError):
                 b.add(w)
+
+
+@pytest.fixture(scope="module")
+def O(oracle_lib):
+    return oracle_lib
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("mode", ["reference", "synthetic"])
+def test_compensated_mode_matches_oracle(sim, O, mode):
+    """Precision mode through full steps (collisions reset the height
+    compensation, the sediment compensation travels with the sediment):
+    every field bit for bit against the oracle's compensated mode."""
+    if mode == "reference":
+        cfg, ks = configs.reference_defaults()
+        cfg.width, cfg.height, cfg.num_plates, cfg.seed = 192, 128, 6, 4
+        cfg.hydraulic_iterations, cfg.thermal_iterations = 6, 2
+    else:
+        cfg, ks = configs.synthetic(128, 128, 5, (0.5, 3.0), 9, hydraulic=5, thermal=4, ccl_every=1)
+        ks.thermalThresholdAngle = 1e-4
+    cfg.precision = _abi.TT_PRECISION_COMPENSATED
+    _say(f"[compensated {mode}] {cfg.width}x{cfg.height}")
+    g = sim.World(cfg, ks).init()
+    o = O.OracleWorld(cfg, ks)
+    o.init()
+    try:
+        for s in range(3):
+            _say(f"[compensated {mode}] step {s}")
+            g.step(1)
+            o.step(1)
+            bad = diff_fields(g, o, tuple(STATE_FIELDS) + tuple(STEP_OUTPUTS))
+            assert not bad, f"step {s}: {bad}"
+            assert g.stats().mass_ledger == o.stats().mass_ledger
+    finally:
+        g.close()
+        o.close()
+
+
+def test_precision_study_matches_oracle(sim, O):
+    """The device study against the oracle's: float and compensated worlds are
+    bit-identical to the oracle's and the fp64 twins perform the same double
+    operations, so the maxima agree exactly and the sums to reduction order."""
+    cfg, ks = configs.synthetic(128, 64, 5, (0.5, 3.0), 3, hydraulic=6, thermal=2, ccl_every=1)
+    ks.thermalThresholdAngle = 1e-4
+    _say("[precision study] 128x64 K=6 T=2, 3 steps")
+    rg = sim.precision_study(cfg, ks, 3)
+    ro = O.precision_study(cfg, ks, 3)
+    for f in ("height", "water", "sediment"):
+        for k in ("max_abs_fp32", "max_abs_comp"):
+            assert rg[f][k] == ro[f][k], (f, k, rg[f][k], ro[f][k])
+        for k in ("rms_fp32", "rms_comp", "rms_ref"):
+            assert rg[f][k] == pytest.approx(ro[f][k], rel=1e-9, abs=1e-300), (f, k)
+    for k in ("mass_start", "mass_ref", "mass_fp32", "mass_comp"):
+        assert rg[k] == pytest.approx(ro[k], rel=1e-12), k
+    for k in ("drift_ref", "drift_fp32", "drift_comp"):
+        assert rg[k] == pytest.approx(ro[k], rel=1e-9, abs=1e-15), k
