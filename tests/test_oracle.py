@@ -586,3 +586,83 @@ def test_separable_uplift_touches_only_cells_in_reach(O):
     changed = h_up != h_no
     assert changed.any()
     assert not (changed & ~reach).any()
+
+
+# --------------------------------------------------------------------------
+# stage G1 / G3 closed forms
+def test_break_score_of_a_square_plate(O):
+    """computeBreakScore (plate_tectonics_kernel.cu:2706-2733) on geometry:
+    a 16x16 square plate has area 256 and 64 boundary sides, so its
+    circularity is 1 - 4*pi*256/64^2 = 1 - pi/4; with the area targets
+    [0, 512] its area score is 1/2."""
+    W = H = 64
+    ids = np.zeros((H, W), np.uint8)
+    ids[8:24, 20:36] = 1
+    w = empty_world(O, W, H, _abi.TT_STAGE_MERGE, targetMinimumPlateArea=0, targetMaximumPlateArea=512)
+    w.set(_abi.FIELD_PLATE_IDS, ids.ravel())
+    set_plates(w, p0=dict(size=W * H - 256, mergeWaitTime=5, used=1),
+               p1=dict(size=256, mergeWaitTime=5, used=1))
+    w.step(1)
+    pd = w.get(_abi.FIELD_PLATE_DATA)
+    assert pd[1]["size"] == 256 and pd[1]["perimeter"] == 64
+    assert pd[1]["circularity"] == pytest.approx(1.0 - np.pi / 4.0, rel=1e-6)
+    assert pd[1]["breakScore"] == pytest.approx(0.5 + 1.0 - np.pi / 4.0, rel=1e-6)
+    # the surrounding plate: same 64 sides, area score clamped to 1
+    assert pd[0]["perimeter"] == 64
+    assert pd[0]["circularity"] == pytest.approx(1.0 - 4.0 * np.pi * (W * H - 256) / 64.0 ** 2, rel=1e-6)
+    assert pd[0]["breakScore"] == pytest.approx(1.0 + pd[0]["circularity"], rel=1e-6)
+    w.close()
+
+
+def _one_plate_kinematics(O, asth, drag, direction, velocity, smooth=0.9):
+    W = H = 64
+    w = empty_world(O, W, H, _abi.TT_STAGE_KINEMATICS, environmentalDragCoefficient=drag,
+                    curlNoiseMultiplier=1.0, velocitySmoothingFactor=smooth, directionSmoothingFactor=smooth)
+    w.set(_abi.FIELD_PLATE_IDS, np.zeros(W * H, np.uint8))
+    w.set(_abi.FIELD_ASTHENOSPHERE, np.tile(np.asarray(asth, np.float32), (W * H, 1)))
+    set_plates(w, p0=dict(size=W * H, used=1, direction=direction, directionSmoothed=direction,
+                          velocity=velocity, velocitySmoothed=velocity, pixelCenter=(0.8, 0.5),
+                          geometricCenter=(31.0, 31.0), mergeWaitTime=3))
+    w.step(1)
+    pd = w.get(_abi.FIELD_PLATE_DATA)[0]
+    w.close()
+    return pd
+
+
+def test_plate_moving_with_the_asthenosphere_keeps_its_velocity(O):
+    """applyPlateMovementChanges (:1863-1925): the asthenosphere sample is
+    biased by 0.2 away from zero (0.1 -> 0.3, 0.2 -> 0.4), which here equals
+    the plate's own velocity (0.6, 0.8) * 0.5, so there is no drag; with no
+    friction the plate keeps speed and direction, and the pixel centre
+    advances by direction * speed modulo 1 (0.8 + 0.3 crosses a cell
+    boundary: hasMoved)."""
+    pd = _one_plate_kinematics(O, (0.1, 0.2), 0.25, (0.6, 0.8), 0.5)
+    assert pd["velocity"] == pytest.approx(0.5, rel=1e-6)
+    assert pd["direction"][0] == pytest.approx(0.6, rel=1e-6) and pd["direction"][1] == pytest.approx(0.8, rel=1e-6)
+    assert pd["pixelCenter"][0] == pytest.approx(0.1, abs=1e-6) and pd["pixelCenter"][1] == pytest.approx(0.9, abs=1e-6)
+    assert pd["hasMoved"] == 1 and pd["mergeWaitTime"] == 2
+
+
+def test_drag_pulls_the_plate_towards_the_asthenosphere(O):
+    """Drag force c * |a - v|^2 along (a - v): the new velocity vector is
+    v + c * |a - v| * (a - v), i.e. a fraction c * |a - v| of the way from
+    the plate's velocity to the (biased, clamped) asthenosphere velocity;
+    smoothing is the exponential average with factor s."""
+    v0, d0 = 0.5, np.array([0.6, 0.8])
+    a = np.array([0.5 + 0.2, -0.5 - 0.2])
+    c, s = 0.1, 0.9
+    pd = _one_plate_kinematics(O, (0.5, -0.5), c, tuple(d0), v0, smooth=s)
+    pv = d0 * v0
+    rel = a - pv
+    nv = pv + c * np.linalg.norm(rel) * rel
+    assert pd["velocity"] == pytest.approx(np.linalg.norm(nv), rel=1e-5)
+    assert pd["direction"][0] == pytest.approx(nv[0] / np.linalg.norm(nv), rel=1e-5)
+    assert pd["direction"][1] == pytest.approx(nv[1] / np.linalg.norm(nv), rel=1e-5)
+    assert pd["velocitySmoothed"] == pytest.approx(s * v0 + (1 - s) * np.linalg.norm(nv), rel=1e-5)
+    ds = s * d0 + (1 - s) * nv / np.linalg.norm(nv)
+    assert pd["directionSmoothed"][0] == pytest.approx(ds[0] / np.linalg.norm(ds), rel=1e-5)
+    # the asthenosphere sample is clamped to [-1, 1] after the bias
+    pd2 = _one_plate_kinematics(O, (2.0, -3.0), c, tuple(d0), v0, smooth=s)
+    rel2 = np.array([1.0, -1.0]) - pv
+    nv2 = pv + c * np.linalg.norm(rel2) * rel2
+    assert pd2["velocity"] == pytest.approx(np.linalg.norm(nv2), rel=1e-5)
