@@ -666,3 +666,49 @@ def test_drag_pulls_the_plate_towards_the_asthenosphere(O):
     rel2 = np.array([1.0, -1.0]) - pv
     nv2 = pv + c * np.linalg.norm(rel2) * rel2
     assert pd2["velocity"] == pytest.approx(np.linalg.norm(nv2), rel=1e-5)
+
+
+def test_subduction_uplift_profiles_closed_form(O):
+    """applyUplift (plate_tectonics_kernel.cu:1662-1738) across a straight
+    oceanic front: plate 0 (lower, so it subducts) moves right into plate 1.
+    The jump flood gives each cell its row's front cell, at distance d.  Cells
+    of the subducting plate scale by 1 + min(log10(1 + 8(d - 1))/1.9085 - 1, 0)
+    * under; cells of the overriding plate scale by 1 + cont(d - range - 10)
+    * max(n . dir0, 0) * over, where n = (1, 0) and dir0 = (1, 0), and for the
+    integer argument cont(x) = exp(-0.02 x^2) (the ridge term
+    0.5 - |frac(|x|) - 0.5| is 0).  Both clamp at minUpliftClamp."""
+    W, H = 64, 64
+    ks_over = dict(subductionUnderUpliftMultiplier=0.01, subductionOverUpliftMultiplier=0.005, subductionOverRange=20)
+    rng = np.random.default_rng(4)
+    h = np.where(np.arange(W)[None, :] < 32, rng.uniform(1000, 1500, (H, W)),
+                 rng.uniform(3000, 3500, (H, W))).astype(F32)
+    ids = np.where(np.arange(W)[None, :] < 32, 0, 1).repeat(H, 0).astype(np.uint8)
+    out = {}
+    for tag, mask in (("collide", _abi.TT_STAGE_COLLIDE), ("uplift", _abi.TT_STAGE_COLLIDE | _abi.TT_STAGE_UPLIFT)):
+        w = empty_world(O, W, H, mask, **ks_over)
+        w.set(_abi.FIELD_HEIGHT, h.ravel())
+        w.set(_abi.FIELD_PLATE_IDS, ids.ravel())
+        set_plates(w, p0=dict(direction=(1.0, 0.0), velocity=1.0, mass=1e6),
+                   p1=dict(direction=(-1.0, 0.0), velocity=1.0, mass=1e6))
+        w.set_iteration(5)
+        w.step(1)
+        out[tag] = (w.get(_abi.FIELD_HEIGHT).reshape(H, W).astype(np.float64),
+                    w.get(_abi.FIELD_DISTANCE)["dist"].reshape(H, W), w.get(_abi.FIELD_PLATE_IDS).reshape(H, W))
+        ks = w.ks
+        w.close()
+    h1, _, id1 = out["collide"]
+    h2, dist, id2 = out["uplift"]
+    assert (id1 == id2).all()
+    lo = float(np.float32(ks.minUpliftClamp))
+    assert lo < 1000.0   # the clamp does not mask the profiles
+    for x in range(2, 30):   # subducting plate 0, front at column 31
+        d = 31 - x
+        assert (dist[:, x] == d).all() and (id2[:, x] == 0).all()
+        m = 1.0 + min(np.log10(1.0 + 8.0 * (d - 1)) / 1.908485019 - 1.0, 0.0) * 0.01
+        np.testing.assert_allclose(h2[:, x], np.maximum(h1[:, x] * m, lo), rtol=2e-6)
+    for x in range(34, 62):  # overriding plate 1, front at column 32
+        d = x - 32
+        assert (dist[:, x] == d).all() and (id2[:, x] == 1).all()
+        xx = d - 20 - 10
+        m = 1.0 + np.exp(-0.02 * xx * xx) * 0.005
+        np.testing.assert_allclose(h2[:, x], np.maximum(h1[:, x] * m, lo), rtol=2e-6)
