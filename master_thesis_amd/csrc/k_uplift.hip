@@ -235,10 +235,13 @@ __global__ __launch_bounds__(256) void k_jfa_pass(tt_dev d, const int *__restric
     out[c] = best;
 }
 
-// The same pass for 4 consecutive cells of a row per thread, when the column
-// offset (step mod W) is a multiple of 4: every tap is then one aligned
-// 16-byte load, so a wavefront moves four times the bytes per load
-// instruction.  Each cell scans its own candidates in the same order.
+// The same pass for 4 consecutive cells of a row per thread: every tap is an
+// aligned 16-byte load, so a wavefront moves four times the bytes per load
+// instruction.  When the column offset (step mod W) is a multiple of 4 the
+// taps at x - s, x, x + s are whole groups; for steps 1..3 each row loads the
+// groups at x - 4, x and x + 4 and the taps are picked from that 12-cell
+// window.  Each cell scans its own candidates in the same order.
+template <bool WINDOW>
 __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restrict__ in, int4 *__restrict__ out,
                                                    int step)
 {
@@ -250,17 +253,25 @@ __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restr
     const int wh = (H << 16) | W;
     const int sx = step % W, sy = step % H;
     const int rows[3] = {tt_wrap1(y - sy, H) * W, y * W, tt_wrap1(y + sy, H) * W};
-    const int cols[3] = {tt_wrap1(x - sx, W), x, tt_wrap1(x + sx, W)};
-    int4 t[3][3];
+    // WINDOW: groups at x - 4, x, x + 4; otherwise at x - s, x, x + s
+    const int gs = WINDOW ? 4 : sx;
+    const int cols[3] = {tt_wrap1(x - gs, W), x, tt_wrap1(x + gs, W)};
+    int t[3][12];
 #pragma unroll
     for (int oy = 0; oy < 3; ++oy)
 #pragma unroll
-        for (int ox = 0; ox < 3; ++ox) t[oy][ox] = in[(rows[oy] + cols[ox]) >> 2];
+        for (int g = 0; g < 3; ++g) {
+            const int4 v = in[(rows[oy] + cols[g]) >> 2];
+            t[oy][4 * g + 0] = v.x;
+            t[oy][4 * g + 1] = v.y;
+            t[oy][4 * g + 2] = v.z;
+            t[oy][4 * g + 3] = v.w;
+        }
     int res[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int me = (y << 16) | (x + k);
-        int best = ((const int *)&t[1][1])[k];
+        int best = t[1][4 + k];
         int bd = 0x7FFFFFFF;
         if (best >= 0) bd = tt_tdist2_pk(me, best, wh);
 #pragma unroll
@@ -268,7 +279,17 @@ __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restr
 #pragma unroll
             for (int ox = 0; ox < 3; ++ox) {
                 if (ox == 1 && oy == 1) continue;
-                const int cand = ((const int *)&t[oy][ox])[k];
+                int cand;
+                if (WINDOW) {
+                    // window index 4 + k + (ox - 1) * sx, in [1, 10] for sx <= 3
+                    const int wi = 4 + k + (ox - 1) * sx;
+                    cand = t[oy][0];
+#pragma unroll
+                    for (int j = 1; j < 12; ++j)
+                        if (j == wi) cand = t[oy][j];
+                } else {
+                    cand = t[oy][4 * ox + k];
+                }
                 if (cand < 0) continue;
                 const int dd = tt_tdist2_pk(me, cand, wh);
                 if (dd < bd) { bd = dd; best = cand; }
@@ -281,14 +302,17 @@ __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restr
 
 hipError_t tt_launch_jfa_pass(hipStream_t s, const tt_dev &d, const int *in, int *out, int step)
 {
-    if ((step % d.W) % 4 == 0) {
-        // N is a multiple of 4 (W is a multiple of 64); 16-byte aligned buffers
-        const unsigned blocks = (unsigned)(((long long)(d.N >> 2) + 255) / 256);
-        hipLaunchKernelGGL(k_jfa_pass4, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
-        return hipGetLastError();
+    // N is a multiple of 4 (W is a multiple of 64); 16-byte aligned buffers
+    const unsigned blocks = (unsigned)(((long long)(d.N >> 2) + 255) / 256);
+    const int sx = step % d.W;
+    if (sx % 4 == 0)
+        hipLaunchKernelGGL(k_jfa_pass4<false>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
+    else if (sx < 4)
+        hipLaunchKernelGGL(k_jfa_pass4<true>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
+    else {
+        const unsigned b1 = (unsigned)(((long long)d.N + 255) / 256);
+        hipLaunchKernelGGL(k_jfa_pass, dim3(b1), dim3(256), 0, s, d, in, out, step);
     }
-    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
-    hipLaunchKernelGGL(k_jfa_pass, dim3(blocks), dim3(256), 0, s, d, in, out, step);
     return hipGetLastError();
 }
 
