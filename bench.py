@@ -325,6 +325,9 @@ def main() -> int:
         probe = probe_hbm(2 << 30, 5)
         result["roofline"]["measured_ceiling"] = probe
         result["roofline"]["frac_of_copy"] = achieved / probe["copy_gbps"] if probe["copy_gbps"] > 0 else None
+        # the same bytes and streams as the kernel, without its stencil and gather
+        result["roofline"]["frac_of_streams"] = (achieved / probe["streams_gbps"]
+                                                 if probe["streams_gbps"] > 0 else None)
 
     if init_state is not None:
         from oracle.oracle_py import OracleWorld

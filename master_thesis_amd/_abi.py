@@ -272,7 +272,7 @@ class PrecisionReport(C.Structure):
 
 class HbmProbe(C.Structure):
     _fields_ = [("copy_gbps", C.c_double), ("read_gbps", C.c_double), ("write_gbps", C.c_double),
-                ("bytes", C.c_int64), ("reps", C.c_int32), ("reserved", C.c_int32)]
+                ("streams_gbps", C.c_double), ("bytes", C.c_int64), ("reps", C.c_int32), ("reserved", C.c_int32)]
 
 
 # numpy-compatible record layouts of the per-cell structs

@@ -1304,8 +1304,8 @@ int tt_probe_hbm(int64_t bytes, int32_t reps, tt_hbm_probe *out)
     if (st == TT_OK && (tt_fill_u32(s, (uint32_t *)a, 0u, n * 4) != hipSuccess ||
                         tt_fill_u32(s, (uint32_t *)b, 0u, n * 4) != hipSuccess))
         st = TT_ERR_DEVICE;
-    double best[3] = {0.0, 0.0, 0.0};
-    for (int which = 0; which < 3 && st == TT_OK; ++which) {
+    double best[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int which = 0; which < 4 && st == TT_OK; ++which) {
         // one untimed launch, then reps timed launches one at a time
         if (tt_launch_probe(s, which, a, b, sink, n) != hipSuccess) { st = TT_ERR_DEVICE; break; }
         for (int r = 0; r < reps; ++r) {
@@ -1316,7 +1316,9 @@ int tt_probe_hbm(int64_t bytes, int32_t reps, tt_hbm_probe *out)
                 st = TT_ERR_DEVICE;
                 break;
             }
-            const double moved = (which == 0 ? 2.0 : 1.0) * (double)bytes;
+            // the stream probe moves 72 bytes per cell of (bytes / 36) & ~3 cells
+            const double moved = which == 3 ? 72.0 * (double)((bytes / 36) & ~3LL)
+                                            : (which == 0 ? 2.0 : 1.0) * (double)bytes;
             if (ms > 0.0f) best[which] = std::max(best[which], moved / (ms * 1e-3) / 1e9);
         }
     }
@@ -1324,6 +1326,7 @@ int tt_probe_hbm(int64_t bytes, int32_t reps, tt_hbm_probe *out)
     out->copy_gbps = best[0];
     out->read_gbps = best[1];
     out->write_gbps = best[2];
+    out->streams_gbps = best[3];
     if (s) (void)hipStreamSynchronize(s);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

@@ -29,7 +29,7 @@ def test_precision_struct_sizes():
 
 
 def test_hbm_probe_struct_size():
-    assert C.sizeof(_abi.HbmProbe) == 40
+    assert C.sizeof(_abi.HbmProbe) == 48
 
 
 def test_server_struct_sizes():

@@ -91,5 +91,5 @@ def test_hbm_probe(sim):
     print("[hbm probe] 64 MiB", flush=True)
     r = sim.probe_hbm(64 << 20, 2)
     assert r["bytes"] == 64 << 20 and r["reps"] == 2
-    for k in ("copy_gbps", "read_gbps", "write_gbps"):
+    for k in ("copy_gbps", "read_gbps", "write_gbps", "streams_gbps"):
         assert 10.0 < r[k] < 100000.0, (k, r[k])
