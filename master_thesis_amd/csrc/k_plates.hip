@@ -1003,6 +1003,13 @@ __device__ __forceinline__ void tt_round_mark(const tt_round &R, uint32_t tile, 
 // back the cells it lowered.  The update is monotone and starts from an upper
 // bound, so the fixed point does not depend on the order of the updates: it
 // equals the oracle's worklist result.
+//
+// Inside a tile each wavefront owns TH / 4 consecutive rows and relaxes them
+// top-down and then bottom-up (Gauss-Seidel in the column direction): a row
+// reads the row the same wavefront has just lowered, so a value crosses the
+// wavefront's rows in one pass instead of one row per pass.  Between two
+// barriers every wavefront makes both passes; a barrier interval in which no
+// cell of the tile changed is the tile's fixed point.
 template <int TH>
 __global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *tiles, const uint8_t *ids, int plate,
                                                      const float *height, float *eff, tt_round R)
@@ -1026,43 +1033,49 @@ __global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *t
     const float SQRT2 = 1.41421356237f;
 #pragma unroll
     for (int j = 0; j < PR; ++j) {
-        const size_t c = (size_t)(ty0 + wv + 4 * j) * W + tx0 + lx;
+        const size_t c = (size_t)(ty0 + wv * PR + j) * W + tx0 + lx;
         in[j] = ids[c] == plate;
         c1[j] = height[c] * 1.0f;
         c2[j] = height[c] * SQRT2;
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < PR; ++j) init[j] = e[wv + 4 * j + 1][lx + 1];
+    for (int j = 0; j < PR; ++j) init[j] = e[wv * PR + j + 1][lx + 1];
     int capped = 1;
-    for (int it = 0; it < TT_SWEEP_CAP; ++it) {
+    for (int it = 0; it < TT_SWEEP_CAP; it += 2) {
         int ch = 0;
 #pragma unroll
-        for (int j = 0; j < PR; ++j) {
-            if (!in[j]) continue;
-            const int r = wv + 4 * j + 1, q = lx + 1;
-            const float old = e[r][q];
-            float md = old;
-            // neighbour order (-1,0),(1,0),(0,-1),(0,1),(-1,-1),(-1,1),(1,-1),(1,1)
-            float t;
-            t = e[r][q - 1] + c1[j]; if (t < md) md = t;
-            t = e[r][q + 1] + c1[j]; if (t < md) md = t;
-            t = e[r - 1][q] + c1[j]; if (t < md) md = t;
-            t = e[r + 1][q] + c1[j]; if (t < md) md = t;
-            t = e[r - 1][q - 1] + c2[j]; if (t < md) md = t;
-            t = e[r + 1][q - 1] + c2[j]; if (t < md) md = t;
-            t = e[r - 1][q + 1] + c2[j]; if (t < md) md = t;
-            t = e[r + 1][q + 1] + c2[j]; if (t < md) md = t;
-            if (md != old) { e[r][q] = md; ch = 1; }
+        for (int jj = 0; jj < 2 * PR; ++jj) {
+            const int j = jj < PR ? jj : 2 * PR - 1 - jj;
+            if (in[j]) {
+                const int r = wv * PR + j + 1, q = lx + 1;
+                const float old = e[r][q];
+                float md = old;
+                // neighbour order (-1,0),(1,0),(0,-1),(0,1),(-1,-1),(-1,1),(1,-1),(1,1)
+                float t;
+                t = e[r][q - 1] + c1[j]; if (t < md) md = t;
+                t = e[r][q + 1] + c1[j]; if (t < md) md = t;
+                t = e[r - 1][q] + c1[j]; if (t < md) md = t;
+                t = e[r + 1][q] + c1[j]; if (t < md) md = t;
+                t = e[r - 1][q - 1] + c2[j]; if (t < md) md = t;
+                t = e[r + 1][q - 1] + c2[j]; if (t < md) md = t;
+                t = e[r - 1][q + 1] + c2[j]; if (t < md) md = t;
+                t = e[r + 1][q + 1] + c2[j]; if (t < md) md = t;
+                if (md != old) { e[r][q] = md; ch = 1; }
+            }
+            // the next row's reads follow this row's writes (also of the
+            // other lanes): wait for the LDS queue, and keep the compiler
+            // from moving those reads up
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         }
         if (!__syncthreads_or(ch)) { capped = 0; break; }
     }
     int wrote = 0;
 #pragma unroll
     for (int j = 0; j < PR; ++j) {
-        const float v = e[wv + 4 * j + 1][lx + 1];
+        const float v = e[wv * PR + j + 1][lx + 1];
         if (in[j] && v != init[j]) {
-            eff[(size_t)(ty0 + wv + 4 * j) * W + tx0 + lx] = v;
+            eff[(size_t)(ty0 + wv * PR + j) * W + tx0 + lx] = v;
             wrote = 1;
         }
     }
