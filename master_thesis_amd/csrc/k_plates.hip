@@ -1008,8 +1008,8 @@ __device__ __forceinline__ void tt_round_mark(const tt_round &R, uint32_t tile, 
 // top-down and then bottom-up (Gauss-Seidel in the column direction): a row
 // reads the row the same wavefront has just lowered, so a value crosses the
 // wavefront's rows in one pass instead of one row per pass.  Between two
-// barriers every wavefront makes both passes; a barrier interval in which no
-// cell of the tile changed is the tile's fixed point.
+// barriers every wavefront makes the down and up passes twice; a barrier
+// interval in which no cell of the tile changed is the tile's fixed point.
 template <int TH>
 __global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *tiles, const uint8_t *ids, int plate,
                                                      const float *height, float *eff, tt_round R)
@@ -1042,11 +1042,11 @@ __global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *t
 #pragma unroll
     for (int j = 0; j < PR; ++j) init[j] = e[wv * PR + j + 1][lx + 1];
     int capped = 1;
-    for (int it = 0; it < TT_SWEEP_CAP; it += 2) {
+    for (int it = 0; it < TT_SWEEP_CAP; it += 4) {
         int ch = 0;
 #pragma unroll
-        for (int jj = 0; jj < 2 * PR; ++jj) {
-            const int j = jj < PR ? jj : 2 * PR - 1 - jj;
+        for (int jj = 0; jj < 4 * PR; ++jj) {
+            const int j = (jj % (2 * PR)) < PR ? (jj % (2 * PR)) : 2 * PR - 1 - (jj % (2 * PR));
             if (in[j]) {
                 const int r = wv * PR + j + 1, q = lx + 1;
                 const float old = e[r][q];
