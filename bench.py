@@ -41,16 +41,25 @@ HBM_PEAK_GBS = 8000.0
 PROFILES = ROOT / "profiles"
 
 
-def hydraulic_bytes_per_cell(first: bool) -> int:
+def hydraulic_bytes_per_cell(first: bool, last: bool) -> int:
     """Compulsory HBM bytes of one fused hydraulic launch per cell.
 
     reads: height 4, water 4, flux 16, and either the step's sediment 4 (first
-    sub-iteration) or velocity 8 + the transported sediment gather 4; writes:
-    height 4, water 4, flux 16, velocity 8, sediment 4.
+    sub-iteration) or the previous sub-iteration's flow water 4 + the
+    transported sediment gather 4; writes: height 4, water 4, flux 16,
+    sediment 4, and the flow water 4 (the velocity 8 on the last
+    sub-iteration).
     """
-    reads = 4 + 4 + 16 + (4 if first else 8 + 4)
-    writes = 4 + 4 + 16 + 8 + 4
+    reads = 4 + 4 + 16 + (4 if first else 4 + 4)
+    writes = 4 + 4 + 16 + 4 + (8 if last else 4)
     return reads + writes
+
+
+def hydraulic_bytes_per_launch_avg(K: int) -> float:
+    """Average of hydraulic_bytes_per_cell over the K launches of a step."""
+    if K <= 0:
+        return 0.0
+    return sum(hydraulic_bytes_per_cell(k == 0, k == K - 1) for k in range(K)) / K
 
 
 def model_bytes_per_cell_step(K: int, T: int, side: int, C: int) -> float:
@@ -268,7 +277,7 @@ def main() -> int:
     K = cfg.hydraulic_iterations
     launches = max(times.hydraulic_launches, 1.0)
     avg_ms = times.ms_hydraulic / launches
-    bpl = N * (hydraulic_bytes_per_cell(True) + (K - 1) * hydraulic_bytes_per_cell(False)) / max(K, 1)
+    bpl = N * hydraulic_bytes_per_launch_avg(K)
     achieved = bpl / (avg_ms * 1e-3) / 1e9
     model_b = model_bytes_per_cell_step(K, cfg.thermal_iterations, max(cfg.width, cfg.height),
                                         cfg.iterations_for_ccl)

@@ -420,7 +420,7 @@ int tt_math_eval_device(int32_t fn, const float *in, float *out, int64_t n);
  * measured ceiling beside the published peak).  Times `reps` launches each of
  * a float4 copy (bytes read + bytes written), a float4 read-only sum, a
  * float4 fill, and a ten-stream copy with the fused hydraulic kernel's bytes
- * per cell (reads and writes of 4, 4, 16, 8 and 4 bytes) over two device
+ * per cell (reads and writes of 4, 4, 16, 4 and 4 bytes) over two device
  * buffers of `bytes` each (a multiple of 16, at most 4 GiB), and reports the
  * best GB/s of each.  Allocates 2 x bytes of device memory for the duration
  * of the call. */
@@ -428,7 +428,7 @@ typedef struct tt_hbm_probe {
     double copy_gbps;
     double read_gbps;
     double write_gbps;
-    double streams_gbps;   /* the ten-stream copy (72 bytes per cell) */
+    double streams_gbps;   /* the ten-stream copy (64 bytes per cell) */
     int64_t bytes;
     int32_t reps;
     int32_t reserved;

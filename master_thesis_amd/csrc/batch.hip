@@ -202,6 +202,7 @@ void fill_desc(tt_bworld &B, const tt_world *w, int T, int K)
         B.sed[i] = w->sed[i];
     }
     B.vel = w->vel;
+    B.wn = w->wn;
     B.frozen = w->tileFrozen;
 }
 

@@ -66,6 +66,7 @@ __global__ __launch_bounds__(256) void k_hydraulic_batch(const tt_bworld *__rest
     A.fin = B.flux[fc];
     A.fout = B.flux[fc ^ 1];
     A.vel = B.vel;
+    A.wn = B.wn;
     A.ch = nullptr;
     A.csprev = nullptr;
     A.csout = nullptr;

@@ -378,33 +378,30 @@ __global__ __launch_bounds__(256) void k_probe_fill(float4 *__restrict__ b, long
 }
 
 // ten streams with the fused hydraulic kernel's bytes per cell: reads of 4, 4,
-// 16, 8 and 4 bytes and writes of the same sizes; the five input arrays are
-// carved from one buffer and the five outputs from the other (36 B per cell
+// 16, 4 and 4 bytes and writes of the same sizes; the five input arrays are
+// carved from one buffer and the five outputs from the other (32 B per cell
 // each)
 __global__ __launch_bounds__(256) void k_probe_streams(const float *__restrict__ a, float *__restrict__ b, long long n)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const float4 *af = reinterpret_cast<const float4 *>(a + 3 * n);
-    const float2 *av = reinterpret_cast<const float2 *>(a + 7 * n);
-    float4 *bf = reinterpret_cast<float4 *>(b + 3 * n);
-    float2 *bv = reinterpret_cast<float2 *>(b + 7 * n);
-    const float h = a[i], w = a[n + i], s = a[2 * n + i];
+    const float4 *af = reinterpret_cast<const float4 *>(a + 4 * n);
+    float4 *bf = reinterpret_cast<float4 *>(b + 4 * n);
+    const float h = a[i], w = a[n + i], s = a[2 * n + i], q = a[3 * n + i];
     const float4 f = af[i];
-    const float2 v = av[i];
     b[i] = h + 1.0f;
     b[n + i] = w + 1.0f;
     b[2 * n + i] = s + 1.0f;
+    b[3 * n + i] = q + 1.0f;
     bf[i] = f;
-    bv[i] = v;
 }
 
 hipError_t tt_launch_probe(hipStream_t s, int which, const float4 *a, float4 *b, float *sink, long long n)
 {
     if (which == 3) {
-        // n float4 per buffer -> cells of 36 bytes, rounded down to a multiple of 4
-        // so that the float4 and float2 sub-arrays stay aligned
-        const long long cells = (n * 16 / 36) & ~3LL;
+        // n float4 per buffer -> cells of 32 bytes (n / 2), a multiple of 4 when
+        // n is even, so that the float4 sub-array stays aligned
+        const long long cells = (n / 2) & ~3LL;
         const unsigned blocks = (unsigned)((cells + 255) / 256);
         hipLaunchKernelGGL(k_probe_streams, dim3(blocks), dim3(256), 0, s, (const float *)a, (float *)b, cells);
         return hipGetLastError();

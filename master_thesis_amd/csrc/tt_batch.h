@@ -19,6 +19,7 @@ struct tt_bworld {
     float4 *flux[2];
     float *sed[2];
     float2 *vel;
+    float *wn;              // water after the flow between sub-iterations
     const uint8_t *frozen;  // per 64x16 tile: unchanged by this step's thermal passes
 };
 

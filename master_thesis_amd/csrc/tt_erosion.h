@@ -200,7 +200,8 @@ struct tt_hyd_args {
     float *wout;
     const float4 *fin;
     float4 *fout;
-    float2 *vel;
+    float2 *vel;          // written by the last sub-iteration only
+    float *wn;            // water after the flow: the previous sub-iteration's in, this one's out (in place)
     const float *sprev;
     float *sout;
     float *ch;            // precision mode: height compensation (in place)
@@ -283,6 +284,27 @@ __device__ __forceinline__ float tt_rain_at(int gx, int gy, int rseed, float rai
     return (tt_fmaxf(0.0f, n) * 1.0f) * rain;
 }
 
+// flow velocity of a cell from the fluxes of the cell and its four
+// neighbours and its water after the flow (flow, plate_tectonics_kernel.cu:
+// 2487-2533): netFlow / max(w, 0.01), clamped to length 50
+__device__ __forceinline__ void tt_flow_velocity(const float4 &Sf, const float4 &Lf, const float4 &Uf, const float4 &Rf,
+                                                 const float4 &Df, float wn, float &vx, float &vy, float &vmag)
+{
+    const float nx = (((Lf.z - Sf.x) + Sf.z) - Rf.x) * 0.5f;
+    const float ny = (((Uf.w - Sf.y) + Sf.y) - Df.w) * 0.5f;
+    const float hh = tt_fmaxf(wn, 0.01f);
+    vx = nx / hh;
+    vy = ny / hh;
+    const float vm = sqrtf(vx * vx + vy * vy);
+    vmag = vm;
+    if (vm > 50.0f) {
+        const float sc = 50.0f / vm;
+        vx = vx * sc;
+        vy = vy * sc;
+        vmag = sqrtf(vx * vx + vy * vy);
+    }
+}
+
 // One sub-iteration on one tile.  FIRST: the water input has no rain yet
 // (rain is added while loading) and the sediment is the step's state;
 // otherwise the water input already holds this sub-iteration's rain (added by
@@ -290,11 +312,14 @@ __device__ __forceinline__ float tt_rain_at(int gx, int gy, int rseed, float rai
 // sub-iteration's output transported along its velocity.  LAST: the water
 // output gets no rain.
 //
-// Every global load of the tile is issued before the first barrier: the
-// velocities and then the sediment taps they address, the heights and water
-// with their halo, and the incoming fluxes of the flux ring.  The barriers
-// wait for LDS traffic only, so these loads stay in flight together instead
-// of forming four dependent memory round trips per wavefront.
+// Between sub-iterations the velocity is not stored: a sub-iteration stores
+// its water after the flow (4 bytes a cell instead of an 8-byte velocity), and
+// the next one recomputes the velocity, bit for bit, from that water and the
+// fluxes it reads anyway (its incoming flux ring).  Only the last
+// sub-iteration writes the velocity field, for the closing transport and the
+// state.  Loads are issued together before the first barrier (heights and
+// water with their halo, the flux ring, the flow water); the sediment taps
+// follow the velocity and are in flight while the new fluxes are computed.
 template <int TH, bool FIRST, bool LAST, bool COMP = false>
 __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_lds<TH> &S, int x0, int y0)
 {
@@ -309,13 +334,11 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
     const int rseed = A.rseed;
     const int gx = x0 + lane;
 
-    // 1. velocities of the own cells, then the sediment they sample
-    float2 pv[RPW];
-    float s4[RPW][4], c4[RPW][4];
-    float wxs[RPW], wys[RPW];
+    // 1. the previous sub-iteration's flow water of the own cells
+    float wnp[RPW];
     if (!FIRST) {
 #pragma unroll
-        for (int r = 0; r < RPW; ++r) pv[r] = A.vel[(y0 + wv * RPW + r) * W + gx];
+        for (int r = 0; r < RPW; ++r) wnp[r] = A.wn[(y0 + wv * RPW + r) * W + gx];
     }
     // 2. heights and water on the tile plus a two-cell halo: rows wv, wv+4,
     // ...; lanes 0..63 take columns 0..63, lanes 0..3 also columns 64..67
@@ -345,28 +368,15 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             cf[k] = A.fin[tt_wrap1(y0 + fy - 1, H) * W + tt_wrap1(x0 + fx - 1, W)];
         }
     }
-    // sediment taps (the state itself on the first sub-iteration)
+    // sediment: the state itself on the first sub-iteration
+    float s4[RPW][4], c4[RPW][4];
+    float wxs[RPW], wys[RPW];
+    if (FIRST) {
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-        const int gy = y0 + wv * RPW + r;
-        const int c = gy * W + gx;
-        if (FIRST) {
+        for (int r = 0; r < RPW; ++r) {
+            const int c = (y0 + wv * RPW + r) * W + gx;
             s4[r][0] = A.sprev[c];
             if (COMP) c4[r][0] = A.csprev[c];
-        } else {
-            const tt_taps t = tt_bilinear_taps((float)gx - pv[r].x * 1.0f, (float)gy - pv[r].y * 1.0f, W, H);
-            wxs[r] = t.wx;
-            wys[r] = t.wy;
-            s4[r][0] = A.sprev[t.a];
-            s4[r][1] = A.sprev[t.b];
-            s4[r][2] = A.sprev[t.c];
-            s4[r][3] = A.sprev[t.d];
-            if (COMP) {
-                c4[r][0] = A.csprev[t.a];
-                c4[r][1] = A.csprev[t.b];
-                c4[r][2] = A.csprev[t.c];
-                c4[r][3] = A.csprev[t.d];
-            }
         }
     }
 #pragma unroll
@@ -386,8 +396,46 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             }
         }
     }
+    if (!FIRST) {
+        // the incoming fluxes, for the previous sub-iteration's velocities
+#pragma unroll
+        for (int k = 0; k < NFR; ++k) {
+            const int i = tid + 256 * k;
+            if (i < NFI) {
+                const int fy = i / TT_FW, fx = i - fy * TT_FW;
+                S.F[fy][fx] = cf[k];
+            }
+        }
+    }
     __syncthreads();
-    // outflow flux on the tile plus a one-cell halo
+    if (!FIRST) {
+        // 4. the previous sub-iteration's velocity of the own cells (as it
+        // computed it) and the sediment taps it addresses
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int ry = wv * RPW + r;
+            const int gy = y0 + ry;
+            const int fy = ry + 1, fx = lane + 1;
+            float vx, vy, vmag;
+            tt_flow_velocity(S.F[fy][fx], S.F[fy][fx - 1], S.F[fy + 1][fx], S.F[fy][fx + 1], S.F[fy - 1][fx], wnp[r],
+                             vx, vy, vmag);
+            const tt_taps t = tt_bilinear_taps((float)gx - vx * 1.0f, (float)gy - vy * 1.0f, W, H);
+            wxs[r] = t.wx;
+            wys[r] = t.wy;
+            s4[r][0] = A.sprev[t.a];
+            s4[r][1] = A.sprev[t.b];
+            s4[r][2] = A.sprev[t.c];
+            s4[r][3] = A.sprev[t.d];
+            if (COMP) {
+                c4[r][0] = A.csprev[t.a];
+                c4[r][1] = A.csprev[t.b];
+                c4[r][2] = A.csprev[t.c];
+                c4[r][3] = A.csprev[t.d];
+            }
+        }
+        __syncthreads();
+    }
+    // 5. outflow flux on the tile plus a one-cell halo
     const float fk = A.fk, g = A.g, L = A.L;
 #pragma unroll
     for (int k = 0; k < NFR; ++k) {
@@ -430,18 +478,8 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         const float dv = 1.0f * (in - out);
         const int ly = ry + 2, lx = lane + 2;
         const float wn = tt_fmaxf(0.0f, S.Wt[ly][lx] + dv);
-        const float nx = (((Lf.z - Sf.x) + Sf.z) - Rf.x) * 0.5f;
-        const float ny = (((Uf.w - Sf.y) + Sf.y) - Df.w) * 0.5f;
-        const float hh = tt_fmaxf(wn, 0.01f);
-        float vx = nx / hh, vy = ny / hh;
-        const float vm = sqrtf(vx * vx + vy * vy);
-        float vmag = vm;
-        if (vm > 50.0f) {
-            const float sc = 50.0f / vm;
-            vx = vx * sc;
-            vy = vy * sc;
-            vmag = sqrtf(vx * vx + vy * vy);
-        }
+        float vx, vy, vmag;
+        tt_flow_velocity(Sf, Lf, Uf, Rf, Df, wn, vx, vy, vmag);
         float wo = tt_fmaxf(0.0f, wn - (tt_fminf(wn, 20.0f) * evap) * 1.0f);
         if (!LAST) wo = wo + tt_rain_at(gx, gy, rseed, rain);
         __builtin_nontemporal_store(wo, &A.wout[c]);
@@ -457,8 +495,12 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             sed = tt_bilinear_mix(s4[r][0], s4[r][1], s4[r][2], s4[r][3], wxs[r], wys[r]);
             if (COMP) cs = tt_bilinear_mix(c4[r][0], c4[r][1], c4[r][2], c4[r][3], wxs[r], wys[r]);
         }
-        __builtin_nontemporal_store(vx, &A.vel[c].x);
-        __builtin_nontemporal_store(vy, &A.vel[c].y);
+        if (LAST) {
+            __builtin_nontemporal_store(vx, &A.vel[c].x);
+            __builtin_nontemporal_store(vy, &A.vel[c].y);
+        } else {
+            __builtin_nontemporal_store(wn, &A.wn[c]);
+        }
         const float mc = S.M[ly][lx];
         // Slope: (m[+1] - m[-1]) / 2.0 in the reference; halving is exact in float
         const float dzdx = (S.M[ly][lx + 1] - S.M[ly][lx - 1]) * 0.5f;

@@ -36,6 +36,7 @@ struct tt_world {
     float4 *flux[2] = {nullptr, nullptr};
     float *sed[2] = {nullptr, nullptr};
     float2 *vel = nullptr;
+    float *wn = nullptr;        // hydraulic: water after the flow, between sub-iterations
     int hc = 0, ic = 0, wc = 0, fc = 0, sc = 0;
     tt_plate_data *pd = nullptr;
     uint32_t *divBitmap = nullptr;

@@ -119,6 +119,7 @@ size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w)
         take((void **)&w->listN[i], 16);
     }
     take((void **)&w->vel, N * 8);
+    take((void **)&w->wn, N * 4);
     take((void **)&w->word, N * 4);
     take((void **)&w->cnt, N);
     take((void **)&w->segCount, (N / 64 + 1) * 4);
@@ -484,7 +485,7 @@ static int step_erode(tt_world *w)
         const int si = it == 0 ? w->sc : (w->sc ^ ((it - 1) & 1)), so = it == 0 ? w->sc : (w->sc ^ (it & 1));
         float *sprev = w->sed[si], *sout = w->sed[so];
         CK(tt_launch_hydraulic(s, d, w->rows, it == 0, it == K - 1, rseed, w->h[hc], w->h[1 - hc], w->water[wc],
-                               w->water[1 - wc], w->flux[fc], w->flux[1 - fc], w->vel, sprev, sout, w->ch,
+                               w->water[1 - wc], w->flux[fc], w->flux[1 - fc], w->vel, w->wn, sprev, sout, w->ch,
                                w->ch ? w->cs[si] : nullptr, w->ch ? w->cs[so] : nullptr));
         hc = 1 - hc;
         wc = 1 - wc;
@@ -624,6 +625,7 @@ static int init_common_zero(tt_world *w)
         CKS(zero(w, w->sed[i], N * 4));
     }
     CKS(zero(w, w->vel, N * 8));
+    CKS(zero(w, w->wn, N * 4));
     if (w->ch) {
         CKS(zero(w, w->ch, N * 4));
         CKS(zero(w, w->cs[0], N * 4));
@@ -858,7 +860,7 @@ static int tune_rows(tt_world *w)
         if (!tt_tile_rows_valid(rows, w->H)) continue;
         auto launch = [&](int k) {
             return tt_launch_hydraulic(w->stream, d, rows, 0, 0, 1, w->h[k & 1], w->h[1 - (k & 1)], w->water[k & 1],
-                                       w->water[1 - (k & 1)], w->flux[k & 1], w->flux[1 - (k & 1)], w->vel,
+                                       w->water[1 - (k & 1)], w->flux[k & 1], w->flux[1 - (k & 1)], w->vel, w->wn,
                                        w->sed[k & 1], w->sed[1 - (k & 1)]);
         };
         CK(launch(0));
@@ -1316,8 +1318,8 @@ int tt_probe_hbm(int64_t bytes, int32_t reps, tt_hbm_probe *out)
                 st = TT_ERR_DEVICE;
                 break;
             }
-            // the stream probe moves 72 bytes per cell of (bytes / 36) & ~3 cells
-            const double moved = which == 3 ? 72.0 * (double)((bytes / 36) & ~3LL)
+            // the stream probe moves 64 bytes per cell of (bytes / 32) & ~3 cells
+            const double moved = which == 3 ? 64.0 * (double)((bytes / 32) & ~3LL)
                                             : (which == 0 ? 2.0 : 1.0) * (double)bytes;
             if (ms > 0.0f) best[which] = std::max(best[which], moved / (ms * 1e-3) / 1e9);
         }

@@ -289,7 +289,7 @@ def precision_study(cfg: _abi.SimConfig, ks: _abi.KernelSettings, steps: int) ->
 def probe_hbm(nbytes: int = 2 << 30, reps: int = 5) -> dict:
     """Measured HBM ceilings (tt_probe_hbm): best GB/s of a float4 copy (read
     plus write bytes), a read-only pass, a fill, and a ten-stream copy with the
-    hydraulic kernel's 72 bytes per cell."""
+    hydraulic kernel's 64 bytes per cell."""
     r = _abi.HbmProbe()
     check(lib().tt_probe_hbm(int(nbytes), int(reps), C.byref(r)), "tt_probe_hbm")
     return {"copy_gbps": r.copy_gbps, "read_gbps": r.read_gbps, "write_gbps": r.write_gbps,
