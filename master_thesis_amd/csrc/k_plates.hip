@@ -636,28 +636,37 @@ __global__ __launch_bounds__(1024) void k_merge_decide(tt_dev d, const uint8_t *
         tt_merge_word(g[k].w, i0 + 12, P, K, cand);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    // the pairing is serial in ascending matrix index; wavefront 0 finds the
+    // non-empty candidate words 64 at a time, lane 0 walks their bits
+    if (threadIdx.x < 64) {
         int merges = 0;
-        for (int wi = 0; wi < TT_NBM_WORDS; ++wi) {
-            unsigned bits = cand[wi];
-            while (bits) {
-                const int bit = __ffs((int)bits) - 1;
-                bits &= bits - 1u;
-                const int i = wi * 32 + bit;
-                const int x = i % TT_MAXP, y = i / TT_MAXP;
-                const bool xIsParent = P[y].mass <= P[x].mass;
-                if (mids[x] != 0) continue;
-                mids[x] = xIsParent ? -1 : TT_MAXP - y;
-                if (mids[y] != 0) { mids[x] = 0; continue; }
-                mids[y] = xIsParent ? TT_MAXP - x : -1;
-                ++merges;
-                const int parent = xIsParent ? x : y;
-                // clearCollisionTypes writes entry min(parent, j) of the 1-row table
-                const int hi = parent < TT_MAXP - 1 ? parent : parent - 1;
-                for (int j = 0; j <= hi; ++j) typeBitmap[j] = 0;
+        for (int w0 = 0; w0 < TT_NBM_WORDS; w0 += 64) {
+            const int wl = w0 + (int)threadIdx.x;
+            unsigned long long nz = __ballot(wl < TT_NBM_WORDS && cand[wl] != 0u);
+            if (threadIdx.x != 0) continue;
+            while (nz) {
+                const int wi = w0 + __ffsll((long long)nz) - 1;
+                nz &= nz - 1ull;
+                unsigned bits = cand[wi];
+                while (bits) {
+                    const int bit = __ffs((int)bits) - 1;
+                    bits &= bits - 1u;
+                    const int i = wi * 32 + bit;
+                    const int x = i % TT_MAXP, y = i / TT_MAXP;
+                    const bool xIsParent = P[y].mass <= P[x].mass;
+                    if (mids[x] != 0) continue;
+                    mids[x] = xIsParent ? -1 : TT_MAXP - y;
+                    if (mids[y] != 0) { mids[x] = 0; continue; }
+                    mids[y] = xIsParent ? TT_MAXP - x : -1;
+                    ++merges;
+                    const int parent = xIsParent ? x : y;
+                    // clearCollisionTypes writes entry min(parent, j) of the 1-row table
+                    const int hi = parent < TT_MAXP - 1 ? parent : parent - 1;
+                    for (int j = 0; j <= hi; ++j) typeBitmap[j] = 0;
+                }
             }
         }
-        ctl->merges = merges;
+        if (threadIdx.x == 0) ctl->merges = merges;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < TT_MAXP; i += 1024) mergeIds[i] = mids[i];
