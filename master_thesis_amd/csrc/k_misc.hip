@@ -161,6 +161,42 @@ hipError_t tt_compact_u8(hipStream_t s, const uint8_t *flags, long long n, uint3
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// unordered compaction: the indices i with flags[i] != 0 in no particular
+// order, for consumers whose result does not depend on it.  Each thread takes
+// 16 flags (one 16-byte load), each block one atomic on the count.
+// ---------------------------------------------------------------------------
+#define TT_COMPACT_ANY_B 4096
+
+__global__ __launch_bounds__(256) void k_compact_any(const uint8_t *__restrict__ flags, long long n,
+                                                     uint32_t *__restrict__ out, uint32_t *__restrict__ count)
+{
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t base;
+    const long long b0 = (long long)blockIdx.x * TT_COMPACT_ANY_B + threadIdx.x * 16;
+    uint32_t f = 0u;
+    if (b0 + 16 <= n) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(flags + b0);
+        const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if ((w4[k >> 2] >> (8 * (k & 3))) & 0xFFu) f |= 1u << k;
+    } else {
+        for (int k = 0; k < 16; ++k)
+            if (b0 + k < n && flags[b0 + k] != 0) f |= 1u << k;
+    }
+    uint32_t tot;
+    const uint32_t pre = tt_block_excl_scan((uint32_t)__builtin_popcount(f), wsum, &tot);
+    if (threadIdx.x == 0) base = tot ? atomicAdd(count, tot) : 0u;
+    __syncthreads();
+    uint32_t pos = base + pre;
+    while (f) {
+        const int k = __builtin_ctz(f);
+        f &= f - 1u;
+        out[pos++] = (uint32_t)(b0 + k);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, uint32_t v, long long n)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -410,5 +446,14 @@ hipError_t tt_launch_probe(hipStream_t s, int which, const float4 *a, float4 *b,
     if (which == 0) hipLaunchKernelGGL(k_probe_copy, dim3(blocks), dim3(256), 0, s, a, b, n);
     else if (which == 1) hipLaunchKernelGGL(k_probe_read, dim3(blocks), dim3(256), 0, s, a, sink, n);
     else hipLaunchKernelGGL(k_probe_fill, dim3(blocks), dim3(256), 0, s, b, n);
+    return hipGetLastError();
+}
+
+hipError_t tt_compact_u8_any(hipStream_t s, const uint8_t *flags, long long n, uint32_t *out_idx, uint32_t *count)
+{
+    hipError_t e = tt_fill_u32(s, count, 0u, 1);
+    if (e != hipSuccess || n <= 0) return e;
+    const long long nb = (n + TT_COMPACT_ANY_B - 1) / TT_COMPACT_ANY_B;
+    hipLaunchKernelGGL(k_compact_any, dim3((unsigned)nb), dim3(256), 0, s, flags, n, out_idx, count);
     return hipGetLastError();
 }

@@ -1262,7 +1262,7 @@ hipError_t tt_launch_flood_cc(hipStream_t s, const tt_dev &d, uint8_t *ids, int 
     const dim3 tiles(d.W / TT_TILE_W, d.H / TT_TILE_H);
     hipLaunchKernelGGL(k_fl_local, tiles, dim3(256), 0, s, d, ids, oldId, lab, rootFlag);
     hipLaunchKernelGGL(k_fl_merge, tiles, dim3(128), 0, s, d, ids, oldId, lab);
-    hipError_t e = tt_compact_u8(s, rootFlag, (long long)d.N, list, nlist, scanTmp);
+    hipError_t e = tt_compact_u8_any(s, rootFlag, (long long)d.N, list, nlist);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_fl_flatten, dim3(1024), dim3(256), 0, s, list, nlist, lab);
     hipLaunchKernelGGL(k_fl_apply, dim3((unsigned)(((long long)d.N + 255) / 256)), dim3(256), 0, s, d, lab, ctl, ids,

@@ -204,39 +204,8 @@ __device__ __forceinline__ int tt_tdist2_pk(int p, int s, int wh)
 
 // JFAPass (plate_tectonics_kernel.cu:1512-1571): candidates at (ox, oy) * step
 // in scan order oy = -1..1, ox = -1..1; a strictly smaller squared distance
-// replaces the current seed.
-__global__ __launch_bounds__(256) void k_jfa_pass(tt_dev d, const int *__restrict__ in, int *__restrict__ out, int step)
-{
-    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
-    if (c >= (uint32_t)d.N) return;
-    int x, y;
-    tt_cell_xy(c, d, x, y);
-    const int W = d.W, H = d.H;
-    const int me = (y << 16) | x, wh = (H << 16) | W;
-    int best = in[c];
-    int bd = 0x7FFFFFFF;
-    if (best >= 0) bd = tt_tdist2_pk(me, best, wh);
-    // (x + k * step) mod W == (x + k * (step mod W)) mod W, and the reduced
-    // offset keeps the single add / subtract wrap in range
-    const int sx = step % W, sy = step % H;
-    const int rows[3] = {tt_wrap1(y - sy, H) * W, y * W, tt_wrap1(y + sy, H) * W};
-    const int cols[3] = {tt_wrap1(x - sx, W), x, tt_wrap1(x + sx, W)};
-#pragma unroll
-    for (int oy = 0; oy < 3; ++oy) {
-#pragma unroll
-        for (int ox = 0; ox < 3; ++ox) {
-            if (ox == 1 && oy == 1) continue;
-            const int cand = in[rows[oy] + cols[ox]];
-            if (cand < 0) continue;
-            const int dd = tt_tdist2_pk(me, cand, wh);
-            if (dd < bd) { bd = dd; best = cand; }
-        }
-    }
-    out[c] = best;
-}
-
-// The same pass for 4 consecutive cells of a row per thread: every tap is an
-// aligned 16-byte load, so a wavefront moves four times the bytes per load
+// replaces the current seed.  4 consecutive cells of a row per thread: every
+// tap is an aligned 16-byte load, so a wavefront moves four times the bytes per load
 // instruction.  When the column offset (step mod W) is a multiple of 4 the
 // taps at x - s, x, x + s are whole groups; for steps 1..3 each row loads the
 // groups at x - 4, x and x + 4 and the taps are picked from that 12-cell
@@ -302,17 +271,17 @@ __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restr
 
 hipError_t tt_launch_jfa_pass(hipStream_t s, const tt_dev &d, const int *in, int *out, int step)
 {
-    // N is a multiple of 4 (W is a multiple of 64); 16-byte aligned buffers
+    // N is a multiple of 4 (W is a multiple of 64); 16-byte aligned buffers.
+    // Steps are powers of two: step mod W is a multiple of 4 for steps >= 4
+    // (W is a multiple of 64) and is 1 or 2 otherwise.
     const unsigned blocks = (unsigned)(((long long)(d.N >> 2) + 255) / 256);
     const int sx = step % d.W;
     if (sx % 4 == 0)
         hipLaunchKernelGGL(k_jfa_pass4<false>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
     else if (sx < 4)
         hipLaunchKernelGGL(k_jfa_pass4<true>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
-    else {
-        const unsigned b1 = (unsigned)(((long long)d.N + 255) / 256);
-        hipLaunchKernelGGL(k_jfa_pass, dim3(b1), dim3(256), 0, s, d, in, out, step);
-    }
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

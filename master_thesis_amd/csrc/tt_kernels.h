@@ -11,6 +11,8 @@ hipError_t tt_scan_u32(hipStream_t s, const uint32_t *in, uint32_t *out, long lo
 // ordered compaction of the indices i < n with flags[i] != 0 (u8 flags)
 hipError_t tt_compact_u8(hipStream_t s, const uint8_t *flags, long long n, uint32_t *out_idx, uint32_t *count,
                          uint32_t *tmp);
+// the same indices in no particular order (flags 16-byte aligned); 2 launches
+hipError_t tt_compact_u8_any(hipStream_t s, const uint8_t *flags, long long n, uint32_t *out_idx, uint32_t *count);
 hipError_t tt_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, long long n);
 hipError_t tt_launch_ledger(hipStream_t s, const float *h, long long n, long long *out);
 

@@ -224,9 +224,9 @@ static int run_ccl(tt_world *w)
     // local roots; per-cell labels are completed by the relabel pass
     CK(tt_launch_ccl_local(s, d, ids, w->labels, w->rootFlag, w->rootCount));
     CK(tt_launch_ccl_merge(s, d, ids, w->labels));
-    CK(tt_compact_u8(s, w->rootFlag, (long long)N, w->locList, w->nloc, w->scanTmp));
+    CK(tt_compact_u8_any(s, w->rootFlag, (long long)N, w->locList, w->nloc));
     CK(tt_launch_ccl_flatten(s, w->locList, w->nloc, w->labels, w->rootCount, w->rootFlag));
-    CK(tt_compact_u8(s, w->rootFlag, (long long)N, w->rootList, w->nroots, w->scanTmp));
+    CK(tt_compact_u8_any(s, w->rootFlag, (long long)N, w->rootList, w->nroots));
     uint32_t nroots = 0;
     CKS(read_u32(w, w->nroots, &nroots));
     w->stats.num_labels = (int)nroots;
@@ -247,7 +247,7 @@ static int run_ccl(tt_world *w)
     }
     CK(tt_launch_frag_assign(s, d, w->fragKey, w->nroots, w->maxCount, ids, w->pd, w->newId));
     CK(tt_launch_relabel(s, d, w->labels, w->newId, ids, w->unFlag));
-    CK(tt_compact_u8(s, w->unFlag, (long long)N, w->list[0], w->listN[0], w->scanTmp));
+    CK(tt_compact_u8_any(s, w->unFlag, (long long)N, w->list[0], w->listN[0]));
     uint32_t n = 0;
     CKS(read_u32(w, w->listN[0], &n));
     w->stats.unassigned_cells = (int)n;
@@ -305,12 +305,12 @@ static int run_split(tt_world *w)
     tt_plate_data hp;
     CK(hipMemcpyAsync(&hp, w->pd + plate, sizeof(hp), hipMemcpyDeviceToHost, s));
     CK(tt_launch_split_prepare(s, d, ids, plate, w->eff, w->unFlag));
-    CK(tt_compact_u8(s, w->unFlag, (long long)N, w->list[0], w->listN[0], w->scanTmp));
+    CK(tt_compact_u8_any(s, w->unFlag, (long long)N, w->list[0], w->listN[0]));
     // tiles holding cells of the plate
     const int th = tt_split_tile_h(d);
     const long long ntiles_all = (long long)(w->W / 64) * (w->H / th);
     CK(tt_launch_split_tiles(s, d, ids, plate, w->vote));
-    CK(tt_compact_u8(s, w->vote, ntiles_all, w->list[1], w->listN[1], w->scanTmp));
+    CK(tt_compact_u8_any(s, w->vote, ntiles_all, w->list[1], w->listN[1]));
     uint32_t ntiles = 0;
     CKS(read_u32(w, w->listN[1], &ntiles));
     const uint32_t *tiles = w->list[1];
