@@ -1282,17 +1282,26 @@ __global__ void k_backtrack(tt_dev d, const float *eff, uint8_t *ids, int px, in
     int cx = px, cy = py;
     bool dead = false;
     int seed = -1;
-    // the first walk is memoryless: reaching a cell it already marked means an
-    // endless cycle (zero-height plateau); canonical outcome: dead end
-    while (eff[at(cx, cy)] > 0.0f) {
-        if (ids[at(cx, cy)] == TT_MAXP) { dead = true; break; }
-        ids[at(cx, cy)] = (uint8_t)TT_MAXP;
+    // Each step issues all of its loads at once (the cell's effort and id, the
+    // neighbours' efforts and, on the second walk, their ids), so a step costs
+    // one memory round trip; the loads are of wrapped, always valid cells.
+    // The first walk is memoryless: reaching a cell it already marked means an
+    // endless cycle (zero-height plateau); canonical outcome: dead end.
+    for (;;) {
+        const size_t c = at(cx, cy);
+        const float ec = eff[c];
+        const uint8_t ic = ids[c];
+        float en[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) en[i] = eff[at(cx + OFF[i][0], cy + OFF[i][1])];
+        if (!(ec > 0.0f)) break;
+        if (ic == TT_MAXP) { dead = true; break; }
+        ids[c] = (uint8_t)TT_MAXP;
         float best = INF;
         int nx = cx, ny = cy;
-        for (int i = 0; i < 8; ++i) {
-            const float v = eff[at(cx + OFF[i][0], cy + OFF[i][1])];
-            if (v < best) { best = v; nx = cx + OFF[i][0]; ny = cy + OFF[i][1]; }
-        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (en[i] < best) { best = en[i]; nx = cx + OFF[i][0]; ny = cy + OFF[i][1]; }
         if (best == INF) { dead = true; break; }
         cx = nx;
         cy = ny;
@@ -1307,17 +1316,29 @@ __global__ void k_backtrack(tt_dev d, const float *eff, uint8_t *ids, int px, in
         pyf = odx;
         cx = px;
         cy = py;
-        while (eff[at(cx, cy)] > 0.0f) {
-            ids[at(cx, cy)] = (uint8_t)TT_MAXP;
+        for (;;) {
+            const size_t c = at(cx, cy);
+            const float ec = eff[c];
+            float en[8];
+            uint8_t in8[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const size_t q = at(cx + OFF[i][0], cy + OFF[i][1]);
+                en[i] = eff[q];
+                in8[i] = ids[q];
+            }
+            if (!(ec > 0.0f)) break;
+            ids[c] = (uint8_t)TT_MAXP;
             float best = INF;
             int nx = cx, ny = cy;
+#pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const int qx = cx + OFF[i][0], qy = cy + OFF[i][1];
-                if (ids[at(qx, qy)] == TT_MAXP) continue;
+                if (in8[i] == TT_MAXP) continue;
                 const float tnx = (float)(qx - px), tny = (float)(qy - py);
                 const float pdot = tnx * odx + tny * ody;
                 if (pdot < 0.0f) continue;
-                const float dv = eff[at(qx, qy)];
+                const float dv = en[i];
                 const float ofx = (float)OFF[i][0], ofy = (float)OFF[i][1];
                 const float ofm = sqrtf(ofx * ofx + ofy * ofy);
                 const float tdx = ofx / ofm, tdy = ofy / ofm;
