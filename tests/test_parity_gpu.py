@@ -316,3 +316,35 @@ def test_png_export(sim, tmp_path):
     for i, c in zip(ids, map(tuple, rgb)):
         assert col.setdefault(int(i), c) == c
     assert len(set(col.values())) == len(col)
+
+
+def test_500_steps_stay_bit_exact(sim, O):
+    """The 500-step horizon of the float tolerance (1e-3 max-abs after 500
+    steps) at a size the oracle runs in seconds: a 128x128 world with the
+    512^2 workload's settings (8 plates, speeds U[0.5,2], K=50, T=20, blur
+    radius 3, CCL every 10 steps) and a plate-area target that makes plates
+    split, stepped 500 times; every field and the statistics are compared
+    every 50 steps, bit for bit."""
+    cfg, ks = configs.synthetic(128, 128, 8, (0.5, 2.0), 21, hydraulic=50, thermal=20, ccl_every=10, uplift_range=3)
+    ks.targetMaximumPlateArea = 1500
+    say("[500 steps] 128x128, 8 plates")
+    g, o = pair(sim, O, cfg, ks)
+    splits = merges = 0
+    try:
+        for block in range(10):
+            for _ in range(50):
+                g.step(1)
+                o.step(1)
+                gs = g.stats()
+                splits += gs.split_plate >= 0
+                merges += gs.merges
+            say(f"[500 steps] step {(block + 1) * 50}")
+            bad = diff_fields(g, o, ALL_FIELDS)
+            assert not bad, f"step {(block + 1) * 50}: {bad}"
+            gs, os_ = g.stats(), o.stats()
+            assert stats_tuple(gs) == stats_tuple(os_)
+            assert gs.mass_ledger == os_.mass_ledger
+    finally:
+        g.close()
+        o.close()
+    assert splits > 0
