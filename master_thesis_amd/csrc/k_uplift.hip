@@ -269,14 +269,81 @@ __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restr
     out[q] = make_int4(res[0], res[1], res[2], res[3]);
 }
 
+// The same pass for R cells of a column chain (rows y0, y0 + s, ...,
+// y0 + (R - 1) s) times 4 cells of a row per thread, when the column offset
+// is a multiple of 4 and H is a multiple of R s: the chain reads R + 2 rows
+// of taps instead of 3 R, each row's taps serving up to three of its cells.
+// Rows are grouped into bands of R s rows; chain (band b, offset r0) starts at
+// row b R s + r0, so the chains cover every row once.
+template <int R>
+__global__ __launch_bounds__(256) void k_jfa_chain(tt_dev d, const int4 *__restrict__ in, int4 *__restrict__ out,
+                                                   int step)
+{
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    const int W = d.W, H = d.H;
+    const uint32_t g4 = (uint32_t)(W >> 2);
+    if (q >= (uint32_t)(d.N >> 2) / R) return;
+    const int x = (int)(q % g4) * 4;
+    const int chain = (int)(q / g4);
+    const int sy = step % H, sx = step % W;
+    const int y0 = (chain / sy) * (R * sy) + chain % sy;
+    const int wh = (H << 16) | W;
+    const int cols[3] = {tt_wrap1(x - sx, W), x, tt_wrap1(x + sx, W)};
+    // rows y0 - s, y0, ..., y0 + R s
+    int t[R + 2][3][4];
+#pragma unroll
+    for (int j = 0; j < R + 2; ++j) {
+        const int row = tt_wrapg(y0 + (j - 1) * sy, H) * W;
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+            const int4 v = in[(row + cols[g]) >> 2];
+            t[j][g][0] = v.x;
+            t[j][g][1] = v.y;
+            t[j][g][2] = v.z;
+            t[j][g][3] = v.w;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int y = y0 + j * sy;   // < H: the chain stays inside its band
+        int res[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int me = (y << 16) | (x + k);
+            int best = t[j + 1][1][k];
+            int bd = 0x7FFFFFFF;
+            if (best >= 0) bd = tt_tdist2_pk(me, best, wh);
+#pragma unroll
+            for (int oy = 0; oy < 3; ++oy) {
+#pragma unroll
+                for (int ox = 0; ox < 3; ++ox) {
+                    if (ox == 1 && oy == 1) continue;
+                    const int cand = t[j + oy][ox][k];
+                    if (cand < 0) continue;
+                    const int dd = tt_tdist2_pk(me, cand, wh);
+                    if (dd < bd) { bd = dd; best = cand; }
+                }
+            }
+            res[k] = best;
+        }
+        out[((size_t)y * W + x) >> 2] = make_int4(res[0], res[1], res[2], res[3]);
+    }
+}
+
 hipError_t tt_launch_jfa_pass(hipStream_t s, const tt_dev &d, const int *in, int *out, int step)
 {
     // N is a multiple of 4 (W is a multiple of 64); 16-byte aligned buffers.
     // Steps are powers of two: step mod W is a multiple of 4 for steps >= 4
     // (W is a multiple of 64) and is 1 or 2 otherwise.
     const unsigned blocks = (unsigned)(((long long)(d.N >> 2) + 255) / 256);
-    const int sx = step % d.W;
-    if (sx % 4 == 0)
+    const int sx = step % d.W, sy = step % d.H;
+    if (sx % 4 == 0 && sy > 0 && d.H % (4 * sy) == 0)
+        hipLaunchKernelGGL(k_jfa_chain<4>, dim3((blocks + 3) / 4), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out,
+                           step);
+    else if (sx % 4 == 0 && sy > 0 && d.H % (2 * sy) == 0)
+        hipLaunchKernelGGL(k_jfa_chain<2>, dim3((blocks + 1) / 2), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out,
+                           step);
+    else if (sx % 4 == 0)
         hipLaunchKernelGGL(k_jfa_pass4<false>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
     else if (sx < 4)
         hipLaunchKernelGGL(k_jfa_pass4<true>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
