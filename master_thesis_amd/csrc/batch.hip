@@ -204,6 +204,7 @@ void fill_desc(tt_bworld &B, const tt_world *w, int T, int K)
     B.vel = w->vel;
     B.wn = w->wn;
     B.frozen = w->tileFrozen;
+    B.chg = w->tileChg;
 }
 
 int launch_erosion(tt_batch *b, int Tmax, int Kmax)

@@ -32,6 +32,7 @@ __global__ __launch_bounds__(256) void k_thermal_batch(const tt_bworld *__restri
     A.hout = B.h[B.hc0 ^ ((t + 1) & 1)];
     A.ch = nullptr;
     A.frozen = B.frozen;
+    A.chg = B.chg;
     A.pass = t;
     tt_thermal_tile<TT_TILE_H>(A, S, (tile % nbx) * TT_TILE_W, (tile / nbx) * TT_TILE_H);
 }

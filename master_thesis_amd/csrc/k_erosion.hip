@@ -84,11 +84,12 @@ hipError_t tt_launch_thermal_frozen(hipStream_t s, const tt_dev &d, int T, const
 }
 
 hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, int rows, const float *hin, float *hout, float *ch,
-                             const uint8_t *frozen, int pass)
+                             const uint8_t *frozen, uint8_t *chg, int pass)
 {
     tt_therm_args A = tt_therm_make(d, hin, hout);
     A.ch = ch;
     A.frozen = rows == TT_TILE_H ? frozen : nullptr;
+    A.chg = rows == TT_TILE_H ? chg : nullptr;
     A.pass = pass;
     const unsigned blocks = (unsigned)((d.W / TT_TILE_W) * (d.H / rows));
     if (ch) {

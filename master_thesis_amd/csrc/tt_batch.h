@@ -21,6 +21,7 @@ struct tt_bworld {
     float2 *vel;
     float *wn;              // water after the flow between sub-iterations
     const uint8_t *frozen;  // per 64x16 tile: unchanged by this step's thermal passes
+    uint8_t *chg;           // per 64x16 tile and pass parity: the thermal pass changed a cell
 };
 
 // tile table entries: (world index << 20) | tile index within the world

@@ -38,6 +38,7 @@ struct tt_therm_args {
     float *hout;
     float *ch;              // precision mode: Kahan compensation of the height (in place)
     const uint8_t *frozen;  // per 64x16 tile: unchanged by every pass of this step (null: unknown)
+    uint8_t *chg;           // per 64x16 tile and pass parity: the pass changed a cell (null: not tracked)
     int pass;               // pass index within the step
 };
 
@@ -69,6 +70,7 @@ __host__ __device__ inline tt_therm_args tt_therm_make(const tt_dev &d, const fl
     A.hout = hout;
     A.ch = nullptr;
     A.frozen = nullptr;
+    A.chg = nullptr;
     A.pass = 0;
     return A;
 }
@@ -80,17 +82,39 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
     const int tid = threadIdx.x;
     const int W = A.W, H = A.H;
     const int lane = tid & 63, wv = tid >> 6;
+    const int nbx = W / TT_TILE_W, nby = H / TT_TILE_H;
+    const int tile = (y0 / TT_TILE_H) * nbx + x0 / TT_TILE_W;
+    uint8_t *chgOut = (A.chg && TH == TT_TILE_H) ? A.chg + (size_t)(A.pass & 1) * (nbx * nby) + tile : nullptr;
     // A frozen tile keeps its heights through every pass of the step: the
     // first pass copies it to the other buffer, later passes find both
     // buffers equal and do nothing.
-    if (A.frozen && TH == TT_TILE_H && A.frozen[(y0 / TT_TILE_H) * (W / TT_TILE_W) + x0 / TT_TILE_W]) {
+    if (A.frozen && TH == TT_TILE_H && A.frozen[tile]) {
         if (A.pass == 0)
 #pragma unroll
             for (int r = 0; r < RPW; ++r) {
                 const int c = (y0 + wv * RPW + r) * W + x0 + lane;
                 A.hout[c] = A.hin[c];
             }
+        if (chgOut && tid == 0) *chgOut = 0;
         return;
+    }
+    // A cell's update reads the heights within two cells (and its own
+    // compensation).  If no cell of this tile or of its eight neighbouring
+    // tiles changed in the previous pass, this pass computes what the previous
+    // one did: nothing changes, and the output buffer (the previous pass's
+    // input) already holds the heights.
+    if (chgOut && A.pass > 0) {
+        const uint8_t *prev = A.chg + (size_t)((A.pass - 1) & 1) * (nbx * nby);
+        const int tx = x0 / TT_TILE_W, ty = y0 / TT_TILE_H;
+        int any = 0;
+        if (tid < 9) {
+            const int dy = tid / 3 - 1, dx = tid % 3 - 1;
+            any = prev[tt_wrap1(ty + dy, nby) * nbx + tt_wrap1(tx + dx, nbx)];
+        }
+        if (!__syncthreads_or(any)) {
+            if (tid == 0) *chgOut = 0;
+            return;
+        }
     }
     float mn = tt_u2f(0x7f800000u), mx = tt_u2f(0xff800000u);
     {
@@ -129,6 +153,7 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
                 const int ry = wv * RPW + r;
                 A.hout[(y0 + ry) * W + x0 + lane] = S.hs[ry + 2][lane + 2];
             }
+            if (chgOut && tid == 0) *chgOut = 0;
             return;
         }
     }
@@ -149,6 +174,7 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
         S.tot[ty][tx] = t;
     }
     __syncthreads();
+    int changed = 0;
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
         const int ry = wv * RPW + r;
@@ -169,6 +195,7 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
         const int cidx = (y0 + ry) * W + x0 + lane;
         float cc = 0.0f;
         if (COMP) cc = A.ch[cidx];
+        const float cc0 = cc;
         if (tc > 0.0f) {
 #pragma unroll
             for (int k = 0; k < 8; ++k)
@@ -183,8 +210,16 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
                 const float q = (amp * (dIn[k] / tIn[k])) * str;
                 v = COMP ? tt_kahan_add(v, cc, q) : v + q;
             }
-        if (COMP) A.ch[cidx] = cc;
+        if (COMP) {
+            changed |= __float_as_uint(cc) != __float_as_uint(cc0);
+            A.ch[cidx] = cc;
+        }
+        changed |= __float_as_uint(v) != __float_as_uint(hc);
         A.hout[cidx] = v;
+    }
+    if (chgOut) {
+        changed = __syncthreads_or(changed);
+        if (tid == 0) *chgOut = changed ? 1 : 0;
     }
 }
 

@@ -63,7 +63,8 @@ int tt_tile_rows_valid(int rows, int H);
 // frozen: per 64x16 tile, unchanged through all thermal passes of the step
 // (tt_launch_thermal_frozen); pass: index of the pass within the step
 hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, int rows, const float *hin, float *hout,
-                             float *ch = nullptr, const uint8_t *frozen = nullptr, int pass = 0);
+                             float *ch = nullptr, const uint8_t *frozen = nullptr, uint8_t *chg = nullptr,
+                             int pass = 0);
 hipError_t tt_launch_thermal_frozen(hipStream_t s, const tt_dev &d, int T, const float *h, float2 *mm,
                                     uint8_t *frozen);
 hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int rows, int first, int last, int rseed,

@@ -80,6 +80,7 @@ struct tt_world {
     uint8_t *act = nullptr;     // per-tile activity flags of the split rounds (2 x tiles)
     float2 *tileMM = nullptr;   // per 64x16 tile: min / max height (thermal)
     uint8_t *tileFrozen = nullptr;  // per 64x16 tile: unchanged by this step's thermal passes
+    uint8_t *tileChg = nullptr;     // per 64x16 tile and pass parity: the thermal pass changed a cell
     float *ch = nullptr;        // precision mode: height compensation
     float *cs[2] = {nullptr, nullptr};  // precision mode: sediment compensation (ping-pong like sed)
 

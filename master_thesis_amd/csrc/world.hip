@@ -164,6 +164,7 @@ size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w)
     take((void **)&w->act, 2 * (N / 1024 + 64));
     take((void **)&w->tileMM, (N / 1024 + 64) * 8);
     take((void **)&w->tileFrozen, N / 1024 + 64);
+    take((void **)&w->tileChg, 2 * (N / 1024 + 64));
     if (c.precision == TT_PRECISION_COMPENSATED) {
         take((void **)&w->ch, N * 4);
         take((void **)&w->cs[0], N * 4);
@@ -475,7 +476,7 @@ static int step_erode(tt_world *w)
     int hc = w->ehc;
     CKS(tt_step_thermal_prep(w));
     for (int it = 0; it < T; ++it) {
-        CK(tt_launch_thermal(s, d, TT_TILE_H, w->h[hc], w->h[1 - hc], w->ch, w->tileFrozen, it));
+        CK(tt_launch_thermal(s, d, TT_TILE_H, w->h[hc], w->h[1 - hc], w->ch, w->tileFrozen, w->tileChg, it));
         hc = 1 - hc;
     }
     CKS(record(w, 3));
