@@ -136,22 +136,47 @@ __global__ __launch_bounds__(256) void k_advect(tt_dev d, const uint8_t *ids, co
     __shared__ uint8_t tile[ADV_TH * ADV_TW];
     __shared__ int pox[256], poy[256];
     __shared__ int dox[256], doy[256];
+    __shared__ uint8_t present[256], useful[256];
+    __shared__ int nuse;
     const int tid = threadIdx.x;
-    const int nd = ctl->n_distinct;
+    int nd = ctl->n_distinct;
     const int R = ctl->max_r;
     pox[tid] = ps[tid].ox;
     poy[tid] = ps[tid].oy;
+    present[tid] = 0;
+    useful[tid] = 0;
     if (tid < nd) { dox[tid] = ctl->dox[tid]; doy[tid] = ctl->doy[tid]; }
     const int x0 = blockIdx.x * TT_TILE_W, y0 = blockIdx.y * TT_TILE_H;
     const bool lds = R <= ADV_R;
+    __syncthreads();
     if (lds) {
         for (int i = tid; i < ADV_TH * ADV_TW; i += 256) {
             const int ly = i / ADV_TW, lx = i - ly * ADV_TW;
             const int gx = tt_wrapg(x0 + lx - ADV_R, d.W), gy = tt_wrapg(y0 + ly - ADV_R, d.H);
-            tile[i] = ids[(size_t)gy * d.W + gx];
+            const uint8_t p = ids[(size_t)gy * d.W + gx];
+            tile[i] = p;
+            present[p] = 1;
         }
     }
     __syncthreads();
+    if (lds) {
+        // A plate arrives at a cell of the tile only from a source cell at
+        // its offset, which lies in the tile plus halo: only the offsets of
+        // the plates present there need to be tried.
+        if (present[tid]) {
+            for (int j = 0; j < nd; ++j)
+                if (dox[j] == pox[tid] && doy[j] == poy[tid]) { useful[j] = 1; break; }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int m = 0;
+            for (int j = 0; j < nd; ++j)
+                if (useful[j]) { dox[m] = dox[j]; doy[m] = doy[j]; ++m; }
+            nuse = m;
+        }
+        __syncthreads();
+        nd = nuse;
+    }
     const int lane = tid & 63, wv = tid >> 6;
     const int segPerRow = d.W / 64;
     int overflow = 0;

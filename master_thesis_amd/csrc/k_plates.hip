@@ -163,29 +163,25 @@ __global__ __launch_bounds__(256) void k_ccl_local(tt_dev d, const uint8_t *ids,
     __shared__ unsigned cnt[TT_TILE_H * TT_TILE_W];
     __shared__ uint8_t tid8[(TT_TILE_H + 1) * (TT_TILE_W + 2)];
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wv = tid >> 6;
     const int x0 = blockIdx.x * TT_TILE_W, y0 = blockIdx.y * TT_TILE_H;
     const int W = d.W;
-    // fast path: one plate id over the whole tile
-    uint8_t own[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) own[k] = ids[(size_t)(y0 + wv * 4 + k) * W + x0 + lane];
-    __shared__ uint8_t first;
-    if (tid == 0) first = own[0];
-    __syncthreads();
-    int same = 1;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) same &= own[k] == first;
-    if (__syncthreads_and(same)) {
-        const uint32_t root = (uint32_t)((size_t)y0 * W + x0);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const size_t c = (size_t)(y0 + wv * 4 + k) * W + x0 + lane;
-            labels[c] = root;
-            rootFlag[c] = (uint8_t)(c == root ? 1 : 0);
+    // fast path: one plate id over the whole tile.  Each thread takes 4
+    // consecutive cells of a row (one 4-byte load, one 16-byte label store and
+    // one 4-byte flag store).
+    {
+        const int fr = tid >> 4, fc = (tid & 15) * 4;
+        const size_t c4 = (size_t)(y0 + fr) * W + x0 + fc;
+        const uint32_t own = *reinterpret_cast<const uint32_t *>(ids + c4);
+        __shared__ uint32_t first;
+        if (tid == 0) first = (own & 0xFFu) * 0x01010101u;
+        __syncthreads();
+        if (__syncthreads_and(own == first)) {
+            const uint32_t root = (uint32_t)((size_t)y0 * W + x0);
+            *reinterpret_cast<uint4 *>(labels + c4) = make_uint4(root, root, root, root);
+            *reinterpret_cast<uint32_t *>(rootFlag + c4) = tid == 0 ? 1u : 0u;
+            if (tid == 0) rootCount[root] = TT_TILE_W * TT_TILE_H;
+            return;
         }
-        if (tid == 0) rootCount[root] = TT_TILE_W * TT_TILE_H;
-        return;
     }
     const int TW2 = TT_TILE_W + 2;
     for (int i = tid; i < (TT_TILE_H + 1) * TW2; i += 256) {
