@@ -347,7 +347,7 @@ __device__ __forceinline__ unsigned tt_det_type(const tt_dev &d, const uint8_t *
     return out;
 }
 
-__global__ __launch_bounds__(256) void k_resolve(tt_dev d, const uint32_t *word, const uint8_t *ids,
+__global__ __launch_bounds__(256) void k_resolve(tt_dev d, const uint32_t *iterp, const uint32_t *word, const uint8_t *ids,
                                                  const float *height, const tt_plate_step *ps,
                                                  const uint32_t *divBitmap, const uint8_t *typeBitmap, float *hW,
                                                  uint8_t *idW, tt_uplift *uplift, uint32_t *accw, uint32_t *hasDiv,
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void k_resolve(tt_dev d, const uint32_t *word,
                 nid = (divBitmap[bit >> 5] & (1u << (bit & 31u))) ? mn : mx;
                 atomicOr(&hasDiv[bit >> 5], 1u << (bit & 31u));
             }
-            const float nz = tt_simplex((float)x, (float)y, (float)d.iter, 0.01f, d.seed);
+            const float nz = tt_simplex((float)x, (float)y, (float)*iterp, 0.01f, d.seed);
             const float target = d.ks.divergence_height_target_min +
                                  nz * (d.ks.divergence_height_target_max - d.ks.divergence_height_target_min);
             hW[c] = tt_lerp(height[c], target, d.ks.divergence_interpolation_factor);
@@ -528,13 +528,13 @@ __global__ __launch_bounds__(256) void k_resolve(tt_dev d, const uint32_t *word,
     if (sf[tid] != 0ull) atomicAdd((unsigned long long *)&fric[tid], sf[tid]);
 }
 
-hipError_t tt_launch_resolve(hipStream_t s, const tt_dev &d, const uint32_t *word, const uint8_t *ids,
+hipError_t tt_launch_resolve(hipStream_t s, const tt_dev &d, const uint32_t *iterp, const uint32_t *word, const uint8_t *ids,
                              const float *height, const tt_plate_step *ps, const uint32_t *divBitmap,
                              const uint8_t *typeBitmap, float *hW, uint8_t *idW, tt_uplift *uplift, uint32_t *accw,
                              uint32_t *hasDiv, long long *fric, uint8_t *srcFlag, uint8_t *contested, int do_uplift)
 {
     const unsigned blocks = (unsigned)(((long long)d.N + 1023) / 1024);
-    hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, s, d, word, ids, height, ps, divBitmap, typeBitmap, hW,
+    hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, s, d, iterp, word, ids, height, ps, divBitmap, typeBitmap, hW,
                        idW, uplift, accw, hasDiv, fric, srcFlag, contested, do_uplift);
     return hipGetLastError();
 }

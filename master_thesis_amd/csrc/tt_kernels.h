@@ -35,7 +35,7 @@ hipError_t tt_launch_keys(hipStream_t s, const tt_dev &d, const uint32_t *word, 
                           const uint32_t *segBase, const uint8_t *ids, const tt_plate_step *ps, const tt_step_ctl *ctl,
                           unsigned long long *keys);
 hipError_t tt_launch_type_matrix(hipStream_t s, const tt_dev &d, const int *typeCounts, uint8_t *typeBitmap);
-hipError_t tt_launch_resolve(hipStream_t s, const tt_dev &d, const uint32_t *word, const uint8_t *ids,
+hipError_t tt_launch_resolve(hipStream_t s, const tt_dev &d, const uint32_t *iterp, const uint32_t *word, const uint8_t *ids,
                              const float *height, const tt_plate_step *ps, const uint32_t *divBitmap,
                              const uint8_t *typeBitmap, float *hW, uint8_t *idW, tt_uplift *uplift, uint32_t *accw,
                              uint32_t *hasDiv, long long *fric, uint8_t *srcFlag, uint8_t *contested, int do_uplift);

@@ -89,9 +89,17 @@ struct tt_world {
     uint8_t *initSeeds = nullptr;
     double *initWts = nullptr;
 
+    uint32_t *dIter = nullptr;  // the step's iteration on the device (read by collision resolution)
+
     // host mirrors
     tt_step_ctl *hctl = nullptr;
     uint32_t *hu32 = nullptr;
+    uint32_t *hIter = nullptr;  // pinned source of dIter
+
+    // the pre-erosion launches captured as graphs, one per (height, id)
+    // buffer parity; replayed while the event timing is off
+    hipGraphExec_t preExec[4] = {};
+    bool preGraphs = true;
     tt_step_stats stats{};
     bool timing = false;
     hipEvent_t ev[8] = {};

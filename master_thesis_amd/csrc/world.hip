@@ -165,6 +165,7 @@ size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w)
     take((void **)&w->tileMM, (N / 1024 + 64) * 8);
     take((void **)&w->tileFrozen, N / 1024 + 64);
     take((void **)&w->tileChg, 2 * (N / 1024 + 64));
+    take((void **)&w->dIter, 4);
     if (c.precision == TT_PRECISION_COMPENSATED) {
         take((void **)&w->ch, N * 4);
         take((void **)&w->cs[0], N * 4);
@@ -353,21 +354,15 @@ static int record(tt_world *w, int slot)
 // after erosion (copy-back, kinematics, accretion, CCL, merge, split,
 // ledger).  A batch of worlds runs the first and last phases per world and the
 // erosion of all its worlds together (batch.hip).
-int tt_step_pre(tt_world *w)
+// the launches of the pre-erosion phase (collisions, propagation, uplift);
+// they depend on the world only through its buffer parities, so a capture
+// per parity replays them
+static int pre_launch(tt_world *w)
 {
     hipStream_t s = w->stream;
     const tt_dev d = make_dev(w);
     const int mask = w->cfg.stage_mask;
     const size_t N = (size_t)w->N;
-    w->stats.overflow_cells = 0;
-    w->stats.complex_sources = 0;
-    w->stats.num_labels = 0;
-    w->stats.unassigned_cells = 0;
-    w->stats.fill_rounds = 0;
-    w->stats.merges = 0;
-    w->stats.split_plate = -1;
-    w->stats.split_new_plate = -1;
-    w->stats.split_iterations = 0;
     CKS(record(w, 0));
     float *hOld = w->h[w->hc], *hNew = w->h[1 - w->hc];
     uint8_t *idOld = w->ids[w->ic], *idNew = w->ids[1 - w->ic];
@@ -383,8 +378,8 @@ int tt_step_pre(tt_world *w)
         CK(tt_scan_u32(s, w->segCount, w->segBase, (long long)(N / 64), w->scanTmp, nullptr));
         CK(tt_launch_keys(s, d, w->word, w->cnt, w->segBase, idOld, w->ps, w->ctl, w->keys));
         CK(tt_launch_type_matrix(s, d, w->typeCounts, w->typeBitmap));
-        CK(tt_launch_resolve(s, d, w->word, idOld, hOld, w->ps, w->divBitmap, w->typeBitmap, hNew, idNew, w->uplift,
-                             w->accw, w->hasDiv, w->fric, w->srcFlag, w->contested, upl ? 1 : 0));
+        CK(tt_launch_resolve(s, d, w->dIter, w->word, idOld, hOld, w->ps, w->divBitmap, w->typeBitmap, hNew, idNew,
+                             w->uplift, w->accw, w->hasDiv, w->fric, w->srcFlag, w->contested, upl ? 1 : 0));
         CK(tt_launch_flip(s, w->divBitmap, w->hasDiv));
         CKS(record(w, 1));
         if (upl) {
@@ -419,10 +414,63 @@ int tt_step_pre(tt_world *w)
         CKS(record(w, 1));
     }
     CKS(record(w, 2));
-    w->ehc = 1 - w->hc;   // erosion works on the written height field
     // the height compensation belongs to a cell's height: it restarts when the
     // collision stage has moved heights between cells
     if (w->ch && (mask & TT_STAGE_COLLIDE)) CKS(zero(w, w->ch, N * 4));
+    return TT_OK;
+}
+
+static void drop_pre_graphs(tt_world *w)
+{
+    for (auto &g : w->preExec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
+static int capture_pre(tt_world *w, hipGraphExec_t *out)
+{
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return TT_ERR_DEVICE;
+    const int st = pre_launch(w);
+    const hipError_t e = hipStreamEndCapture(w->stream, &g);
+    int res = (st == TT_OK && e == hipSuccess && g) ? TT_OK : TT_ERR_DEVICE;
+    if (res == TT_OK && hipGraphInstantiate(out, g, nullptr, nullptr, 0) != hipSuccess) res = TT_ERR_DEVICE;
+    if (g) (void)hipGraphDestroy(g);
+    if (res != TT_OK) {
+        (void)hipGetLastError();
+        *out = nullptr;
+    }
+    return res;
+}
+
+int tt_step_pre(tt_world *w)
+{
+    w->stats.overflow_cells = 0;
+    w->stats.complex_sources = 0;
+    w->stats.num_labels = 0;
+    w->stats.unassigned_cells = 0;
+    w->stats.fill_rounds = 0;
+    w->stats.merges = 0;
+    w->stats.split_plate = -1;
+    w->stats.split_new_plate = -1;
+    w->stats.split_iterations = 0;
+    // the step's iteration for the device (the collision resolution's
+    // divergence noise reads it there, so captured launches stay valid)
+    *w->hIter = w->iter;
+    CK(hipMemcpyAsync(w->dIter, w->hIter, 4, hipMemcpyHostToDevice, w->stream));
+    if (w->preGraphs && !w->timing) {
+        hipGraphExec_t &g = w->preExec[w->hc * 2 + w->ic];
+        if (!g && capture_pre(w, &g) != TT_OK) w->preGraphs = false;   // launches from now on
+        if (g) {
+            CK(hipGraphLaunch(g, w->stream));
+        } else {
+            CKS(pre_launch(w));
+        }
+    } else {
+        CKS(pre_launch(w));
+    }
+    w->ehc = 1 - w->hc;   // erosion works on the written height field
     return TT_OK;
 }
 
@@ -658,6 +706,7 @@ static int init_counts(tt_world *w)
 int tt_world_do_init(tt_world *w)
 {
     hipStream_t s = w->stream;
+    drop_pre_graphs(w);   // a reset may change the seed, an argument of the captured launches
     w->iter = 0;
     w->hc = w->ic = w->wc = w->fc = w->sc = 0;
     CKS(init_common_zero(w));
@@ -1005,7 +1054,8 @@ int tt_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, tt_w
     for (auto &a : w->allocs) { *a.ptr = base + off; off += a.bytes; }
     if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void **)&w->hctl, sizeof(tt_step_ctl), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&w->hu32, TT_ROUND_BATCH * 4, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&w->hu32, TT_ROUND_BATCH * 4, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&w->hIter, 64, hipHostMallocDefault) != hipSuccess) {
         tt_world_destroy(w);
         set_error("stream / pinned memory allocation failed");
         return TT_ERR_DEVICE;
@@ -1062,6 +1112,7 @@ int tt_world_set_settings(tt_world *w, const tt_kernel_settings *ks)
 {
     if (!w || !ks) return TT_ERR_INVALID;
     w->ks = *ks;
+    drop_pre_graphs(w);   // the settings are arguments of the captured launches
     return TT_OK;
 }
 
@@ -1252,8 +1303,10 @@ int tt_world_destroy(tt_world *w)
     for (int i = 0; i < 8; ++i)
         if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
     if (w->stream) (void)hipStreamDestroy(w->stream);
+    drop_pre_graphs(w);
     if (w->hctl) (void)hipHostFree(w->hctl);
     if (w->hu32) (void)hipHostFree(w->hu32);
+    if (w->hIter) (void)hipHostFree(w->hIter);
     if (w->base) (void)hipFree(w->base);
     delete w;
     return TT_OK;
