@@ -10,7 +10,17 @@ __global__ __launch_bounds__(256) void k_thermal(tt_therm_args A)
 {
     __shared__ tt_therm_lds<TH> S;
     const int nbx = A.W / TT_TILE_W, nby = A.H / TH;
-    const int b = tt_xcd_tile(blockIdx.x, nbx * nby);
+    // bands of 4 tile rows dealt round-robin to the XCDs: neighbouring tiles
+    // share an L2, and the rough parts of the terrain (the tiles that do not
+    // return early) spread over all XCDs
+    const int nb = nbx * nby, band = 4 * nbx, nbands = nby / 4;
+    int b = blockIdx.x;
+    if ((nby & 3) == 0 && (nbands & 7) == 0) {
+        const int x = b & 7, j = b >> 3;
+        b = (x + 8 * (j / band)) * band + j % band;
+    } else {
+        b = tt_xcd_tile(blockIdx.x, nb);
+    }
     tt_thermal_tile<TH, COMP>(A, S, (b % nbx) * TT_TILE_W, (b / nbx) * TH);
 }
 
