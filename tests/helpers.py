@@ -50,3 +50,17 @@ def ulp_diff(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     ai = np.where(ai < 0, np.int64(-2**31) - ai, ai)
     bi = np.where(bi < 0, np.int64(-2**31) - bi, bi)
     return np.abs(ai - bi)
+
+
+def banded_continental_heights(rng, a, b, H=64, W=64):
+    """Continental crust at or above the propagation threshold (13000) on
+    columns 31-a[y]..31 of plate 0 and 32..32+b[y] of plate 1, with one cell of
+    each band exactly at the threshold; continental crust just below it
+    (12990..12999) elsewhere."""
+    h = rng.uniform(12990, 12999, (H, W)).astype(np.float32)
+    for y in range(H):
+        h[y, 31 - a[y]:32] = rng.uniform(13500, 14500, a[y] + 1)
+        h[y, 32:33 + b[y]] = rng.uniform(13500, 14500, b[y] + 1)
+        h[y, 31 - rng.integers(0, a[y] + 1)] = 13000.0
+        h[y, 32 + rng.integers(0, b[y] + 1)] = 13000.0
+    return h

@@ -10,6 +10,7 @@ import pytest
 
 from master_thesis_amd import _abi, configs
 from tests.golden import noise_restatement as nr
+from tests.helpers import banded_continental_heights
 
 F32 = np.float32
 ALL = _abi.TT_STAGE_ALL
@@ -712,3 +713,246 @@ def test_subduction_uplift_profiles_closed_form(O):
         xx = d - 20 - 10
         m = 1.0 + np.exp(-0.02 * xx * xx) * 0.005
         np.testing.assert_allclose(h2[:, x], np.maximum(h1[:, x] * m, lo), rtol=2e-6)
+
+
+# --------------------------------------------------------------------------
+# continental collisions: propagation (C1/C2), the continental branch of
+# applyUplift (C4), and the friction-loss input of the plate kinematics
+# --------------------------------------------------------------------------
+def _continental_front(O, h, mask, **ks_over):
+    """Plate 0 (columns 0..31) moves right into plate 1 (columns 32..63),
+    speed 1, pixel centres (0.5, 0.5); one step of the stages in mask."""
+    W, H = 64, 64
+    ids = np.where(np.arange(W)[None, :] < 32, 0, 1).repeat(H, 0).astype(np.uint8)
+    w = empty_world(O, W, H, mask, **ks_over)
+    w.set(_abi.FIELD_HEIGHT, h.ravel())
+    w.set(_abi.FIELD_PLATE_IDS, ids.ravel())
+    set_plates(w, p0=dict(direction=(1.0, 0.0), velocity=1.0, mass=1e7, pixelCenter=(0.5, 0.5)),
+               p1=dict(direction=(-1.0, 0.0), velocity=1.0, mass=1e7, pixelCenter=(0.5, 0.5)))
+    w.set_iteration(5)
+    w.step(1)
+    return w
+
+
+def test_continental_propagation_walks_back_to_the_threshold(O):
+    """continentalPropagation (plate_tectonics_kernel.cu:975-1027) walks a
+    continental source cell back along each colliding plate's direction, cell
+    by cell (a DDA on the pixel grid), until the pre-step height falls below
+    continentalPropThreshold, and returns that cell; after
+    continentalPropMaxIters steps it returns the last cell walked.  Head on at
+    speed 1 the two collision columns 31 and 32 are the sources.  Plate 0's walk
+    goes left, plate 1's right; both sources of a row reach the same two
+    targets 30-a[y] and 33+b[y].  moveContinentalCollisions (:1087-1185) gives
+    each target half of each source's cumulative height (two of the four
+    counts) and the pair of plates; combinePropagatedCollisions (:1187-1218)
+    marks the targets continental and the sources consumed (inv 1)."""
+    rng = np.random.default_rng(11)
+    a = rng.integers(0, 12, 64)
+    b = rng.integers(0, 12, 64)
+    h = banded_continental_heights(rng, a, b)
+    w = _continental_front(O, h, _abi.TT_STAGE_COLLIDE | _abi.TT_STAGE_UPLIFT)
+    up = w.get(_abi.FIELD_UPLIFT).reshape(64, 64)
+    pr = w.get(_abi.FIELD_PROPAGATED).reshape(64, 64)
+    st = w.stats()
+    w.close()
+    # the sources: continental collisions (types bit 0), cum = the lower height
+    assert (up["types"][:, [31, 32]] & 1 == 1).all()
+    assert np.array_equal(up["cum"][:, 31], np.minimum(h[:, 30], h[:, 32]))
+    assert np.array_equal(up["cum"][:, 32], np.minimum(h[:, 31], h[:, 33]))
+    assert st.complex_sources == 128
+    for y in range(64):
+        t0, t1 = 30 - a[y], 33 + b[y]
+        half = F32(F32(up["cum"][y, 31] / F32(2)) + F32(up["cum"][y, 32] / F32(2)))
+        for t in (t0, t1):
+            assert pr["cum"][y, t] == half
+            assert pr["types"][y, t] == 1
+            assert sorted([pr["inv"][y, t] & 0xFF, (pr["inv"][y, t] >> 8) & 0xFF]) == [0xFE, 0xFF]
+            assert pr["inv"][y, t] >> 16 == 0
+        assert pr["inv"][y, 31] == 1 and pr["inv"][y, 32] == 1
+        seeds = np.nonzero(pr["inv"][y] >> 8)[0].tolist()
+        assert seeds == sorted({t0, t1})
+
+
+def test_continental_propagation_stops_after_max_iterations(O):
+    """With continentalPropMaxIters = 4 and continental crust further back, the
+    walk ends after four cells: source 31 reaches 27 (plate 0) and 35 (plate 1),
+    source 32 reaches 28 and 36, each target taking half of its source's
+    cumulative height."""
+    rng = np.random.default_rng(12)
+    h = banded_continental_heights(rng, np.full(64, 10), np.full(64, 10))
+    w = _continental_front(O, h, _abi.TT_STAGE_COLLIDE | _abi.TT_STAGE_UPLIFT, continentalPropMaxIters=4)
+    up = w.get(_abi.FIELD_UPLIFT).reshape(64, 64)
+    pr = w.get(_abi.FIELD_PROPAGATED).reshape(64, 64)
+    w.close()
+    for y in range(64):
+        assert np.nonzero(pr["inv"][y] >> 8)[0].tolist() == [27, 28, 35, 36]
+        for src, ts in ((31, (27, 35)), (32, (28, 36))):
+            for t in ts:
+                assert pr["cum"][y, t] == up["cum"][y, src] / F32(2)
+
+
+def test_continental_uplift_closed_form(O):
+    """applyUplift's continental branch (plate_tectonics_kernel.cu:1662-1738,
+    formula :1646-1650): every cell whose nearest jump-flood seed is a
+    continental target gains cont(d) * cum * continentalUpliftMultiplier and is
+    clamped above at maxUpliftClamp, whatever its plate.  With straight target
+    columns 21 and 42 the distance d is the toroidal column distance, and for
+    integer d cont(d) = exp(-0.02 d^2) (the ridge term vanishes)."""
+    rng = np.random.default_rng(13)
+    h = banded_continental_heights(rng, np.full(64, 9), np.full(64, 9))
+    clamp = 14050.0
+    out = {}
+    for tag, mask in (("collide", _abi.TT_STAGE_COLLIDE), ("uplift", _abi.TT_STAGE_COLLIDE | _abi.TT_STAGE_UPLIFT)):
+        w = _continental_front(O, h, mask, continentalUpliftMultiplier=0.01, maxUpliftClamp=clamp)
+        out[tag] = (w.get(_abi.FIELD_HEIGHT).reshape(64, 64).astype(np.float64),
+                    w.get(_abi.FIELD_DISTANCE)["dist"].reshape(64, 64),
+                    w.get(_abi.FIELD_PROPAGATED).reshape(64, 64))
+        w.close()
+    h1 = out["collide"][0]
+    h2, dist, pr = out["uplift"]
+    x = np.arange(64)
+    dx = np.minimum.reduce([np.abs(x - 21), 64 - np.abs(x - 21), np.abs(x - 42), 64 - np.abs(x - 42)])
+    assert (dist == dx[None, :]).all()
+    near = np.where(np.minimum(np.abs(x - 21), 64 - np.abs(x - 21)) < np.minimum(np.abs(x - 42), 64 - np.abs(x - 42)), 21, 42)
+    cum = pr["cum"][np.arange(64)[:, None], near[None, :]].astype(np.float64)
+    add = np.exp(-0.02 * dx[None, :] ** 2.0) * cum * 0.01
+    exp = np.minimum(h1 + add, clamp)
+    assert (exp == clamp).any() and (exp < clamp).any()
+    err = np.abs(h2 - exp)
+    assert (err <= 2e-3 + 1e-6 * exp).all(), err.max()
+    # the gain is visible: up to ~140 at the seeds, under 0.1 at 21 columns
+    assert (h2 - h1)[h2 < clamp].max() > 50.0
+
+
+@pytest.mark.parametrize("case", ["continental", "mixed"])
+def test_friction_loss_slows_the_colliding_plates(O, case):
+    """applyInelasticCollision -> applyElasticVelocityChange
+    (plate_tectonics_kernel.cu:397-467) adds, for every collision cell and
+    every plate in it, velocity * coefficient * (source height / plate mass) to
+    the plate's friction; applyPlateMovementChanges (:1863-1925) takes the
+    friction off the speed (clamped at 0) before the drag (0 here).  The
+    coefficient is the continental one when the pair's collision is
+    continental, the accretion one when it is a subduction whose lower plate is
+    continental at the cell (types 3/4), else the subduction one."""
+    rng = np.random.default_rng(14)
+    kc, ks_, ka = 0.7, 0.3, 0.45
+    h = rng.uniform(12000, 13000, (64, 64)).astype(F32)
+    acc_rows = np.zeros(64, bool)
+    if case == "mixed":
+        # plate 0 oceanic, except on every fourth row: a subduction of plate 0
+        # (0x0D) with accretion rows where its cells are continental
+        acc_rows = np.arange(64) % 4 == 0
+        h[:, :32] = np.where(acc_rows[:, None], rng.uniform(10500, 11500, (64, 32)),
+                             rng.uniform(1000, 5000, (64, 32))).astype(F32)
+    w = _continental_front(O, h, _abi.TT_STAGE_COLLIDE | _abi.TT_STAGE_KINEMATICS,
+                           frictionCoefficientContinental=kc, frictionCoefficientSubduction=ks_,
+                           frictionCoefficientAccretion=ka, environmentalDragCoefficient=0.0)
+    pd = w.get(_abi.FIELD_PLATE_DATA)
+    tb = int(w.get(_abi.FIELD_COLLISION_TYPES)[0])
+    w.close()
+    coef = np.full(64, kc) if case == "continental" else np.where(acc_rows, ka, ks_)
+    assert tb == (0x03 if case == "continental" else 0x0D)
+    hd = h.astype(np.float64)
+    # collision cells 31 and 32: plate 0 from columns 30, 31, plate 1 from 32, 33
+    loss0 = (coef * (hd[:, 30] + hd[:, 31])).sum() / 1e7
+    loss1 = (coef * (hd[:, 32] + hd[:, 33])).sum() / 1e7
+    assert 0.01 < loss0 < 0.95 and 0.01 < loss1 < 0.95
+    assert pd[0]["velocity"] == pytest.approx(1.0 - loss0, rel=2e-6)
+    assert pd[1]["velocity"] == pytest.approx(1.0 - loss1, rel=2e-6)
+    assert tuple(pd[0]["direction"]) == (1.0, 0.0) and tuple(pd[1]["direction"]) == (-1.0, 0.0)
+
+
+def test_reference_init_nested_voronoi(O):
+    """Reference-mode initialisation: generatePlateCenters and
+    generateVoronoiSeeds (plate_tectonic_sim.cu:873-928) draw positions as
+    (height-scaled, width-scaled) pairs; each seed of a generation takes the id
+    of its nearest seed of the previous generation (toroidal, first on ties),
+    and initPlateIDs (plate_tectonics_kernel.cu:31-66) gives each cell the id
+    of its nearest seed of the last generation.  The draws come from SplitMix64
+    in the order: centres, generations, then per plate speed and direction.
+    Checked by brute force; the connected-component relabel at the end of the
+    initialisation may give split-off fragments new ids, so the ids are
+    compared on the cells that keep an original id."""
+    W, H, P, seed = 64, 48, 5, 3
+    gens = (12, 40)
+    cfg, ks = configs.reference_defaults()
+    cfg.width, cfg.height, cfg.num_plates, cfg.seed = W, H, P, seed
+    cfg.init_mode = _abi.TT_INIT_REFERENCE
+    cfg.num_voronoi_gens = len(gens)
+    for i, n in enumerate(gens):
+        cfg.voronoi_gens[i] = n
+    w = O.OracleWorld(cfg, ks)
+    w.init()
+    got = w.get(_abi.FIELD_PLATE_IDS).reshape(H, W)
+    pd = w.get(_abi.FIELD_PLATE_DATA)
+    w.close()
+
+    rng = SplitMix64(seed)
+
+    def draw():
+        return F32(rng.u01() * F32(H)), F32(rng.u01() * F32(W))
+
+    def nearest(px, py, sites):
+        best, out = np.inf, 0
+        for sx, sy, sid in sites:
+            dx, dy = abs(F32(px) - sx), abs(F32(py) - sy)
+            wx, wy = min(dx, F32(W) - dx), min(dy, F32(H) - dy)
+            d2 = F32(F32(wx * wx) + F32(wy * wy))
+            if d2 < best:
+                best, out = d2, sid
+        return out
+
+    centres = [(*draw(), i) for i in range(P)]
+    sites = centres
+    for n in gens:
+        new = []
+        for _ in range(n):
+            sx, sy = draw()
+            new.append((sx, sy, nearest(sx, sy, sites)))
+        sites = new
+    exp = np.array([[nearest(F32(x), F32(y), sites) for x in range(W)] for y in range(H)], np.uint8)
+    keep = got < P
+    assert keep.mean() > 0.9
+    assert np.array_equal(got[keep], exp[keep])
+    assert len(set(s[2] for s in sites)) > 1
+    for i, (cx, cy, _) in enumerate(centres):
+        assert pd[i]["pixelCenter"][0] == cx - np.floor(cx) and pd[i]["pixelCenter"][1] == cy - np.floor(cy)
+        v = F32(F32(F32(F32(rng.u01() * F32(2)) - F32(1)) + F32(1)) / F32(2))
+        xd, yd = F32(F32(rng.u01() * F32(2)) - F32(1)), F32(F32(rng.u01() * F32(2)) - F32(1))
+        m = F32(np.sqrt(F32(F32(xd * xd) + F32(yd * yd))))
+        assert pd[i]["velocity"] == v
+        assert pd[i]["direction"][0] == F32(xd / m) and pd[i]["direction"][1] == F32(yd / m)
+
+
+def test_reference_init_heights_are_simplex_fbm(O):
+    """initHeightmap (plate_tectonics_kernel.cu:68-99): four octaves of
+    simplex noise, seeds seed+o, frequencies 0.01 * 10^o and amplitudes
+    0.9 * 0.1^o; each octave's value v is damped by 1 + |grad v| (forward
+    differences with step 0.001), remapped from [-1, 1] to [0, 1], and the sum
+    maps to 5000 + 10000 * value.  Composed here from the independent numpy
+    simplex (itself pinned by the golden vectors)."""
+    W, H, P, seed = 24, 16, 3, 7
+    cfg, ks = configs.reference_defaults()
+    cfg.width, cfg.height, cfg.num_plates, cfg.seed = W, H, P, seed
+    cfg.init_mode = _abi.TT_INIT_REFERENCE
+    cfg.num_voronoi_gens = 0
+    w = O.OracleWorld(cfg, ks)
+    w.init()
+    got = w.get(_abi.FIELD_HEIGHT).reshape(H, W)
+    w.close()
+    d = F32(0.001)
+    exp = np.zeros((H, W), np.float32)
+    for y in range(H):
+        for x in range(W):
+            value, amp, freq = F32(0), F32(0.9), F32(0.01)
+            for o in range(4):
+                v = nr.simplex(F32(x), F32(y), F32(0), freq, seed + o)
+                vx = nr.simplex(F32(F32(x) + d), F32(y), F32(0), freq, seed + o)
+                vy = nr.simplex(F32(x), F32(F32(y) + d), F32(0), freq, seed + o)
+                gx, gy = F32(F32(vx - v) / d), F32(F32(vy - v) / d)
+                q = F32(v / F32(F32(1) + F32(np.sqrt(F32(F32(gx * gx) + F32(gy * gy))))))
+                value = F32(value + F32(F32(F32(q + F32(1)) / F32(2)) * amp))
+                amp, freq = F32(amp * F32(0.1)), F32(freq * F32(10))
+            exp[y, x] = F32(F32(5000) + F32(value * F32(10000)))
+    np.testing.assert_allclose(got, exp, rtol=1e-6)
+    assert (got > 5000).all() and (got < 15000).all()

@@ -16,7 +16,8 @@ import numpy as np
 import pytest
 
 from master_thesis_amd import _abi, configs
-from tests.helpers import STATE_FIELDS, STEP_OUTPUTS, diff_fields, get, put, transfer_state
+from tests.helpers import (STATE_FIELDS, STEP_OUTPUTS, banded_continental_heights, diff_fields, get, put,
+                           transfer_state)
 
 pytestmark = pytest.mark.gpu
 ALL_FIELDS = tuple(STATE_FIELDS) + tuple(STEP_OUTPUTS)
@@ -201,6 +202,70 @@ def test_empty_world(sim, O):
     cfg, ks = configs.synthetic(64, 64, 1, (0.5, 1.0), 1, hydraulic=2, thermal=2)
     cfg.init_mode = _abi.TT_INIT_EMPTY
     run_parity(sim, O, cfg, ks, 2, "empty")
+
+
+def _front_case(case):
+    """The continental-front worlds of the oracle's closed-form pins: plate 0
+    (columns 0..31) moves right into plate 1, speed 1."""
+    rng = np.random.default_rng(21)
+    ks_over = {}
+    up = _abi.TT_STAGE_COLLIDE | _abi.TT_STAGE_UPLIFT
+    if case == "propagation":
+        h, mask = banded_continental_heights(rng, rng.integers(0, 12, 64), rng.integers(0, 12, 64)), up
+    elif case == "max_iters":
+        h, mask = banded_continental_heights(rng, np.full(64, 10), np.full(64, 10)), up
+        ks_over = dict(continentalPropMaxIters=4)
+    elif case == "uplift_clamp":
+        h, mask = banded_continental_heights(rng, np.full(64, 9), np.full(64, 9)), up
+        ks_over = dict(continentalUpliftMultiplier=0.01, maxUpliftClamp=14050.0)
+    elif case == "friction":
+        h = rng.uniform(12000, 13000, (64, 64)).astype(np.float32)
+        rows = np.arange(64) % 4 == 0
+        h[:, :32] = np.where(rows[:, None], rng.uniform(10500, 11500, (64, 32)),
+                             rng.uniform(1000, 5000, (64, 32))).astype(np.float32)
+        mask = _abi.TT_STAGE_COLLIDE | _abi.TT_STAGE_KINEMATICS
+        ks_over = dict(frictionCoefficientContinental=0.7, frictionCoefficientSubduction=0.3,
+                       frictionCoefficientAccretion=0.45, environmentalDragCoefficient=0.0)
+    else:   # the whole step on the propagation world
+        h, mask = banded_continental_heights(rng, rng.integers(0, 12, 64), rng.integers(0, 12, 64)), _abi.TT_STAGE_ALL
+    return h, mask, ks_over
+
+
+@pytest.mark.parametrize("case", ["propagation", "max_iters", "uplift_clamp", "friction", "all_stages"])
+def test_continental_front_matches_oracle(sim, O, case):
+    h, mask, ks_over = _front_case(case)
+    cfg, ks = configs.synthetic(64, 64, 2, (0.5, 1.0), 1, hydraulic=2, thermal=2, ccl_every=1)
+    cfg.init_mode = _abi.TT_INIT_EMPTY
+    cfg.stage_mask = mask
+    for k, v in ks_over.items():
+        setattr(ks, k, v)
+    say(f"[front {case}] init")
+    g, o = pair(sim, O, cfg, ks)
+    try:
+        ids = np.where(np.arange(64)[None, :] < 32, 0, 1).repeat(64, 0).astype(np.uint8)
+        pd = get(o, _abi.FIELD_PLATE_DATA)
+        for p, d in ((0, 1.0), (1, -1.0)):
+            pd[p]["direction"] = (d, 0.0)
+            pd[p]["velocity"] = 1.0
+            pd[p]["mass"] = 1e7
+            pd[p]["pixelCenter"] = (0.5, 0.5)
+        for wld in (g, o):
+            put(wld, _abi.FIELD_HEIGHT, h.ravel())
+            put(wld, _abi.FIELD_PLATE_IDS, ids.ravel())
+            put(wld, _abi.FIELD_PLATE_DATA, pd)
+            wld.set_iteration(5)
+        for s in range(3 if case == "all_stages" else 1):
+            say(f"[front {case}] step {s}")
+            g.step(1)
+            o.step(1)
+            bad = diff_fields(g, o, ALL_FIELDS)
+            assert not bad, f"{case} step {s}: {bad}"
+            assert stats_tuple(g.stats()) == stats_tuple(o.stats())
+        if case != "friction":
+            assert o.stats().complex_sources > 0
+    finally:
+        g.close()
+        o.close()
 
 
 def test_resume_from_transferred_state(sim, O):
