@@ -30,7 +30,11 @@ __global__ __launch_bounds__(256) void k_hydraulic(tt_hyd_args A)
     __shared__ tt_hyd_lds<TH> S;
     const int nbx = A.W / TT_TILE_W, nby = A.H / TH;
     const int b = tt_xcd_tile(blockIdx.x, nbx * nby);
-    tt_hydraulic_tile<TH, FIRST, LAST, COMP>(A, S, (b % nbx) * TT_TILE_W, (b / nbx) * TH);
+    const int x0 = (b % nbx) * TT_TILE_W, y0 = (b / nbx) * TH;
+    if (x0 >= 2 && x0 + TT_TILE_W + 2 <= A.W && y0 >= 2 && y0 + TH + 2 <= A.H)
+        tt_hydraulic_tile<TH, FIRST, LAST, COMP, true>(A, S, x0, y0);
+    else
+        tt_hydraulic_tile<TH, FIRST, LAST, COMP, false>(A, S, x0, y0);
 }
 
 int tt_tile_rows_valid(int rows, int H) { return (rows == 8 || rows == 16 || rows == 32) && (H % rows) == 0; }

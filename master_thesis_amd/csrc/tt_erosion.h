@@ -299,6 +299,27 @@ __device__ __forceinline__ tt_taps tt_bilinear_taps(float tx, float ty, int W, i
     return t;
 }
 
+// As tt_bilinear_taps; when no sample of the wavefront has its 2x2 footprint
+// across the grid edge (the common case), the indices are formed without the
+// wrap, which gives the same indices there.
+__device__ __forceinline__ tt_taps tt_bilinear_taps_nowrap(float tx, float ty, int W, int H)
+{
+    const float fx = __builtin_floorf(tx), fy = __builtin_floorf(ty);
+    const int x0 = (int)fx, y0 = (int)fy;
+    const bool inside = (unsigned)x0 < (unsigned)(W - 1) && (unsigned)y0 < (unsigned)(H - 1);
+    if (__builtin_amdgcn_ballot_w64(!inside) == 0) {
+        tt_taps t;
+        t.wx = tx - (float)x0;
+        t.wy = ty - (float)y0;
+        t.a = y0 * W + x0;
+        t.b = t.a + 1;
+        t.c = t.a + W;
+        t.d = t.c + 1;
+        return t;
+    }
+    return tt_bilinear_taps(tx, ty, W, H);
+}
+
 __device__ __forceinline__ float tt_bilinear_mix(float s00, float s10, float s01, float s11, float wx, float wy)
 {
     const float I0 = (1.0f - wx) * s00 + wx * s10;
@@ -310,6 +331,18 @@ __device__ __forceinline__ float tt_bilinear(const float *__restrict__ s, float 
 {
     const tt_taps t = tt_bilinear_taps(tx, ty, W, H);
     return tt_bilinear_mix(s[t.a], s[t.b], s[t.c], s[t.d], t.wx, t.wy);
+}
+
+// tt_sinf for x >= 0.03 (finite or not): when the reduction's quadrant count
+// is 0 for every lane of the wavefront, the reduced argument is x itself
+// (x - 0 * pi/2 pieces) and tt_sinf is the sine polynomial of x; the other
+// wavefronts take tt_sinf.  Same bits either way.
+__device__ __forceinline__ float tt_sinf_wave(float x)
+{
+    const double xd = (double)x;
+    const double k = tt_rint_small(xd * 6.36619772367581382433e-01);
+    if (__builtin_amdgcn_ballot_w64(!(k == 0.0)) == 0) return (float)tt_sin_poly(xd);
+    return tt_sinf(x);
 }
 
 // rain amount of a cell: max(0, discreteNoise((x, 0, y), 1, seed + iter)) * dt * rain
@@ -355,7 +388,13 @@ __device__ __forceinline__ void tt_flow_velocity(const float4 &Sf, const float4 
 // state.  Loads are issued together before the first barrier (heights and
 // water with their halo, the flux ring, the flow water); the sediment taps
 // follow the velocity and are in flight while the new fluxes are computed.
-template <int TH, bool FIRST, bool LAST, bool COMP = false>
+//
+// The kernel is bound by its vector ALU work more than by HBM, so addressing
+// is kept off the vector units where it can be: the wavefront index is read
+// into a scalar register (row arithmetic runs on the scalar unit), and IN
+// (a tile whose two-cell halo does not cross the grid edge) drops the wrap of
+// every halo and flux-ring index.
+template <int TH, bool FIRST, bool LAST, bool COMP = false, bool IN = false>
 __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_lds<TH> &S, int x0, int y0)
 {
     constexpr int RPW = TH / 4;
@@ -364,7 +403,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
     constexpr int NHR = (TH + 4 + 3) / 4;        // halo rows per wavefront
     const int tid = threadIdx.x;
     const int W = A.W, H = A.H;
-    const int lane = tid & 63, wv = tid >> 6;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const float rain = A.rain;
     const int rseed = A.rseed;
     const int gx = x0 + lane;
@@ -377,14 +416,14 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
     }
     // 2. heights and water on the tile plus a two-cell halo: rows wv, wv+4,
     // ...; lanes 0..63 take columns 0..63, lanes 0..3 also columns 64..67
-    const int gxa = tt_wrap1(x0 + lane - 2, W);
-    const int gxb = tt_wrap1(x0 + 64 + lane - 2, W);
+    const int gxa = IN ? x0 + lane - 2 : tt_wrap1(x0 + lane - 2, W);
+    const int gxb = IN ? x0 + 62 + lane : tt_wrap1(x0 + 64 + lane - 2, W);
     float hm[NHR], hw[NHR], hm2[NHR], hw2[NHR];
 #pragma unroll
     for (int j = 0; j < NHR; ++j) {
         const int ly = wv + 4 * j;
         if (ly < TH + 4) {
-            const int row = tt_wrap1(y0 + ly - 2, H) * W;
+            const int row = (IN ? y0 + ly - 2 : tt_wrap1(y0 + ly - 2, H)) * W;
             hm[j] = A.hin[row + gxa];
             hw[j] = A.win[row + gxa];
             if (lane < TT_EW - 64) {
@@ -400,7 +439,8 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         const int i = tid + 256 * k;
         if (i < NFI) {
             const int fy = i / TT_FW, fx = i - fy * TT_FW;
-            cf[k] = A.fin[tt_wrap1(y0 + fy - 1, H) * W + tt_wrap1(x0 + fx - 1, W)];
+            cf[k] = IN ? A.fin[(y0 + fy - 1) * W + x0 + fx - 1]
+                       : A.fin[tt_wrap1(y0 + fy - 1, H) * W + tt_wrap1(x0 + fx - 1, W)];
         }
     }
     // sediment: the state itself on the first sub-iteration
@@ -418,7 +458,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
     for (int j = 0; j < NHR; ++j) {
         const int ly = wv + 4 * j;
         if (ly < TH + 4) {
-            const int gy = tt_wrap1(y0 + ly - 2, H);
+            const int gy = IN ? y0 + ly - 2 : tt_wrap1(y0 + ly - 2, H);
             float m = hm[j], w = hw[j];
             if (FIRST) w = w + tt_rain_at(gxa, gy, rseed, rain);
             S.M[ly][lane] = m;
@@ -454,7 +494,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             float vx, vy, vmag;
             tt_flow_velocity(S.F[fy][fx], S.F[fy][fx - 1], S.F[fy + 1][fx], S.F[fy][fx + 1], S.F[fy - 1][fx], wnp[r],
                              vx, vy, vmag);
-            const tt_taps t = tt_bilinear_taps((float)gx - vx * 1.0f, (float)gy - vy * 1.0f, W, H);
+            const tt_taps t = tt_bilinear_taps_nowrap((float)gx - vx * 1.0f, (float)gy - vy * 1.0f, W, H);
             wxs[r] = t.wx;
             wys[r] = t.wy;
             s4[r][0] = A.sprev[t.a];
@@ -541,7 +581,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         const float dzdx = (S.M[ly][lx + 1] - S.M[ly][lx - 1]) * 0.5f;
         const float dzdy = (S.M[ly + 1][lx] - S.M[ly - 1][lx]) * 0.5f;
         const float slope = sqrtf(dzdx * dzdx + dzdy * dzdy);
-        float C = (cap * tt_sinf(tt_fmaxf(slope, 0.03f))) * vmag;
+        float C = (cap * tt_sinf_wave(tt_fmaxf(slope, 0.03f))) * vmag;
         C = tt_fminf(C, 5.0f);
         float hnew;
         float chv = 0.0f;
