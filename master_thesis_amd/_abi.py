@@ -213,7 +213,8 @@ class StepStats(C.Structure):
         ("split_iterations", C.c_int32),
         ("complex_sources", C.c_int32),
         ("tile_rows", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("hydraulic_div_const", C.c_int32),
+        ("reserved", C.c_int32 * 1),
         ("mass_ledger", C.c_double),
     ]
 

@@ -195,6 +195,8 @@ void fill_desc(tt_bworld &B, const tt_world *w, int T, int K)
     B.fk = 1.0f * k.hydrationPipeCrossSection;
     B.g = k.gravity;
     B.L = k.hydrationPipeLength;
+    B.rL = w->rL;
+    B.divL = w->divL;
     for (int i = 0; i < 2; ++i) {
         B.h[i] = w->h[i];
         B.water[i] = w->water[i];

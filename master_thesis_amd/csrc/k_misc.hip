@@ -457,3 +457,35 @@ hipError_t tt_compact_u8_any(hipStream_t s, const uint8_t *flags, long long n, u
     hipLaunchKernelGGL(k_compact_any, dim3((unsigned)nb), dim3(256), 0, s, flags, n, out_idx, count);
     return hipGetLastError();
 }
+
+// Every float x that tt_div_safe accepts (zeros and 2^-100 <= |x| <= 2^100),
+// all 2^32 bit patterns visited: counts the x for which tt_div_const differs
+// in any bit from the IEEE quotient x / L.
+__global__ __launch_bounds__(256) void k_check_divisor(float L, float rL, uint32_t *__restrict__ bad)
+{
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    uint32_t n = 0;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < (1ull << 32);
+         i += stride) {
+        const float x = __uint_as_float((uint32_t)i);
+        if (!tt_div_safe(x)) continue;
+        n += __float_as_uint(x / L) != __float_as_uint(tt_div_const(x, L, rL)) ? 1u : 0u;
+    }
+    if (n) atomicAdd(bad, n);
+}
+
+hipError_t tt_check_divisor(hipStream_t s, float L, float rL, uint32_t *bad)
+{
+    uint32_t *d = nullptr;
+    hipError_t e = hipMalloc((void **)&d, sizeof(uint32_t));
+    if (e != hipSuccess) return e;
+    e = tt_fill_u32(s, d, 0u, 1);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_check_divisor, dim3(8192), dim3(256), 0, s, L, rL, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(bad, d, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    const hipError_t f = hipFree(d);
+    return e != hipSuccess ? e : f;
+}

@@ -14,6 +14,8 @@ struct tt_bworld {
     int rseed;             // seed + iteration (rain noise)
     float talus, amp, str; // thermal settings
     float rain, evap, cap, ke, kd, fk, g, L;  // hydraulic settings
+    float rL;                                 // RN(1 / L)
+    int divL;                                 // the division by L may take tt_div_const
     float *h[2];
     float *water[2];
     float4 *flux[2];

@@ -123,7 +123,30 @@ struct tt_dev {
     int wsh;                // l - 1
     double axs, ays;        // 256 / W, 256 / H (noise scale)
     tt_kernel_settings ks;
+    float rL;               // RN(1 / hydrationPipeLength)
+    int divL;               // 1: tt_div_const is exact for the pipe length (checked)
 };
+
+// x / L as RN(x * rL) corrected by one fused multiply-add with the negated
+// residual L * q0 - x, rL = RN(1/L).
+// For numerators accepted by tt_div_safe the correction is exact and the
+// result is the correctly rounded quotient (Markstein's theorem); whether it
+// equals IEEE x / L for every such x is checked per divisor over all 2^32
+// bit patterns (k_check_divisor) before it is used.
+__device__ __forceinline__ float tt_div_const(float x, float L, float rL)
+{
+    const float q0 = x * rL;
+    const float e = __builtin_fmaf(L, q0, -x);
+    return __builtin_fmaf(-e, rL, q0);
+}
+
+// 2^-100 <= |x| <= 2^100: the quotient and the residual stay normal (two
+// compares; zeros, NaN and infinities are left to the IEEE division)
+__device__ __forceinline__ int tt_div_safe(float x)
+{
+    const float a = __builtin_fabsf(x);
+    return (int)(a >= 0x1p-100f) & (int)(a <= 0x1p100f);
+}
 
 // Row and column of a cell index c < 2^31 without a division: with
 // m = ceil(2^(31+l) / W) and 2^(l-1) < W <= 2^l, m W - 2^(31+l) < W <= 2^l,

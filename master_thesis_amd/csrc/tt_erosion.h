@@ -229,6 +229,8 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
 struct tt_hyd_args {
     int W, H, rseed;
     float rain, evap, cap, ke, kd, fk, g, L;
+    float rL;             // RN(1 / L)
+    int divL;             // 1: the division by L may take tt_div_const
     const float *hin;
     float *hout;
     const float *win;
@@ -264,6 +266,8 @@ __host__ __device__ inline void tt_hyd_settings(const tt_dev &d, int rseed, tt_h
     A.fk = 1.0f * d.ks.hydrationPipeCrossSection;
     A.g = d.ks.gravity;
     A.L = d.ks.hydrationPipeLength;
+    A.rL = d.rL;
+    A.divL = d.divL;
 }
 
 // bilinear sample with toroidal wrap (interpolate, cuda_helper.cuh:49-65).
@@ -520,11 +524,28 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         const int ly = fy + 1, lx = fx + 1;
         const float mc = S.M[ly][lx], wc = S.Wt[ly][lx];
         const float sc = mc + wc;
+        const float n0 = g * ((sc - S.M[ly][lx - 1]) - S.Wt[ly][lx - 1]);
+        const float n1 = g * ((sc - S.M[ly + 1][lx]) - S.Wt[ly + 1][lx]);
+        const float n2 = g * ((sc - S.M[ly][lx + 1]) - S.Wt[ly][lx + 1]);
+        const float n3 = g * ((sc - S.M[ly - 1][lx]) - S.Wt[ly - 1][lx]);
+        float q0, q1, q2, q3;
+        const int safe = tt_div_safe(n0) & tt_div_safe(n1) & tt_div_safe(n2) & tt_div_safe(n3);
+        if (A.divL && __builtin_amdgcn_ballot_w64(!safe) == 0) {
+            q0 = tt_div_const(n0, L, A.rL);
+            q1 = tt_div_const(n1, L, A.rL);
+            q2 = tt_div_const(n2, L, A.rL);
+            q3 = tt_div_const(n3, L, A.rL);
+        } else {
+            q0 = n0 / L;
+            q1 = n1 / L;
+            q2 = n2 / L;
+            q3 = n3 / L;
+        }
         float4 o;
-        o.x = tt_fmaxf(0.0f, cf[k].x + fk * ((g * ((sc - S.M[ly][lx - 1]) - S.Wt[ly][lx - 1])) / L));
-        o.y = tt_fmaxf(0.0f, cf[k].y + fk * ((g * ((sc - S.M[ly + 1][lx]) - S.Wt[ly + 1][lx])) / L));
-        o.z = tt_fmaxf(0.0f, cf[k].z + fk * ((g * ((sc - S.M[ly][lx + 1]) - S.Wt[ly][lx + 1])) / L));
-        o.w = tt_fmaxf(0.0f, cf[k].w + fk * ((g * ((sc - S.M[ly - 1][lx]) - S.Wt[ly - 1][lx])) / L));
+        o.x = tt_fmaxf(0.0f, cf[k].x + fk * q0);
+        o.y = tt_fmaxf(0.0f, cf[k].y + fk * q1);
+        o.z = tt_fmaxf(0.0f, cf[k].z + fk * q2);
+        o.w = tt_fmaxf(0.0f, cf[k].w + fk * q3);
         const float tot = (((o.x + o.y) + o.z) + o.w) + 1e-6f;
         float K = tt_fminf(1.0f, wc / (tot * 1.0f + 1e-6f));
         K = tt_fmaxf(K, 0.01f);

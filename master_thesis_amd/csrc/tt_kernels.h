@@ -120,3 +120,5 @@ hipError_t tt_launch_flood_cc(hipStream_t s, const tt_dev &d, uint8_t *ids, int 
 hipError_t tt_launch_split_finalize(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *n,
                                     uint8_t *ids, int oldId, int newId, tt_plate_data *pd);
 hipError_t tt_launch_probe(hipStream_t s, int which, const float4 *a, float4 *b, float *sink, long long n);
+// counts (into *bad, host memory) the numerators for which tt_div_const(x, L, rL) differs from x / L
+hipError_t tt_check_divisor(hipStream_t s, float L, float rL, uint32_t *bad);

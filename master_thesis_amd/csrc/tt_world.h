@@ -90,6 +90,8 @@ struct tt_world {
     double *initWts = nullptr;
 
     uint32_t *dIter = nullptr;  // the step's iteration on the device (read by collision resolution)
+    float rL = 0.0f;            // RN(1 / hydrationPipeLength)
+    int divL = 0;               // 1: the hydraulic division by the pipe length may take tt_div_const
 
     // host mirrors
     tt_step_ctl *hctl = nullptr;

@@ -194,8 +194,37 @@ static tt_dev make_dev(const tt_world *w)
     d.axs = 256.0 / (double)w->W;
     d.ays = 256.0 / (double)w->H;
     d.ks = w->ks;
+    d.rL = w->rL;
+    d.divL = w->divL;
     tt_dev_set_divisor(d);
     return d;
+}
+
+// The hydraulic flux divides by the pipe length.  tt_div_const replaces that
+// IEEE division once it has been checked, on the device, to give the same bits
+// for every numerator it may receive; the verdict is kept per divisor for the
+// process.  Divisors outside [2^-20, 2^20] keep the IEEE division.
+static std::mutex g_divMu;
+static std::map<uint32_t, int> g_divOk;
+
+static int set_divisor(tt_world *w)
+{
+    const float L = w->ks.hydrationPipeLength;
+    volatile float one = 1.0f;
+    w->rL = one / L;
+    w->divL = 0;
+    if (!(L >= 0x1p-20f && L <= 0x1p20f)) return TT_OK;
+    uint32_t key;
+    std::memcpy(&key, &L, 4);
+    std::lock_guard<std::mutex> lk(g_divMu);
+    auto it = g_divOk.find(key);
+    if (it == g_divOk.end()) {
+        uint32_t bad = 1;
+        CK(tt_check_divisor(w->stream, L, w->rL, &bad));
+        it = g_divOk.emplace(key, bad == 0 ? 1 : 0).first;
+    }
+    w->divL = it->second;
+    return TT_OK;
 }
 
 static int sync_ctl(tt_world *w)
@@ -594,6 +623,7 @@ int tt_step_post(tt_world *w)
     w->iter++;
     w->stats.iteration = w->iter;
     w->stats.tile_rows = w->rows;
+    w->stats.hydraulic_div_const = w->divL;
     if (w->timing) {
         float t[8] = {0};
         for (int i = 1; i <= 7; ++i) CK(hipEventElapsedTime(&t[i], w->ev[i - 1], w->ev[i]));
@@ -707,6 +737,7 @@ int tt_world_do_init(tt_world *w)
 {
     hipStream_t s = w->stream;
     drop_pre_graphs(w);   // a reset may change the seed, an argument of the captured launches
+    CKS(set_divisor(w));
     w->iter = 0;
     w->hc = w->ic = w->wc = w->fc = w->sc = 0;
     CKS(init_common_zero(w));
@@ -1085,6 +1116,7 @@ int tt_world_init(tt_world *w)
     if (w->rows == 0) CKS(tune_rows(w));
     CKS(tt_world_do_init(w));
     w->stats.tile_rows = w->rows;
+    w->stats.hydraulic_div_const = w->divL;
     return TT_OK;
 }
 
@@ -1113,7 +1145,7 @@ int tt_world_set_settings(tt_world *w, const tt_kernel_settings *ks)
     if (!w || !ks) return TT_ERR_INVALID;
     w->ks = *ks;
     drop_pre_graphs(w);   // the settings are arguments of the captured launches
-    return TT_OK;
+    return set_divisor(w);
 }
 
 int tt_world_download(tt_world *w, int32_t field, void *host, size_t bytes)

@@ -268,6 +268,17 @@ def test_continental_front_matches_oracle(sim, O, case):
         o.close()
 
 
+@pytest.mark.parametrize("L", [1000.0, 7.0, 0.3, 2.0 ** 25])
+def test_pipe_length_division_matches_oracle(sim, O, L):
+    """The flux division by the pipe length takes the checked reciprocal form
+    for divisors in [2^-20, 2^20] (the check passes for these) and IEEE
+    division otherwise; either way the bits are the oracle's."""
+    cfg, ks = configs.synthetic(128, 128, 6, (0.5, 3.0), 3, hydraulic=6, thermal=2, ccl_every=1)
+    ks.hydrationPipeLength = L
+    seen = run_parity(sim, O, cfg, ks, 2, f"pipe length {L}")
+    assert seen[-1].hydraulic_div_const == (0 if L > 2.0 ** 20 else 1)
+
+
 def test_resume_from_transferred_state(sim, O):
     """A GPU world's state handed to the oracle mid-run continues identically
     (snapshot/resume through the field upload/download interface)."""
