@@ -232,7 +232,7 @@ typedef struct tt_step_stats {
     int32_t complex_sources;     /* continental sources handled in canonical serial order */
     int32_t tile_rows;           /* rows of the hydraulic tiles in use (tt_sim_config.tile_rows or tuned) */
     int32_t hydraulic_div_const; /* 1: the flux division by the pipe length uses the checked reciprocal form */
-    int32_t reserved[1];
+    int32_t hydraulic_sqrt_fast; /* 1: square roots of normal arguments use the checked hardware-root form */
     double mass_ledger;          /* sum of height over the grid, exact fixed-point, fp64 */
 } tt_step_stats;
 

@@ -214,7 +214,7 @@ class StepStats(C.Structure):
         ("complex_sources", C.c_int32),
         ("tile_rows", C.c_int32),
         ("hydraulic_div_const", C.c_int32),
-        ("reserved", C.c_int32 * 1),
+        ("hydraulic_sqrt_fast", C.c_int32),
         ("mass_ledger", C.c_double),
     ]
 

@@ -197,6 +197,7 @@ void fill_desc(tt_bworld &B, const tt_world *w, int T, int K)
     B.L = k.hydrationPipeLength;
     B.rL = w->rL;
     B.divL = w->divL;
+    B.sqrtOk = w->sqrtOk;
     for (int i = 0; i < 2; ++i) {
         B.h[i] = w->h[i];
         B.water[i] = w->water[i];

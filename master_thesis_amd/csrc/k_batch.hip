@@ -61,6 +61,7 @@ __global__ __launch_bounds__(256) void k_hydraulic_batch(const tt_bworld *__rest
     A.L = B.L;
     A.rL = B.rL;
     A.divL = B.divL;
+    A.sqrtOk = B.sqrtOk;
     const int hc = B.hc1 ^ (k & 1), wc = B.wc ^ (k & 1), fc = B.fc ^ (k & 1);
     A.hin = B.h[hc];
     A.hout = B.h[hc ^ 1];

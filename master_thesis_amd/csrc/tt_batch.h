@@ -16,6 +16,7 @@ struct tt_bworld {
     float rain, evap, cap, ke, kd, fk, g, L;  // hydraulic settings
     float rL;                                 // RN(1 / L)
     int divL;                                 // the division by L may take tt_div_const
+    int sqrtOk;                               // square roots may take tt_sqrt_normal
     float *h[2];
     float *water[2];
     float4 *flux[2];

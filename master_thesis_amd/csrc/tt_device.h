@@ -125,6 +125,7 @@ struct tt_dev {
     tt_kernel_settings ks;
     float rL;               // RN(1 / hydrationPipeLength)
     int divL;               // 1: tt_div_const is exact for the pipe length (checked)
+    int sqrtOk;             // 1: tt_sqrt_normal is exact over its range (checked)
 };
 
 // x / L as RN(x * rL) corrected by one fused multiply-add with the negated
@@ -138,6 +139,28 @@ __device__ __forceinline__ float tt_div_const(float x, float L, float rL)
     const float q0 = x * rL;
     const float e = __builtin_fmaf(L, q0, -x);
     return __builtin_fmaf(-e, rL, q0);
+}
+
+// Square root of 2^-96 <= x <= FLT_MAX: the hardware root (within 1 ulp) moved
+// to a neighbour when the fused residual x - s' * s shows that the neighbour
+// s' is the better root.  This is the correction that the IEEE square root of
+// the compiler applies; below 2^-96 it also scales the argument, and it treats
+// zeros and infinity apart, which this range leaves out.  Whether it equals
+// the IEEE square root over the whole range is checked once per process
+// (k_check_sqrt) before it is used.
+__device__ __forceinline__ float tt_sqrt_normal(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float up = __uint_as_float(__float_as_uint(s) + 1u);
+    float t = __builtin_fmaf(-dn, s, x) <= 0.0f ? dn : s;
+    t = __builtin_fmaf(-up, s, x) > 0.0f ? up : t;
+    return t;
+}
+
+__device__ __forceinline__ int tt_sqrt_in_range(float x)
+{
+    return (int)(x >= 0x1p-96f) & (int)(x <= 0x1.fffffep127f);
 }
 
 // 2^-100 <= |x| <= 2^100: the quotient and the residual stay normal (two

@@ -231,6 +231,7 @@ struct tt_hyd_args {
     float rain, evap, cap, ke, kd, fk, g, L;
     float rL;             // RN(1 / L)
     int divL;             // 1: the division by L may take tt_div_const
+    int sqrtOk;           // 1: square roots of normal arguments may take tt_sqrt_normal
     const float *hin;
     float *hout;
     const float *win;
@@ -268,6 +269,15 @@ __host__ __device__ inline void tt_hyd_settings(const tt_dev &d, int rseed, tt_h
     A.L = d.ks.hydrationPipeLength;
     A.rL = d.rL;
     A.divL = d.divL;
+    A.sqrtOk = d.sqrtOk;
+}
+
+// IEEE square root; a wavefront whose arguments all lie in [2^-96, FLT_MAX]
+// takes tt_sqrt_normal when it has been checked (ok)
+__device__ __forceinline__ float tt_sqrt_wave(float x, int ok)
+{
+    if (ok && __builtin_amdgcn_ballot_w64(!tt_sqrt_in_range(x)) == 0) return tt_sqrt_normal(x);
+    return sqrtf(x);
 }
 
 // bilinear sample with toroidal wrap (interpolate, cuda_helper.cuh:49-65).
@@ -360,20 +370,44 @@ __device__ __forceinline__ float tt_rain_at(int gx, int gy, int rseed, float rai
 // neighbours and its water after the flow (flow, plate_tectonics_kernel.cu:
 // 2487-2533): netFlow / max(w, 0.01), clamped to length 50
 __device__ __forceinline__ void tt_flow_velocity(const float4 &Sf, const float4 &Lf, const float4 &Uf, const float4 &Rf,
-                                                 const float4 &Df, float wn, float &vx, float &vy, float &vmag)
+                                                 const float4 &Df, float wn, float &vx, float &vy, float &vmag,
+                                                 int sqrtOk)
 {
     const float nx = (((Lf.z - Sf.x) + Sf.z) - Rf.x) * 0.5f;
     const float ny = (((Uf.w - Sf.y) + Sf.y) - Df.w) * 0.5f;
     const float hh = tt_fmaxf(wn, 0.01f);
     vx = nx / hh;
     vy = ny / hh;
-    const float vm = sqrtf(vx * vx + vy * vy);
+    const float vm = tt_sqrt_wave(vx * vx + vy * vy, sqrtOk);
     vmag = vm;
     if (vm > 50.0f) {
         const float sc = 50.0f / vm;
         vx = vx * sc;
         vy = vy * sc;
         vmag = sqrtf(vx * vx + vy * vy);
+    }
+}
+
+// The velocity alone, as tt_flow_velocity computes it: a squared magnitude of
+// at most 2500 has a root of at most 50 (the root is correctly rounded and
+// monotone), so no clamp applies and the root is not needed; a wavefront with
+// a larger (or NaN) one computes it.
+__device__ __forceinline__ void tt_flow_velocity_dir(const float4 &Sf, const float4 &Lf, const float4 &Uf,
+                                                     const float4 &Rf, const float4 &Df, float wn, float &vx,
+                                                     float &vy)
+{
+    const float nx = (((Lf.z - Sf.x) + Sf.z) - Rf.x) * 0.5f;
+    const float ny = (((Uf.w - Sf.y) + Sf.y) - Df.w) * 0.5f;
+    const float hh = tt_fmaxf(wn, 0.01f);
+    vx = nx / hh;
+    vy = ny / hh;
+    const float s2 = vx * vx + vy * vy;
+    if (__builtin_amdgcn_ballot_w64(!(s2 <= 2500.0f)) == 0) return;
+    const float vm = sqrtf(s2);
+    if (vm > 50.0f) {
+        const float sc = 50.0f / vm;
+        vx = vx * sc;
+        vy = vy * sc;
     }
 }
 
@@ -495,9 +529,9 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             const int ry = wv * RPW + r;
             const int gy = y0 + ry;
             const int fy = ry + 1, fx = lane + 1;
-            float vx, vy, vmag;
-            tt_flow_velocity(S.F[fy][fx], S.F[fy][fx - 1], S.F[fy + 1][fx], S.F[fy][fx + 1], S.F[fy - 1][fx], wnp[r],
-                             vx, vy, vmag);
+            float vx, vy;
+            tt_flow_velocity_dir(S.F[fy][fx], S.F[fy][fx - 1], S.F[fy + 1][fx], S.F[fy][fx + 1], S.F[fy - 1][fx],
+                                 wnp[r], vx, vy);
             const tt_taps t = tt_bilinear_taps_nowrap((float)gx - vx * 1.0f, (float)gy - vy * 1.0f, W, H);
             wxs[r] = t.wx;
             wys[r] = t.wy;
@@ -575,7 +609,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         const int ly = ry + 2, lx = lane + 2;
         const float wn = tt_fmaxf(0.0f, S.Wt[ly][lx] + dv);
         float vx, vy, vmag;
-        tt_flow_velocity(Sf, Lf, Uf, Rf, Df, wn, vx, vy, vmag);
+        tt_flow_velocity(Sf, Lf, Uf, Rf, Df, wn, vx, vy, vmag, A.sqrtOk);
         float wo = tt_fmaxf(0.0f, wn - (tt_fminf(wn, 20.0f) * evap) * 1.0f);
         if (!LAST) wo = wo + tt_rain_at(gx, gy, rseed, rain);
         __builtin_nontemporal_store(wo, &A.wout[c]);
@@ -601,7 +635,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         // Slope: (m[+1] - m[-1]) / 2.0 in the reference; halving is exact in float
         const float dzdx = (S.M[ly][lx + 1] - S.M[ly][lx - 1]) * 0.5f;
         const float dzdy = (S.M[ly + 1][lx] - S.M[ly - 1][lx]) * 0.5f;
-        const float slope = sqrtf(dzdx * dzdx + dzdy * dzdy);
+        const float slope = tt_sqrt_wave(dzdx * dzdx + dzdy * dzdy, A.sqrtOk);
         float C = (cap * tt_sinf_wave(tt_fmaxf(slope, 0.03f))) * vmag;
         C = tt_fminf(C, 5.0f);
         float hnew;

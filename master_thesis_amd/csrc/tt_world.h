@@ -92,6 +92,7 @@ struct tt_world {
     uint32_t *dIter = nullptr;  // the step's iteration on the device (read by collision resolution)
     float rL = 0.0f;            // RN(1 / hydrationPipeLength)
     int divL = 0;               // 1: the hydraulic division by the pipe length may take tt_div_const
+    int sqrtOk = 0;             // 1: hydraulic square roots of normal arguments may take tt_sqrt_normal
 
     // host mirrors
     tt_step_ctl *hctl = nullptr;

@@ -196,6 +196,7 @@ static tt_dev make_dev(const tt_world *w)
     d.ks = w->ks;
     d.rL = w->rL;
     d.divL = w->divL;
+    d.sqrtOk = w->sqrtOk;
     tt_dev_set_divisor(d);
     return d;
 }
@@ -203,9 +204,12 @@ static tt_dev make_dev(const tt_world *w)
 // The hydraulic flux divides by the pipe length.  tt_div_const replaces that
 // IEEE division once it has been checked, on the device, to give the same bits
 // for every numerator it may receive; the verdict is kept per divisor for the
-// process.  Divisors outside [2^-20, 2^20] keep the IEEE division.
+// process.  Divisors outside [2^-20, 2^20] keep the IEEE division.  Likewise
+// tt_sqrt_normal replaces the IEEE square root once it has been checked over
+// its whole range (one verdict for the process).
 static std::mutex g_divMu;
 static std::map<uint32_t, int> g_divOk;
+static int g_sqrtOk = -1;
 
 static int set_divisor(tt_world *w)
 {
@@ -213,10 +217,16 @@ static int set_divisor(tt_world *w)
     volatile float one = 1.0f;
     w->rL = one / L;
     w->divL = 0;
+    std::lock_guard<std::mutex> lk(g_divMu);
+    if (g_sqrtOk < 0) {
+        uint32_t bad = 1;
+        CK(tt_check_sqrt(w->stream, &bad));
+        g_sqrtOk = bad == 0 ? 1 : 0;
+    }
+    w->sqrtOk = g_sqrtOk;
     if (!(L >= 0x1p-20f && L <= 0x1p20f)) return TT_OK;
     uint32_t key;
     std::memcpy(&key, &L, 4);
-    std::lock_guard<std::mutex> lk(g_divMu);
     auto it = g_divOk.find(key);
     if (it == g_divOk.end()) {
         uint32_t bad = 1;
@@ -624,6 +634,7 @@ int tt_step_post(tt_world *w)
     w->stats.iteration = w->iter;
     w->stats.tile_rows = w->rows;
     w->stats.hydraulic_div_const = w->divL;
+    w->stats.hydraulic_sqrt_fast = w->sqrtOk;
     if (w->timing) {
         float t[8] = {0};
         for (int i = 1; i <= 7; ++i) CK(hipEventElapsedTime(&t[i], w->ev[i - 1], w->ev[i]));
@@ -1117,6 +1128,7 @@ int tt_world_init(tt_world *w)
     CKS(tt_world_do_init(w));
     w->stats.tile_rows = w->rows;
     w->stats.hydraulic_div_const = w->divL;
+    w->stats.hydraulic_sqrt_fast = w->sqrtOk;
     return TT_OK;
 }
 

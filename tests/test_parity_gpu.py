@@ -277,6 +277,7 @@ def test_pipe_length_division_matches_oracle(sim, O, L):
     ks.hydrationPipeLength = L
     seen = run_parity(sim, O, cfg, ks, 2, f"pipe length {L}")
     assert seen[-1].hydraulic_div_const == (0 if L > 2.0 ** 20 else 1)
+    assert seen[-1].hydraulic_sqrt_fast == 1
 
 
 def test_resume_from_transferred_state(sim, O):
