@@ -276,7 +276,8 @@ __host__ __device__ inline void tt_hyd_settings(const tt_dev &d, int rseed, tt_h
 // takes tt_sqrt_normal when it has been checked (ok)
 __device__ __forceinline__ float tt_sqrt_wave(float x, int ok)
 {
-    if (ok && __builtin_amdgcn_ballot_w64(!tt_sqrt_in_range(x)) == 0) return tt_sqrt_normal(x);
+    if (__builtin_amdgcn_readfirstlane(ok) && __builtin_amdgcn_ballot_w64(!tt_sqrt_in_range(x)) == 0)
+        return tt_sqrt_normal(x);
     return sqrtf(x);
 }
 
@@ -359,10 +360,14 @@ __device__ __forceinline__ float tt_sinf_wave(float x)
     return tt_sinf(x);
 }
 
-// rain amount of a cell: max(0, discreteNoise((x, 0, y), 1, seed + iter)) * dt * rain
+// rain amount of a cell: max(0, discreteNoise((x, 0, y), 1, seed + iter)) * dt * rain.
+// For a cell (0 <= x, y < 2^24) the lattice point is the cell itself, and the
+// lattice sum ((x*1723 + 0*93241) + y*149812) + 3824 + seed of tt_random_grid
+// loses its middle term: adding +0 to the non-negative x*1723 changes nothing.
 __device__ __forceinline__ float tt_rain_at(int gx, int gy, int rseed, float rain)
 {
-    const float n = tt_discrete_noise((float)gx, 0.0f, (float)gy, 1.0f, rseed);
+    const float s = (((float)gx * 1723.0f + (float)gy * 149812.0f) + 3824.0f) + (float)rseed;
+    const float n = tt_random_float(tt_f2u32_sat(s)) * 2.0f - 1.0f;
     return (tt_fmaxf(0.0f, n) * 1.0f) * rain;
 }
 
