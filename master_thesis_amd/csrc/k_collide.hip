@@ -108,20 +108,37 @@ __device__ __forceinline__ void tt_write_collision(const tt_dev &d, const uint8_
 
 // arrivals at destination (gx, gy): the set of plates p with ids[g - o_p] == p
 // (one candidate per distinct offset); returns the count.
-template <bool LDS>
-__device__ __forceinline__ int tt_arrivals(const tt_dev &d, const uint8_t *tile, const uint8_t *ids, int lx, int ly,
-                                           int gx, int gy, const int *dox, const int *doy, int nd, const int *pox,
-                                           const int *poy, tt_pset &set)
+__device__ __forceinline__ int tt_arrivals(const tt_dev &d, const uint8_t *ids, int gx, int gy, const int *dox,
+                                           const int *doy, int nd, const int *pox, const int *poy, tt_pset &set)
 {
     int n = 0;
     set.clear();
     for (int j = 0; j < nd; ++j) {
         const int ox = dox[j], oy = doy[j];
-        int p;
-        if (LDS) p = tile[(ly - oy + ADV_R) * ADV_TW + (lx - ox + ADV_R)];
-        else p = ids[(size_t)tt_wrapg(gy - oy, d.H) * d.W + tt_wrapg(gx - ox, d.W)];
+        const int p = ids[(size_t)tt_wrapg(gy - oy, d.H) * d.W + tt_wrapg(gx - ox, d.W)];
         if (pox[p] == ox && poy[p] == oy) {
             set.add(p);
+            ++n;
+        }
+    }
+    return n;
+}
+
+// offset key of |ox|, |oy| <= ADV_R
+__device__ __forceinline__ uint32_t tt_adv_key(int ox, int oy) { return ((uint32_t)(oy + 128) << 8) | (uint32_t)(ox + 128); }
+
+// The same set from the LDS tile, whose entries hold (offset key of the
+// plate << 8) | plate: the source of offset j is entry base - del[j], and its
+// plate arrives exactly when the plate's own offset key is key[j].
+__device__ __forceinline__ int tt_arrivals_lds(const uint32_t *tileK, int base, const uint32_t *key, const int *del,
+                                               int nd, tt_pset &set)
+{
+    int n = 0;
+    set.clear();
+    for (int j = 0; j < nd; ++j) {
+        const uint32_t e = tileK[base - del[j]];
+        if ((e >> 8) == key[j]) {
+            set.add((int)(e & 0xFFu));
             ++n;
         }
     }
@@ -133,9 +150,11 @@ __global__ __launch_bounds__(256) void k_advect(tt_dev d, const uint8_t *ids, co
                                                 const uint8_t *typeBitmap, uint32_t *word, uint8_t *cnt,
                                                 uint32_t *segCount, int *typeCounts, tt_step_ctl *ctlw)
 {
-    __shared__ uint8_t tile[ADV_TH * ADV_TW];
+    __shared__ uint32_t tileK[ADV_TH * ADV_TW];
     __shared__ int pox[256], poy[256];
     __shared__ int dox[256], doy[256];
+    __shared__ uint32_t dkey[256];
+    __shared__ int ddel[256];
     __shared__ uint8_t present[256], useful[256];
     __shared__ int nuse;
     const int tid = threadIdx.x;
@@ -150,11 +169,13 @@ __global__ __launch_bounds__(256) void k_advect(tt_dev d, const uint8_t *ids, co
     const bool lds = R <= ADV_R;
     __syncthreads();
     if (lds) {
+        // every plate's offset is within ADV_R here (R <= ADV_R), so its key is defined
         for (int i = tid; i < ADV_TH * ADV_TW; i += 256) {
             const int ly = i / ADV_TW, lx = i - ly * ADV_TW;
-            const int gx = tt_wrapg(x0 + lx - ADV_R, d.W), gy = tt_wrapg(y0 + ly - ADV_R, d.H);
+            // |x0 + lx - ADV_R| < 2W and |y0 + ly - ADV_R| < 2H: one conditional wrap
+            const int gx = tt_wrap1(x0 + lx - ADV_R, d.W), gy = tt_wrap1(y0 + ly - ADV_R, d.H);
             const uint8_t p = ids[(size_t)gy * d.W + gx];
-            tile[i] = p;
+            tileK[i] = (tt_adv_key(pox[p], poy[p]) << 8) | p;
             present[p] = 1;
         }
     }
@@ -171,7 +192,11 @@ __global__ __launch_bounds__(256) void k_advect(tt_dev d, const uint8_t *ids, co
         if (tid == 0) {
             int m = 0;
             for (int j = 0; j < nd; ++j)
-                if (useful[j]) { dox[m] = dox[j]; doy[m] = doy[j]; ++m; }
+                if (useful[j]) {
+                    dkey[m] = tt_adv_key(dox[j], doy[j]);
+                    ddel[m] = doy[j] * ADV_TW + dox[j];
+                    ++m;
+                }
             nuse = m;
         }
         __syncthreads();
@@ -185,8 +210,8 @@ __global__ __launch_bounds__(256) void k_advect(tt_dev d, const uint8_t *ids, co
         const int gx = x0 + lane, gy = y0 + ly;
         const size_t c = (size_t)gy * d.W + gx;
         tt_pset set;
-        const int n = lds ? tt_arrivals<true>(d, tile, ids, lane, ly, gx, gy, dox, doy, nd, pox, poy, set)
-                          : tt_arrivals<false>(d, tile, ids, lane, ly, gx, gy, dox, doy, nd, pox, poy, set);
+        const int n = lds ? tt_arrivals_lds(tileK, (ly + ADV_R) * ADV_TW + lane + ADV_R, dkey, ddel, nd, set)
+                          : tt_arrivals(d, ids, gx, gy, dox, doy, nd, pox, poy, set);
         uint32_t wd = 0;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
