@@ -21,15 +21,16 @@
 #define ADV_R 8
 #define ADV_TW (TT_TILE_W + 2 * ADV_R)
 #define ADV_TH (TT_TILE_H + 2 * ADV_R)
+#define ADV_K (2 * ADV_R + 1)   // offsets per axis within ADV_R
 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_plate_prep(const tt_plate_data *pd, tt_plate_step *ps, tt_step_ctl *ctl,
                                                     long long *fric)
 {
     __shared__ int sox[256], soy[256];
-    __shared__ int nd, maxr;
+    __shared__ int nd, maxr, zero;
     const int p = threadIdx.x;
-    if (p == 0) { nd = 0; maxr = 0; }
+    if (p == 0) { nd = 0; maxr = 0; zero = 0; }
     tt_plate_step st;
     if (p < TT_MAXP) {
         const tt_plate_data q = pd[p];
@@ -63,12 +64,14 @@ __global__ __launch_bounds__(256) void k_plate_prep(const tt_plate_data *pd, tt_
             ctl->doy[k] = st.oy;
             const int r = max(abs(st.ox), abs(st.oy));
             atomicMax(&maxr, r);
+            if (r == 0) zero = 1;
         }
     }
     __syncthreads();
     if (p == 0) {
         ctl->n_distinct = nd;
         ctl->max_r = maxr;
+        ctl->zero_distinct = zero;
         ctl->overflow_cells = 0;
         ctl->complex_sources = 0;
     }
@@ -84,8 +87,10 @@ hipError_t tt_launch_plate_prep(hipStream_t s, const tt_plate_data *pd, tt_plate
 // ---------------------------------------------------------------------------
 // collision typing helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void tt_write_collision(const tt_dev &d, const uint8_t *bitmap, float hA, float hB, int pA,
-                                                   int pB, int *counts)
+// writeCollision: the counter (pair * 5 + kind) that the pair's collision
+// increments, or -1 when the pair's type is kept (the cooldown early exit)
+__device__ __forceinline__ int tt_collision_key(const tt_dev &d, const uint8_t *bitmap, float hA, float hB, int pA,
+                                                int pB)
 {
     const bool cond = pA < pB;
     const int mn = cond ? pA : pB, mx = cond ? pB : pA;
@@ -94,7 +99,7 @@ __device__ __forceinline__ void tt_write_collision(const tt_dev &d, const uint8_
     const unsigned packed = bitmap[t];
     // `packed >> 0 & 1 && packed >> 4 & 0x0F < cooldown` (plate_tectonics_kernel.cu:813)
     const unsigned cmpbit = (0x0F < d.ks.collisionTypeUpdateCooldown) ? 1u : 0u;
-    if ((packed & 1u) && ((packed >> 4) & cmpbit)) return;
+    if ((packed & 1u) && ((packed >> 4) & cmpbit)) return -1;
     const bool minCont = hmn >= d.ks.continentalCrustThreshold;
     const bool maxCont = hmx >= d.ks.continentalCrustThreshold;
     int k;
@@ -103,13 +108,30 @@ __device__ __forceinline__ void tt_write_collision(const tt_dev &d, const uint8_
     else if (maxCont) k = 0;             // subductionsAContinental
     else if (hmn <= hmx) k = 1;          // subductionsAOceanic
     else k = 3;                          // subductionsBOceanic
-    atomicAdd(&counts[t * 5 + k], 1);
+    return (int)(t * 5u) + k;
+}
+
+// counts[key] += 1 for every lane with key >= 0, as one atomic per distinct key
+// of the wavefront (neighbouring collision cells mostly share the pair and the
+// kind; integer sums do not depend on the order).  Called by the whole
+// wavefront.
+__device__ __forceinline__ void tt_wave_count(int key, int *counts)
+{
+    unsigned long long pending = __builtin_amdgcn_ballot_w64(key >= 0);
+    const int lane = (int)(threadIdx.x & 63u);
+    while (pending) {
+        const int first = (int)__builtin_ctzll(pending);
+        const int leader = __builtin_amdgcn_readlane(key, first);
+        const unsigned long long same = __builtin_amdgcn_ballot_w64(key == leader);
+        if (lane == first) atomicAdd(&counts[leader], (int)__popcll(same));
+        pending &= ~same;
+    }
 }
 
 // arrivals at destination (gx, gy): the set of plates p with ids[g - o_p] == p
 // (one candidate per distinct offset); returns the count.
 __device__ __forceinline__ int tt_arrivals(const tt_dev &d, const uint8_t *ids, int gx, int gy, const int *dox,
-                                           const int *doy, int nd, const int *pox, const int *poy, tt_pset &set)
+                                           const int *doy, int nd, const int *pox, const int *poy, tt_min4 &set)
 {
     int n = 0;
     set.clear();
@@ -117,7 +139,7 @@ __device__ __forceinline__ int tt_arrivals(const tt_dev &d, const uint8_t *ids, 
         const int ox = dox[j], oy = doy[j];
         const int p = ids[(size_t)tt_wrapg(gy - oy, d.H) * d.W + tt_wrapg(gx - ox, d.W)];
         if (pox[p] == ox && poy[p] == oy) {
-            set.add(p);
+            set.add((unsigned)p);
             ++n;
         }
     }
@@ -131,14 +153,14 @@ __device__ __forceinline__ uint32_t tt_adv_key(int ox, int oy) { return ((uint32
 // plate << 8) | plate: the source of offset j is entry base - del[j], and its
 // plate arrives exactly when the plate's own offset key is key[j].
 __device__ __forceinline__ int tt_arrivals_lds(const uint32_t *tileK, int base, const uint32_t *key, const int *del,
-                                               int nd, tt_pset &set)
+                                               int nd, tt_min4 &set)
 {
     int n = 0;
     set.clear();
     for (int j = 0; j < nd; ++j) {
         const uint32_t e = tileK[base - del[j]];
         if ((e >> 8) == key[j]) {
-            set.add((int)(e & 0xFFu));
+            set.add(e & 0xFFu);
             ++n;
         }
     }
@@ -153,55 +175,50 @@ __global__ __launch_bounds__(256) void k_advect(tt_dev d, const uint8_t *ids, co
     __shared__ uint32_t tileK[ADV_TH * ADV_TW];
     __shared__ int pox[256], poy[256];
     __shared__ int dox[256], doy[256];
-    __shared__ uint32_t dkey[256];
-    __shared__ int ddel[256];
-    __shared__ uint8_t present[256], useful[256];
+    __shared__ uint32_t dkey[ADV_K * ADV_K];
+    __shared__ int ddel[ADV_K * ADV_K];
+    __shared__ uint8_t offUsed[ADV_K * ADV_K];
     __shared__ int nuse;
     const int tid = threadIdx.x;
     int nd = ctl->n_distinct;
     const int R = ctl->max_r;
+    const int zeroDistinct = ctl->zero_distinct;
     pox[tid] = ps[tid].ox;
     poy[tid] = ps[tid].oy;
-    present[tid] = 0;
-    useful[tid] = 0;
+    for (int i = tid; i < ADV_K * ADV_K; i += 256) offUsed[i] = 0;
+    if (tid == 0) nuse = 0;
     if (tid < nd) { dox[tid] = ctl->dox[tid]; doy[tid] = ctl->doy[tid]; }
     const int x0 = blockIdx.x * TT_TILE_W, y0 = blockIdx.y * TT_TILE_H;
     const bool lds = R <= ADV_R;
     __syncthreads();
     if (lds) {
-        // every plate's offset is within ADV_R here (R <= ADV_R), so its key is defined
+        // Every plate's offset is within ADV_R here (R <= ADV_R), so its key
+        // is defined.  A plate arrives at a cell of the tile only from a
+        // source cell at its offset, which lies in the tile plus halo: only
+        // the offsets of the plates present there need to be tried.  These
+        // are among the distinct offsets of the plates 0..254; the "no plate"
+        // entry 255 (offset (0, 0)) adds its offset only when that is one of
+        // them.  The order of the list does not matter (the arrivals are a set).
         for (int i = tid; i < ADV_TH * ADV_TW; i += 256) {
             const int ly = i / ADV_TW, lx = i - ly * ADV_TW;
             // |x0 + lx - ADV_R| < 2W and |y0 + ly - ADV_R| < 2H: one conditional wrap
             const int gx = tt_wrap1(x0 + lx - ADV_R, d.W), gy = tt_wrap1(y0 + ly - ADV_R, d.H);
             const uint8_t p = ids[(size_t)gy * d.W + gx];
-            tileK[i] = (tt_adv_key(pox[p], poy[p]) << 8) | p;
-            present[p] = 1;
+            const int ox = pox[p], oy = poy[p];
+            tileK[i] = (tt_adv_key(ox, oy) << 8) | p;
+            if (p != TT_MAXP || zeroDistinct) offUsed[(oy + ADV_R) * ADV_K + ox + ADV_R] = 1;
         }
+        __syncthreads();
+        for (int i = tid; i < ADV_K * ADV_K; i += 256)
+            if (offUsed[i]) {
+                const int ox = i % ADV_K - ADV_R, oy = i / ADV_K - ADV_R;
+                const int m = atomicAdd(&nuse, 1);
+                dkey[m] = tt_adv_key(ox, oy);
+                ddel[m] = oy * ADV_TW + ox;
+            }
     }
     __syncthreads();
-    if (lds) {
-        // A plate arrives at a cell of the tile only from a source cell at
-        // its offset, which lies in the tile plus halo: only the offsets of
-        // the plates present there need to be tried.
-        if (present[tid]) {
-            for (int j = 0; j < nd; ++j)
-                if (dox[j] == pox[tid] && doy[j] == poy[tid]) { useful[j] = 1; break; }
-        }
-        __syncthreads();
-        if (tid == 0) {
-            int m = 0;
-            for (int j = 0; j < nd; ++j)
-                if (useful[j]) {
-                    dkey[m] = tt_adv_key(dox[j], doy[j]);
-                    ddel[m] = doy[j] * ADV_TW + dox[j];
-                    ++m;
-                }
-            nuse = m;
-        }
-        __syncthreads();
-        nd = nuse;
-    }
+    if (lds) nd = nuse;
     const int lane = tid & 63, wv = tid >> 6;
     const int segPerRow = d.W / 64;
     int overflow = 0;
@@ -209,21 +226,21 @@ __global__ __launch_bounds__(256) void k_advect(tt_dev d, const uint8_t *ids, co
         const int ly = wv * 4 + k;
         const int gx = x0 + lane, gy = y0 + ly;
         const size_t c = (size_t)gy * d.W + gx;
-        tt_pset set;
+        tt_min4 set;
         const int n = lds ? tt_arrivals_lds(tileK, (ly + ADV_R) * ADV_TW + lane + ADV_R, dkey, ddel, nd, set)
                           : tt_arrivals(d, ids, gx, gy, dox, doy, nd, pox, poy, set);
+        const unsigned sl[4] = {set.s0, set.s1, set.s2, set.s3};
         uint32_t wd = 0;
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int p = set.pop_min();
-            if (p < 256) wd |= (uint32_t)(0xFFu - (uint32_t)p) << (8 * s);
-        }
+        for (int s = 0; s < 4; ++s)
+            if (sl[s] < 256u) wd |= (0xFFu - sl[s]) << (8 * s);
         word[c] = wd;
         cnt[c] = (uint8_t)(n < 255 ? n : 255);
         if (n > 4) ++overflow;
         int sum = n;
         for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
         if (lane == 0) segCount[(size_t)gy * segPerRow + blockIdx.x] = (uint32_t)sum;
+        int key[6] = {-1, -1, -1, -1, -1, -1};
         if (wd > 0xFFu && ((wd >> 8) & 0xFFu) != 0u) {
             // B1: pair counts (determineCollisionType, plate_tectonics_kernel.cu:838-888)
             int P[4];
@@ -234,20 +251,22 @@ __global__ __launch_bounds__(256) void k_advect(tt_dev d, const uint8_t *ids, co
                             ? height[(size_t)tt_wrapg(gy - poy[P[s]], d.H) * d.W + tt_wrapg(gx - pox[P[s]], d.W)]
                             : 0.0f;
             if (P[3] != TT_MAXP) {
-                tt_write_collision(d, typeBitmap, hh[0], hh[1], P[0], P[1], typeCounts);
-                tt_write_collision(d, typeBitmap, hh[0], hh[2], P[0], P[2], typeCounts);
-                tt_write_collision(d, typeBitmap, hh[0], hh[3], P[0], P[3], typeCounts);
-                tt_write_collision(d, typeBitmap, hh[1], hh[2], P[1], P[2], typeCounts);
-                tt_write_collision(d, typeBitmap, hh[1], hh[3], P[1], P[3], typeCounts);
-                tt_write_collision(d, typeBitmap, hh[2], hh[3], P[2], P[3], typeCounts);
+                key[0] = tt_collision_key(d, typeBitmap, hh[0], hh[1], P[0], P[1]);
+                key[1] = tt_collision_key(d, typeBitmap, hh[0], hh[2], P[0], P[2]);
+                key[2] = tt_collision_key(d, typeBitmap, hh[0], hh[3], P[0], P[3]);
+                key[3] = tt_collision_key(d, typeBitmap, hh[1], hh[2], P[1], P[2]);
+                key[4] = tt_collision_key(d, typeBitmap, hh[1], hh[3], P[1], P[3]);
+                key[5] = tt_collision_key(d, typeBitmap, hh[2], hh[3], P[2], P[3]);
             } else if (P[2] != TT_MAXP) {
-                tt_write_collision(d, typeBitmap, hh[0], hh[1], P[0], P[1], typeCounts);
-                tt_write_collision(d, typeBitmap, hh[0], hh[2], P[0], P[2], typeCounts);
-                tt_write_collision(d, typeBitmap, hh[1], hh[2], P[1], P[2], typeCounts);
+                key[0] = tt_collision_key(d, typeBitmap, hh[0], hh[1], P[0], P[1]);
+                key[1] = tt_collision_key(d, typeBitmap, hh[0], hh[2], P[0], P[2]);
+                key[2] = tt_collision_key(d, typeBitmap, hh[1], hh[2], P[1], P[2]);
             } else {
-                tt_write_collision(d, typeBitmap, hh[0], hh[1], P[0], P[1], typeCounts);
+                key[0] = tt_collision_key(d, typeBitmap, hh[0], hh[1], P[0], P[1]);
             }
         }
+#pragma unroll
+        for (int s = 0; s < 6; ++s) tt_wave_count(key[s], typeCounts);
     }
     if (overflow) atomicAdd(&ctlw->overflow_cells, overflow);
 }

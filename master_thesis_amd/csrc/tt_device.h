@@ -48,6 +48,7 @@ struct tt_step_ctl {
     // advection
     int n_distinct;
     int max_r;
+    int zero_distinct;   // some plate (0..254) has the offset (0, 0)
     int dox[256], doy[256];
     // outputs of the step read back by the host
     int overflow_cells;

@@ -57,26 +57,17 @@ __device__ __forceinline__ bool tt_insert_pair(uint32_t &inv, int a, int b)
 }
 
 // plate-id bit set of up to 256 plates held in four 64-bit registers
-struct tt_pset {
-    unsigned long long m0, m1, m2, m3;
-    __device__ __forceinline__ void clear() { m0 = m1 = m2 = m3 = 0ull; }
-    __device__ __forceinline__ void add(int p)
+// The four smallest of a set of distinct plate ids, ascending (256 for
+// missing entries): an insertion through a min/max chain.
+struct tt_min4 {
+    unsigned s0, s1, s2, s3;
+    __device__ __forceinline__ void clear() { s0 = s1 = s2 = s3 = 256u; }
+    __device__ __forceinline__ void add(unsigned p)
     {
-        const unsigned long long b = 1ull << (p & 63);
-        switch (p >> 6) {
-            case 0: m0 |= b; break;
-            case 1: m1 |= b; break;
-            case 2: m2 |= b; break;
-            default: m3 |= b; break;
-        }
-    }
-    // remove and return the smallest member (256 if empty)
-    __device__ __forceinline__ int pop_min()
-    {
-        if (m0) { const int b = __ffsll((long long)m0) - 1; m0 &= m0 - 1; return b; }
-        if (m1) { const int b = __ffsll((long long)m1) - 1; m1 &= m1 - 1; return 64 + b; }
-        if (m2) { const int b = __ffsll((long long)m2) - 1; m2 &= m2 - 1; return 128 + b; }
-        if (m3) { const int b = __ffsll((long long)m3) - 1; m3 &= m3 - 1; return 192 + b; }
-        return 256;
+        unsigned t = p, a;
+        a = min(s0, t); t = max(s0, t); s0 = a;
+        a = min(s1, t); t = max(s1, t); s1 = a;
+        a = min(s2, t); t = max(s2, t); s2 = a;
+        s3 = min(s3, t);
     }
 };
