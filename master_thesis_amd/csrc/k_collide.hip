@@ -398,21 +398,31 @@ __global__ __launch_bounds__(256) void k_resolve(tt_dev d, const uint32_t *iterp
                                                  long long *fric, uint8_t *srcFlag, uint8_t *contested, int do_uplift)
 {
     __shared__ unsigned long long sf[256];
+    __shared__ tt_plate_step sps[256];
     const int tid = threadIdx.x;
     sf[tid] = 0ull;
+    sps[tid] = ps[tid];
+    // the four cells' collision words are loaded together
+    uint32_t cvs[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long long c = (long long)blockIdx.x * 1024 + k * 256 + tid;
+        cvs[k] = c < d.N ? word[c] : 0u;
+    }
     __syncthreads();
+    ps = sps;
     const int W = d.W, H = d.H;
     for (int k = 0; k < 4; ++k) {
         const long long c = (long long)blockIdx.x * 1024 + k * 256 + tid;
         if (c >= d.N) break;
         int x, y;
         tt_cell_xy((uint32_t)c, d, x, y);
-        const uint32_t cv = word[c];
+        const uint32_t cv = cvs[k];
         if (cv == 0) {
             // divergence (processDivergence, plate_tectonics_kernel.cu:301-362)
             const int prev = ids[c];
             const tt_plate_step pp = ps[prev];
-            const uint32_t ov = word[(size_t)tt_wrapg(y - pp.oy, H) * W + tt_wrapg(x - pp.ox, W)];
+            const uint32_t ov = word[(size_t)tt_wrap_near(y - pp.oy, H) * W + tt_wrap_near(x - pp.ox, W)];
             const int opp = (int)(uint8_t)((TT_MAXP - ov) & 0xFFu);
             int nid = prev;
             if (ov != 0 && ov <= 0xFFu && opp != prev && prev != TT_MAXP) {
@@ -435,7 +445,7 @@ __global__ __launch_bounds__(256) void k_resolve(tt_dev d, const uint32_t *iterp
             // movement (processMovement, :586-609)
             const tt_plate_step pp = ps[P[0]];
             idW[c] = (uint8_t)P[0];
-            hW[c] = height[(size_t)tt_wrapg(y - pp.oy, H) * W + tt_wrapg(x - pp.ox, W)];
+            hW[c] = height[(size_t)tt_wrap_near(y - pp.oy, H) * W + tt_wrap_near(x - pp.ox, W)];
             if (do_uplift) srcFlag[c] = 0;
             continue;
         }
@@ -443,10 +453,10 @@ __global__ __launch_bounds__(256) void k_resolve(tt_dev d, const uint32_t *iterp
         tt_plate_step PD[4];
         for (int s = 0; s < 4; ++s) PD[s] = ps[P[s]];  // ps[255] is the default plate
         float prevH[4];
-        prevH[0] = height[(size_t)tt_wrapg(y - PD[0].oy, H) * W + tt_wrapg(x - PD[0].ox, W)];
-        prevH[1] = height[(size_t)tt_wrapg(y - PD[1].oy, H) * W + tt_wrapg(x - PD[1].ox, W)];
-        prevH[2] = P[2] != TT_MAXP ? height[(size_t)tt_wrapg(y - PD[2].oy, H) * W + tt_wrapg(x - PD[2].ox, W)] : 0.0f;
-        prevH[3] = P[3] != TT_MAXP ? height[(size_t)tt_wrapg(y - PD[3].oy, H) * W + tt_wrapg(x - PD[3].ox, W)] : 0.0f;
+        prevH[0] = height[(size_t)tt_wrap_near(y - PD[0].oy, H) * W + tt_wrap_near(x - PD[0].ox, W)];
+        prevH[1] = height[(size_t)tt_wrap_near(y - PD[1].oy, H) * W + tt_wrap_near(x - PD[1].ox, W)];
+        prevH[2] = P[2] != TT_MAXP ? height[(size_t)tt_wrap_near(y - PD[2].oy, H) * W + tt_wrap_near(x - PD[2].ox, W)] : 0.0f;
+        prevH[3] = P[3] != TT_MAXP ? height[(size_t)tt_wrap_near(y - PD[3].oy, H) * W + tt_wrap_near(x - PD[3].ox, W)] : 0.0f;
         unsigned types[6] = {0, 0, 0, 0, 0, 0};
         types[0] = tt_det_type(d, typeBitmap, P[0], P[1], prevH[0], prevH[1]);
         if (P[2] != TT_MAXP) {
@@ -517,14 +527,14 @@ __global__ __launch_bounds__(256) void k_resolve(tt_dev d, const uint32_t *iterp
             if (ia == own && t == 4) {
                 const int bx = x - PD[ib].ox, by = y - PD[ib].oy;
                 size_t tgt;
-                if (bx != x || by != y) tgt = (size_t)tt_wrapg(by, H) * W + tt_wrapg(bx, W);
-                else tgt = (size_t)tt_wrapg(y + PD[ia].oy, H) * W + tt_wrapg(x + PD[ia].ox, W);
+                if (bx != x || by != y) tgt = (size_t)tt_wrap_near(by, H) * W + tt_wrap_near(bx, W);
+                else tgt = (size_t)tt_wrap_near(y + PD[ia].oy, H) * W + tt_wrap_near(x + PD[ia].ox, W);
                 atomicMax(&accw[tgt], (uint32_t)(c + 1));
             } else if (ib == own && t == 3) {
                 const int ax_ = x - PD[ia].ox, ay_ = y - PD[ia].oy;
                 size_t tgt;
-                if (ax_ != x || ay_ != y) tgt = (size_t)tt_wrapg(ay_, H) * W + tt_wrapg(ax_, W);
-                else tgt = (size_t)tt_wrapg(y + PD[ib].oy, H) * W + tt_wrapg(x + PD[ib].ox, W);
+                if (ax_ != x || ay_ != y) tgt = (size_t)tt_wrap_near(ay_, H) * W + tt_wrap_near(ax_, W);
+                else tgt = (size_t)tt_wrap_near(y + PD[ib].oy, H) * W + tt_wrap_near(x + PD[ib].ox, W);
                 atomicMax(&accw[tgt], (uint32_t)(c + 1));
             }
             if (ia == own) {
@@ -554,12 +564,12 @@ __global__ __launch_bounds__(256) void k_resolve(tt_dev d, const uint32_t *iterp
             if (hc >= d.ks.continentalPropThreshold) {
                 int tx, ty;
                 tt_cont_prop(d, height, x, y, ps[prim].dirx, ps[prim].diry, ps[prim].pcx, ps[prim].pcy, hc, tx, ty);
-                if (tx != x || ty != y) { remote = true; contested[(size_t)tt_wrapg(ty, H) * W + tt_wrapg(tx, W)] = 1; }
+                if (tx != x || ty != y) { remote = true; contested[(size_t)tt_wrap_near(ty, H) * W + tt_wrap_near(tx, W)] = 1; }
                 for (int ci = 0; ci < 4; ++ci) {
                     const int sec = tt_slot_plate(cv, ci);
                     if (sec == TT_MAXP || sec == prim) continue;
                     tt_cont_prop(d, height, x, y, ps[sec].dirx, ps[sec].diry, ps[sec].pcx, ps[sec].pcy, hc, tx, ty);
-                    if (tx != x || ty != y) { remote = true; contested[(size_t)tt_wrapg(ty, H) * W + tt_wrapg(tx, W)] = 1; }
+                    if (tx != x || ty != y) { remote = true; contested[(size_t)tt_wrap_near(ty, H) * W + tt_wrap_near(tx, W)] = 1; }
                 }
             }
             srcFlag[c] = remote ? 2 : 1;

@@ -87,6 +87,11 @@ __device__ __forceinline__ int tt_wrapg(int v, int n)
     int r = v % n;
     return r + (n & (r >> 31));
 }
+// the general wrap, with a conditional add / subtract when -n <= v < 2n
+__device__ __forceinline__ int tt_wrap_near(int v, int n)
+{
+    return ((unsigned)(v + n) < (unsigned)(3 * n)) ? tt_wrap1(v, n) : tt_wrapg(v, n);
+}
 
 // XCD-aware block order: the dispatcher hands consecutive block ids to the 8
 // XCDs round-robin; remapping so that each XCD receives a contiguous range of

@@ -283,13 +283,9 @@ __device__ __forceinline__ float tt_sqrt_wave(float x, int ok)
 
 // bilinear sample with toroidal wrap (interpolate, cuda_helper.cuh:49-65).
 // |v| <= 50 keeps the sample column within [-51, W + 50], where a single
-// conditional wrap is exact; other columns (non-finite velocities) take the
-// general modulo.  IEEE floor is exact, so the hardware floor equals the
-// oracle's bit-level floor.
-__device__ __forceinline__ int tt_wrap_near(int v, int n)
-{
-    return ((unsigned)(v + n) < (unsigned)(3 * n)) ? tt_wrap1(v, n) : tt_wrapg(v, n);
-}
+// conditional wrap is exact (tt_wrap_near); other columns (non-finite
+// velocities) take the general modulo.  IEEE floor is exact, so the hardware
+// floor equals the oracle's bit-level floor.
 
 // the four taps and the weights of a sample, split from the mix so that a
 // caller can issue the tap loads early
