@@ -766,6 +766,11 @@ __global__ __launch_bounds__(256) void k_count(tt_dev d, const int *mergeIds, co
         tt_count_run R = tt_count_run{0, 0, 0, 0, 0, 0, 0, 0u, 0u, 0u};
         int cur = -1;
         const double twopi = 6.283185307179586;
+        // neighbouring cells sample the same noise texel (16 cells per texel
+        // at 4096 columns): the texel's vector is computed when it changes
+        const int sy = (int)((double)y * d.ays);
+        int csx = -1;
+        float cax = 0.0f, cay = 0.0f;
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const int x = x0 + j;
@@ -785,10 +790,13 @@ __global__ __launch_bounds__(256) void k_count(tt_dev d, const int *mergeIds, co
                                  (unsigned)(nb3 != pid);
             if (mode == 0) {
                 R.per += cnt;
-                float ax, ay;
-                tt_asth_at(noise, x, y, d.axs, d.ays, ax, ay);
-                R.ax += tt_fix(ax, TT_ASTH_BITS);
-                R.ay += tt_fix(ay, TT_ASTH_BITS);
+                const int sx = (int)((double)x * d.axs);
+                if (sx != csx) {
+                    tt_asth_texel(noise, sx, sy, cax, cay);
+                    csx = sx;
+                }
+                R.ax += tt_fix(cax, TT_ASTH_BITS);
+                R.ay += tt_fix(cay, TT_ASTH_BITS);
                 if (x % 10 == 0 && y % 10 == 0) {
                     const double agx = (twopi * (double)x) / (double)W, agy = (twopi * (double)y) / (double)H;
                     R.s0 += tt_fix_d(tt_sin_d(agx), TT_ANG_BITS);

@@ -109,13 +109,18 @@ __device__ __forceinline__ float tt_noise_at(const uint8_t *noise, int x, int y)
 {
     return (float)noise[tt_wrapg(y, 256) * 256 + tt_wrapg(x, 256)] / 255.0f;
 }
-__device__ __forceinline__ void tt_asth_at(const uint8_t *noise, int x, int y, double xs, double ys, float &ax, float &ay)
+// the vector of noise texel (sx, sy); cell (x, y) samples texel
+// ((int)(x * 256 / W), (int)(y * 256 / H)) in double
+__device__ __forceinline__ void tt_asth_texel(const uint8_t *noise, int sx, int sy, float &ax, float &ay)
 {
-    const int sx = (int)((double)x * xs), sy = (int)((double)y * ys);
     const float left = tt_noise_at(noise, sx - 1, sy), right = tt_noise_at(noise, sx + 1, sy);
     const float bottom = tt_noise_at(noise, sx, sy - 1), top = tt_noise_at(noise, sx, sy + 1);
     ay = (float)((double)(right - left) * 0.5);
     ax = (float)(-((double)(top - bottom) * 0.5));
+}
+__device__ __forceinline__ void tt_asth_at(const uint8_t *noise, int x, int y, double xs, double ys, float &ax, float &ay)
+{
+    tt_asth_texel(noise, (int)((double)x * xs), (int)((double)y * ys), ax, ay);
 }
 
 // ---------------------------------------------------------------------------
