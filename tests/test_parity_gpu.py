@@ -280,6 +280,42 @@ def test_pipe_length_division_matches_oracle(sim, O, L):
     assert seen[-1].hydraulic_sqrt_fast == 1
 
 
+@pytest.mark.parametrize("scale", [1e-38, 0.0])
+def test_extreme_magnitudes_take_the_ieee_paths(sim, O, scale):
+    """Heights and water of tiny magnitude (subnormal differences), or a flat
+    world (height 5, no water: zero differences): the flux numerators and
+    square-root arguments leave the ranges of the checked fast forms, so the
+    wavefronts take IEEE division and square root; the bits still match the
+    oracle."""
+    W = H = 128
+    cfg, ks = configs.synthetic(W, H, 2, (0.5, 1.0), 1, hydraulic=4, thermal=2, ccl_every=1)
+    cfg.init_mode = _abi.TT_INIT_EMPTY
+    cfg.stage_mask = _abi.TT_STAGE_THERMAL | _abi.TT_STAGE_HYDRAULIC
+    ks.hydrationRainfall = 0.0
+    rng = np.random.default_rng(5)
+    if scale > 0.0:
+        h = (rng.uniform(0.0, 1.0, (H, W)) * scale).astype(np.float32)
+        water = (rng.uniform(0.0, 1.0, (H, W)) * scale).astype(np.float32)
+    else:
+        h = np.full((H, W), 5.0, np.float32)
+        water = np.zeros((H, W), np.float32)
+    say(f"[extreme {scale}] init")
+    g, o = pair(sim, O, cfg, ks)
+    try:
+        for wld in (g, o):
+            put(wld, _abi.FIELD_HEIGHT, h.ravel())
+            put(wld, _abi.FIELD_WATER, water.ravel())
+        for s in range(2):
+            say(f"[extreme {scale}] step {s}")
+            g.step(1)
+            o.step(1)
+            bad = diff_fields(g, o, ALL_FIELDS)
+            assert not bad, f"scale {scale} step {s}: {bad}"
+    finally:
+        g.close()
+        o.close()
+
+
 def test_resume_from_transferred_state(sim, O):
     """A GPU world's state handed to the oracle mid-run continues identically
     (snapshot/resume through the field upload/download interface)."""
