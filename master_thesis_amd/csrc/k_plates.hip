@@ -394,6 +394,50 @@ hipError_t tt_launch_frag_sort(hipStream_t s, unsigned long long *fragKey, const
     return hipGetLastError();
 }
 
+// One phase of the all-ascending bitonic network over the first n keys; keys
+// at indices >= n count as +infinity.  Every compare-exchange puts the smaller
+// key at the lower index, so a real key never moves to an index >= n and the
+// virtual ones never move below it.  flip: the pairs (b k + o, b k + k - 1 - o)
+// that merge two sorted halves of a block of k; otherwise the pairs (i, i + j)
+// in blocks of 2j.
+__global__ __launch_bounds__(256) void k_sort_phase(unsigned long long *a, const uint32_t *np, long long pairs,
+                                                    long long k, long long j, int flip)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= pairs) return;
+    const long long n = (long long)*np;
+    long long i, l;
+    if (flip) {
+        const long long h = k >> 1, b = t / h, o = t - b * h;
+        i = b * k + o;
+        l = b * k + k - 1 - o;
+    } else {
+        const long long b = t / j, o = t - b * j;
+        i = b * 2 * j + o;
+        l = i + j;
+    }
+    if (l >= n) return;
+    const unsigned long long x = a[i], y = a[l];
+    if (y < x) {
+        a[i] = y;
+        a[l] = x;
+    }
+}
+
+hipError_t tt_sort_keys_large(hipStream_t s, unsigned long long *key, const uint32_t *nroots, uint32_t n)
+{
+    long long cap = 2;
+    while (cap < (long long)n) cap <<= 1;
+    const long long pairs = cap >> 1;
+    const unsigned blocks = (unsigned)((pairs + 255) / 256);
+    for (long long k = 2; k <= cap; k <<= 1) {
+        hipLaunchKernelGGL(k_sort_phase, dim3(blocks), dim3(256), 0, s, key, nroots, pairs, k, 0LL, 1);
+        for (long long j = k >> 2; j > 0; j >>= 1)
+            hipLaunchKernelGGL(k_sort_phase, dim3(blocks), dim3(256), 0, s, key, nroots, pairs, k, j, 0);
+    }
+    return hipGetLastError();
+}
+
 // createLabelIdLookup in canonical serial order
 __global__ __launch_bounds__(256) void k_frag_assign(tt_dev d, const unsigned long long *fragKey,
                                                      const uint32_t *nroots, const int *maxCount, const uint8_t *ids,

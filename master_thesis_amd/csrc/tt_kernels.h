@@ -88,6 +88,8 @@ hipError_t tt_launch_frag_build(hipStream_t s, const tt_dev &d, const uint32_t *
                                 uint32_t nroots_host, const uint32_t *rootCount, const uint8_t *ids, unsigned long long *fragKey,
                                 int *maxCount, uint8_t *newId);
 hipError_t tt_launch_frag_sort(hipStream_t s, unsigned long long *fragKey, const uint32_t *nroots, int capacity);
+// ascending sort of the first *nroots keys (n on the host) for more keys than one block holds
+hipError_t tt_sort_keys_large(hipStream_t s, unsigned long long *key, const uint32_t *nroots, uint32_t n);
 hipError_t tt_launch_frag_assign(hipStream_t s, const tt_dev &d, const unsigned long long *fragKey,
                                  const uint32_t *nroots, const int *maxCount, const uint8_t *ids, tt_plate_data *pd,
                                  uint8_t *newId);

@@ -279,12 +279,7 @@ static int run_ccl(tt_world *w)
         while ((uint32_t)cap < nroots) cap <<= 1;
         CK(tt_launch_frag_sort(s, w->fragKey, w->nroots, cap));
     } else {
-        std::vector<unsigned long long> hk(nroots);
-        CK(hipMemcpyAsync(hk.data(), w->fragKey, (size_t)nroots * 8, hipMemcpyDeviceToHost, s));
-        CK(hipStreamSynchronize(s));
-        std::sort(hk.begin(), hk.end());
-        CK(hipMemcpyAsync(w->fragKey, hk.data(), (size_t)nroots * 8, hipMemcpyHostToDevice, s));
-        CK(hipStreamSynchronize(s));
+        CK(tt_sort_keys_large(s, w->fragKey, w->nroots, nroots));
     }
     CK(tt_launch_frag_assign(s, d, w->fragKey, w->nroots, w->maxCount, ids, w->pd, w->newId));
     CK(tt_launch_relabel(s, d, w->labels, w->newId, ids, w->unFlag));

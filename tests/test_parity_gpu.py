@@ -316,6 +316,38 @@ def test_extreme_magnitudes_take_the_ieee_paths(sim, O, scale):
         o.close()
 
 
+def test_ccl_with_thousands_of_components(sim, O):
+    """Random plate ids on a 256x256 grid leave about 20,000 CCL components,
+    more than one block sorts in LDS: the fragment order comes from the
+    multi-pass bitonic network, and the relabel and fill match the oracle."""
+    W = H = 256
+    cfg, ks = configs.synthetic(W, H, 2, (0.5, 1.0), 1, hydraulic=1, thermal=1, ccl_every=1)
+    cfg.init_mode = _abi.TT_INIT_EMPTY
+    cfg.stage_mask = _abi.TT_STAGE_CCL
+    rng = np.random.default_rng(17)
+    ids = rng.integers(0, 12, (H, W)).astype(np.uint8)
+    say("[many components] init")
+    g, o = pair(sim, O, cfg, ks)
+    try:
+        pd = get(o, _abi.FIELD_PLATE_DATA)
+        for p in range(12):
+            pd[p]["size"] = int((ids == p).sum())
+            pd[p]["used"] = 1
+        for wld in (g, o):
+            put(wld, _abi.FIELD_PLATE_IDS, ids.ravel())
+            put(wld, _abi.FIELD_PLATE_DATA, pd)
+        say("[many components] step")
+        g.step(1)
+        o.step(1)
+        bad = diff_fields(g, o, ALL_FIELDS)
+        assert not bad, bad
+        assert stats_tuple(g.stats()) == stats_tuple(o.stats())
+        assert o.stats().num_labels > 4096
+    finally:
+        g.close()
+        o.close()
+
+
 def test_resume_from_transferred_state(sim, O):
     """A GPU world's state handed to the oracle mid-run continues identically
     (snapshot/resume through the field upload/download interface)."""
