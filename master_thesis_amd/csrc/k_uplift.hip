@@ -210,7 +210,10 @@ __device__ __forceinline__ int tt_tdist2_pk(int p, int s, int wh)
 // taps at x - s, x, x + s are whole groups; for steps 1..3 each row loads the
 // groups at x - 4, x and x + 4 and the taps are picked from that 12-cell
 // window.  Each cell scans its own candidates in the same order.
-template <bool WINDOW>
+// SX: 0 when the column offset is a multiple of 4 (taps are whole groups);
+// 1, 2 or 3 for the window form (the offset fixed at compile time, so each tap
+// is a fixed window entry)
+template <int SX>
 __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restrict__ in, int4 *__restrict__ out,
                                                    int step)
 {
@@ -222,7 +225,8 @@ __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restr
     const int wh = (H << 16) | W;
     const int sx = step % W, sy = step % H;
     const int rows[3] = {tt_wrap1(y - sy, H) * W, y * W, tt_wrap1(y + sy, H) * W};
-    // WINDOW: groups at x - 4, x, x + 4; otherwise at x - s, x, x + s
+    // window: groups at x - 4, x, x + 4; otherwise at x - s, x, x + s
+    constexpr bool WINDOW = SX > 0;
     const int gs = WINDOW ? 4 : sx;
     const int cols[3] = {tt_wrap1(x - gs, W), x, tt_wrap1(x + gs, W)};
     int t[3][12];
@@ -250,12 +254,8 @@ __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restr
                 if (ox == 1 && oy == 1) continue;
                 int cand;
                 if (WINDOW) {
-                    // window index 4 + k + (ox - 1) * sx, in [1, 10] for sx <= 3
-                    const int wi = 4 + k + (ox - 1) * sx;
-                    cand = t[oy][0];
-#pragma unroll
-                    for (int j = 1; j < 12; ++j)
-                        if (j == wi) cand = t[oy][j];
+                    // window index 4 + k + (ox - 1) * SX, in [1, 10]
+                    cand = t[oy][4 + k + (ox - 1) * SX];
                 } else {
                     cand = t[oy][4 * ox + k];
                 }
@@ -344,9 +344,13 @@ hipError_t tt_launch_jfa_pass(hipStream_t s, const tt_dev &d, const int *in, int
         hipLaunchKernelGGL(k_jfa_chain<2>, dim3((blocks + 1) / 2), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out,
                            step);
     else if (sx % 4 == 0)
-        hipLaunchKernelGGL(k_jfa_pass4<false>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
-    else if (sx < 4)
-        hipLaunchKernelGGL(k_jfa_pass4<true>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
+        hipLaunchKernelGGL(k_jfa_pass4<0>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
+    else if (sx == 1)
+        hipLaunchKernelGGL(k_jfa_pass4<1>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
+    else if (sx == 2)
+        hipLaunchKernelGGL(k_jfa_pass4<2>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
+    else if (sx == 3)
+        hipLaunchKernelGGL(k_jfa_pass4<3>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
