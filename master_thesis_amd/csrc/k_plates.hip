@@ -1324,38 +1324,107 @@ hipError_t tt_launch_flood_cc(hipStream_t s, const tt_dev &d, uint8_t *ids, int 
     return hipGetLastError();
 }
 
-// BacktrackPath (plate_tectonics_kernel.cu:2883-2983), one thread
-__global__ void k_backtrack(tt_dev d, const float *eff, uint8_t *ids, int px, int py, int newId, tt_step_ctl *ctl)
+// BacktrackPath (plate_tectonics_kernel.cu:2883-2983).  The walks are serial:
+// each step depends on the last.  A step reads the cell and its eight
+// neighbours, so instead of one memory round trip per step one wavefront keeps
+// a 64 x 64 window of efforts and plate ids in LDS (one round trip of 16-byte
+// and 4-byte loads fills it) and walks inside it; the window is re-centred on
+// the walk when the walk reaches its border, about every 30 steps.  All 64
+// lanes walk the same path in lockstep from the same LDS values; every lane
+// writes a mark to the window, lane 0 alone to global memory.  The window's
+// left column is a multiple of 4, so a 4-cell group never straddles the wrap
+// (W is a multiple of 64).  Requires W, H >= 64 (no cell appears twice in a
+// window).
+#define TT_BT_WIN 64
+struct tt_bt_win {
+    float e[TT_BT_WIN][TT_BT_WIN];
+    uint8_t id[TT_BT_WIN][TT_BT_WIN];
+};
+
+// fill the window whose top-left cell is the wrapped cell (wxw, wyw), wxw % 4 == 0
+__device__ __forceinline__ void tt_bt_fill(tt_bt_win &S, const tt_dev &d, const float *eff, const uint8_t *ids,
+                                           int wxw, int wyw)
 {
-    if (threadIdx.x != 0) return;
+    __syncthreads();  // the marks stored so far are visible to the loads
+    const int lane = threadIdx.x;
+    const int g = lane & 15, r0 = lane >> 4;
+    int col = wxw + 4 * g;
+    col -= (col >= d.W) ? d.W : 0;
+    float4 e[16];
+    uint32_t v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        int row = wyw + r0 + 4 * k;
+        row -= (row >= d.H) ? d.H : 0;
+        const size_t q = (size_t)row * d.W + col;
+        e[k] = *(const float4 *)(eff + q);
+        v[k] = *(const uint32_t *)(ids + q);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        *(float4 *)&S.e[r0 + 4 * k][4 * g] = e[k];
+        *(uint32_t *)&S.id[r0 + 4 * k][4 * g] = v[k];
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void k_backtrack(tt_dev d, const float *eff, uint8_t *ids, int px, int py, int newId,
+                                                  tt_step_ctl *ctl)
+{
+    __shared__ tt_bt_win S;
     const int W = d.W, H = d.H;
+    const bool t0 = threadIdx.x == 0;
     const float INF = tt_u2f(0x7f800000u);
     const int OFF[8][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}, {-1, -1}, {-1, 1}, {1, -1}, {1, 1}};
     auto at = [&](int x, int y) { return (size_t)tt_wrapg(y, H) * W + tt_wrapg(x, W); };
     const uint8_t oldAtPoint = ids[at(px, py)];
     int cx = px, cy = py;
+    // window origin: unwrapped (wx, wy) and wrapped (wxw, wyw)
+    int wx = 0, wy = 0, wxw = 0, wyw = 0;
+    auto recentre = [&]() {
+        wx = (cx - TT_BT_WIN / 2) & ~3;
+        wy = cy - TT_BT_WIN / 2;
+        wxw = tt_wrapg(wx, W);
+        wyw = tt_wrapg(wy, H);
+        tt_bt_fill(S, d, eff, ids, wxw, wyw);
+    };
+    recentre();
+    // re-centre the window when (cx, cy) and its neighbours are not inside it
+    auto cover = [&]() {
+        const int lx = cx - wx, ly = cy - wy;
+        if (lx < 1 || lx > TT_BT_WIN - 2 || ly < 1 || ly > TT_BT_WIN - 2) recentre();
+    };
+    auto mark = [&](int lx, int ly) {
+        S.id[ly][lx] = (uint8_t)TT_MAXP;
+        if (t0) {
+            int c = wxw + lx, r = wyw + ly;
+            c -= (c >= W) ? W : 0;
+            r -= (r >= H) ? H : 0;
+            ids[(size_t)r * W + c] = (uint8_t)TT_MAXP;
+        }
+    };
     bool dead = false;
     int seed = -1;
-    // Each step issues all of its loads at once (the cell's effort and id, the
-    // neighbours' efforts and, on the second walk, their ids), so a step costs
-    // one memory round trip; the loads are of wrapped, always valid cells.
     // The first walk is memoryless: reaching a cell it already marked means an
     // endless cycle (zero-height plateau); canonical outcome: dead end.
     for (;;) {
-        const size_t c = at(cx, cy);
-        const float ec = eff[c];
-        const uint8_t ic = ids[c];
-        float en[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) en[i] = eff[at(cx + OFF[i][0], cy + OFF[i][1])];
+        cover();
+        const int lx = cx - wx, ly = cy - wy;
+        const float ec = S.e[ly][lx];
+        const uint8_t ic = S.id[ly][lx];
         if (!(ec > 0.0f)) break;
         if (ic == TT_MAXP) { dead = true; break; }
-        ids[c] = (uint8_t)TT_MAXP;
+        mark(lx, ly);
         float best = INF;
         int nx = cx, ny = cy;
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (en[i] < best) { best = en[i]; nx = cx + OFF[i][0]; ny = cy + OFF[i][1]; }
+        for (int i = 0; i < 8; ++i) {
+            const float en = S.e[ly + OFF[i][1]][lx + OFF[i][0]];
+            const bool take = en < best;
+            best = take ? en : best;
+            nx = take ? cx + OFF[i][0] : nx;
+            ny = take ? cy + OFF[i][1] : ny;
+        }
         if (best == INF) { dead = true; break; }
         cx = nx;
         cy = ny;
@@ -1368,43 +1437,44 @@ __global__ void k_backtrack(tt_dev d, const float *eff, uint8_t *ids, int px, in
         const float odx = obx / om, ody = oby / om;
         pxf = -ody;
         pyf = odx;
+        // the direction bias of each neighbour is the same at every step
+        float bias[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float ofx = (float)OFF[i][0], ofy = (float)OFF[i][1];
+            const float ofm = sqrtf(ofx * ofx + ofy * ofy);
+            const float tdx = ofx / ofm, tdy = ofy / ofm;
+            const float dot = tdx * odx + tdy * ody;
+            bias[i] = -dot * 50.0f;
+        }
         cx = px;
         cy = py;
         for (;;) {
-            const size_t c = at(cx, cy);
-            const float ec = eff[c];
-            float en[8];
-            uint8_t in8[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const size_t q = at(cx + OFF[i][0], cy + OFF[i][1]);
-                en[i] = eff[q];
-                in8[i] = ids[q];
-            }
+            cover();
+            const int lx = cx - wx, ly = cy - wy;
+            const float ec = S.e[ly][lx];
             if (!(ec > 0.0f)) break;
-            ids[c] = (uint8_t)TT_MAXP;
+            mark(lx, ly);
             float best = INF;
             int nx = cx, ny = cy;
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const int qx = cx + OFF[i][0], qy = cy + OFF[i][1];
-                if (in8[i] == TT_MAXP) continue;
+                const uint8_t iq = S.id[ly + OFF[i][1]][lx + OFF[i][0]];
+                const float dv = S.e[ly + OFF[i][1]][lx + OFF[i][0]];
                 const float tnx = (float)(qx - px), tny = (float)(qy - py);
                 const float pdot = tnx * odx + tny * ody;
-                if (pdot < 0.0f) continue;
-                const float dv = en[i];
-                const float ofx = (float)OFF[i][0], ofy = (float)OFF[i][1];
-                const float ofm = sqrtf(ofx * ofx + ofy * ofy);
-                const float tdx = ofx / ofm, tdy = ofy / ofm;
-                const float dot = tdx * odx + tdy * ody;
-                const float bias = -dot * 50.0f;
-                const float score = dv + bias;
-                if (score < best) { best = score; nx = qx; ny = qy; }
+                const float score = dv + bias[i];
+                const bool take = (iq != TT_MAXP) & !(pdot < 0.0f) & (score < best);
+                best = take ? score : best;
+                nx = take ? qx : nx;
+                ny = take ? qy : ny;
             }
             if (best != INF) { cx = nx; cy = ny; }
             else { dead = true; break; }
         }
     }
+    if (!t0) return;
     if (!dead) {
         int sx = px, sy = py;
         for (int i = 1; i <= 100 && ids[at(sx, sy)] != oldAtPoint; ++i) {
@@ -1426,6 +1496,7 @@ __global__ void k_backtrack(tt_dev d, const float *eff, uint8_t *ids, int px, in
 hipError_t tt_launch_backtrack(hipStream_t s, const tt_dev &d, const float *eff, uint8_t *ids, int px, int py,
                                int newId, tt_step_ctl *ctl)
 {
+    if (d.W < TT_BT_WIN || d.H < TT_BT_WIN) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_backtrack, dim3(1), dim3(64), 0, s, d, eff, ids, px, py, newId, ctl);
     return hipGetLastError();
 }
