@@ -126,6 +126,19 @@ def test_splits(sim, O):
     assert any(s.split_plate >= 0 for s in seen)
 
 
+@pytest.mark.parametrize("W,H", [(64, 64), (192, 64), (64, 192)])
+def test_split_walks_on_narrow_worlds(sim, O, W, H):
+    """Splits on worlds 64 cells across: the cut-line walk's 64 x 64 window
+    spans the whole side, and the walk wraps around the torus and is
+    re-centred along the long side."""
+    cfg, ks = configs.reference_defaults()
+    cfg.width, cfg.height, cfg.num_plates, cfg.seed = W, H, 3, 5
+    cfg.num_voronoi_gens = 0
+    ks.targetMaximumPlateArea = 300
+    seen = run_parity(sim, O, cfg, ks, 8, f"split walks {W}x{H}")
+    assert any(s.split_plate >= 0 for s in seen)
+
+
 def test_merges(sim, O):
     cfg, ks = configs.synthetic(128, 64, 10, (0.2, 0.6), 4, hydraulic=3, thermal=2, ccl_every=1)
     ks.mergeMinVelocity = 1.0
