@@ -122,20 +122,35 @@ __device__ __forceinline__ unsigned tt_ccl_edges(uint8_t id, uint8_t tl, uint8_t
 }
 
 // union-find in LDS (32-bit parents), linking the larger root under the
-// smaller with atomicMin as in unionStep (ccl.cu:60-85)
-__device__ __forceinline__ unsigned tt_find32(const unsigned *par, unsigned a)
+// smaller with atomicMin as in unionStep (ccl.cu:60-85).  A parent is never
+// larger than its child and always lies in the child's component, so the root
+// of a component is its smallest cell whatever the order of the links.
+//
+// Find with path halving: each visited node is pointed at its grandparent, an
+// ancestor in the same component, so chains stay inside their components and
+// shrink.  A concurrent atomicMin on the node (by a thread that found it as a
+// root before it was linked) may be overwritten; that thread then retries its
+// union from the node's old parent, so the link is made there.  Without
+// halving, a row of one id links into a chain of its whole length and the
+// tile's finds walk chains of up to a thousand cells.
+__device__ __forceinline__ unsigned tt_find32h(unsigned *par, unsigned a)
 {
-    unsigned p = par[a];
-    while (p != a) { a = p; p = par[a]; }
-    return a;
+    for (;;) {
+        const unsigned p = par[a];
+        if (p == a) return a;
+        const unsigned gp = par[p];
+        if (gp == p) return p;
+        par[a] = gp;
+        a = gp;
+    }
 }
 
 __device__ __forceinline__ void tt_union32(unsigned *par, unsigned a, unsigned b)
 {
     bool done = false;
     while (!done) {
-        a = tt_find32(par, a);
-        b = tt_find32(par, b);
+        a = tt_find32h(par, a);
+        b = tt_find32h(par, b);
         if (a < b) {
             const unsigned old = atomicMin(&par[b], a);
             done = old == b;
@@ -207,7 +222,7 @@ __global__ __launch_bounds__(256) void k_ccl_local(tt_dev d, const uint8_t *ids,
     }
     __syncthreads();
     for (int i = tid; i < TT_TILE_H * TT_TILE_W; i += 256) {
-        const unsigned r = tt_find32(par, (unsigned)i);
+        const unsigned r = tt_find32h(par, (unsigned)i);
         atomicAdd(&cnt[r], 1u);
         const int ry = (int)(r / TT_TILE_W), rx = (int)(r % TT_TILE_W);
         const int ly = i / TT_TILE_W, lx = i - ly * TT_TILE_W;
@@ -1247,7 +1262,7 @@ __global__ __launch_bounds__(256) void k_fl_local(tt_dev d, const uint8_t *ids, 
             rootFlag[c] = 0;
             continue;
         }
-        const unsigned r = tt_find32(par, (unsigned)i);
+        const unsigned r = tt_find32h(par, (unsigned)i);
         lab[c] = (uint32_t)((size_t)(y0 + (int)(r >> 6)) * W + x0 + (int)(r & 63));
         rootFlag[c] = r == (unsigned)i ? 1 : 0;
     }
