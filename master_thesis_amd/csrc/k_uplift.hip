@@ -202,6 +202,22 @@ __device__ __forceinline__ int tt_tdist2_pk(int p, int s, int wh)
     return __builtin_amdgcn_sdot2(dd, dd, 0, false);
 }
 
+// Power-of-two sides: u = (a - b) mod n is the 16-bit difference masked to
+// n - 1, and min(u, n - u) equals min(|a - b|, n - |a - b|) (for a - b < 0,
+// u = n - |a - b|), one instruction fewer than the absolute value.
+typedef unsigned short tt_u2 __attribute__((ext_vector_type(2)));
+template <bool P2>
+__device__ __forceinline__ int tt_tdist2(int p, int s, int wh)
+{
+    if (!P2) return tt_tdist2_pk(p, s, wh);
+    const tt_u2 a = __builtin_bit_cast(tt_u2, p), b = __builtin_bit_cast(tt_u2, s);
+    const tt_u2 n = __builtin_bit_cast(tt_u2, wh);
+    const unsigned m = __builtin_bit_cast(unsigned, n - (unsigned short)1);
+    const tt_u2 u = __builtin_bit_cast(tt_u2, __builtin_bit_cast(unsigned, a - b) & m);
+    const tt_s2 t = __builtin_bit_cast(tt_s2, __builtin_elementwise_min(u, n - u));
+    return __builtin_amdgcn_sdot2(t, t, 0, false);
+}
+
 // JFAPass (plate_tectonics_kernel.cu:1512-1571): candidates at (ox, oy) * step
 // in scan order oy = -1..1, ox = -1..1; a strictly smaller squared distance
 // replaces the current seed.  4 consecutive cells of a row per thread: every
@@ -213,7 +229,7 @@ __device__ __forceinline__ int tt_tdist2_pk(int p, int s, int wh)
 // SX: 0 when the column offset is a multiple of 4 (taps are whole groups);
 // 1, 2 or 3 for the window form (the offset fixed at compile time, so each tap
 // is a fixed window entry)
-template <int SX>
+template <int SX, bool P2>
 __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restrict__ in, int4 *__restrict__ out,
                                                    int step)
 {
@@ -246,7 +262,7 @@ __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restr
         const int me = (y << 16) | (x + k);
         int best = t[1][4 + k];
         int bd = 0x7FFFFFFF;
-        if (best >= 0) bd = tt_tdist2_pk(me, best, wh);
+        if (best >= 0) bd = tt_tdist2<P2>(me, best, wh);
 #pragma unroll
         for (int oy = 0; oy < 3; ++oy) {
 #pragma unroll
@@ -260,7 +276,7 @@ __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restr
                     cand = t[oy][4 * ox + k];
                 }
                 if (cand < 0) continue;
-                const int dd = tt_tdist2_pk(me, cand, wh);
+                const int dd = tt_tdist2<P2>(me, cand, wh);
                 if (dd < bd) { bd = dd; best = cand; }
             }
         }
@@ -275,7 +291,7 @@ __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restr
 // of taps instead of 3 R, each row's taps serving up to three of its cells.
 // Rows are grouped into bands of R s rows; chain (band b, offset r0) starts at
 // row b R s + r0, so the chains cover every row once.
-template <int R>
+template <int R, bool P2>
 __global__ __launch_bounds__(256) void k_jfa_chain(tt_dev d, const int4 *__restrict__ in, int4 *__restrict__ out,
                                                    int step)
 {
@@ -312,7 +328,7 @@ __global__ __launch_bounds__(256) void k_jfa_chain(tt_dev d, const int4 *__restr
             const int me = (y << 16) | (x + k);
             int best = t[j + 1][1][k];
             int bd = 0x7FFFFFFF;
-            if (best >= 0) bd = tt_tdist2_pk(me, best, wh);
+            if (best >= 0) bd = tt_tdist2<P2>(me, best, wh);
 #pragma unroll
             for (int oy = 0; oy < 3; ++oy) {
 #pragma unroll
@@ -320,7 +336,7 @@ __global__ __launch_bounds__(256) void k_jfa_chain(tt_dev d, const int4 *__restr
                     if (ox == 1 && oy == 1) continue;
                     const int cand = t[j + oy][ox][k];
                     if (cand < 0) continue;
-                    const int dd = tt_tdist2_pk(me, cand, wh);
+                    const int dd = tt_tdist2<P2>(me, cand, wh);
                     if (dd < bd) { bd = dd; best = cand; }
                 }
             }
@@ -330,6 +346,29 @@ __global__ __launch_bounds__(256) void k_jfa_chain(tt_dev d, const int4 *__restr
     }
 }
 
+template <bool P2>
+static hipError_t launch_jfa(hipStream_t s, const tt_dev &d, const int *in, int *out, int step, int sx, int sy,
+                             unsigned blocks)
+{
+    if (sx % 4 == 0 && sy > 0 && d.H % (4 * sy) == 0)
+        hipLaunchKernelGGL((k_jfa_chain<4, P2>), dim3((blocks + 3) / 4), dim3(256), 0, s, d, (const int4 *)in,
+                           (int4 *)out, step);
+    else if (sx % 4 == 0 && sy > 0 && d.H % (2 * sy) == 0)
+        hipLaunchKernelGGL((k_jfa_chain<2, P2>), dim3((blocks + 1) / 2), dim3(256), 0, s, d, (const int4 *)in,
+                           (int4 *)out, step);
+    else if (sx % 4 == 0)
+        hipLaunchKernelGGL((k_jfa_pass4<0, P2>), dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
+    else if (sx == 1)
+        hipLaunchKernelGGL((k_jfa_pass4<1, P2>), dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
+    else if (sx == 2)
+        hipLaunchKernelGGL((k_jfa_pass4<2, P2>), dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
+    else if (sx == 3)
+        hipLaunchKernelGGL((k_jfa_pass4<3, P2>), dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 hipError_t tt_launch_jfa_pass(hipStream_t s, const tt_dev &d, const int *in, int *out, int step)
 {
     // N is a multiple of 4 (W is a multiple of 64); 16-byte aligned buffers.
@@ -337,23 +376,9 @@ hipError_t tt_launch_jfa_pass(hipStream_t s, const tt_dev &d, const int *in, int
     // (W is a multiple of 64) and is 1 or 2 otherwise.
     const unsigned blocks = (unsigned)(((long long)(d.N >> 2) + 255) / 256);
     const int sx = step % d.W, sy = step % d.H;
-    if (sx % 4 == 0 && sy > 0 && d.H % (4 * sy) == 0)
-        hipLaunchKernelGGL(k_jfa_chain<4>, dim3((blocks + 3) / 4), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out,
-                           step);
-    else if (sx % 4 == 0 && sy > 0 && d.H % (2 * sy) == 0)
-        hipLaunchKernelGGL(k_jfa_chain<2>, dim3((blocks + 1) / 2), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out,
-                           step);
-    else if (sx % 4 == 0)
-        hipLaunchKernelGGL(k_jfa_pass4<0>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
-    else if (sx == 1)
-        hipLaunchKernelGGL(k_jfa_pass4<1>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
-    else if (sx == 2)
-        hipLaunchKernelGGL(k_jfa_pass4<2>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
-    else if (sx == 3)
-        hipLaunchKernelGGL(k_jfa_pass4<3>, dim3(blocks), dim3(256), 0, s, d, (const int4 *)in, (int4 *)out, step);
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
+    const bool p2 = (d.W & (d.W - 1)) == 0 && (d.H & (d.H - 1)) == 0;
+    return p2 ? launch_jfa<true>(s, d, in, out, step, sx, sy, blocks)
+              : launch_jfa<false>(s, d, in, out, step, sx, sy, blocks);
 }
 
 // ---------------------------------------------------------------------------
