@@ -830,6 +830,7 @@ __global__ __launch_bounds__(256) void k_count(tt_dev d, const int *mergeIds, co
         const int sy = (int)((double)y * d.ays);
         int csx = -1;
         float cax = 0.0f, cay = 0.0f;
+        long long fax = 0, fay = 0;  // the texel's vector in fixed point
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const int x = x0 + j;
@@ -852,10 +853,12 @@ __global__ __launch_bounds__(256) void k_count(tt_dev d, const int *mergeIds, co
                 const int sx = (int)((double)x * d.axs);
                 if (sx != csx) {
                     tt_asth_texel(noise, sx, sy, cax, cay);
+                    fax = tt_fix(cax, TT_ASTH_BITS);
+                    fay = tt_fix(cay, TT_ASTH_BITS);
                     csx = sx;
                 }
-                R.ax += tt_fix(cax, TT_ASTH_BITS);
-                R.ay += tt_fix(cay, TT_ASTH_BITS);
+                R.ax += fax;
+                R.ay += fay;
                 if (x % 10 == 0 && y % 10 == 0) {
                     const double agx = (twopi * (double)x) / (double)W, agy = (twopi * (double)y) / (double)H;
                     R.s0 += tt_fix_d(tt_sin_d(agx), TT_ANG_BITS);

@@ -370,12 +370,9 @@ TT_HD long long tt_fix(float v, int bits)
     const double lim = 4611686018427387904.0;  // 2^62
     if (d >= lim) return 4611686018427387904LL;
     if (d <= -lim) return -4611686018427387904LL;
-    // round half to even on a double that already has all bits of v
-    const double ad = tt_fabs(d);
-    long long q = (long long)ad;
-    const double frac = ad - (double)q;
-    if (frac > 0.5 || (frac == 0.5 && (q & 1))) q += 1;
-    return d < 0.0 ? -q : q;
+    // round half to even (rint in the default rounding mode) of a double that
+    // already has all bits of v; |d| < 2^62 converts exactly
+    return (long long)__builtin_rint(d);
 }
 
 TT_HD long long tt_fix_d(double v, int bits)
@@ -385,11 +382,7 @@ TT_HD long long tt_fix_d(double v, int bits)
     const double lim = 4611686018427387904.0;  // 2^62
     if (d >= lim) return 4611686018427387904LL;
     if (d <= -lim) return -4611686018427387904LL;
-    const double ad = tt_fabs(d);
-    long long q = (long long)ad;
-    const double frac = ad - (double)q;
-    if (frac > 0.5 || (frac == 0.5 && (q & 1))) q += 1;
-    return d < 0.0 ? -q : q;
+    return (long long)__builtin_rint(d);
 }
 
 TT_HD float tt_unfix(long long q, int bits) { return (float)((double)q * tt_pow2i(-bits)); }

@@ -956,3 +956,27 @@ def test_reference_init_heights_are_simplex_fbm(O):
             exp[y, x] = F32(F32(5000) + F32(value * F32(10000)))
     np.testing.assert_allclose(got, exp, rtol=1e-6)
     assert (got > 5000).all() and (got < 15000).all()
+
+
+def test_mass_ledger_rounds_half_to_even(O):
+    """The mass ledger is the sum of every height in fixed point at 2^-20,
+    each rounded half to even (tt_fix, shared by the oracle and the device
+    library); checked against numpy's rint, an independent rounding, on
+    heights that include exact halves of the fixed-point step, of both signs."""
+    W = H = 64
+    n = W * H
+    w = empty_world(O, W, H, 0)
+    rng = np.random.default_rng(7)
+    h = rng.uniform(-2e4, 2e4, n).astype(np.float32)
+    k = rng.integers(-2**20, 2**20, n // 4)
+    h[: n // 4] = ((k + 0.5) * 2.0**-20).astype(np.float32)          # exact halves
+    h[n // 4: n // 2] = (k * 2.0**-20).astype(np.float32)            # exact integers
+    w.set(_abi.FIELD_HEIGHT, h)
+    w.step(1)
+    hh = w.get(_abi.FIELD_HEIGHT).astype(np.float64)
+    assert np.count_nonzero(np.abs(hh * 2.0**20 - np.floor(hh * 2.0**20)) == 0.5) > n // 8
+    q = int(np.rint(hh * 2.0**20).astype(np.int64).sum())
+    assert w.stats().mass_ledger == float(q) * 2.0**-20
+    # a rounding of halves away from zero differs on these heights
+    away = int((np.sign(hh) * np.floor(np.abs(hh) * 2.0**20 + 0.5)).astype(np.int64).sum())
+    assert away != q
