@@ -831,6 +831,13 @@ __global__ __launch_bounds__(256) void k_count(tt_dev d, const int *mergeIds, co
         int csx = -1;
         float cax = 0.0f, cay = 0.0f;
         long long fax = 0, fay = 0;  // the texel's vector in fixed point
+        // the row's angle terms of the centre sums, the same for all 16 cells
+        long long fs2 = 0, fs3 = 0;
+        if (mode == 0 && y % 10 == 0) {
+            const double agy = (twopi * (double)y) / (double)H;
+            fs2 = tt_fix_d(tt_sin_d(agy), TT_ANG_BITS);
+            fs3 = tt_fix_d(tt_cos_d(agy), TT_ANG_BITS);
+        }
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const int x = x0 + j;
@@ -860,11 +867,11 @@ __global__ __launch_bounds__(256) void k_count(tt_dev d, const int *mergeIds, co
                 R.ax += fax;
                 R.ay += fay;
                 if (x % 10 == 0 && y % 10 == 0) {
-                    const double agx = (twopi * (double)x) / (double)W, agy = (twopi * (double)y) / (double)H;
+                    const double agx = (twopi * (double)x) / (double)W;
                     R.s0 += tt_fix_d(tt_sin_d(agx), TT_ANG_BITS);
                     R.s1 += tt_fix_d(tt_cos_d(agx), TT_ANG_BITS);
-                    R.s2 += tt_fix_d(tt_sin_d(agy), TT_ANG_BITS);
-                    R.s3 += tt_fix_d(tt_cos_d(agy), TT_ANG_BITS);
+                    R.s2 += fs2;
+                    R.s3 += fs3;
                     R.acnt += 1u;
                 }
             } else if (cnt) {
