@@ -4,6 +4,7 @@ by the library."""
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -12,7 +13,9 @@ import numpy as np
 from master_thesis_amd import _abi
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "liboracle.so"
+# TT_ORACLE_LIB names another build of the same source (a copy with a mistake
+# planted on purpose, to show that the pins in tests/ catch it)
+LIB = Path(os.environ.get("TT_ORACLE_LIB", HERE / "liboracle.so"))
 _lib = None
 
 
