@@ -133,7 +133,7 @@ typedef struct tt_sim_config {
     int32_t height;                /* multiple of 16, >= 64 */
     uint32_t seed;
     int32_t num_plates;            /* 1..254 */
-    int32_t num_voronoi_gens;      /* TT_INIT_REFERENCE only, 0..4 */
+    int32_t num_voronoi_gens;      /* TT_INIT_REFERENCE only, 0..4 (longer lists: tt_world_reset) */
     int32_t voronoi_gens[4];
     int32_t iterations_for_ccl;    /* CCL when step % iterations_for_ccl == 0 */
     int32_t hydraulic_iterations;  /* default 10 */
@@ -264,11 +264,19 @@ int tt_config_parse_json(const char *text, tt_sim_config *cfg, tt_kernel_setting
 int tt_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, tt_world **out);
 /* initialize(numStartingPlates, numVoronoiSeeds), plate_tectonic_sim.h:115 */
 int tt_world_init(tt_world *w);
-/* resetSim(seed, numStartingPlates, numVoronoiSeeds), plate_tectonic_sim.h:119 */
-int tt_world_reset(tt_world *w, uint32_t seed, int32_t num_plates);
+/* resetSim(seed, numStartingPlates, numVoronoiSeeds), plate_tectonic_sim.h:119:
+ * re-initialises the world with a new seed, plate count and list of Voronoi
+ * generations (num_gens entries, any number; each at least 1 seed, the last at
+ * most 65536).  With the world's own seed this is also
+ * initialize(numStartingPlates, numVoronoiSeeds), plate_tectonic_sim.h:115, for
+ * lists longer than the 4 slots of tt_sim_config. */
+int tt_world_reset(tt_world *w, uint32_t seed, int32_t num_plates, const int32_t *voronoi_gens, int32_t num_gens);
 /* executeIteration() x n, plate_tectonic_sim.h:117; returns after the device
  * work is complete (the reference synchronises at sim.cu:117-119). */
 int tt_world_step(tt_world *w, int32_t n);
+/* The world's configuration and settings (e.g. of a world made by
+ * tt_world_load or a server job); either pointer may be NULL. */
+int tt_world_config(tt_world *w, tt_sim_config *cfg, tt_kernel_settings *ks);
 /* Upload of new KernelSettings (copyHostKernelSettingsToDevice, kernel_settings.cu:7-9). */
 int tt_world_set_settings(tt_world *w, const tt_kernel_settings *ks);
 int tt_world_download(tt_world *w, int32_t field, void *host, size_t bytes);
@@ -302,6 +310,40 @@ int tt_world_diagnostics_json(tt_world *w, char *buf, size_t cap, size_t *len);
  * loaded from a snapshot continues bit-identically. */
 int tt_world_save(tt_world *w, const char *path);
 int tt_world_load(const char *path, tt_world **out);
+
+/* ---- host-side generators (the static members of PlateTectonicSim,
+ * plate_tectonic_sim.h:121-130) ------------------------------------------------
+ * The reference draws from std::default_random_engine, whose stream is
+ * implementation-defined (MSVC); these draw from SplitMix64 instead.
+ * *rng_state is the generator state, advanced by every draw; a world's
+ * initialisation starts from *rng_state = seed and calls
+ * tt_generate_plate_centers, then tt_generate_voronoi_seeds per generation,
+ * then tt_generate_plate_data, in that order.  Each U[0, b) draw is
+ * (next() >> 40) * 2^-24 * b in float.  No device work. */
+
+/* struct VoronoiSeed (types/voronoi_seed.h): position (x, y) and plate id. */
+typedef struct tt_voronoi_seed {
+    float position[2];
+    uint8_t id;
+    uint8_t pad_[3];
+} tt_voronoi_seed;
+
+/* generatePlateCenters(generator, numPlates, width, height),
+ * plate_tectonic_sim.cu:873-890: out[i] = ((U[0,height), U[0,width)), id i),
+ * x drawn over the height range as in the reference. */
+int tt_generate_plate_centers(uint64_t *rng_state, int32_t num_plates, int32_t width, int32_t height,
+                              tt_voronoi_seed *out);
+/* generateVoronoiSeeds(generator, centerSeeds, numSeeds, width, height),
+ * plate_tectonic_sim.cu:892-928: num_seeds new seeds, each with the id of its
+ * nearest (toroidal, first on ties) seed among centers. */
+int tt_generate_voronoi_seeds(uint64_t *rng_state, const tt_voronoi_seed *centers, int32_t num_centers,
+                              int32_t num_seeds, int32_t width, int32_t height, tt_voronoi_seed *out);
+/* generatePlateData(generator, plateCenters, numStartingPlates),
+ * plate_tectonic_sim.cu:816-842: out[0..254] = PlateData() defaults, with
+ * pixelCenter = frac(center), velocity (U(-1,1)+1)/2 and a normalised random
+ * direction for the first num_plates entries. */
+int tt_generate_plate_data(uint64_t *rng_state, const tt_voronoi_seed *centers, int32_t num_plates,
+                           tt_plate_data *out);
 
 /* ---- batches: many worlds stepped together (no reference counterpart; the
  * batched form of executeIteration for many independent worlds) ------------

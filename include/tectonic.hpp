@@ -58,13 +58,15 @@ public:
     PlateTectonicSim &operator=(const PlateTectonicSim &) = delete;
 
     /* initialize(int numStartingPlates, const std::vector<int>& numVoronoiSeeds),
-     * plate_tectonic_sim.h:115 */
+     * plate_tectonic_sim.h:115; any number of Voronoi generations */
     void initialize(int numStartingPlates, const std::vector<int> &numVoronoiSeeds)
     {
-        configure(numStartingPlates, numVoronoiSeeds);
         release();
-        check(tt_world_create(&cfg_, &ks_, &world_), "tt_world_create");
-        check(tt_world_init(world_), "tt_world_init");
+        tt_sim_config c = cfg_;
+        c.num_plates = numStartingPlates;
+        c.num_voronoi_gens = 0;
+        check(tt_world_create(&c, &ks_, &world_), "tt_world_create");
+        reset(cfg_.seed, numStartingPlates, numVoronoiSeeds);
     }
 
     /* executeIteration(), plate_tectonic_sim.h:117 */
@@ -76,11 +78,41 @@ public:
     }
 
     /* resetSim(unsigned seed, int numStartingPlates, const std::vector<int>&),
-     * plate_tectonic_sim.h:119 */
+     * plate_tectonic_sim.h:119: the world keeps its device buffers */
     void resetSim(unsigned seed, int numStartingPlates, const std::vector<int> &numVoronoiSeeds)
     {
         cfg_.seed = seed;
-        initialize(numStartingPlates, numVoronoiSeeds);
+        if (!world_) initialize(numStartingPlates, numVoronoiSeeds);
+        else reset(seed, numStartingPlates, numVoronoiSeeds);
+    }
+
+    /* The static generators (plate_tectonic_sim.h:121-130).  The reference
+     * passes a std::default_random_engine&; here the generator is a SplitMix64
+     * state, advanced in place (a world starts from state = seed). */
+    static std::vector<tt_voronoi_seed> generatePlateCenters(uint64_t &generator, int numPlates, int width, int height)
+    {
+        std::vector<tt_voronoi_seed> out((size_t)(numPlates > 0 ? numPlates : 0));
+        check(tt_generate_plate_centers(&generator, numPlates, width, height, out.data()), "tt_generate_plate_centers");
+        return out;
+    }
+    static std::vector<tt_voronoi_seed> generateVoronoiSeeds(uint64_t &generator,
+                                                             const std::vector<tt_voronoi_seed> &centerSeeds,
+                                                             int numSeeds, int width, int height)
+    {
+        std::vector<tt_voronoi_seed> out((size_t)(numSeeds > 0 ? numSeeds : 0));
+        check(tt_generate_voronoi_seeds(&generator, centerSeeds.data(), (int32_t)centerSeeds.size(), numSeeds, width,
+                                        height, out.data()), "tt_generate_voronoi_seeds");
+        return out;
+    }
+    static std::vector<tt_plate_data> generatePlateData(uint64_t &generator,
+                                                        const std::vector<tt_voronoi_seed> &plateCenters,
+                                                        int numStartingPlates)
+    {
+        if ((size_t)numStartingPlates > plateCenters.size()) throw Error("fewer centres than plates", TT_ERR_INVALID);
+        std::vector<tt_plate_data> out(TT_MAX_PLATES);
+        check(tt_generate_plate_data(&generator, plateCenters.data(), numStartingPlates, out.data()),
+              "tt_generate_plate_data");
+        return out;
     }
 
     /* copyHostKernelSettingsToDevice, kernel_settings.cu:7-9 */
@@ -131,11 +163,14 @@ public:
     const tt_kernel_settings &kernelSettings() const { return ks_; }
 
 private:
-    void configure(int plates, const std::vector<int> &gens)
+    void reset(unsigned seed, int plates, const std::vector<int> &gens)
     {
+        const std::vector<int32_t> g(gens.begin(), gens.end());
+        check(tt_world_reset(world_, seed, plates, g.data(), (int32_t)g.size()), "tt_world_reset");
+        cfg_.seed = seed;
         cfg_.num_plates = plates;
-        cfg_.num_voronoi_gens = (int32_t)(gens.size() < 4 ? gens.size() : 4);
-        for (int i = 0; i < cfg_.num_voronoi_gens; ++i) cfg_.voronoi_gens[i] = gens[(size_t)i];
+        cfg_.num_voronoi_gens = (int32_t)(g.size() < 4 ? g.size() : 4);
+        for (int i = 0; i < cfg_.num_voronoi_gens; ++i) cfg_.voronoi_gens[i] = g[(size_t)i];
     }
     void requireWorld() const
     {

@@ -279,6 +279,14 @@ class HbmProbe(C.Structure):
 # numpy-compatible record layouts of the per-cell structs
 import numpy as np  # noqa: E402
 
+class VoronoiSeed(C.Structure):
+    """struct VoronoiSeed (types/voronoi_seed.h) = tt_voronoi_seed."""
+    _fields_ = [("position", C.c_float * 2), ("id", C.c_uint8), ("pad_", C.c_uint8 * 3)]
+
+
+VORONOI_SEED_DTYPE = np.dtype([("position", "<f4", 2), ("id", "u1"), ("pad", "u1", 3)])
+assert VORONOI_SEED_DTYPE.itemsize == 12 and C.sizeof(VoronoiSeed) == 12
+
 UPLIFT_DTYPE = np.dtype([("cum", "<f4"), ("types", "u1"), ("pad", "u1", 3)])
 PROPAGATED_DTYPE = np.dtype([("cum", "<f4"), ("inv", "<u4"), ("types", "u1"), ("pad", "u1", 3)])
 DISTANCE_DTYPE = np.dtype([("dist", "<i4"), ("orig", "<u4")])
@@ -356,8 +364,15 @@ def lib() -> C.CDLL:
     L.tt_config_parse_json.argtypes = [C.c_char_p, C.POINTER(SimConfig), C.POINTER(KernelSettings)]
     L.tt_world_create.argtypes = [C.POINTER(SimConfig), C.POINTER(KernelSettings), C.POINTER(vp)]
     L.tt_world_init.argtypes = [vp]
-    L.tt_world_reset.argtypes = [vp, C.c_uint32, C.c_int32]
+    L.tt_world_reset.argtypes = [vp, C.c_uint32, C.c_int32, C.POINTER(C.c_int32), C.c_int32]
+    L.tt_generate_plate_centers.argtypes = [C.POINTER(C.c_uint64), C.c_int32, C.c_int32, C.c_int32,
+                                            C.POINTER(VoronoiSeed)]
+    L.tt_generate_voronoi_seeds.argtypes = [C.POINTER(C.c_uint64), C.POINTER(VoronoiSeed), C.c_int32, C.c_int32,
+                                            C.c_int32, C.c_int32, C.POINTER(VoronoiSeed)]
+    L.tt_generate_plate_data.argtypes = [C.POINTER(C.c_uint64), C.POINTER(VoronoiSeed), C.c_int32,
+                                         C.POINTER(PlateData)]
     L.tt_world_step.argtypes = [vp, C.c_int32]
+    L.tt_world_config.argtypes = [vp, C.POINTER(SimConfig), C.POINTER(KernelSettings)]
     L.tt_world_set_settings.argtypes = [vp, C.POINTER(KernelSettings)]
     L.tt_world_download.argtypes = [vp, C.c_int32, vp, C.c_size_t]
     L.tt_world_upload.argtypes = [vp, C.c_int32, vp, C.c_size_t]

@@ -1006,21 +1006,26 @@ hipError_t tt_launch_finalize(hipStream_t s, const tt_dev &d, tt_plate_data *pd,
 // ---------------------------------------------------------------------------
 // G6: split
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_split_prepare(tt_dev d, const uint8_t *ids, int plate, float *eff,
-                                                       uint8_t *flag)
+// initEffortToBoundary (:2824-2836): +inf on the plate, 0 elsewhere; the
+// plate's cells are flagged for the compaction, and a plate cell whose height
+// (the relaxation's cost) is negative or NaN is reported: such a cost has no
+// fixed point, and the split is abandoned.
+__global__ __launch_bounds__(256) void k_split_prepare(tt_dev d, const uint8_t *ids, int plate, const float *height,
+                                                       float *eff, uint8_t *flag, tt_step_ctl *ctl)
 {
     const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
     if (c >= d.N) return;
     const bool in = ids[c] == plate;
     eff[c] = in ? tt_u2f(0x7f800000u) : 0.0f;
     flag[c] = in ? 1 : 0;
+    if (in && !(height[c] >= 0.0f)) ctl->split_negative = 1;
 }
 
-hipError_t tt_launch_split_prepare(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, float *eff,
-                                   uint8_t *flag)
+hipError_t tt_launch_split_prepare(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, const float *height,
+                                   float *eff, uint8_t *flag, tt_step_ctl *ctl)
 {
     const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
-    hipLaunchKernelGGL(k_split_prepare, dim3(blocks), dim3(256), 0, s, d, ids, plate, eff, flag);
+    hipLaunchKernelGGL(k_split_prepare, dim3(blocks), dim3(256), 0, s, d, ids, plate, height, eff, flag, ctl);
     return hipGetLastError();
 }
 

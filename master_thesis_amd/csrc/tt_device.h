@@ -64,6 +64,7 @@ struct tt_step_ctl {
     int largest_value, largest_id, heaviest_id, n_plates;
     float heaviest_value;
     int split_seed;      // cell where the split's new id starts, -1 if none
+    int split_negative;  // some cell of the plate to split has a height below 0 (or NaN)
     long long ledger;
 };
 

@@ -107,8 +107,8 @@ hipError_t tt_launch_count(hipStream_t s, const tt_dev &d, const int *mergeIds, 
 hipError_t tt_launch_finalize(hipStream_t s, const tt_dev &d, tt_plate_data *pd, const long long *acc, int mode,
                               tt_step_ctl *ctl);
 // split
-hipError_t tt_launch_split_prepare(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, float *eff,
-                                   uint8_t *flag);
+hipError_t tt_launch_split_prepare(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, const float *height,
+                                   float *eff, uint8_t *flag, tt_step_ctl *ctl);
 int tt_split_tile_h(const tt_dev &d);
 hipError_t tt_launch_split_tiles(hipStream_t s, const tt_dev &d, const uint8_t *ids, int plate, uint8_t *flag);
 hipError_t tt_launch_relax_tiles(hipStream_t s, const tt_dev &d, const uint32_t *tiles, unsigned ntiles,

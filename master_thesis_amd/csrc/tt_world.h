@@ -8,6 +8,10 @@
 #include "tt_device.h"
 
 
+// entries of a world's Voronoi seed table (the last generation of
+// numVoronoiSeeds, or the plate centres)
+constexpr int TT_MAX_SEEDS = 65536;
+
 struct tt_alloc {
     void **ptr;
     size_t bytes;
@@ -19,6 +23,7 @@ struct tt_alloc {
 struct tt_world {
     tt_sim_config cfg{};
     tt_kernel_settings ks{};
+    std::vector<int32_t> gens;   // numVoronoiSeeds: seed counts of the Voronoi generations
     int W = 0, H = 0, N = 0;
     uint32_t iter = 0;
     bool initialized = false;

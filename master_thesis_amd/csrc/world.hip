@@ -172,7 +172,7 @@ size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w)
         take((void **)&w->cs[1], N * 4);
     }
     take((void **)&w->scanTmp, (N / 1024 + tt_scan_tmp_words((long long)N) + 4096) * 4);
-    take((void **)&w->initSeeds, 4096 * 12);
+    take((void **)&w->initSeeds, TT_MAX_SEEDS * 12);
     take((void **)&w->initWts, 128 * 8);
     return total;
 }
@@ -340,15 +340,20 @@ static int run_split(tt_world *w)
     const size_t N = (size_t)w->N;
     tt_plate_data hp;
     CK(hipMemcpyAsync(&hp, w->pd + plate, sizeof(hp), hipMemcpyDeviceToHost, s));
-    CK(tt_launch_split_prepare(s, d, ids, plate, w->eff, w->unFlag));
+    CKS(zero(w, &w->ctl->split_negative, 4));
+    CK(tt_launch_split_prepare(s, d, ids, plate, w->h[w->hc], w->eff, w->unFlag, w->ctl));
     CK(tt_compact_u8_any(s, w->unFlag, (long long)N, w->list[0], w->listN[0]));
     // tiles holding cells of the plate
     const int th = tt_split_tile_h(d);
     const long long ntiles_all = (long long)(w->W / 64) * (w->H / th);
     CK(tt_launch_split_tiles(s, d, ids, plate, w->vote));
     CK(tt_compact_u8_any(s, w->vote, ntiles_all, w->list[1], w->listN[1]));
-    uint32_t ntiles = 0;
-    CKS(read_u32(w, w->listN[1], &ntiles));
+    // one read-back: the tile count and the negative-cost flag
+    CK(hipMemcpyAsync(w->hu32, w->listN[1], 4, hipMemcpyDeviceToHost, s));
+    CK(hipMemcpyAsync(w->hu32 + 1, &w->ctl->split_negative, 4, hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    const uint32_t ntiles = w->hu32[0];
+    if (w->hu32[1]) return TT_OK;   // no fixed point of the relaxation: the split is abandoned
     const uint32_t *tiles = w->list[1];
     const float *height = w->h[w->hc];
     int rounds = 0;
@@ -664,32 +669,72 @@ static int upload_seeds(tt_world *w, const std::vector<float> &sx, const std::ve
                         const std::vector<uint8_t> &sid)
 {
     const size_t n = sx.size();
-    if (n > 4096) { set_error("too many Voronoi seeds (max 4096)"); return TT_ERR_UNSUPPORTED; }
-    std::vector<uint8_t> blob(4096 * 12, 0);
+    if (n > TT_MAX_SEEDS) { set_error("too many Voronoi seeds"); return TT_ERR_UNSUPPORTED; }
+    std::vector<uint8_t> blob(TT_MAX_SEEDS * 12, 0);
     memcpy(blob.data(), sx.data(), n * 4);
-    memcpy(blob.data() + 4096 * 4, sy.data(), n * 4);
-    memcpy(blob.data() + 4096 * 8, sid.data(), n);
+    memcpy(blob.data() + TT_MAX_SEEDS * 4, sy.data(), n * 4);
+    memcpy(blob.data() + TT_MAX_SEEDS * 8, sid.data(), n);
     CK(hipMemcpyAsync(w->initSeeds, blob.data(), blob.size(), hipMemcpyHostToDevice, w->stream));
     CK(hipStreamSynchronize(w->stream));
     const tt_dev d = make_dev(w);
     const float *dsx = (const float *)w->initSeeds;
-    const float *dsy = (const float *)(w->initSeeds + 4096 * 4);
-    const uint8_t *dsid = (const uint8_t *)(w->initSeeds + 4096 * 8);
+    const float *dsy = (const float *)(w->initSeeds + TT_MAX_SEEDS * 4);
+    const uint8_t *dsid = (const uint8_t *)(w->initSeeds + TT_MAX_SEEDS * 8);
     CK(tt_launch_init_plate_ids(w->stream, d, dsx, dsy, dsid, (int)n, w->ids[w->ic]));
     return TT_OK;
 }
 
-static void gen_plate_data(tt_world *w, SplitMix &rng, const std::vector<float> &cx, const std::vector<float> &cy,
-                           bool reference_speed, std::vector<tt_plate_data> &pd)
+// generatePlateCenters (plate_tectonic_sim.cu:873-890): plate i sits at
+// (U[0, height), U[0, width)) -- x drawn over the height range and y over the
+// width range, as in the reference -- with id i.
+static void gen_centers(SplitMix &rng, int n, int W, int H, std::vector<tt_voronoi_seed> &out)
 {
-    const int P = w->cfg.num_plates;
+    out.assign((size_t)n, tt_voronoi_seed{});
+    for (int i = 0; i < n; ++i) {
+        out[i].position[0] = rng.u01() * (float)H;
+        out[i].position[1] = rng.u01() * (float)W;
+        out[i].id = (uint8_t)i;
+    }
+}
+
+// generateVoronoiSeeds (plate_tectonic_sim.cu:892-928): each new seed takes the
+// id of its nearest seed of the previous generation (toroidal squared
+// distance, first index on ties).
+static void gen_seeds(SplitMix &rng, const std::vector<tt_voronoi_seed> &prev, int n, int W, int H,
+                      std::vector<tt_voronoi_seed> &out)
+{
+    out.assign((size_t)n, tt_voronoi_seed{});
+    for (int i = 0; i < n; ++i) {
+        const float px = rng.u01() * (float)H;
+        const float py = rng.u01() * (float)W;
+        float best = 3.402823466e38f;
+        uint8_t cl = 0;
+        for (const tt_voronoi_seed &q : prev) {
+            const float dx = tt_fabsf(px - q.position[0]), dy = tt_fabsf(py - q.position[1]);
+            const float wx = tt_fminf(dx, (float)W - dx), wy = tt_fminf(dy, (float)H - dy);
+            const float d2 = wx * wx + wy * wy;
+            if (d2 < best) { cl = q.id; best = d2; }
+        }
+        out[i].position[0] = px;
+        out[i].position[1] = py;
+        out[i].id = cl;
+    }
+}
+
+// generatePlateData (plate_tectonic_sim.cu:816-842): pixel centre = fractional
+// part of the centre, speed (U(-1,1) + 1) / 2 (or U[lo, hi] for the synthetic
+// worlds), direction = normalised pair of U(-1,1); 256 entries (the
+// reference's 255 plus the spare slot of the device table).
+static void gen_plate_data(SplitMix &rng, const std::vector<tt_voronoi_seed> &centers, int P, bool reference_speed,
+                           float lo, float hi, std::vector<tt_plate_data> &pd)
+{
     pd.assign(256, default_plate());
     for (int i = 0; i < P; ++i) {
         tt_plate_data &p = pd[i];
-        p.pixelCenter[0] = cx[i] - tt_floorf(cx[i]);
-        p.pixelCenter[1] = cy[i] - tt_floorf(cy[i]);
+        p.pixelCenter[0] = centers[i].position[0] - tt_floorf(centers[i].position[0]);
+        p.pixelCenter[1] = centers[i].position[1] - tt_floorf(centers[i].position[1]);
         if (reference_speed) p.velocity = ((rng.u01() * 2.0f - 1.0f) + 1.0f) / 2.0f;
-        else p.velocity = w->cfg.speed_min + (w->cfg.speed_max - w->cfg.speed_min) * rng.u01();
+        else p.velocity = lo + (hi - lo) * rng.u01();
         p.velocitySmoothed = p.velocity;
         const float xd = rng.u01() * 2.0f - 1.0f;
         const float yd = rng.u01() * 2.0f - 1.0f;
@@ -760,44 +805,28 @@ int tt_world_do_init(tt_world *w)
     }
     SplitMix rng{(uint64_t)w->cfg.seed};
     const int P = w->cfg.num_plates;
-    std::vector<float> cx(P), cy(P);
-    std::vector<uint8_t> cid(P);
-    for (int i = 0; i < P; ++i) {
-        cx[i] = rng.u01() * (float)w->H;
-        cy[i] = rng.u01() * (float)w->W;
-        cid[i] = (uint8_t)i;
-    }
-    std::vector<float> sx = cx, sy = cy;
-    std::vector<uint8_t> sid = cid;
+    std::vector<tt_voronoi_seed> centers, seeds, next;
+    gen_centers(rng, P, w->W, w->H, centers);
+    seeds = centers;
     if (mode == TT_INIT_REFERENCE) {
-        for (int g = 0; g < w->cfg.num_voronoi_gens; ++g) {
-            const int n = w->cfg.voronoi_gens[g];
-            std::vector<float> nx(n), ny(n);
-            std::vector<uint8_t> ni(n);
-            for (int i = 0; i < n; ++i) {
-                const float px = rng.u01() * (float)w->H;
-                const float py = rng.u01() * (float)w->W;
-                float best = 3.402823466e38f;
-                uint8_t cl = 0;
-                for (size_t j = 0; j < sx.size(); ++j) {
-                    const float dx = tt_fabsf(px - sx[j]), dy = tt_fabsf(py - sy[j]);
-                    const float wx = tt_fminf(dx, (float)w->W - dx), wy = tt_fminf(dy, (float)w->H - dy);
-                    const float d2 = wx * wx + wy * wy;
-                    if (d2 < best) { cl = sid[j]; best = d2; }
-                }
-                nx[i] = px;
-                ny[i] = py;
-                ni[i] = cl;
-            }
-            sx.swap(nx);
-            sy.swap(ny);
-            sid.swap(ni);
+        // every generation of numVoronoiSeeds in turn; fewer seeds than plates
+        // fall back to the plate centres (plate_tectonic_sim.cu:720-728)
+        for (int n : w->gens) {
+            gen_seeds(rng, seeds, n, w->W, w->H, next);
+            seeds.swap(next);
         }
-        if (sx.size() < cx.size()) { sx = cx; sy = cy; sid = cid; }
+        if (seeds.size() < centers.size()) seeds = centers;
     }
     std::vector<tt_plate_data> pd;
-    gen_plate_data(w, rng, cx, cy, mode == TT_INIT_REFERENCE, pd);
+    gen_plate_data(rng, centers, P, mode == TT_INIT_REFERENCE, w->cfg.speed_min, w->cfg.speed_max, pd);
     CK(hipMemcpyAsync(w->pd, pd.data(), 256 * sizeof(tt_plate_data), hipMemcpyHostToDevice, s));
+    std::vector<float> sx(seeds.size()), sy(seeds.size());
+    std::vector<uint8_t> sid(seeds.size());
+    for (size_t i = 0; i < seeds.size(); ++i) {
+        sx[i] = seeds[i].position[0];
+        sy[i] = seeds[i].position[1];
+        sid[i] = seeds[i].id;
+    }
     if (mode == TT_INIT_SYNTHETIC) {
         double wts[65], tot = 0.0;
         for (int k = -32; k <= 32; ++k) { wts[k + 32] = tt_exp_core(-(double)(k * k) / 128.0); tot = tot + wts[k + 32]; }
@@ -994,6 +1023,19 @@ int tt_validate_config(const tt_sim_config *c)
     return TT_OK;
 }
 
+// Seed counts of the Voronoi generations: each at least 1; the last one is
+// uploaded to the device's seed table of TT_MAX_SEEDS entries.
+static int validate_gens(const std::vector<int32_t> &gens)
+{
+    for (int32_t n : gens)
+        if (n < 1) { set_error("every Voronoi generation needs at least one seed"); return TT_ERR_INVALID; }
+    if (!gens.empty() && gens.back() > TT_MAX_SEEDS) {
+        set_error("the last Voronoi generation has more than " + std::to_string(TT_MAX_SEEDS) + " seeds");
+        return TT_ERR_UNSUPPORTED;
+    }
+    return TT_OK;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -1076,6 +1118,11 @@ int tt_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, tt_w
     w->H = cfg->height;
     w->N = cfg->width * cfg->height;
     w->rows = cfg->tile_rows;
+    w->gens.assign(cfg->voronoi_gens, cfg->voronoi_gens + cfg->num_voronoi_gens);
+    if (validate_gens(w->gens) != TT_OK) {
+        delete w;
+        return TT_ERR_INVALID;
+    }
     CK(hipGetDevice(&w->device));
     tt_world_layout(*cfg, true, w);
     size_t total = 0;
@@ -1127,15 +1174,74 @@ int tt_world_init(tt_world *w)
     return TT_OK;
 }
 
-int tt_world_reset(tt_world *w, uint32_t seed, int32_t num_plates)
+int tt_world_reset(tt_world *w, uint32_t seed, int32_t num_plates, const int32_t *voronoi_gens, int32_t num_gens)
 {
-    if (!w) return TT_ERR_INVALID;
+    if (!w || num_gens < 0 || (num_gens > 0 && !voronoi_gens)) return TT_ERR_INVALID;
     tt_sim_config c = w->cfg;
     c.seed = seed;
     c.num_plates = num_plates;
+    c.num_voronoi_gens = 0;   // the generations live in w->gens (any number)
     CKS(tt_validate_config(&c));
+    std::vector<int32_t> gens(voronoi_gens, voronoi_gens + num_gens);
+    CKS(validate_gens(gens));
+    c.num_voronoi_gens = num_gens <= 4 ? num_gens : 4;
+    for (int i = 0; i < c.num_voronoi_gens; ++i) c.voronoi_gens[i] = gens[(size_t)i];
     w->cfg = c;
-    return tt_world_do_init(w);
+    w->gens = gens;
+    if (w->rows == 0) CKS(tune_rows(w));
+    CKS(tt_world_do_init(w));
+    w->stats.tile_rows = w->rows;
+    w->stats.hydraulic_div_const = w->divL;
+    w->stats.hydraulic_sqrt_fast = w->sqrtOk;
+    return TT_OK;
+}
+
+int tt_generate_plate_centers(uint64_t *rng_state, int32_t num_plates, int32_t width, int32_t height,
+                              tt_voronoi_seed *out)
+{
+    if (!rng_state || num_plates < 0 || (num_plates > 0 && !out) || width < 1 || height < 1) return TT_ERR_INVALID;
+    SplitMix rng{*rng_state};
+    std::vector<tt_voronoi_seed> v;
+    gen_centers(rng, num_plates, width, height, v);
+    std::copy(v.begin(), v.end(), out);
+    *rng_state = rng.s;
+    return TT_OK;
+}
+
+int tt_generate_voronoi_seeds(uint64_t *rng_state, const tt_voronoi_seed *centers, int32_t num_centers,
+                              int32_t num_seeds, int32_t width, int32_t height, tt_voronoi_seed *out)
+{
+    if (!rng_state || num_centers < 0 || num_seeds < 0 || (num_centers > 0 && !centers) || (num_seeds > 0 && !out) ||
+        width < 1 || height < 1)
+        return TT_ERR_INVALID;
+    SplitMix rng{*rng_state};
+    const std::vector<tt_voronoi_seed> prev(centers, centers + num_centers);
+    std::vector<tt_voronoi_seed> v;
+    gen_seeds(rng, prev, num_seeds, width, height, v);
+    std::copy(v.begin(), v.end(), out);
+    *rng_state = rng.s;
+    return TT_OK;
+}
+
+int tt_generate_plate_data(uint64_t *rng_state, const tt_voronoi_seed *centers, int32_t num_plates,
+                           tt_plate_data *out)
+{
+    if (!rng_state || !out || num_plates < 0 || num_plates > MAXP || (num_plates > 0 && !centers)) return TT_ERR_INVALID;
+    SplitMix rng{*rng_state};
+    const std::vector<tt_voronoi_seed> c(centers, centers + num_plates);
+    std::vector<tt_plate_data> pd;
+    gen_plate_data(rng, c, num_plates, true, 0.0f, 0.0f, pd);
+    std::copy(pd.begin(), pd.begin() + MAXP, out);
+    *rng_state = rng.s;
+    return TT_OK;
+}
+
+int tt_world_config(tt_world *w, tt_sim_config *cfg, tt_kernel_settings *ks)
+{
+    if (!w) return TT_ERR_INVALID;
+    if (cfg) *cfg = w->cfg;
+    if (ks) *ks = w->ks;
+    return TT_OK;
 }
 
 int tt_world_step(tt_world *w, int32_t n)
@@ -1187,13 +1293,33 @@ int tt_world_upload(tt_world *w, int32_t field, const void *host, size_t bytes)
 }
 
 // ---- snapshot file ------------------------------------------------------
-// "TTSNAP1\n", u32 header bytes, tt_sim_config, tt_kernel_settings, u32
+// "TTSNAP2\n", u32 header bytes, tt_sim_config, tt_kernel_settings, u32
 // iteration, tt_step_stats, u32 field count; per field: i32 id, u64 bytes, u32
-// CRC32 of the bytes, the bytes.
+// CRC32 of the bytes (zlib crc32_z over the whole length), the bytes.  Worlds
+// in TT_PRECISION_COMPENSATED also store their Kahan compensation terms (ids
+// SNAP_HEIGHT_COMP and SNAP_SEDIMENT_COMP), which carry over between steps.
+// Version-1 files (no compensation fields) still load.
 static const int kSnapFields[] = {TT_FIELD_HEIGHT, TT_FIELD_PLATE_IDS, TT_FIELD_WATER, TT_FIELD_FLUX,
                                   TT_FIELD_VELOCITY, TT_FIELD_SEDIMENT, TT_FIELD_UPLIFT, TT_FIELD_PLATE_DATA,
                                   TT_FIELD_DIVERGENCE_BITMAP, TT_FIELD_COLLISION_TYPES, TT_FIELD_LABELS};
-static const char kSnapMagic[8] = {'T', 'T', 'S', 'N', 'A', 'P', '1', '\n'};
+enum { SNAP_HEIGHT_COMP = 1000, SNAP_SEDIMENT_COMP = 1001 };
+static const char kSnapMagic[8] = {'T', 'T', 'S', 'N', 'A', 'P', '2', '\n'};
+static const char kSnapMagicV1[8] = {'T', 'T', 'S', 'N', 'A', 'P', '1', '\n'};
+
+static uint32_t snap_crc(const uint8_t *p, size_t n) { return (uint32_t)crc32_z(0L, p, (z_size_t)n); }
+
+// device buffer of a snapshot field id (public fields and the compensation terms)
+static bool snap_ptr(tt_world *w, int id, void **p, size_t *bytes)
+{
+    const size_t N = (size_t)w->N;
+    if (id == SNAP_HEIGHT_COMP || id == SNAP_SEDIMENT_COMP) {
+        if (!w->ch) return false;
+        *p = id == SNAP_HEIGHT_COMP ? (void *)w->ch : (void *)w->cs[w->sc];
+        *bytes = N * 4;
+        return true;
+    }
+    return field_ptr(w, id, p, bytes, false);
+}
 
 struct snap_header {
     tt_sim_config cfg;
@@ -1215,21 +1341,35 @@ int tt_world_save(tt_world *w, const char *path)
     hd.ks = w->ks;
     hd.iteration = w->iter;
     hd.stats = w->stats;
-    hd.nfields = (uint32_t)(sizeof kSnapFields / sizeof kSnapFields[0]);
+    std::vector<int> ids(std::begin(kSnapFields), std::end(kSnapFields));
+    if (w->ch) {
+        ids.push_back(SNAP_HEIGHT_COMP);
+        ids.push_back(SNAP_SEDIMENT_COMP);
+    }
+    hd.nfields = (uint32_t)ids.size();
     const uint32_t hb = (uint32_t)sizeof hd;
     bool ok = fwrite(kSnapMagic, 8, 1, f) == 1 && fwrite(&hb, 4, 1, f) == 1 && fwrite(&hd, sizeof hd, 1, f) == 1;
     std::vector<uint8_t> buf;
-    for (int fid : kSnapFields) {
+    for (int fid : ids) {
         if (!ok) break;
         void *p;
         size_t b;
-        if (!field_ptr(w, fid, &p, &b, false)) { ok = false; break; }
+        if (!snap_ptr(w, fid, &p, &b)) { ok = false; break; }
         buf.resize(b);
-        const int st = tt_world_download(w, fid, buf.data(), b);
-        if (st != TT_OK) { fclose(f); return st; }
+        if (fid == SNAP_HEIGHT_COMP || fid == SNAP_SEDIMENT_COMP) {
+            if (hipMemcpyAsync(buf.data(), p, b, hipMemcpyDeviceToHost, w->stream) != hipSuccess ||
+                hipStreamSynchronize(w->stream) != hipSuccess) {
+                fclose(f);
+                set_error("downloading a compensation field failed");
+                return TT_ERR_DEVICE;
+            }
+        } else {
+            const int st = tt_world_download(w, fid, buf.data(), b);
+            if (st != TT_OK) { fclose(f); return st; }
+        }
         const int32_t id = fid;
         const uint64_t bytes = b;
-        const uint32_t crc = (uint32_t)crc32(0L, buf.data(), (uInt)b);
+        const uint32_t crc = snap_crc(buf.data(), b);
         ok = fwrite(&id, 4, 1, f) == 1 && fwrite(&bytes, 8, 1, f) == 1 && fwrite(&crc, 4, 1, f) == 1 &&
              fwrite(buf.data(), 1, b, f) == b;
     }
@@ -1247,10 +1387,10 @@ int tt_world_load(const char *path, tt_world **out)
     char magic[8];
     uint32_t hb = 0;
     snap_header hd;
-    if (fread(magic, 8, 1, f) != 1 || memcmp(magic, kSnapMagic, 8) != 0 || fread(&hb, 4, 1, f) != 1 ||
-        hb != sizeof hd || fread(&hd, sizeof hd, 1, f) != 1) {
+    if (fread(magic, 8, 1, f) != 1 || (memcmp(magic, kSnapMagic, 8) != 0 && memcmp(magic, kSnapMagicV1, 8) != 0) ||
+        fread(&hb, 4, 1, f) != 1 || hb != sizeof hd || fread(&hd, sizeof hd, 1, f) != 1) {
         fclose(f);
-        set_error(std::string(path) + " is not a version-1 world snapshot");
+        set_error(std::string(path) + " is not a world snapshot (version 1 or 2)");
         return TT_ERR_INVALID;
     }
     // read and check every field before any device work
@@ -1270,7 +1410,7 @@ int tt_world_load(const char *path, tt_world **out)
         b.id = id;
         b.data.resize((size_t)bytes);
         if (fread(b.data.data(), 1, (size_t)bytes, f) != (size_t)bytes ||
-            (uint32_t)crc32(0L, b.data.data(), (uInt)bytes) != crc) {
+            snap_crc(b.data.data(), (size_t)bytes) != crc) {
             fclose(f);
             set_error(std::string(path) + ": field " + std::to_string(id) + " is truncated or corrupt");
             return TT_ERR_INVALID;
@@ -1286,6 +1426,19 @@ int tt_world_load(const char *path, tt_world **out)
     int st = tt_world_init(w);
     for (const blob &b : blobs) {
         if (st != TT_OK) break;
+        if (b.id == SNAP_HEIGHT_COMP || b.id == SNAP_SEDIMENT_COMP) {
+            void *p;
+            size_t bytes;
+            if (!snap_ptr(w, b.id, &p, &bytes) || bytes != b.data.size()) {
+                set_error(std::string(path) + ": compensation field of a world that is not compensated, or bad size");
+                st = TT_ERR_INVALID;
+            } else if (hipMemcpyAsync(p, b.data.data(), bytes, hipMemcpyHostToDevice, w->stream) != hipSuccess ||
+                       hipStreamSynchronize(w->stream) != hipSuccess) {
+                set_error("uploading a compensation field failed");
+                st = TT_ERR_DEVICE;
+            }
+            continue;
+        }
         st = tt_world_upload(w, b.id, b.data.data(), b.data.size());
     }
     if (st != TT_OK) {

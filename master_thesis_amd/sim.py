@@ -42,9 +42,16 @@ class World:
         check(lib().tt_world_init(self.h), "tt_world_init")
         return self
 
-    def reset(self, seed: int, plates: int) -> None:
-        check(lib().tt_world_reset(self.h, seed, plates), "tt_world_reset")
+    def reset(self, seed: int, plates: int, voronoi_gens: Sequence[int] = ()) -> None:
+        """resetSim(seed, numStartingPlates, numVoronoiSeeds): re-initialise
+        with any number of Voronoi generations."""
+        gens = [int(g) for g in voronoi_gens]
+        arr = (C.c_int32 * max(1, len(gens)))(*gens)
+        check(lib().tt_world_reset(self.h, seed, plates, arr, len(gens)), "tt_world_reset")
         self.cfg.seed, self.cfg.num_plates = seed, plates
+        self.cfg.num_voronoi_gens = min(4, len(gens))
+        for i in range(self.cfg.num_voronoi_gens):
+            self.cfg.voronoi_gens[i] = gens[i]
 
     def step(self, n: int = 1) -> None:
         check(lib().tt_world_step(self.h, n), "tt_world_step")
@@ -128,8 +135,7 @@ class World:
         return w._refresh_geometry()
 
     def _refresh_geometry(self) -> "World":
-        d = self.diagnostics()
-        self.cfg.width, self.cfg.height = d["width"], d["height"]
+        check(lib().tt_world_config(self.h, C.byref(self.cfg), C.byref(self.ks)), "tt_world_config")
         self.n = self.cfg.width * self.cfg.height
         return self
 
@@ -215,7 +221,7 @@ class Server:
         r = self.result(jid, 0)
         cfg = _abi.SimConfig()
         cfg.width, cfg.height = r.width, r.height
-        return World._adopt(h, cfg)
+        return World._adopt(h, cfg)._refresh_geometry()
 
     def drain(self, timeout_ms: int = -1) -> None:
         check(lib().tt_server_drain(self.h, int(timeout_ms)), "tt_server_drain")
@@ -317,10 +323,14 @@ class PlateTectonicSim:
         self._world: World | None = None
 
     def initialize(self, numStartingPlates: int, numVoronoiSeeds: Sequence[int]) -> None:
-        self._configure(numStartingPlates, numVoronoiSeeds)
         if self._world is not None:
             self._world.close()
-        self._world = World(self._cfg, self._ks).init()
+        cfg = self._cfg.copy()
+        cfg.num_plates = numStartingPlates
+        cfg.num_voronoi_gens = 0
+        self._world = World(cfg, self._ks)
+        self._world.reset(self._cfg.seed, numStartingPlates, numVoronoiSeeds)
+        self._cfg = self._world.cfg.copy()
 
     def executeIteration(self) -> None:
         if self._world is None:
@@ -329,14 +339,43 @@ class PlateTectonicSim:
 
     def resetSim(self, seed: int, numStartingPlates: int, numVoronoiSeeds: Sequence[int]) -> None:
         self._cfg.seed = seed
-        self.initialize(numStartingPlates, numVoronoiSeeds)
+        if self._world is None:
+            self.initialize(numStartingPlates, numVoronoiSeeds)
+        else:
+            self._world.reset(seed, numStartingPlates, numVoronoiSeeds)
+            self._cfg = self._world.cfg.copy()
 
-    def _configure(self, plates: int, gens: Sequence[int]) -> None:
-        gens = list(gens)[:4]
-        self._cfg.num_plates = plates
-        self._cfg.num_voronoi_gens = len(gens)
-        for i, g in enumerate(gens):
-            self._cfg.voronoi_gens[i] = int(g)
+    # static generators of the reference class (plate_tectonic_sim.h:121-130);
+    # the generator is a SplitMix64 state, a one-element list advanced in place
+    @staticmethod
+    def generatePlateCenters(generator: list, numPlates: int, width: int, height: int) -> np.ndarray:
+        st = C.c_uint64(generator[0])
+        out = (_abi.VoronoiSeed * max(1, numPlates))()
+        check(lib().tt_generate_plate_centers(C.byref(st), numPlates, width, height, out), "tt_generate_plate_centers")
+        generator[0] = st.value
+        return np.frombuffer(bytes(out), _abi.VORONOI_SEED_DTYPE)[:numPlates].copy()
+
+    @staticmethod
+    def generateVoronoiSeeds(generator: list, centerSeeds: np.ndarray, numSeeds: int, width: int,
+                             height: int) -> np.ndarray:
+        st = C.c_uint64(generator[0])
+        c = np.ascontiguousarray(centerSeeds, _abi.VORONOI_SEED_DTYPE)
+        out = (_abi.VoronoiSeed * max(1, numSeeds))()
+        check(lib().tt_generate_voronoi_seeds(C.byref(st), c.ctypes.data_as(C.POINTER(_abi.VoronoiSeed)), len(c),
+                                              numSeeds, width, height, out), "tt_generate_voronoi_seeds")
+        generator[0] = st.value
+        return np.frombuffer(bytes(out), _abi.VORONOI_SEED_DTYPE)[:numSeeds].copy()
+
+    @staticmethod
+    def generatePlateData(generator: list, plateCenters: np.ndarray, numStartingPlates: int) -> np.ndarray:
+        st = C.c_uint64(generator[0])
+        c = np.ascontiguousarray(plateCenters, _abi.VORONOI_SEED_DTYPE)
+        out = np.zeros(_abi.MAX_PLATES, _abi.PLATE_DTYPE)
+        check(lib().tt_generate_plate_data(C.byref(st), c.ctypes.data_as(C.POINTER(_abi.VoronoiSeed)),
+                                           numStartingPlates, out.ctypes.data_as(C.POINTER(_abi.PlateData))),
+              "tt_generate_plate_data")
+        generator[0] = st.value
+        return out
 
     @property
     def world(self) -> World:

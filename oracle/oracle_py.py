@@ -34,6 +34,7 @@ def lib():
         L.oracle_world_create.argtypes = [C.POINTER(_abi.SimConfig), C.POINTER(_abi.KernelSettings), C.c_char_p]
         L.oracle_world_init.argtypes = [vp]
         L.oracle_world_step.argtypes = [vp, C.c_int]
+        L.oracle_world_set_gens.argtypes = [vp, C.POINTER(C.c_int32), C.c_int]
         L.oracle_world_get.argtypes = [vp, C.c_int, vp, C.c_size_t]
         L.oracle_world_set.argtypes = [vp, C.c_int, vp, C.c_size_t]
         L.oracle_world_set_iteration.argtypes = [vp, C.c_uint32]
@@ -65,6 +66,12 @@ class OracleWorld:
 
     def step(self, n: int = 1):
         self._ok(lib().oracle_world_step(self.h, n), "step")
+
+    def set_gens(self, gens):
+        """numVoronoiSeeds (any length) for the next init()."""
+        g = [int(x) for x in gens]
+        arr = (C.c_int32 * max(1, len(g)))(*g)
+        self._ok(lib().oracle_world_set_gens(self.h, arr, len(g)), "set_gens")
 
     def get(self, field: int) -> np.ndarray:
         dt, shape = _abi.field_spec(field, self.n)

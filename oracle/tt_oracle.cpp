@@ -107,6 +107,7 @@ struct World {
     std::vector<DistBuf> dist;
     tt_step_stats stats;
     std::vector<uint8_t> noise;  // 256x256 asthenosphere source (R channel of noise.png)
+    std::vector<int> gens;       // numVoronoiSeeds (any number of generations)
     int noiseW = 256, noiseH = 256;
 
     int idx(int x, int y) const { return wrapc(y, H) * W + wrapc(x, W); }
@@ -299,8 +300,7 @@ void init_reference(World &w)
     std::vector<V2> spos = cpos;
     std::vector<uint8_t> sid = cid;
     // generateVoronoiSeeds generations (plate_tectonic_sim.cu:892-928)
-    for (int g = 0; g < w.cfg.num_voronoi_gens; ++g) {
-        const int n = w.cfg.voronoi_gens[g];
+    for (const int n : w.gens) {
         std::vector<V2> np(n);
         std::vector<uint8_t> ni(n);
         for (int i = 0; i < n; ++i) {
@@ -1500,6 +1500,23 @@ void stage_split(World &w)
     const uint8_t old = (uint8_t)plate;
     const int ptx = (int)(w.pd[plate].geometricCenter[0] + 0.5f);
     const int pty = (int)(w.pd[plate].geometricCenter[1] + 0.5f);
+    // A negative (or NaN) cost on the plate has no fixed point: the reference
+    // relaxes until its cap of 10000 rounds (plate_tectonic_sim.cu:944-966)
+    // and goes on with whatever it holds.  Canonical outcome: the new id is
+    // taken (selectUnusedPlateId) and the split is abandoned, like a dead end.
+    bool negative = false;
+    for (size_t c = 0; c < N && !negative; ++c)
+        if (w.ids[c] == old && !(w.height[c] >= 0.0f)) negative = true;
+    if (negative) {
+        for (int i = 0; i < MAXP; ++i)
+            if (!w.pd[i].used) {
+                w.pd[i].used = 1;
+                w.stats.split_plate = plate;
+                w.stats.split_new_plate = i;
+                break;
+            }
+        return;
+    }
     // effort to boundary: 0 outside the plate, +inf inside, relaxed to the fixed point
     const float INF = tt_u2f(0x7f800000u);
     std::vector<float> eff(N, 0.0f);
@@ -1995,6 +2012,7 @@ void *oracle_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks
     w.stats.split_new_plate = -1;
     w.noise.assign(256 * 256, 0);
     if (noise256) memcpy(w.noise.data(), noise256, 256 * 256);
+    w.gens.assign(cfg->voronoi_gens, cfg->voronoi_gens + std::max(0, std::min(4, cfg->num_voronoi_gens)));
     return h;
 }
 
@@ -2005,6 +2023,14 @@ int oracle_world_init(void *hh)
     if (w.cfg.init_mode == TT_INIT_SYNTHETIC) init_synthetic(w);
     else if (w.cfg.init_mode == TT_INIT_REFERENCE) init_reference(w);
     else init_asthenosphere(w);
+    return TT_OK;
+}
+
+// numVoronoiSeeds of any length for the next oracle_world_init
+int oracle_world_set_gens(void *hh, const int32_t *gens, int n)
+{
+    if (!hh || n < 0 || (n > 0 && !gens)) return TT_ERR_INVALID;
+    ((ow_handle *)hh)->w.gens.assign(gens, gens + n);
     return TT_OK;
 }
 

@@ -1,6 +1,8 @@
 """The C++ facade (include/tectonic.hpp) compiles against the C ABI, links
-against libtectonic.so and reports a missing device as an exception instead
-of exiting (the reference exits from CUDA_ERROR_CHECK)."""
+against libtectonic.so, reports a missing device as an exception instead of
+exiting (the reference exits from CUDA_ERROR_CHECK), and on a GPU drives the
+reference's host sequence (main.cpp:135-154: construct, initialize,
+executeIteration, resetSim) with a five-generation numVoronoiSeeds list."""
 import shutil
 import subprocess
 
@@ -13,30 +15,62 @@ SRC = r'''
 #include "tectonic.hpp"
 int main()
 {
-    tt::PlateTectonicSim sim(512, 512, 1000u);
+    tt::PlateTectonicSim sim(512, 256, 1000u);
     if (sim.config().width != 512 || sim.config().num_plates != 16) return 2;
+    // the static generators run on the host
+    uint64_t gen = 1000u;
+    auto centres = tt::PlateTectonicSim::generatePlateCenters(gen, 8, 512, 256);
+    auto seeds = tt::PlateTectonicSim::generateVoronoiSeeds(gen, centres, 40, 512, 256);
+    auto pd = tt::PlateTectonicSim::generatePlateData(gen, centres, 8);
+    if (centres.size() != 8 || seeds.size() != 40 || pd.size() != 255 || pd[0].velocity <= 0.0f) return 3;
     try {
-        sim.initialize(8, {});
+        sim.initialize(8, {20, 40, 60, 80, 100});
     } catch (const tt::Error &e) {
         std::printf("error %d\n", e.status());
         return 0;
     }
     sim.executeIteration();
-    std::printf("ran %u\n", sim.iteration());
+    sim.executeIteration();
+    const unsigned it = sim.iteration();
+    sim.resetSim(7u, 10, {30, 60, 90, 120, 150});
+    if (sim.iteration() != 0 || sim.config().seed != 7u || sim.config().num_plates != 10) return 4;
+    sim.executeIteration();
+    const auto ids = sim.download<uint8_t>(TT_FIELD_PLATE_IDS);
+    std::printf("ran %u then %u, %zu ids\n", it, sim.iteration(), ids.size());
     return 0;
 }
 '''
 
 
-@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
-def test_facade_builds_and_links(tmp_path, tectonic_lib):
+def build_host(tmp_path):
     src = tmp_path / "host.cpp"
     src.write_text(SRC)
     exe = tmp_path / "host"
     root = _abi.PKG_DIR.parent
     subprocess.run(["g++", "-std=c++17", "-O1", f"-I{root / 'include'}", str(src), "-o", str(exe),
                     f"-L{_abi.PKG_DIR}", "-ltectonic", f"-Wl,-rpath,{_abi.PKG_DIR}"], check=True)
+    return exe
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_facade_builds_and_links(tmp_path, tectonic_lib):
+    exe = build_host(tmp_path)
+    import ctypes as C
+    n = C.c_int(0)
+    tectonic_lib.tt_device_count(C.byref(n))
+    if n.value > 0:
+        pytest.skip("a device is present: test_facade_steps_on_the_device runs the host")
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    # without a GPU the constructor works and initialize() throws a device error
-    assert r.stdout.startswith("error -2") or r.stdout.startswith("ran 1")
+    # without a GPU the constructor and the generators work, initialize() throws a device error
+    assert r.stdout.startswith("error -2"), r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_facade_steps_on_the_device(tmp_path, tectonic_lib):
+    print("[facade] build and run the host", flush=True)
+    exe = build_host(tmp_path)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    assert r.stdout.startswith("ran 2 then 1, 131072 ids"), r.stdout
