@@ -458,6 +458,13 @@ int tt_world_stage_times(tt_world *w, tt_stage_times *out);
 enum { TT_MATH_EXPF = 0, TT_MATH_LOG10F = 1, TT_MATH_SINF = 2, TT_MATH_COSF = 3,
        TT_MATH_SIMPLEX = 4, TT_MATH_DISCRETE = 5, TT_MATH_FMODF = 6, TT_MATH_ATAN2 = 7 };
 int tt_math_eval_device(int32_t fn, const float *in, float *out, int64_t n);
+/* Allocation-free variant of tt_math_eval_device (no counterpart in the
+ * reference): for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Stages the
+ * host arrays through a static device buffer of 262144 elements per GPU
+ * (4 MiB, held until the process ends); n above that is TT_ERR_INVALID.  Same
+ * results as tt_math_eval_device.  Calls that overlap on one GPU race. */
+int tt_math_eval_device_static(int32_t fn, const float *in, float *out, int64_t n);
 
 /* ---- HBM ceiling probe (no reference counterpart: gives the roofline a
  * measured ceiling beside the published peak).  Times `reps` launches each of

@@ -384,6 +384,7 @@ def lib() -> C.CDLL:
     L.tt_world_set_timing.argtypes = [vp, C.c_int32]
     L.tt_world_stage_times.argtypes = [vp, C.POINTER(StageTimes)]
     L.tt_math_eval_device.argtypes = [C.c_int32, vp, vp, C.c_int64]
+    L.tt_math_eval_device_static.argtypes = [C.c_int32, vp, vp, C.c_int64]
     L.tt_world_set_iteration.argtypes = [vp, C.c_uint32]
     L.tt_batch_create.argtypes = [C.POINTER(vp)]
     L.tt_batch_add.argtypes = [vp, vp]

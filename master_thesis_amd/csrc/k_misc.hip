@@ -1,6 +1,9 @@
 // k_misc.hip -- device-wide scan and ordered compaction (wavefront ballot /
 // prefix primitives), world initialisation kernels and the device math
 // self-test.
+#include <mutex>
+#include <string>
+
 #include "tt_kernels.h"
 
 // ---------------------------------------------------------------------------
@@ -387,6 +390,67 @@ hipError_t tt_launch_math_eval(hipStream_t s, int fn, const float *in, float *ou
 {
     hipLaunchKernelGGL(k_math_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fn, in, out, n);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// tt_math_eval_device_static: tt_math_eval_device without device allocation.
+// The staged arguments and results live in `scratch`, a file-scope device
+// array of TT_MATH_STATIC_MAX elements' worth (4 MiB: up to 3 input floats and
+// 1 output float per element) that each GPU holds until the process ends.  In
+// host code the array's name is only a stand-in: its device address on the
+// current GPU comes from hipGetSymbolAddress, taken once per device at the
+// first call there and kept in g_scratch.  Every call on one GPU shares that
+// GPU's array and overwrites it whole for the elements it uses (nothing needs
+// clearing); calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+#define TT_MATH_STATIC_MAX 262144
+#define TT_SCRATCH_DEVICES 64
+static __device__ __align__(256) float scratch[4 * TT_MATH_STATIC_MAX];
+static std::mutex g_scratchMu;
+static float *g_scratch[TT_SCRATCH_DEVICES] = {};
+
+void tt_set_error(const std::string &msg);
+
+// device address of `scratch` on the current device `dev`, nullptr on failure
+static float *scratch_on(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            tt_set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch[dev] = (float *)ptr;
+    }
+    return g_scratch[dev];
+}
+
+extern "C" int tt_math_eval_device_static(int32_t fn, const float *in, float *out, int64_t n)
+{
+    if (!in || !out || n < 0) return TT_ERR_INVALID;
+    if (fn < TT_MATH_EXPF || fn > TT_MATH_ATAN2) return TT_ERR_INVALID;
+    if (n > TT_MATH_STATIC_MAX) {
+        tt_set_error("tt_math_eval_device_static: n = " + std::to_string(n) + " exceeds its static buffer of " +
+                     std::to_string(TT_MATH_STATIC_MAX) + " elements");
+        return TT_ERR_INVALID;
+    }
+    if (n == 0) return TT_OK;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) { tt_set_error("no HIP device"); return TT_ERR_DEVICE; }
+    if (dev >= TT_SCRATCH_DEVICES) { tt_set_error("device index beyond the static table"); return TT_ERR_UNSUPPORTED; }
+    float *base = scratch_on(dev);
+    if (!base) return TT_ERR_DEVICE;
+    const int per = (fn == TT_MATH_SIMPLEX || fn == TT_MATH_DISCRETE) ? 3 : (fn == TT_MATH_FMODF || fn == TT_MATH_ATAN2) ? 2 : 1;
+    float *din = base, *dout = base + 3 * TT_MATH_STATIC_MAX;
+    if (hipMemcpy(din, in, (size_t)n * per * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        tt_launch_math_eval(0, fn, din, dout, n) != hipSuccess || hipStreamSynchronize(0) != hipSuccess ||
+        hipMemcpy(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        tt_set_error("device math evaluation failed");
+        return TT_ERR_DEVICE;
+    }
+    return TT_OK;
 }
 
 // ---------------------------------------------------------------------------
