@@ -71,25 +71,39 @@ def model_bytes_per_cell_step(K: int, T: int, side: int, C: int) -> float:
 
 
 def load_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    profile (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by the
+    counters' reading of a copy kernel of known size); counters cannot be read
+    from inside this process."""
     p = PROFILES / "hydraulic_traffic.json"
     if p.exists():
         try:
-            return json.loads(p.read_text()).get("bytes_per_launch")
+            return json.loads(p.read_text()).get("bytes_per_launch"), "profiles/hydraulic_traffic.json"
         except Exception:
-            return None
-    return None
+            return None, None
+    return None, None
 
 
-def round64(x: float) -> int:
-    return max(64, int(round(x / 64.0)) * 64)
+def load_step_bytes():
+    """Measured HBM bytes of one whole step (all kernels of the step, from the
+    committed counter profile of the same workload), or None."""
+    p = PROFILES / "step_4096_hbm_bytes.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get("bytes_per_step"), "profiles/step_4096_hbm_bytes.json"
+        except Exception:
+            return None, None
+    return None, None
 
 
-def mixed_world(rng, lo: int, hi: int, pmin: int, pmax: int, seed: int, cfgs):
-    side = round64(float(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
-    plates = int(rng.integers(pmin, pmax + 1))
-    cfg, ks = cfgs.synthetic(side, side, plates, (0.5, 3.0), seed, hydraulic=50, thermal=20, ccl_every=1,
-                             uplift_range=5)
-    return cfg, ks
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def run_batch_mode(args) -> dict:
@@ -97,8 +111,7 @@ def run_batch_mode(args) -> dict:
     plates U{4..40}): the same worlds stepped as one batch and one at a time."""
     from master_thesis_amd import configs
     from master_thesis_amd.sim import Batch, World
-    rng = np.random.default_rng(args.seed)
-    specs = [mixed_world(rng, 256, 2048, 4, 40, args.seed + i, configs) for i in range(args.batch_worlds)]
+    specs = configs.batch_mix(args.batch_worlds, args.seed)
     cells = sum(c.width * c.height for c, _ in specs)
     worlds = [World(c, k).init() for c, k in specs]
     warm = max(args.warmup, 1)   # the first batched step captures the erosion graph
@@ -144,7 +157,7 @@ def run_serve_mode(args) -> dict:
     t = 0.0
     for i in range(args.serve_requests):
         t += float(rng.exponential(1.0 / args.serve_rate))
-        cfg, ks = mixed_world(rng, 256, 4096, 4, 48, args.seed + 10000 + i, configs)
+        cfg, ks = configs.mixed_world(rng, 256, 4096, 4, 48, args.seed + 10000 + i)
         steps = int(rng.integers(50, 1001))
         reqs.append((t, cfg, ks, steps))
     budget = int(args.serve_budget_gb * (1 << 30))
@@ -174,6 +187,9 @@ def run_serve_mode(args) -> dict:
         "worlds_per_s": rep.worlds_per_s, "latency_p50_s": rep.latency_p50_s, "latency_p99_s": rep.latency_p99_s,
         "latency_mean_s": rep.latency_mean_s, "wall_s": rep.wall_s, "cell_updates_per_s": work / rep.wall_s,
         "bytes_peak_gb": rep.bytes_peak / (1 << 30), "batch_steps": int(rep.batch_steps),
+        "offered_worlds_per_s": args.serve_rate,
+        "offered_over_capacity": args.serve_rate / rep.worlds_per_s if rep.worlds_per_s > 0 else None,
+        "arrival_span_s": reqs[-1][0] if reqs else 0.0,
     }
 
 
@@ -186,13 +202,14 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-512", action="store_true", help="skip the CPU run of the 512^2 workload")
     ap.add_argument("--tile-rows", type=int, default=0, help="erosion tile rows (0: autotuned)")
     ap.add_argument("--mode", choices=["plain", "batch", "serve", "precision", "all"], default="all",
                     help="plain: the 4096^2 step (the metric); batch / serve: the batched and served forms; "
                          "all: the plain line with batched and served samples added")
     ap.add_argument("--batch-worlds", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=10)
-    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=100)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--precision-side", type=int, default=8192)
@@ -269,8 +286,17 @@ def main() -> int:
         init_state = {f: world.download(f) for f in STATE_FIELDS}
     world.step(args.warmup)
     world.set_timing(True)
+    # the K timed steps in up to 5 equal segments (one timed region; the
+    # segment times give the spread)
+    nseg = max(1, min(5, args.steps))
+    bounds = [round(i * args.steps / nseg) for i in range(nseg + 1)]
+    seg_ms = []
     t0 = time.perf_counter()
-    world.step(args.steps)  # returns after the device work is complete
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        ts = time.perf_counter()
+        world.step(b - a)  # returns after the device work is complete
+        if b > a:
+            seg_ms.append(1000.0 * (time.perf_counter() - ts) / (b - a))
     t1 = time.perf_counter()
     times = world.stage_times()
     stats = world.stats()
@@ -312,10 +338,12 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": load_traffic(),
+            "traffic": None,
             "bytes_per_launch": bpl,
             "avg_launch_ms": avg_ms,
         },
+        "ms_per_step_segments": seg_ms,
+        "ms_per_step_median_segment": float(np.median(seg_ms)) if seg_ms else None,
         "stages_ms_per_step": {
             "collide": times.ms_collide / args.steps, "uplift": times.ms_uplift / args.steps,
             "thermal": times.ms_thermal / args.steps, "hydraulic": times.ms_hydraulic / args.steps,
@@ -323,13 +351,22 @@ def main() -> int:
             "split": times.ms_split / args.steps, "events_total": times.ms_total / args.steps,
         },
         "step_model": {
-            "bytes_per_cell_step": model_b,
-            "frac_of_hbm": model_b * N * args.steps / wall / (HBM_PEAK_GBS * 1e9),
+            "reference_pass_bytes_per_cell_step": model_b,
+            "note": "byte model of the reference's separate passes (B = 135 + 8P + 8T + 128K + 10/C); the fused "
+                    "kernels move less, so it is not a measure of HBM use",
         },
         "final_stats": {"plates": stats.number_of_plates, "labels": stats.num_labels,
                         "iteration": stats.iteration, "tile_rows": stats.tile_rows},
     }
     world.close()
+    traffic, tsrc = load_traffic()
+    result["roofline"]["traffic"] = traffic
+    result["roofline"]["traffic_source"] = tsrc
+    sb, ssrc = load_step_bytes()
+    if sb:
+        step_s = wall / args.steps
+        result["step_hbm"] = {"bytes_per_step": sb, "source": ssrc, "achieved_gbps": sb / step_s / 1e9,
+                              "frac_of_peak": sb / step_s / 1e9 / HBM_PEAK_GBS}
     if not args.no_hbm_probe:
         # measured ceilings beside the published peak: a float4 copy has the
         # same 50/50 read/write mix as the hydraulic kernel
@@ -342,23 +379,45 @@ def main() -> int:
                                                  if probe["streams_gbps"] > 0 else None)
 
     if init_state is not None:
+        # the CPU restatement timed on this host: 3 steps of the same world from
+        # the same initial state (the metric's workload), and the 512^2 workload
+        # in full (100 steps), median of 3 runs
         from oracle.oracle_py import OracleWorld
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         ow = OracleWorld(cfg, ks)
         for f, a in init_state.items():
             ow.set(f, a)
         ow.set_iteration(0)
+        csteps = 3
         c0 = time.perf_counter()
-        ow.step(1)
+        ow.step(csteps)
         c1 = time.perf_counter()
+        ow.close()
         result["cpu_baseline"] = {
-            "value": N / (c1 - c0),
+            "value": N * csteps / (c1 - c0),
             "unit": "cell-updates/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"1 step of the same {cfg.width}x{cfg.height} world from the same initial state "
+            "cpu_model": cpu_model(),
+            "sample": f"{csteps} steps of the same {cfg.width}x{cfg.height} world from the same initial state "
                       f"(CPU restatement, OpenMP, {threads} threads)",
         }
+        if not args.no_cpu_512:
+            c512, k512 = configs.world_512(seed=args.seed)
+            n512, s512 = c512.width * c512.height, configs.STEPS["world_512"]
+            runs = []
+            for _ in range(3):
+                o5 = OracleWorld(c512, k512)
+                o5.init()
+                r0 = time.perf_counter()
+                o5.step(s512)
+                runs.append(n512 * s512 / (time.perf_counter() - r0))
+                o5.close()
+            result["cpu_world_512"] = {
+                "value": float(np.median(runs)), "runs": runs, "unit": "cell-updates/s", "cores": threads,
+                "kind": "port", "cpu_model": cpu_model(),
+                "sample": f"512x512, 8 plates, {s512} steps (K=50, T=20, CCL every 10), median of 3 runs",
+            }
     if args.mode == "all":
         for key, fn in (("batched", run_batch_mode), ("serving", run_serve_mode)):
             try:

@@ -17,6 +17,8 @@ generation_settings.h:67-76):
 """
 from __future__ import annotations
 
+import numpy as np
+
 from ._abi import KernelSettings, SimConfig, TT_INIT_SYNTHETIC, TT_STAGE_ALL
 
 
@@ -64,4 +66,23 @@ def world_8192(seed: int = 1):
     return synthetic(8192, 8192, 64, (0.5, 3.0), seed, ccl_every=1, uplift_range=5)
 
 
-STEPS = {"world_512": 100, "world_4096": 500, "world_8192": 300}
+def round64(x: float) -> int:
+    return max(64, int(round(x / 64.0)) * 64)
+
+
+def mixed_world(rng, lo: int, hi: int, pmin: int, pmax: int, seed: int):
+    """A square world with side log-uniform in [lo, hi] rounded to a multiple
+    of 64 and plates uniform in {pmin..pmax}; the erosion of world_4096."""
+    side = round64(float(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+    plates = int(rng.integers(pmin, pmax + 1))
+    return synthetic(side, side, plates, (0.5, 3.0), seed, hydraulic=50, thermal=20, ccl_every=1, uplift_range=5)
+
+
+def batch_mix(n: int = 256, seed: int = 1):
+    """The batched workload: n worlds, sides log-uniform in [256, 2048]
+    (multiples of 64), plates U{4..40}, world seeds seed + i (200 steps)."""
+    rng = np.random.default_rng(seed)
+    return [mixed_world(rng, 256, 2048, 4, 40, seed + i) for i in range(n)]
+
+
+STEPS = {"world_512": 100, "world_4096": 500, "world_8192": 300, "batch_mix": 200}

@@ -195,6 +195,23 @@ def test_jfa_single_seed_is_exact(O):
         assert (orig == sy * W + sx).all()
 
 
+def test_jfa_distance_rounds_squared_distances_beyond_2_pow_24(O):
+    """JFAComputeDistance (plate_tectonics_kernel.cu:1573-1600) stores
+    (int)sqrtf((float)(dx*dx + dy*dy)); on a 16384-wide world the squared
+    distances pass 2^24, the float conversion rounds, and some distances
+    exceed the exact integer root by one."""
+    W, H = 16384, 256
+    m = np.zeros((H, W), bool)
+    m[0, 0] = True
+    dist, orig = O.jfa(m)
+    d2 = _torus_d2(W, H, 0, 0)
+    assert d2.max() > 2 ** 24
+    assert np.array_equal(dist, np.sqrt(d2.astype(F32)).astype(np.int32))
+    exact = np.floor(np.sqrt(d2.astype(np.float64))).astype(np.int32)
+    assert (dist != exact).sum() > 50
+    assert (orig == 0).all()
+
+
 def test_jfa_multi_seed_bounds(O):
     rng = np.random.default_rng(3)
     W, H = 96, 64
