@@ -45,14 +45,16 @@ PROFILES = ROOT / "profiles"
 def hydraulic_bytes_per_cell(first: bool, last: bool) -> int:
     """Compulsory HBM bytes of one fused hydraulic launch per cell.
 
-    reads: height 4, water 4, flux 16, and either the step's sediment 4 (first
-    sub-iteration) or the previous sub-iteration's flow water 4 + the
-    transported sediment gather 4; writes: height 4, water 4, flux 16,
-    sediment 4, and the flow water 4 (the velocity 8 on the last
-    sub-iteration).
+    reads: height 4, water 4 (the step's state water on the first
+    sub-iteration, else the previous sub-iteration's water after its flow),
+    flux 16, and the step's sediment 4 (first) or the transported-sediment
+    gather 4; writes: height 4, water 4 (the flow water, or the state water on
+    the last sub-iteration), flux 16, sediment 4, and the velocity 8 on the
+    last sub-iteration.
     """
-    reads = 4 + 4 + 16 + (4 if first else 4 + 4)
-    writes = 4 + 4 + 16 + 4 + (8 if last else 4)
+    del first
+    reads = 4 + 4 + 16 + 4
+    writes = 4 + 4 + 16 + 4 + (8 if last else 0)
     return reads + writes
 
 

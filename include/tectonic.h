@@ -143,7 +143,10 @@ typedef struct tt_sim_config {
     float speed_max;
     int32_t stage_mask;            /* TT_STAGE_* */
     int32_t tile_rows;             /* rows of the hydraulic tiles: 8, 16 or 32 (dividing height);
-                                      0 = chosen by the tile autotuner (thermal tiles: 16 rows) */
+                                      0 = chosen by the tile autotuner (cached per size class; without
+                                      a cached choice the first step times its sub-iterations at each
+                                      height and keeps the fastest); -1 = tune even when cached.
+                                      Thermal tiles have 16 rows (their skip flags are per 64x16 tile). */
     int32_t uplift_mode;           /* TT_UPLIFT_JFA (the reference's live path) or TT_UPLIFT_SEPARABLE */
     int32_t precision;             /* TT_PRECISION_FP32 (reference arithmetic, bit-exact with the oracle)
                                       or TT_PRECISION_COMPENSATED (Kahan height / sediment updates) */

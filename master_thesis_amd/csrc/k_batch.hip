@@ -62,15 +62,14 @@ __global__ __launch_bounds__(256) void k_hydraulic_batch(const tt_bworld *__rest
     A.rL = B.rL;
     A.divL = B.divL;
     A.sqrtOk = B.sqrtOk;
-    const int hc = B.hc1 ^ (k & 1), wc = B.wc ^ (k & 1), fc = B.fc ^ (k & 1);
+    A.sin003 = B.sin003;
+    const int hc = B.hc1 ^ (k & 1), fc = B.fc ^ (k & 1);
     A.hin = B.h[hc];
     A.hout = B.h[hc ^ 1];
-    A.win = B.water[wc];
-    A.wout = B.water[wc ^ 1];
+    tt_hyd_water_bufs(B.water[0], B.water[1], B.wn, B.wc, k, B.K, &A.win, &A.wout);
     A.fin = B.flux[fc];
     A.fout = B.flux[fc ^ 1];
     A.vel = B.vel;
-    A.wn = B.wn;
     A.ch = nullptr;
     A.csprev = nullptr;
     A.csout = nullptr;

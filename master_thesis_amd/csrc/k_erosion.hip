@@ -129,12 +129,11 @@ static void launch_hyd(hipStream_t s, unsigned blocks, bool first, bool last, co
 
 hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int rows, int first, int last, int rseed,
                                const float *hin, float *hout, const float *win, float *wout, const float4 *fin,
-                               float4 *fout, float2 *vel, float *wn, const float *sprev, float *sout, float *ch,
+                               float4 *fout, float2 *vel, const float *sprev, float *sout, float *ch,
                                const float *csprev, float *csout)
 {
     tt_hyd_args A;
     tt_hyd_settings(d, rseed, A);
-    A.wn = wn;
     A.hin = hin;
     A.hout = hout;
     A.win = win;

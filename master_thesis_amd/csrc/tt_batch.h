@@ -17,6 +17,7 @@ struct tt_bworld {
     float rL;                                 // RN(1 / L)
     int divL;                                 // the division by L may take tt_div_const
     int sqrtOk;                               // square roots may take tt_sqrt_normal
+    float sin003;                             // tt_sinf(0.03f)
     float *h[2];
     float *water[2];
     float4 *flux[2];

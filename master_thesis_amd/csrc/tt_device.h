@@ -68,6 +68,27 @@ struct tt_step_ctl {
     long long ledger;
 };
 
+// Water buffers of hydraulic sub-iteration k of K.  Between sub-iterations
+// only the water after the flow (before evaporation) is stored: the next
+// sub-iteration evaporates it and adds its rain while loading, and recomputes
+// the previous velocity from it.  The first sub-iteration reads the state
+// water water[wc]; sub-iteration k < K - 1 writes its flow water to X
+// (k even) or Y (k odd), with X = water[wc ^ 1] and Y = flow; the last writes
+// the state water after evaporation into water[wc] (free once the first has
+// read it), or into water[wc ^ 1] when it is also the first.  The state water
+// index after the step is tt_hyd_water_index_after(wc, K).
+__host__ __device__ inline void tt_hyd_water_bufs(float *w0, float *w1, float *flow, int wc, int k, int K,
+                                                  const float **win, float **wout)
+{
+    float *water[2] = {w0, w1};
+    float *X = water[wc ^ 1], *Y = flow;
+    *win = k == 0 ? water[wc] : (((k - 1) & 1) ? Y : X);
+    if (k == K - 1) *wout = K == 1 ? water[wc ^ 1] : water[wc];
+    else *wout = (k & 1) ? Y : X;
+}
+
+__host__ __device__ inline int tt_hyd_water_index_after(int wc, int K) { return K == 1 ? wc ^ 1 : wc; }
+
 // uint8 plate id arithmetic of the reference (MAX_PLATE_COUNT - byte)
 __device__ __forceinline__ int tt_slot_plate(uint32_t word, int slot)
 {

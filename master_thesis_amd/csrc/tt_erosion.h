@@ -232,14 +232,14 @@ struct tt_hyd_args {
     float rL;             // RN(1 / L)
     int divL;             // 1: the division by L may take tt_div_const
     int sqrtOk;           // 1: square roots of normal arguments may take tt_sqrt_normal
+    float sin003;         // tt_sinf(0.03f): the capacity's sine below the slope threshold
     const float *hin;
     float *hout;
-    const float *win;
-    float *wout;
+    const float *win;     // first sub-iteration: the state water; later: the previous one's flow water
+    float *wout;          // last sub-iteration: the state water after evaporation; earlier: the flow water
     const float4 *fin;
     float4 *fout;
     float2 *vel;          // written by the last sub-iteration only
-    float *wn;            // water after the flow: the previous sub-iteration's in, this one's out (in place)
     const float *sprev;
     float *sout;
     float *ch;            // precision mode: height compensation (in place)
@@ -270,7 +270,12 @@ __host__ __device__ inline void tt_hyd_settings(const tt_dev &d, int rseed, tt_h
     A.rL = d.rL;
     A.divL = d.divL;
     A.sqrtOk = d.sqrtOk;
+    A.sin003 = tt_sinf(0.03f);
 }
+
+// Squared slopes up to this bound have a square root below 0.03f (the root is
+// monotone and 0.00089 < 0.03f^2), so max(slope, 0.03f) is 0.03f there.
+#define TT_SLOPE2_SMALL 0.00089f
 
 // IEEE square root; a wavefront whose arguments all lie in [2^-96, FLT_MAX]
 // takes tt_sqrt_normal when it has been checked (ok)
@@ -356,6 +361,12 @@ __device__ __forceinline__ float tt_sinf_wave(float x)
     return tt_sinf(x);
 }
 
+// evaporate (plate_tectonics_kernel.cu:2587-2593) with dt = 1
+__device__ __forceinline__ float tt_evaporate(float w, float evap)
+{
+    return tt_fmaxf(0.0f, w - (tt_fminf(w, 20.0f) * evap) * 1.0f);
+}
+
 // rain amount of a cell: max(0, discreteNoise((x, 0, y), 1, seed + iter)) * dt * rain.
 // For a cell (0 <= x, y < 2^24) the lattice point is the cell itself, and the
 // lattice sum ((x*1723 + 0*93241) + y*149812) + 3824 + seed of tt_random_grid
@@ -412,20 +423,21 @@ __device__ __forceinline__ void tt_flow_velocity_dir(const float4 &Sf, const flo
     }
 }
 
-// One sub-iteration on one tile.  FIRST: the water input has no rain yet
-// (rain is added while loading) and the sediment is the step's state;
-// otherwise the water input already holds this sub-iteration's rain (added by
-// the previous one on its final write) and the sediment is the previous
-// sub-iteration's output transported along its velocity.  LAST: the water
-// output gets no rain.
+// One sub-iteration on one tile.  FIRST: the water input is the step's
+// state water and the sediment is the step's state; otherwise the water input
+// is the previous sub-iteration's water after its flow (before evaporation),
+// which this one evaporates while loading, and the sediment is the previous
+// sub-iteration's output transported along its velocity.  Every sub-iteration
+// adds its rain while loading.  LAST: the water output is the state water
+// after evaporation; otherwise it is the water after the flow.
 //
-// Between sub-iterations the velocity is not stored: a sub-iteration stores
-// its water after the flow (4 bytes a cell instead of an 8-byte velocity), and
-// the next one recomputes the velocity, bit for bit, from that water and the
-// fluxes it reads anyway (its incoming flux ring).  Only the last
+// Between sub-iterations only that flow water is stored (4 bytes a cell; the
+// evaporated water and the velocity are functions of it): the next
+// sub-iteration evaporates it, and recomputes the velocity, bit for bit, from
+// it and the fluxes it reads anyway (its incoming flux ring).  Only the last
 // sub-iteration writes the velocity field, for the closing transport and the
 // state.  Loads are issued together before the first barrier (heights and
-// water with their halo, the flux ring, the flow water); the sediment taps
+// water with their halo, the flux ring, the own flow water); the sediment taps
 // follow the velocity and are in flight while the new fluxes are computed.
 //
 // The kernel is bound by its vector ALU work more than by HBM, so addressing
@@ -451,7 +463,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
     float wnp[RPW];
     if (!FIRST) {
 #pragma unroll
-        for (int r = 0; r < RPW; ++r) wnp[r] = A.wn[(y0 + wv * RPW + r) * W + gx];
+        for (int r = 0; r < RPW; ++r) wnp[r] = A.win[(y0 + wv * RPW + r) * W + gx];
     }
     // 2. heights and water on the tile plus a two-cell halo: rows wv, wv+4,
     // ...; lanes 0..63 take columns 0..63, lanes 0..3 also columns 64..67
@@ -471,17 +483,32 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             }
         }
     }
-    // 3. incoming fluxes of the flux ring
+    // 3. incoming fluxes of the flux ring.  Item i = tid + 256 k sits at row
+    // i / TT_FW, column i % TT_FW of the ring; 256 = 3 TT_FW + 58, so the
+    // coordinates of item k follow from those of item k - 1 by additions.
+    int fyk[NFR], fxk[NFR];
+    {
+        int fy = tid / TT_FW, fx = tid - (tid / TT_FW) * TT_FW;
+#pragma unroll
+        for (int k = 0; k < NFR; ++k) {
+            fyk[k] = fy;
+            fxk[k] = fx;
+            fx += 256 - 3 * TT_FW;
+            fy += 3;
+            if (fx >= TT_FW) { fx -= TT_FW; fy += 1; }
+        }
+    }
     float4 cf[NFR];
 #pragma unroll
     for (int k = 0; k < NFR; ++k) {
         const int i = tid + 256 * k;
         if (i < NFI) {
-            const int fy = i / TT_FW, fx = i - fy * TT_FW;
+            const int fy = fyk[k], fx = fxk[k];
             cf[k] = IN ? A.fin[(y0 + fy - 1) * W + x0 + fx - 1]
                        : A.fin[tt_wrap1(y0 + fy - 1, H) * W + tt_wrap1(x0 + fx - 1, W)];
         }
     }
+    const float evap = A.evap;
     // sediment: the state itself on the first sub-iteration
     float s4[RPW][4], c4[RPW][4];
     float wxs[RPW], wys[RPW];
@@ -498,13 +525,17 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         const int ly = wv + 4 * j;
         if (ly < TH + 4) {
             const int gy = IN ? y0 + ly - 2 : tt_wrap1(y0 + ly - 2, H);
+            // the water this sub-iteration starts from: the state water plus
+            // rain (first), or the previous flow water evaporated plus rain
             float m = hm[j], w = hw[j];
-            if (FIRST) w = w + tt_rain_at(gxa, gy, rseed, rain);
+            if (!FIRST) w = tt_evaporate(w, evap);
+            w = w + tt_rain_at(gxa, gy, rseed, rain);
             S.M[ly][lane] = m;
             S.Wt[ly][lane] = w;
             if (lane < TT_EW - 64) {
                 float w2 = hw2[j];
-                if (FIRST) w2 = w2 + tt_rain_at(gxb, gy, rseed, rain);
+                if (!FIRST) w2 = tt_evaporate(w2, evap);
+                w2 = w2 + tt_rain_at(gxb, gy, rseed, rain);
                 S.M[ly][64 + lane] = hm2[j];
                 S.Wt[ly][64 + lane] = w2;
             }
@@ -515,10 +546,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
 #pragma unroll
         for (int k = 0; k < NFR; ++k) {
             const int i = tid + 256 * k;
-            if (i < NFI) {
-                const int fy = i / TT_FW, fx = i - fy * TT_FW;
-                S.F[fy][fx] = cf[k];
-            }
+            if (i < NFI) S.F[fyk[k]][fxk[k]] = cf[k];
         }
     }
     __syncthreads();
@@ -555,7 +583,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
     for (int k = 0; k < NFR; ++k) {
         const int i = tid + 256 * k;
         if (i >= NFI) break;
-        const int fy = i / TT_FW, fx = i - fy * TT_FW;
+        const int fy = fyk[k], fx = fxk[k];
         const int ly = fy + 1, lx = fx + 1;
         const float mc = S.M[ly][lx], wc = S.Wt[ly][lx];
         const float sc = mc + wc;
@@ -582,16 +610,23 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         o.z = tt_fmaxf(0.0f, cf[k].z + fk * q2);
         o.w = tt_fmaxf(0.0f, cf[k].w + fk * q3);
         const float tot = (((o.x + o.y) + o.z) + o.w) + 1e-6f;
-        float K = tt_fminf(1.0f, wc / (tot * 1.0f + 1e-6f));
-        K = tt_fmaxf(K, 0.01f);
-        o.x = o.x * K;
-        o.y = o.y * K;
-        o.z = o.z * K;
-        o.w = o.w * K;
+        const float den = tot * 1.0f + 1e-6f;
+        // K = max(min(1, wc / den), 0.01): wc >= den gives a quotient of at
+        // least 1 (the division is correctly rounded), so K = 1 and the
+        // scaling is the identity; a wavefront where that holds in every lane
+        // skips the division and the products
+        if (__builtin_amdgcn_ballot_w64(!(wc >= den)) != 0) {
+            float K = tt_fminf(1.0f, wc / den);
+            K = tt_fmaxf(K, 0.01f);
+            o.x = o.x * K;
+            o.y = o.y * K;
+            o.z = o.z * K;
+            o.w = o.w * K;
+        }
         S.F[fy][fx] = o;
     }
     __syncthreads();
-    const float evap = A.evap, cap = A.cap, ke = A.ke, kd = A.kd;
+    const float cap = A.cap, ke = A.ke, kd = A.kd;
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
         const int ry = wv * RPW + r;
@@ -611,9 +646,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         const float wn = tt_fmaxf(0.0f, S.Wt[ly][lx] + dv);
         float vx, vy, vmag;
         tt_flow_velocity(Sf, Lf, Uf, Rf, Df, wn, vx, vy, vmag, A.sqrtOk);
-        float wo = tt_fmaxf(0.0f, wn - (tt_fminf(wn, 20.0f) * evap) * 1.0f);
-        if (!LAST) wo = wo + tt_rain_at(gx, gy, rseed, rain);
-        __builtin_nontemporal_store(wo, &A.wout[c]);
+        __builtin_nontemporal_store(LAST ? tt_evaporate(wn, evap) : wn, &A.wout[c]);
         __builtin_nontemporal_store(Sf.x, &A.fout[c].x);
         __builtin_nontemporal_store(Sf.y, &A.fout[c].y);
         __builtin_nontemporal_store(Sf.z, &A.fout[c].z);
@@ -629,15 +662,18 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         if (LAST) {
             __builtin_nontemporal_store(vx, &A.vel[c].x);
             __builtin_nontemporal_store(vy, &A.vel[c].y);
-        } else {
-            __builtin_nontemporal_store(wn, &A.wn[c]);
         }
         const float mc = S.M[ly][lx];
         // Slope: (m[+1] - m[-1]) / 2.0 in the reference; halving is exact in float
         const float dzdx = (S.M[ly][lx + 1] - S.M[ly][lx - 1]) * 0.5f;
         const float dzdy = (S.M[ly + 1][lx] - S.M[ly - 1][lx]) * 0.5f;
-        const float slope = tt_sqrt_wave(dzdx * dzdx + dzdy * dzdy, A.sqrtOk);
-        float C = (cap * tt_sinf_wave(tt_fmaxf(slope, 0.03f))) * vmag;
+        // sin(max(slope, 0.03)): a wavefront whose slopes all lie below 0.03
+        // takes the constant; the others compute the root and the sine
+        const float s2 = dzdx * dzdx + dzdy * dzdy;
+        float sn = A.sin003;
+        if (__builtin_amdgcn_ballot_w64(!(s2 <= TT_SLOPE2_SMALL)) != 0)
+            sn = tt_sinf_wave(tt_fmaxf(tt_sqrt_wave(s2, A.sqrtOk), 0.03f));
+        float C = (cap * sn) * vmag;
         C = tt_fminf(C, 5.0f);
         float hnew;
         float chv = 0.0f;

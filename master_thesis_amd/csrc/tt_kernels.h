@@ -69,7 +69,7 @@ hipError_t tt_launch_thermal_frozen(hipStream_t s, const tt_dev &d, int T, const
                                     uint8_t *frozen);
 hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int rows, int first, int last, int rseed,
                                const float *hin, float *hout, const float *win, float *wout, const float4 *fin,
-                               float4 *fout, float2 *vel, float *wn, const float *sprev, float *sout, float *ch = nullptr,
+                               float4 *fout, float2 *vel, const float *sprev, float *sout, float *ch = nullptr,
                                const float *csprev = nullptr, float *csout = nullptr);
 hipError_t tt_launch_transport(hipStream_t s, const tt_dev &d, const float *sprev, const float2 *vel, float *sout,
                                const float *csprev = nullptr, float *csout = nullptr);

@@ -28,7 +28,10 @@ struct tt_world {
     uint32_t iter = 0;
     bool initialized = false;
     int device = 0;
-    int rows = 0;            // rows of the hydraulic tiles (0 until resolved)
+    int rows = 0;            // rows of the hydraulic tiles (0 or -1 until resolved)
+    bool tuning = false;     // the tile height is tuned on the next step with enough sub-iterations
+    hipEvent_t tuneEv[6] = {};
+    int tuneRows[3] = {}, tuneCount[3] = {}, tuneGroups = 0;
     int ehc = 0;             // height buffer index during erosion
     hipStream_t stream = nullptr;
     char *base = nullptr;

@@ -529,7 +529,7 @@ int tt_step_hydraulic_count(const tt_world *w)
 void tt_step_erode_indices(tt_world *w, int T, int K, int *sedLast)
 {
     w->ehc ^= (T + K) & 1;
-    w->wc ^= K & 1;
+    w->wc = tt_hyd_water_index_after(w->wc, K);
     w->fc ^= K & 1;
     *sedLast = K > 0 ? (w->sc ^ ((K - 1) & 1)) : w->sc;
 }
@@ -568,17 +568,35 @@ static int step_erode(tt_world *w)
     }
     CKS(record(w, 3));
     const int rseed = (int)(w->cfg.seed + w->iter);
-    int wc = w->wc, fc = w->fc;
+    // online tile tuning: groups of G middle sub-iterations per candidate height
+    int G = 0, ncand = 0;
+    if (w->tuning) {
+        for (int rows : {16, 8, 32})
+            if (tt_tile_rows_valid(rows, w->H)) w->tuneRows[ncand++] = rows;
+        G = ncand > 0 ? std::min(16, (K - 2) / ncand) : 0;
+        if (G < 4) G = 0;
+    }
+    int fc = w->fc;
     for (int it = 0; it < K; ++it) {
         const int si = it == 0 ? w->sc : (w->sc ^ ((it - 1) & 1)), so = it == 0 ? w->sc : (w->sc ^ (it & 1));
         float *sprev = w->sed[si], *sout = w->sed[so];
-        CK(tt_launch_hydraulic(s, d, w->rows, it == 0, it == K - 1, rseed, w->h[hc], w->h[1 - hc], w->water[wc],
-                               w->water[1 - wc], w->flux[fc], w->flux[1 - fc], w->vel, w->wn, sprev, sout, w->ch,
-                               w->ch ? w->cs[si] : nullptr, w->ch ? w->cs[so] : nullptr));
+        const float *win;
+        float *wout;
+        tt_hyd_water_bufs(w->water[0], w->water[1], w->wn, w->wc, it, K, &win, &wout);
+        const int grp = (G > 0 && it >= 1 && it < 1 + G * ncand) ? (it - 1) / G : -1;
+        const int rows = grp >= 0 ? w->tuneRows[grp] : w->rows;
+        if (grp >= 0 && (it - 1) % G == 0) CK(hipEventRecord(w->tuneEv[2 * grp], s));
+        CK(tt_launch_hydraulic(s, d, rows, it == 0, it == K - 1, rseed, w->h[hc], w->h[1 - hc], win, wout,
+                               w->flux[fc], w->flux[1 - fc], w->vel, sprev, sout, w->ch, w->ch ? w->cs[si] : nullptr,
+                               w->ch ? w->cs[so] : nullptr));
+        if (grp >= 0 && (it - 1) % G == G - 1) {
+            CK(hipEventRecord(w->tuneEv[2 * grp + 1], s));
+            w->tuneCount[grp] = G;
+        }
         hc = 1 - hc;
-        wc = 1 - wc;
         fc = 1 - fc;
     }
+    w->tuneGroups = G > 0 ? ncand : 0;
     CKS(record(w, 4));
     int sedLast = 0;
     tt_step_erode_indices(w, T, K, &sedLast);
@@ -586,6 +604,8 @@ static int step_erode(tt_world *w)
     CKS(record(w, 5));
     return TT_OK;
 }
+
+static int tune_finish(tt_world *w);
 
 int tt_step_post(tt_world *w)
 {
@@ -618,6 +638,7 @@ int tt_step_post(tt_world *w)
     CK(tt_launch_ledger(s, w->h[w->hc], (long long)N, &w->ctl->ledger));
     CKS(record(w, 7));
     CKS(sync_ctl(w));
+    if (w->tuneGroups > 0) CKS(tune_finish(w));
     const tt_step_ctl &c = *w->hctl;
     w->stats.overflow_cells = (mask & TT_STAGE_COLLIDE) ? c.overflow_cells : 0;
     w->stats.complex_sources = ((mask & TT_STAGE_COLLIDE) && upl) ? c.complex_sources : 0;
@@ -952,46 +973,43 @@ void tune_save()
 }
 }  // namespace
 
-static int tune_rows(tt_world *w)
+// A cached choice for the world's size class sets the tile height at once.
+// Without one (or with tile_rows = -1) the world starts on 16-row tiles and
+// tunes online: its first step with enough hydraulic sub-iterations runs
+// groups of its middle sub-iterations at each candidate height (all heights
+// give the same bits), timed with events on the world's stream, and
+// tune_finish keeps the fastest per launch for the size class.
+static int tune_rows(tt_world *w, bool force)
 {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     if (!g_tune_loaded) tune_load();
     const std::string key = tune_key(w);
     const auto it = g_tune.find(key);
-    if (it != g_tune.end() && tt_tile_rows_valid(it->second, w->H)) {
+    if (!force && it != g_tune.end() && tt_tile_rows_valid(it->second, w->H)) {
         w->rows = it->second;
+        w->tuning = false;
         return TT_OK;
     }
-    // defined contents for the timing launches (initialisation overwrites them)
-    CKS(init_common_zero(w));
-    for (int i = 0; i < 2; ++i) CKS(zero(w, w->h[i], (size_t)w->N * 4));
-    const tt_dev d = make_dev(w);
-    hipEvent_t e0, e1;
-    CK(hipEventCreate(&e0));
-    CK(hipEventCreate(&e1));
-    int best = 16;
+    w->rows = 16;
+    w->tuning = true;
+    return TT_OK;
+}
+
+static int tune_finish(tt_world *w)
+{
+    int best = w->rows;
     float bestMs = 1e30f;
-    const int cand[3] = {16, 8, 32};
-    for (int rows : cand) {
-        if (!tt_tile_rows_valid(rows, w->H)) continue;
-        auto launch = [&](int k) {
-            return tt_launch_hydraulic(w->stream, d, rows, 0, 0, 1, w->h[k & 1], w->h[1 - (k & 1)], w->water[k & 1],
-                                       w->water[1 - (k & 1)], w->flux[k & 1], w->flux[1 - (k & 1)], w->vel, w->wn,
-                                       w->sed[k & 1], w->sed[1 - (k & 1)]);
-        };
-        CK(launch(0));
-        CK(hipEventRecord(e0, w->stream));
-        for (int k = 1; k <= 4; ++k) CK(launch(k));
-        CK(hipEventRecord(e1, w->stream));
-        CK(hipEventSynchronize(e1));
+    for (int i = 0; i < w->tuneGroups; ++i) {
         float ms = 0.0f;
-        CK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < bestMs) { bestMs = ms; best = rows; }
+        CK(hipEventElapsedTime(&ms, w->tuneEv[2 * i], w->tuneEv[2 * i + 1]));
+        const float per = ms / (float)w->tuneCount[i];
+        if (per < bestMs) { bestMs = per; best = w->tuneRows[i]; }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    w->tuneGroups = 0;
+    w->tuning = false;
     w->rows = best;
-    g_tune[key] = best;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[tune_key(w)] = best;
     tune_save();
     return TT_OK;
 }
@@ -1016,8 +1034,8 @@ int tt_validate_config(const tt_sim_config *c)
         set_error("precision must be TT_PRECISION_FP32 or TT_PRECISION_COMPENSATED");
         return TT_ERR_INVALID;
     }
-    if (c->tile_rows != 0 && !tt_tile_rows_valid(c->tile_rows, c->height)) {
-        set_error("tile_rows must be 0, or 8, 16 or 32 dividing the height");
+    if (c->tile_rows != 0 && c->tile_rows != -1 && !tt_tile_rows_valid(c->tile_rows, c->height)) {
+        set_error("tile_rows must be 0 or -1 (tuned), or 8, 16 or 32 dividing the height");
         return TT_ERR_INVALID;
     }
     return TT_OK;
@@ -1146,6 +1164,8 @@ int tt_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, tt_w
     }
     for (int i = 0; i < 8; ++i)
         if (hipEventCreate(&w->ev[i]) != hipSuccess) { tt_world_destroy(w); return TT_ERR_DEVICE; }
+    for (int i = 0; i < 6; ++i)
+        if (hipEventCreate(&w->tuneEv[i]) != hipSuccess) { tt_world_destroy(w); return TT_ERR_DEVICE; }
     memset(&w->stats, 0, sizeof(w->stats));
     memset(&w->times, 0, sizeof(w->times));
     w->stats.split_plate = w->stats.split_new_plate = -1;
@@ -1166,7 +1186,7 @@ int tt_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, tt_w
 int tt_world_init(tt_world *w)
 {
     if (!w) return TT_ERR_INVALID;
-    if (w->rows == 0) CKS(tune_rows(w));
+    if (w->rows <= 0) CKS(tune_rows(w, w->cfg.tile_rows < 0));
     CKS(tt_world_do_init(w));
     w->stats.tile_rows = w->rows;
     w->stats.hydraulic_div_const = w->divL;
@@ -1188,7 +1208,7 @@ int tt_world_reset(tt_world *w, uint32_t seed, int32_t num_plates, const int32_t
     for (int i = 0; i < c.num_voronoi_gens; ++i) c.voronoi_gens[i] = gens[(size_t)i];
     w->cfg = c;
     w->gens = gens;
-    if (w->rows == 0) CKS(tune_rows(w));
+    if (w->rows <= 0) CKS(tune_rows(w, w->cfg.tile_rows < 0));
     CKS(tt_world_do_init(w));
     w->stats.tile_rows = w->rows;
     w->stats.hydraulic_div_const = w->divL;
@@ -1494,6 +1514,8 @@ int tt_world_destroy(tt_world *w)
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     for (int i = 0; i < 8; ++i)
         if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);
+    for (int i = 0; i < 6; ++i)
+        if (w->tuneEv[i]) (void)hipEventDestroy(w->tuneEv[i]);
     if (w->stream) (void)hipStreamDestroy(w->stream);
     drop_pre_graphs(w);
     if (w->hctl) (void)hipHostFree(w->hctl);

@@ -185,6 +185,16 @@ def test_tile_rows(sim, O, rows):
     assert all(s.tile_rows == rows for s in seen)
 
 
+def test_online_tuning_step_matches_oracle(sim, O):
+    """tile_rows = -1: the first step runs groups of its sub-iterations at 16,
+    8 and 32 rows to time them; the step's bits do not depend on the heights,
+    and the tuned height is one of the candidates."""
+    cfg, ks = configs.synthetic(384, 384, 7, (0.5, 3.0), 4, hydraulic=26, thermal=3, ccl_every=1)
+    cfg.tile_rows = -1
+    seen = run_parity(sim, O, cfg, ks, 2, "online tuning")
+    assert seen[0].tile_rows in (8, 16, 32) and seen[1].tile_rows == seen[0].tile_rows
+
+
 def test_tuned_tile_rows_are_valid(sim):
     cfg, ks = configs.synthetic(256, 192, 4, (0.5, 3.0), 3, hydraulic=2, thermal=1)
     with sim.World(cfg, ks).init() as g:
