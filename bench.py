@@ -206,6 +206,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cpu-512", action="store_true", help="skip the CPU run of the 512^2 workload")
     ap.add_argument("--tile-rows", type=int, default=0, help="erosion tile rows (0: autotuned)")
+    ap.add_argument("--uplift", choices=["jfa", "separable"], default="jfa",
+                    help="uplift spread: the jump flood (the reference's live path) or the bounded-radius "
+                         "separable pass (radius = blur radius)")
     ap.add_argument("--mode", choices=["plain", "batch", "serve", "precision", "all"], default="all",
                     help="plain: the 4096^2 step (the metric); batch / serve: the batched and served forms; "
                          "all: the plain line with batched and served samples added")
@@ -281,6 +284,7 @@ def main() -> int:
     if args.side != 4096:
         cfg.width = cfg.height = args.side
     cfg.tile_rows = args.tile_rows
+    cfg.uplift_mode = _abi.TT_UPLIFT_SEPARABLE if args.uplift == "separable" else _abi.TT_UPLIFT_JFA
     N = cfg.width * cfg.height
     world = World(cfg, ks).init()
     init_state = None
@@ -328,7 +332,9 @@ def main() -> int:
         "config": {
             "workload": f"plain step, {cfg.width}x{cfg.height}, {cfg.num_plates} plates, speed U[0.5,3], "
                         f"K={K} hydraulic, T={cfg.thermal_iterations} thermal, CCL every "
-                        f"{cfg.iterations_for_ccl} step, blur radius {ks.upliftRange}",
+                        f"{cfg.iterations_for_ccl} step, blur radius {ks.upliftRange}"
+                        + (", separable uplift" if args.uplift == "separable" else ""),
+            "uplift": args.uplift,
             "side": cfg.width, "plates": cfg.num_plates, "hydraulic_iterations": K,
             "thermal_iterations": cfg.thermal_iterations, "ccl_every": cfg.iterations_for_ccl,
             "seed": cfg.seed, "parallelism": "single world per GPU",

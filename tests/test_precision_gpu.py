@@ -108,3 +108,22 @@ def test_precision_study_matches_oracle(sim, O):
         assert rg[k] == pytest.approx(ro[k], rel=1e-12), k
     for k in ("drift_ref", "drift_fp32", "drift_comp"):
         assert rg[k] == pytest.approx(ro[k], rel=1e-9, abs=1e-15), k
+
+
+def test_precision_study_at_8192(sim):
+    """The precision workload's geometry (8192x8192, 64 plates, K = 50,
+    T = 20) for two erosion-only steps: every statistic finite, both float
+    variants still well correlated with the fp64 twin (errors far below the
+    fields' own spread) and the same mass drift as the twin."""
+    cfg, ks = configs.world_8192(seed=1)
+    _say("[precision study] 8192x8192, 64 plates, 2 steps")
+    r = sim.precision_study(cfg, ks, 2)
+    _say(r)
+    for f in ("height", "water", "sediment"):
+        for k, v in r[f].items():
+            assert math.isfinite(v), (f, k)
+    h = r["height"]
+    assert h["rms_ref"] > 0.0
+    assert h["rms_fp32"] < 0.5 * h["rms_ref"] and h["rms_comp"] < 0.5 * h["rms_ref"]
+    assert abs(r["drift_fp32"] - r["drift_ref"]) < 1e-3 and abs(r["drift_comp"] - r["drift_ref"]) < 1e-3
+    assert r["ms_fp32"] > 0 and r["ms_comp"] > 0 and r["ms_ref"] > 0
