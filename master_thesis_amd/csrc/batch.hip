@@ -16,6 +16,8 @@
 // bodies, and nothing is shared between members.
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -42,8 +44,68 @@ std::string tt_get_error();
         if (s_ != TT_OK) return s_; \
     } while (0)
 
+// Host threads that live as long as the batch: run(f) has every thread (and
+// the caller) execute f once and returns when all have finished.
+struct tt_pool {
+    std::vector<std::thread> th;
+    std::mutex mu;
+    std::condition_variable wake, idle;
+    std::function<void()> job;
+    uint64_t gen = 0;
+    int busy = 0;
+    bool stop = false;
+
+    void start(int n, int device)
+    {
+        for (int i = 0; i < n; ++i)
+            th.emplace_back([this, device]() {
+                (void)hipSetDevice(device);
+                uint64_t seen = 0;
+                for (;;) {
+                    std::function<void()> f;
+                    {
+                        std::unique_lock<std::mutex> lk(mu);
+                        wake.wait(lk, [&] { return stop || gen != seen; });
+                        if (stop) return;
+                        seen = gen;
+                        f = job;
+                    }
+                    f();
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (--busy == 0) idle.notify_all();
+                }
+            });
+    }
+    void run(const std::function<void()> &f)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            job = f;
+            busy = (int)th.size();
+            ++gen;
+        }
+        wake.notify_all();
+        f();
+        std::unique_lock<std::mutex> lk(mu);
+        idle.wait(lk, [&] { return busy == 0; });
+    }
+    void shutdown()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        wake.notify_all();
+        for (auto &t : th) t.join();
+        th.clear();
+        stop = false;
+    }
+    ~tt_pool() { shutdown(); }
+};
+
 struct tt_batch {
     int device = 0;
+    tt_pool pool;               // threads - 1 workers beside the calling thread
     hipStream_t stream = nullptr;
     hipEvent_t erosionDone = nullptr;
     std::vector<tt_world *> worlds;
@@ -135,18 +197,21 @@ int rebuild(tt_batch *b)
     return TT_OK;
 }
 
-// runs f(world index) for every member from a pool of host threads
+// runs f(world index) for every member on the batch's host threads
 template <typename F>
 int parallel(tt_batch *b, F f)
 {
     const int n = (int)b->worlds.size();
     const int nt = std::max(1, std::min(b->threads, n));
+    if ((int)b->pool.th.size() != nt - 1) {
+        b->pool.shutdown();
+        b->pool.start(nt - 1, b->device);
+    }
     std::atomic<int> next{0};
     std::atomic<int> status{TT_OK};
     std::mutex mu;
     std::string msg;
     auto work = [&]() {
-        (void)hipSetDevice(b->device);
         for (;;) {
             const int i = next.fetch_add(1);
             if (i >= n || status.load() != TT_OK) break;
@@ -160,13 +225,8 @@ int parallel(tt_batch *b, F f)
             }
         }
     };
-    if (nt == 1) {
-        work();
-    } else {
-        std::vector<std::thread> pool;
-        for (int t = 0; t < nt; ++t) pool.emplace_back(work);
-        for (auto &t : pool) t.join();
-    }
+    if (nt == 1) work();
+    else b->pool.run(work);
     if (status.load() != TT_OK) tt_set_error(msg);
     return status.load();
 }
@@ -371,6 +431,7 @@ int tt_batch_destroy(tt_batch *b)
     if (!b) return TT_ERR_INVALID;
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
+    b->pool.shutdown();
     drop_graph(b);
     free_tables(b);
     for (auto &e : b->preDone)

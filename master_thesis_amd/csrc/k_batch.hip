@@ -82,12 +82,24 @@ __global__ __launch_bounds__(256) void k_hydraulic_batch(const tt_bworld *__rest
     }
     const int x0 = (tile % nbx) * TT_TILE_W, y0 = (tile / nbx) * TH;
     const bool first = k == 0, last = k == B.K - 1;
-    if (first) {
-        if (last) tt_hydraulic_tile<TH, true, true>(A, S, x0, y0);
-        else tt_hydraulic_tile<TH, true, false>(A, S, x0, y0);
+    // interior tiles (two-cell halo inside the grid) take the wrap-free body
+    const bool in = x0 >= 2 && x0 + TT_TILE_W + 2 <= B.W && y0 >= 2 && y0 + TH + 2 <= B.H;
+    if (in) {
+        if (first) {
+            if (last) tt_hydraulic_tile<TH, true, true, false, true>(A, S, x0, y0);
+            else tt_hydraulic_tile<TH, true, false, false, true>(A, S, x0, y0);
+        } else {
+            if (last) tt_hydraulic_tile<TH, false, true, false, true>(A, S, x0, y0);
+            else tt_hydraulic_tile<TH, false, false, false, true>(A, S, x0, y0);
+        }
     } else {
-        if (last) tt_hydraulic_tile<TH, false, true>(A, S, x0, y0);
-        else tt_hydraulic_tile<TH, false, false>(A, S, x0, y0);
+        if (first) {
+            if (last) tt_hydraulic_tile<TH, true, true>(A, S, x0, y0);
+            else tt_hydraulic_tile<TH, true, false>(A, S, x0, y0);
+        } else {
+            if (last) tt_hydraulic_tile<TH, false, true>(A, S, x0, y0);
+            else tt_hydraulic_tile<TH, false, false>(A, S, x0, y0);
+        }
     }
 }
 
