@@ -119,9 +119,12 @@ def run_batch_mode(args) -> dict:
     warm = max(args.warmup, 1)   # the first batched step captures the erosion graph
     with Batch(worlds) as b:
         b.step(warm)
+        t_warm = b.stage_times()
         t0 = time.perf_counter()
         b.step(args.batch_steps)
         tb = time.perf_counter() - t0
+        t_all = b.stage_times()
+    phases = {k: (t_all[k] - t_warm[k]) / args.batch_steps for k in ("ms_pre", "ms_erosion", "ms_post")}
     for w in worlds:
         w.close()
     worlds = [World(c, k).init() for c, k in specs]
@@ -144,6 +147,7 @@ def run_batch_mode(args) -> dict:
         "batched_worlds_per_s": n * args.batch_steps / tb / args.batch_steps,
         "plain_worlds_per_s": n * args.batch_steps / tp / args.batch_steps,
         "speedup": tp / tb,
+        "batched_phases_ms_per_step": phases,
     }
 
 

@@ -367,6 +367,16 @@ int tt_batch_step(tt_batch *b, int32_t steps);
 int tt_batch_set_threads(tt_batch *b, int32_t threads);
 /* 1 (default): replay the erosion passes from a captured hipGraph; 0: launch them directly */
 int tt_batch_set_graph(tt_batch *b, int32_t enabled);
+/* Device time of the batch's phases summed over its steps (HIP events): from
+ * the start of a step until every member's pre-erosion phase is done, the
+ * batched erosion, and from the end of the erosion until every member's
+ * post-erosion phase is done. */
+typedef struct tt_batch_times {
+    double ms_pre, ms_erosion, ms_post;
+    int64_t steps;
+    double reserved[4];
+} tt_batch_times;
+int tt_batch_stage_times(tt_batch *b, tt_batch_times *out);
 int tt_batch_destroy(tt_batch *b);
 
 /* ---- serving: world-generation jobs under an HBM budget (no reference

@@ -279,6 +279,11 @@ class HbmProbe(C.Structure):
 # numpy-compatible record layouts of the per-cell structs
 import numpy as np  # noqa: E402
 
+class BatchTimes(C.Structure):
+    _fields_ = [("ms_pre", C.c_double), ("ms_erosion", C.c_double), ("ms_post", C.c_double),
+                ("steps", C.c_int64), ("reserved", C.c_double * 4)]
+
+
 class VoronoiSeed(C.Structure):
     """struct VoronoiSeed (types/voronoi_seed.h) = tt_voronoi_seed."""
     _fields_ = [("position", C.c_float * 2), ("id", C.c_uint8), ("pad_", C.c_uint8 * 3)]
@@ -394,6 +399,7 @@ def lib() -> C.CDLL:
     L.tt_batch_set_threads.argtypes = [vp, C.c_int32]
     L.tt_batch_set_graph.argtypes = [vp, C.c_int32]
     L.tt_batch_destroy.argtypes = [vp]
+    L.tt_batch_stage_times.argtypes = [vp, C.POINTER(BatchTimes)]
     L.tt_server_create.argtypes = [C.c_uint64, C.c_int32, C.POINTER(vp)]
     L.tt_server_set_keep.argtypes = [vp, C.c_int32]
     L.tt_server_submit.argtypes = [vp, C.POINTER(SimConfig), C.POINTER(KernelSettings), C.c_int32,

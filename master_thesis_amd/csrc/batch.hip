@@ -108,8 +108,10 @@ struct tt_batch {
     tt_pool pool;               // threads - 1 workers beside the calling thread
     hipStream_t stream = nullptr;
     hipEvent_t erosionDone = nullptr;
+    hipEvent_t tStart = nullptr, tPre = nullptr, tEro = nullptr, tEnd = nullptr;   // phase timing
+    tt_batch_times times{};
     std::vector<tt_world *> worlds;
-    std::vector<hipEvent_t> preDone;
+    std::vector<hipEvent_t> preDone, postDone;
     std::vector<int> sedLast, Ts, Ks;
     int threads = 16;
     bool dirty = true;
@@ -189,6 +191,9 @@ int rebuild(tt_batch *b)
     }
     b->preDone.resize(n, nullptr);
     for (auto &e : b->preDone)
+        if (!e) BCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->postDone.resize(n, nullptr);
+    for (auto &e : b->postDone)
         if (!e) BCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     b->sedLast.assign(n, 0);
     b->Ts.assign(n, 0);
@@ -315,6 +320,7 @@ int batch_step_once(tt_batch *b)
     const int n = (int)b->worlds.size();
     if (n == 0) return TT_OK;
     if (b->dirty) BCKS(rebuild(b));
+    BCK(hipEventRecord(b->tStart, b->stream));
     BCKS(parallel(b, [&](int i) -> int {
         tt_world *w = b->worlds[i];
         w->timing = false;
@@ -333,9 +339,11 @@ int batch_step_once(tt_batch *b)
         Kmax = std::max(Kmax, b->Ks[i]);
         fill_desc(b->hDesc[i], w, b->Ts[i], b->Ks[i]);
     }
+    BCK(hipEventRecord(b->tPre, b->stream));
     BCK(hipMemcpyAsync(b->dDesc, b->hDesc, (size_t)n * sizeof(tt_bworld), hipMemcpyHostToDevice, b->stream));
     BCKS(run_erosion(b, Tmax, Kmax));
     BCK(hipEventRecord(b->erosionDone, b->stream));
+    BCK(hipEventRecord(b->tEro, b->stream));
     for (int i = 0; i < n; ++i) {
         tt_world *w = b->worlds[i];
         tt_step_erode_indices(w, b->Ts[i], b->Ks[i], &b->sedLast[i]);
@@ -344,10 +352,22 @@ int batch_step_once(tt_batch *b)
     BCKS(parallel(b, [&](int i) -> int {
         tt_world *w = b->worlds[i];
         if (b->Ks[i] > 0) BCKS(tt_step_transport(w, b->sedLast[i]));
-        return tt_step_post(w);
+        BCKS(tt_step_post(w));
+        BCK(hipEventRecord(b->postDone[i], w->stream));
+        return TT_OK;
     }));
+    for (int i = 0; i < n; ++i) BCK(hipStreamWaitEvent(b->stream, b->postDone[i], 0));
+    BCK(hipEventRecord(b->tEnd, b->stream));
     // the next step overwrites the pinned descriptors: the copy must be done
     BCK(hipStreamSynchronize(b->stream));
+    float a = 0.0f, e = 0.0f, c = 0.0f;
+    BCK(hipEventElapsedTime(&a, b->tStart, b->tPre));
+    BCK(hipEventElapsedTime(&e, b->tPre, b->tEro));
+    BCK(hipEventElapsedTime(&c, b->tEro, b->tEnd));
+    b->times.ms_pre += a;
+    b->times.ms_erosion += e;
+    b->times.ms_post += c;
+    b->times.steps += 1;
     return TT_OK;
 }
 
@@ -362,7 +382,9 @@ int tt_batch_create(tt_batch **out)
     tt_batch *b = new (std::nothrow) tt_batch();
     if (!b) return TT_ERR_NOMEM;
     if (hipGetDevice(&b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->erosionDone, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&b->erosionDone, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&b->tStart) != hipSuccess || hipEventCreate(&b->tPre) != hipSuccess ||
+        hipEventCreate(&b->tEro) != hipSuccess || hipEventCreate(&b->tEnd) != hipSuccess) {
         delete b;
         tt_set_error("no HIP device for the batch");
         return TT_ERR_DEVICE;
@@ -426,6 +448,13 @@ int tt_batch_step(tt_batch *b, int32_t steps)
     return TT_OK;
 }
 
+int tt_batch_stage_times(tt_batch *b, tt_batch_times *out)
+{
+    if (!b || !out) return TT_ERR_INVALID;
+    *out = b->times;
+    return TT_OK;
+}
+
 int tt_batch_destroy(tt_batch *b)
 {
     if (!b) return TT_ERR_INVALID;
@@ -436,7 +465,10 @@ int tt_batch_destroy(tt_batch *b)
     free_tables(b);
     for (auto &e : b->preDone)
         if (e) (void)hipEventDestroy(e);
-    if (b->erosionDone) (void)hipEventDestroy(b->erosionDone);
+    for (auto &e : b->postDone)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {b->erosionDone, b->tStart, b->tPre, b->tEro, b->tEnd})
+        if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return TT_OK;

@@ -89,6 +89,21 @@ __host__ __device__ inline void tt_hyd_water_bufs(float *w0, float *w1, float *f
 
 __host__ __device__ inline int tt_hyd_water_index_after(int wc, int K) { return K == 1 ? wc ^ 1 : wc; }
 
+// Element i of a buffer addressed as base + 32-bit byte offset: with a
+// wavefront-uniform base the compiler uses the scalar-base addressing mode
+// (SGPR base + VGPR offset) instead of per-lane 64-bit address arithmetic.
+// Valid while i * sizeof(T) < 2^32 (sides of at most 16384 cells).
+template <typename T>
+__device__ __forceinline__ const T &tt_at(const T *base, uint32_t i)
+{
+    return *(const T *)((const char *)base + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T *tt_ptr(T *base, uint32_t i)
+{
+    return (T *)((char *)base + i * (uint32_t)sizeof(T));
+}
+
 // uint8 plate id arithmetic of the reference (MAX_PLATE_COUNT - byte)
 __device__ __forceinline__ int tt_slot_plate(uint32_t word, int slot)
 {

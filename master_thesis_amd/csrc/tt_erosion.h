@@ -327,7 +327,7 @@ __device__ __forceinline__ tt_taps tt_bilinear_taps_nowrap(float tx, float ty, i
         tt_taps t;
         t.wx = tx - (float)x0;
         t.wy = ty - (float)y0;
-        t.a = y0 * W + x0;
+        t.a = (int)(__umul24((uint32_t)y0, (uint32_t)W) + (uint32_t)x0);   // 0 <= y0, x0 < 2^24
         t.b = t.a + 1;
         t.c = t.a + W;
         t.d = t.c + 1;
@@ -463,7 +463,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
     float wnp[RPW];
     if (!FIRST) {
 #pragma unroll
-        for (int r = 0; r < RPW; ++r) wnp[r] = A.win[(y0 + wv * RPW + r) * W + gx];
+        for (int r = 0; r < RPW; ++r) wnp[r] = tt_at(A.win, (uint32_t)((y0 + wv * RPW + r) * W + gx));
     }
     // 2. heights and water on the tile plus a two-cell halo: rows wv, wv+4,
     // ...; lanes 0..63 take columns 0..63, lanes 0..3 also columns 64..67
@@ -475,11 +475,11 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         const int ly = wv + 4 * j;
         if (ly < TH + 4) {
             const int row = (IN ? y0 + ly - 2 : tt_wrap1(y0 + ly - 2, H)) * W;
-            hm[j] = A.hin[row + gxa];
-            hw[j] = A.win[row + gxa];
+            hm[j] = tt_at(A.hin, (uint32_t)(row + gxa));
+            hw[j] = tt_at(A.win, (uint32_t)(row + gxa));
             if (lane < TT_EW - 64) {
-                hm2[j] = A.hin[row + gxb];
-                hw2[j] = A.win[row + gxb];
+                hm2[j] = tt_at(A.hin, (uint32_t)(row + gxb));
+                hw2[j] = tt_at(A.win, (uint32_t)(row + gxb));
             }
         }
     }
@@ -504,8 +504,10 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         const int i = tid + 256 * k;
         if (i < NFI) {
             const int fy = fyk[k], fx = fxk[k];
-            cf[k] = IN ? A.fin[(y0 + fy - 1) * W + x0 + fx - 1]
-                       : A.fin[tt_wrap1(y0 + fy - 1, H) * W + tt_wrap1(x0 + fx - 1, W)];
+            // rows and columns below 2^24: 24-bit products (full rate)
+            cf[k] = tt_at(A.fin, IN ? __umul24((uint32_t)(y0 + fy - 1), (uint32_t)W) + (uint32_t)(x0 + fx - 1)
+                                    : __umul24((uint32_t)tt_wrap1(y0 + fy - 1, H), (uint32_t)W) +
+                                          (uint32_t)tt_wrap1(x0 + fx - 1, W));
         }
     }
     const float evap = A.evap;
@@ -516,7 +518,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
             const int c = (y0 + wv * RPW + r) * W + gx;
-            s4[r][0] = A.sprev[c];
+            s4[r][0] = tt_at(A.sprev, (uint32_t)c);
             if (COMP) c4[r][0] = A.csprev[c];
         }
     }
@@ -564,10 +566,10 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             const tt_taps t = tt_bilinear_taps_nowrap((float)gx - vx * 1.0f, (float)gy - vy * 1.0f, W, H);
             wxs[r] = t.wx;
             wys[r] = t.wy;
-            s4[r][0] = A.sprev[t.a];
-            s4[r][1] = A.sprev[t.b];
-            s4[r][2] = A.sprev[t.c];
-            s4[r][3] = A.sprev[t.d];
+            s4[r][0] = tt_at(A.sprev, (uint32_t)t.a);
+            s4[r][1] = tt_at(A.sprev, (uint32_t)t.b);
+            s4[r][2] = tt_at(A.sprev, (uint32_t)t.c);
+            s4[r][3] = tt_at(A.sprev, (uint32_t)t.d);
             if (COMP) {
                 c4[r][0] = A.csprev[t.a];
                 c4[r][1] = A.csprev[t.b];
@@ -646,11 +648,12 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         const float wn = tt_fmaxf(0.0f, S.Wt[ly][lx] + dv);
         float vx, vy, vmag;
         tt_flow_velocity(Sf, Lf, Uf, Rf, Df, wn, vx, vy, vmag, A.sqrtOk);
-        __builtin_nontemporal_store(LAST ? tt_evaporate(wn, evap) : wn, &A.wout[c]);
-        __builtin_nontemporal_store(Sf.x, &A.fout[c].x);
-        __builtin_nontemporal_store(Sf.y, &A.fout[c].y);
-        __builtin_nontemporal_store(Sf.z, &A.fout[c].z);
-        __builtin_nontemporal_store(Sf.w, &A.fout[c].w);
+        __builtin_nontemporal_store(LAST ? tt_evaporate(wn, evap) : wn, tt_ptr(A.wout, (uint32_t)c));
+        float4 *fo = tt_ptr(A.fout, (uint32_t)c);
+        __builtin_nontemporal_store(Sf.x, &fo->x);
+        __builtin_nontemporal_store(Sf.y, &fo->y);
+        __builtin_nontemporal_store(Sf.z, &fo->z);
+        __builtin_nontemporal_store(Sf.w, &fo->w);
         float sed, cs = 0.0f;
         if (FIRST) {
             sed = s4[r][0];
@@ -660,8 +663,9 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             if (COMP) cs = tt_bilinear_mix(c4[r][0], c4[r][1], c4[r][2], c4[r][3], wxs[r], wys[r]);
         }
         if (LAST) {
-            __builtin_nontemporal_store(vx, &A.vel[c].x);
-            __builtin_nontemporal_store(vy, &A.vel[c].y);
+            float2 *vo = tt_ptr(A.vel, (uint32_t)c);
+            __builtin_nontemporal_store(vx, &vo->x);
+            __builtin_nontemporal_store(vy, &vo->y);
         }
         const float mc = S.M[ly][lx];
         // Slope: (m[+1] - m[-1]) / 2.0 in the reference; halving is exact in float
@@ -703,8 +707,8 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             A.ch[c] = chv;
             A.csout[c] = cs;
         }
-        __builtin_nontemporal_store(hnew, &A.hout[c]);
-        __builtin_nontemporal_store(sed, &A.sout[c]);
+        __builtin_nontemporal_store(hnew, tt_ptr(A.hout, (uint32_t)c));
+        __builtin_nontemporal_store(sed, tt_ptr(A.sout, (uint32_t)c));
     }
 }
 

@@ -85,6 +85,7 @@ struct tt_world {
     long long *acc = nullptr;
     float *eff = nullptr;
     uint32_t *rchg = nullptr;   // per-round change counters of the split fixed points (TT_RCHG)
+    int lastRounds = 28;        // rounds the last split relaxation needed (first batch length)
     uint8_t *act = nullptr;     // per-tile activity flags of the split rounds (2 x tiles)
     float2 *tileMM = nullptr;   // per 64x16 tile: min / max height (thermal)
     uint8_t *tileFrozen = nullptr;  // per 64x16 tile: unchanged by this step's thermal passes

@@ -308,22 +308,29 @@ static int run_ccl(tt_world *w)
 // ---------------------------------------------------------------------------
 // Runs rounds of a tiled fixed-point kernel in batches, with one host read per
 // batch: a round that changed nothing ends the loop (later rounds of the batch
-// exit at once).  Round numbers continue across batches; the counters of a
-// batch live in one half of the TT_RCHG ring and are cleared before it runs.
+// exit at once).  Round r counts its changes in rchg[r % TT_RCHG]; a batch
+// clears its own counters (never the previous round's) before it runs.  The
+// first batch has the length the world's last relaxation needed plus two;
+// later batches double, up to TT_ROUND_BATCH rounds.
 template <typename Launch>
 static int run_rounds(tt_world *w, Launch launch, int *rounds_out)
 {
-    const int B = TT_ROUND_BATCH;
     const long long ntiles_all = (long long)(w->W / 64) * (w->H / tt_split_tile_h(make_dev(w)));
     CKS(zero(w, w->act, (size_t)(2 * ntiles_all)));
-    for (int batch = 0; batch < 100000; ++batch) {
-        uint32_t *half = w->rchg + (batch & 1) * B;
-        CKS(zero(w, half, B * 4));
-        CK(launch(batch * B, B));
-        CK(hipMemcpyAsync(w->hu32, half, B * 4, hipMemcpyDeviceToHost, w->stream));
+    int B = std::max(4, std::min(TT_ROUND_BATCH, w->lastRounds + 2));
+    for (int r0 = 0; r0 < 3200000; r0 += B, B = std::min(TT_ROUND_BATCH, 2 * B)) {
+        const int a = r0 % TT_RCHG, n1 = std::min(B, TT_RCHG - a);
+        CKS(zero(w, w->rchg + a, (size_t)n1 * 4));
+        if (B > n1) CKS(zero(w, w->rchg, (size_t)(B - n1) * 4));
+        CK(launch(r0, B));
+        CK(hipMemcpyAsync(w->hu32, w->rchg, TT_RCHG * 4, hipMemcpyDeviceToHost, w->stream));
         CK(hipStreamSynchronize(w->stream));
         for (int r = 0; r < B; ++r)
-            if (w->hu32[r] == 0u) { *rounds_out = batch * B + r + 1; return TT_OK; }
+            if (w->hu32[(r0 + r) % TT_RCHG] == 0u) {
+                *rounds_out = r0 + r + 1;
+                w->lastRounds = r0 + r + 1;
+                return TT_OK;
+            }
     }
     set_error("split fixed point did not converge");
     return TT_ERR_STATE;
@@ -1156,7 +1163,7 @@ int tt_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, tt_w
     for (auto &a : w->allocs) { *a.ptr = base + off; off += a.bytes; }
     if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void **)&w->hctl, sizeof(tt_step_ctl), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&w->hu32, TT_ROUND_BATCH * 4, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&w->hu32, TT_RCHG * 4, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&w->hIter, 64, hipHostMallocDefault) != hipSuccess) {
         tt_world_destroy(w);
         set_error("stream / pinned memory allocation failed");

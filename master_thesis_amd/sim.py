@@ -168,6 +168,13 @@ class Batch:
     def step(self, n: int = 1) -> None:
         check(lib().tt_batch_step(self.h, n), "tt_batch_step")
 
+    def stage_times(self) -> dict:
+        """Device time (ms, summed over steps) of the pre-erosion phases, the
+        batched erosion and the post-erosion phases."""
+        t = _abi.BatchTimes()
+        check(lib().tt_batch_stage_times(self.h, C.byref(t)), "tt_batch_stage_times")
+        return {"ms_pre": t.ms_pre, "ms_erosion": t.ms_erosion, "ms_post": t.ms_post, "steps": int(t.steps)}
+
     def __len__(self) -> int:
         n = C.c_int32()
         check(lib().tt_batch_size(self.h, C.byref(n)), "tt_batch_size")
