@@ -309,8 +309,11 @@ enum { TT_EXPORT_HEIGHT16 = 0, TT_EXPORT_PLATES_RGB = 1, TT_EXPORT_COLLISION_CLA
  * *len (the text length) is set; otherwise cap must exceed *len. */
 int tt_world_diagnostics_json(tt_world *w, char *buf, size_t cap, size_t *len);
 /* Snapshot: every state field, the configuration, the settings and the
- * iteration in one file (magic "TTSNAP", version 1, CRC32 per field).  A world
- * loaded from a snapshot continues bit-identically. */
+ * iteration in one file (magic "TTSNAP2", a CRC32 per field over its whole
+ * length; worlds in TT_PRECISION_COMPENSATED also keep their compensation
+ * terms).  tt_world_load checks every CRC before it creates the world and
+ * also reads version-1 files.  A world loaded from a snapshot continues
+ * bit-identically. */
 int tt_world_save(tt_world *w, const char *path);
 int tt_world_load(const char *path, tt_world **out);
 
