@@ -38,9 +38,10 @@ __global__ __launch_bounds__(256) void k_thermal_batch(const tt_bworld *__restri
 }
 
 template <int TH>
-__global__ __launch_bounds__(256) void k_hydraulic_batch(const tt_bworld *__restrict__ worlds,
+__global__ __launch_bounds__(64 * tt_hyd_waves<TH>()) void k_hydraulic_batch(const tt_bworld *__restrict__ worlds,
                                                          const uint32_t *__restrict__ tiles, int k)
 {
+    constexpr int NW = tt_hyd_waves<TH>();
     __shared__ tt_hyd_lds<TH> S;
     const uint32_t e = tiles[tt_xcd_tile(blockIdx.x, gridDim.x)];
     const tt_bworld &B = worlds[e >> 20];
@@ -86,19 +87,19 @@ __global__ __launch_bounds__(256) void k_hydraulic_batch(const tt_bworld *__rest
     const bool in = x0 >= 2 && x0 + TT_TILE_W + 2 <= B.W && y0 >= 2 && y0 + TH + 2 <= B.H;
     if (in) {
         if (first) {
-            if (last) tt_hydraulic_tile<TH, true, true, false, true>(A, S, x0, y0);
-            else tt_hydraulic_tile<TH, true, false, false, true>(A, S, x0, y0);
+            if (last) tt_hydraulic_tile<TH, true, true, false, true, NW>(A, S, x0, y0);
+            else tt_hydraulic_tile<TH, true, false, false, true, NW>(A, S, x0, y0);
         } else {
-            if (last) tt_hydraulic_tile<TH, false, true, false, true>(A, S, x0, y0);
-            else tt_hydraulic_tile<TH, false, false, false, true>(A, S, x0, y0);
+            if (last) tt_hydraulic_tile<TH, false, true, false, true, NW>(A, S, x0, y0);
+            else tt_hydraulic_tile<TH, false, false, false, true, NW>(A, S, x0, y0);
         }
     } else {
         if (first) {
-            if (last) tt_hydraulic_tile<TH, true, true>(A, S, x0, y0);
-            else tt_hydraulic_tile<TH, true, false>(A, S, x0, y0);
+            if (last) tt_hydraulic_tile<TH, true, true, false, false, NW>(A, S, x0, y0);
+            else tt_hydraulic_tile<TH, true, false, false, false, NW>(A, S, x0, y0);
         } else {
-            if (last) tt_hydraulic_tile<TH, false, true>(A, S, x0, y0);
-            else tt_hydraulic_tile<TH, false, false>(A, S, x0, y0);
+            if (last) tt_hydraulic_tile<TH, false, true, false, false, NW>(A, S, x0, y0);
+            else tt_hydraulic_tile<TH, false, false, false, false, NW>(A, S, x0, y0);
         }
     }
 }
@@ -114,8 +115,8 @@ hipError_t tt_launch_hydraulic_batch(hipStream_t s, int rows, const tt_bworld *w
                                      int ntiles, int k)
 {
     if (ntiles <= 0) return hipSuccess;
-    if (rows == 8) hipLaunchKernelGGL(k_hydraulic_batch<8>, dim3((unsigned)ntiles), dim3(256), 0, s, worlds, tiles, k);
-    else if (rows == 32) hipLaunchKernelGGL(k_hydraulic_batch<32>, dim3((unsigned)ntiles), dim3(256), 0, s, worlds, tiles, k);
-    else hipLaunchKernelGGL(k_hydraulic_batch<16>, dim3((unsigned)ntiles), dim3(256), 0, s, worlds, tiles, k);
+    if (rows == 8) hipLaunchKernelGGL(k_hydraulic_batch<8>, dim3((unsigned)ntiles), dim3(64 * tt_hyd_waves<8>()), 0, s, worlds, tiles, k);
+    else if (rows == 32) hipLaunchKernelGGL(k_hydraulic_batch<32>, dim3((unsigned)ntiles), dim3(64 * tt_hyd_waves<32>()), 0, s, worlds, tiles, k);
+    else hipLaunchKernelGGL(k_hydraulic_batch<16>, dim3((unsigned)ntiles), dim3(64 * tt_hyd_waves<16>()), 0, s, worlds, tiles, k);
     return hipGetLastError();
 }

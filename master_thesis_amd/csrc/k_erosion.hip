@@ -25,16 +25,17 @@ __global__ __launch_bounds__(256) void k_thermal(tt_therm_args A)
 }
 
 template <int TH, bool FIRST, bool LAST, bool COMP>
-__global__ __launch_bounds__(256) void k_hydraulic(tt_hyd_args A)
+__global__ __launch_bounds__(64 * tt_hyd_waves<TH>()) void k_hydraulic(tt_hyd_args A)
 {
+    constexpr int NW = tt_hyd_waves<TH>();
     __shared__ tt_hyd_lds<TH> S;
     const int nbx = A.W / TT_TILE_W, nby = A.H / TH;
     const int b = tt_xcd_tile(blockIdx.x, nbx * nby);
     const int x0 = (b % nbx) * TT_TILE_W, y0 = (b / nbx) * TH;
     if (x0 >= 2 && x0 + TT_TILE_W + 2 <= A.W && y0 >= 2 && y0 + TH + 2 <= A.H)
-        tt_hydraulic_tile<TH, FIRST, LAST, COMP, true>(A, S, x0, y0);
+        tt_hydraulic_tile<TH, FIRST, LAST, COMP, true, NW>(A, S, x0, y0);
     else
-        tt_hydraulic_tile<TH, FIRST, LAST, COMP, false>(A, S, x0, y0);
+        tt_hydraulic_tile<TH, FIRST, LAST, COMP, false, NW>(A, S, x0, y0);
 }
 
 int tt_tile_rows_valid(int rows, int H) { return (rows == 8 || rows == 16 || rows == 32) && (H % rows) == 0; }
@@ -121,10 +122,11 @@ hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, int rows, const flo
 template <int TH, bool COMP>
 static void launch_hyd(hipStream_t s, unsigned blocks, bool first, bool last, const tt_hyd_args &A)
 {
-    if (first && last) hipLaunchKernelGGL((k_hydraulic<TH, true, true, COMP>), dim3(blocks), dim3(256), 0, s, A);
-    else if (first) hipLaunchKernelGGL((k_hydraulic<TH, true, false, COMP>), dim3(blocks), dim3(256), 0, s, A);
-    else if (last) hipLaunchKernelGGL((k_hydraulic<TH, false, true, COMP>), dim3(blocks), dim3(256), 0, s, A);
-    else hipLaunchKernelGGL((k_hydraulic<TH, false, false, COMP>), dim3(blocks), dim3(256), 0, s, A);
+    const dim3 block(64 * tt_hyd_waves<TH>());
+    if (first && last) hipLaunchKernelGGL((k_hydraulic<TH, true, true, COMP>), dim3(blocks), block, 0, s, A);
+    else if (first) hipLaunchKernelGGL((k_hydraulic<TH, true, false, COMP>), dim3(blocks), block, 0, s, A);
+    else if (last) hipLaunchKernelGGL((k_hydraulic<TH, false, true, COMP>), dim3(blocks), block, 0, s, A);
+    else hipLaunchKernelGGL((k_hydraulic<TH, false, false, COMP>), dim3(blocks), block, 0, s, A);
 }
 
 hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int rows, int first, int last, int rseed,

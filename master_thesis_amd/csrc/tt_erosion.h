@@ -445,13 +445,22 @@ __device__ __forceinline__ void tt_flow_velocity_dir(const float4 &Sf, const flo
 // into a scalar register (row arithmetic runs on the scalar unit), and IN
 // (a tile whose two-cell halo does not cross the grid edge) drops the wrap of
 // every halo and flux-ring index.
-template <int TH, bool FIRST, bool LAST, bool COMP = false, bool IN = false>
+// Wavefronts per hydraulic block: two own rows per wavefront at every tile
+// height.  A thread then needs at most 64 VGPRs, so a SIMD holds eight
+// wavefronts (a 16-row block has 29 KB of LDS: four 512-thread blocks fit a
+// CU); with four own rows per wavefront the 16-row block needed 85 VGPRs and a
+// SIMD held five.
+template <int TH> constexpr int tt_hyd_waves() { return TH / 2; }
+
+template <int TH, bool FIRST, bool LAST, bool COMP = false, bool IN = false, int NW = 4>
 __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_lds<TH> &S, int x0, int y0)
 {
-    constexpr int RPW = TH / 4;
+    static_assert(TH % NW == 0, "each wavefront owns whole rows of the tile");
+    constexpr int NT = 64 * NW;                  // threads of the block
+    constexpr int RPW = TH / NW;                 // own rows per wavefront
     constexpr int NFI = (TH + 2) * TT_FW;        // flux items (tile plus a one-cell halo)
-    constexpr int NFR = (NFI + 255) / 256;       // flux items per thread
-    constexpr int NHR = (TH + 4 + 3) / 4;        // halo rows per wavefront
+    constexpr int NFR = (NFI + NT - 1) / NT;     // flux items per thread
+    constexpr int NHR = (TH + 4 + NW - 1) / NW;  // halo rows per wavefront
     const int tid = threadIdx.x;
     const int W = A.W, H = A.H;
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -465,14 +474,15 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
 #pragma unroll
         for (int r = 0; r < RPW; ++r) wnp[r] = tt_at(A.win, (uint32_t)((y0 + wv * RPW + r) * W + gx));
     }
-    // 2. heights and water on the tile plus a two-cell halo: rows wv, wv+4,
-    // ...; lanes 0..63 take columns 0..63, lanes 0..3 also columns 64..67
+    // 2. heights and water on the tile plus a two-cell halo: rows wv,
+    // wv + NW, ...; lanes 0..63 take columns 0..63, lanes 0..3 also columns
+    // 64..67
     const int gxa = IN ? x0 + lane - 2 : tt_wrap1(x0 + lane - 2, W);
     const int gxb = IN ? x0 + 62 + lane : tt_wrap1(x0 + 64 + lane - 2, W);
     float hm[NHR], hw[NHR], hm2[NHR], hw2[NHR];
 #pragma unroll
     for (int j = 0; j < NHR; ++j) {
-        const int ly = wv + 4 * j;
+        const int ly = wv + NW * j;
         if (ly < TH + 4) {
             const int row = (IN ? y0 + ly - 2 : tt_wrap1(y0 + ly - 2, H)) * W;
             hm[j] = tt_at(A.hin, (uint32_t)(row + gxa));
@@ -483,9 +493,10 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             }
         }
     }
-    // 3. incoming fluxes of the flux ring.  Item i = tid + 256 k sits at row
-    // i / TT_FW, column i % TT_FW of the ring; 256 = 3 TT_FW + 58, so the
-    // coordinates of item k follow from those of item k - 1 by additions.
+    // 3. incoming fluxes of the flux ring.  Item i = tid + NT k sits at row
+    // i / TT_FW, column i % TT_FW of the ring; NT = (NT / TT_FW) TT_FW +
+    // NT % TT_FW, so the coordinates of item k follow from those of item
+    // k - 1 by additions.
     int fyk[NFR], fxk[NFR];
     {
         int fy = tid / TT_FW, fx = tid - (tid / TT_FW) * TT_FW;
@@ -493,15 +504,15 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         for (int k = 0; k < NFR; ++k) {
             fyk[k] = fy;
             fxk[k] = fx;
-            fx += 256 - 3 * TT_FW;
-            fy += 3;
+            fx += NT % TT_FW;
+            fy += NT / TT_FW;
             if (fx >= TT_FW) { fx -= TT_FW; fy += 1; }
         }
     }
     float4 cf[NFR];
 #pragma unroll
     for (int k = 0; k < NFR; ++k) {
-        const int i = tid + 256 * k;
+        const int i = tid + NT * k;
         if (i < NFI) {
             const int fy = fyk[k], fx = fxk[k];
             // rows and columns below 2^24: 24-bit products (full rate)
@@ -524,7 +535,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
     }
 #pragma unroll
     for (int j = 0; j < NHR; ++j) {
-        const int ly = wv + 4 * j;
+        const int ly = wv + NW * j;
         if (ly < TH + 4) {
             const int gy = IN ? y0 + ly - 2 : tt_wrap1(y0 + ly - 2, H);
             // the water this sub-iteration starts from: the state water plus
@@ -547,7 +558,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
         // the incoming fluxes, for the previous sub-iteration's velocities
 #pragma unroll
         for (int k = 0; k < NFR; ++k) {
-            const int i = tid + 256 * k;
+            const int i = tid + NT * k;
             if (i < NFI) S.F[fyk[k]][fxk[k]] = cf[k];
         }
     }
@@ -583,7 +594,7 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
     const float fk = A.fk, g = A.g, L = A.L;
 #pragma unroll
     for (int k = 0; k < NFR; ++k) {
-        const int i = tid + 256 * k;
+        const int i = tid + NT * k;
         if (i >= NFI) break;
         const int fy = fyk[k], fx = fxk[k];
         const int ly = fy + 1, lx = fx + 1;
