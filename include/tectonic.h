@@ -472,7 +472,8 @@ int tt_world_stage_times(tt_world *w, tt_stage_times *out);
 /* ---- device self-test of the deterministic math (no reference counterpart:
  * used by the parity tests to show device and host math agree bit for bit) -- */
 enum { TT_MATH_EXPF = 0, TT_MATH_LOG10F = 1, TT_MATH_SINF = 2, TT_MATH_COSF = 3,
-       TT_MATH_SIMPLEX = 4, TT_MATH_DISCRETE = 5, TT_MATH_FMODF = 6, TT_MATH_ATAN2 = 7 };
+       TT_MATH_SIMPLEX = 4, TT_MATH_DISCRETE = 5, TT_MATH_FMODF = 6, TT_MATH_ATAN2 = 7,
+       TT_MATH_DIV_RCP = 8 /* pairs (a, b): a / b from RN(1 / b) as the thermal passes divide */ };
 int tt_math_eval_device(int32_t fn, const float *in, float *out, int64_t n);
 /* Allocation-free variant of tt_math_eval_device (no counterpart in the
  * reference): for callers that cannot allocate device memory (code under HIP

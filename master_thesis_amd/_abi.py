@@ -59,6 +59,7 @@ FIELD_DISTANCE = 16
 
 TT_MATH_EXPF, TT_MATH_LOG10F, TT_MATH_SINF, TT_MATH_COSF = 0, 1, 2, 3
 TT_MATH_SIMPLEX, TT_MATH_DISCRETE, TT_MATH_FMODF, TT_MATH_ATAN2 = 4, 5, 6, 7
+TT_MATH_DIV_RCP = 8
 
 KS_FIELDS = [
     ("inelasticCollisionMultiplierContinental", "f", 1.0),

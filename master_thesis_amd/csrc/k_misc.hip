@@ -381,6 +381,7 @@ __global__ __launch_bounds__(256) void k_math_eval(int fn, const float *in, floa
         case TT_MATH_DISCRETE: r = tt_discrete_noise(in[3 * i], in[3 * i + 1], in[3 * i + 2], 1.0f, 1001); break;
         case TT_MATH_FMODF: r = tt_fmodf(in[2 * i], in[2 * i + 1]); break;
         case TT_MATH_ATAN2: r = (float)tt_atan2((double)in[2 * i], (double)in[2 * i + 1]); break;
+        case TT_MATH_DIV_RCP: r = tt_div_by_rcp(in[2 * i], in[2 * i + 1], 1.0f / in[2 * i + 1]); break;
         default: break;
     }
     out[i] = r;
@@ -430,7 +431,7 @@ static float *scratch_on(int dev)
 extern "C" int tt_math_eval_device_static(int32_t fn, const float *in, float *out, int64_t n)
 {
     if (!in || !out || n < 0) return TT_ERR_INVALID;
-    if (fn < TT_MATH_EXPF || fn > TT_MATH_ATAN2) return TT_ERR_INVALID;
+    if (fn < TT_MATH_EXPF || fn > TT_MATH_DIV_RCP) return TT_ERR_INVALID;
     if (n > TT_MATH_STATIC_MAX) {
         tt_set_error("tt_math_eval_device_static: n = " + std::to_string(n) + " exceeds its static buffer of " +
                      std::to_string(TT_MATH_STATIC_MAX) + " elements");
@@ -442,7 +443,8 @@ extern "C" int tt_math_eval_device_static(int32_t fn, const float *in, float *ou
     if (dev >= TT_SCRATCH_DEVICES) { tt_set_error("device index beyond the static table"); return TT_ERR_UNSUPPORTED; }
     float *base = scratch_on(dev);
     if (!base) return TT_ERR_DEVICE;
-    const int per = (fn == TT_MATH_SIMPLEX || fn == TT_MATH_DISCRETE) ? 3 : (fn == TT_MATH_FMODF || fn == TT_MATH_ATAN2) ? 2 : 1;
+    const int per = (fn == TT_MATH_SIMPLEX || fn == TT_MATH_DISCRETE) ? 3
+                    : (fn == TT_MATH_FMODF || fn == TT_MATH_ATAN2 || fn == TT_MATH_DIV_RCP) ? 2 : 1;
     float *din = base, *dout = base + 3 * TT_MATH_STATIC_MAX;
     if (hipMemcpy(din, in, (size_t)n * per * 4, hipMemcpyHostToDevice) != hipSuccess ||
         tt_launch_math_eval(0, fn, din, dout, n) != hipSuccess || hipStreamSynchronize(0) != hipSuccess ||

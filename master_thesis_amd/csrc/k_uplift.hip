@@ -85,17 +85,17 @@ hipError_t tt_launch_propagate_local(hipStream_t s, const tt_dev &d, const uint3
 }
 
 // One thread applies the listed sources in ascending cell order, exactly as the
-// reference program would run them one after another.
-__global__ void k_propagate_serial(tt_dev d, const uint32_t *list, const uint32_t *count, const uint32_t *word,
-                                   const tt_uplift *uplift, const float *height, const tt_plate_step *ps,
-                                   tt_prop *prop)
+// reference program would run them one after another: the walks and every
+// read-modify-write of a target's record go to global memory.  Used when the
+// sources or their distinct targets exceed what k_prop_apply holds in LDS.
+__device__ void tt_propagate_serial_global(const tt_dev &d, const uint32_t *list, uint32_t n, const uint32_t *word,
+                                           const tt_uplift *uplift, const float *height, const tt_plate_step *ps,
+                                           tt_prop *prop)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint32_t n = *count;
     for (uint32_t li = 0; li < n; ++li) {
         const uint32_t c = list[li];
         int x, y;
-    tt_cell_xy((uint32_t)c, d, x, y);
+        tt_cell_xy((uint32_t)c, d, x, y);
         const uint32_t cv = word[c];
         const tt_uplift U = uplift[c];
         const int prim = tt_slot_plate(cv, (int)((U.types >> 1) & 3u));
@@ -138,11 +138,200 @@ __global__ void k_propagate_serial(tt_dev d, const uint32_t *list, const uint32_
     }
 }
 
+// The canonical serial pass in three kernels' worth of work, with the same
+// result.  What makes the one-thread pass slow is memory latency: every step
+// of a voxel-march walk and every update of a target is a dependent global
+// access.  The walks depend only on the heights, so they run in parallel
+// (k_prop_targets: one thread per walk).  Then one block gives every distinct
+// target cell a slot in LDS and one thread applies the sources in ascending
+// cell order to those slots; only the order-dependent part stays serial, at
+// LDS latency.
+//
+// Request (li, k): source li's walk k (0: its primary plate, 1 + ci: the
+// plate in collision slot ci, when valid and not the primary).  A target is
+// kept as its cell and as its offset from the source before wrapping: the
+// reference tells positions apart by unwrapped coordinates (two offsets can
+// wrap to one cell on worlds narrower than two walks).
+#define TT_PROP_INVALID 0xFFFFFFFFu
+#define TT_PROP_SLOTS TT_PROP_SLOT_WORDS    // distinct target cells held in LDS
+#define TT_PROP_CHUNK 512     // sources staged in LDS at a time
+
+__global__ __launch_bounds__(256) void k_prop_targets(tt_dev d, const uint32_t *list, const uint32_t *count,
+                                                      const uint32_t *word, const tt_uplift *uplift,
+                                                      const float *height, const tt_plate_step *ps, int cap,
+                                                      uint32_t *tcell, uint16_t *toff)
+{
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t li = t / 5u, k = t - 5u * li;
+    const uint32_t n = *count;
+    if (n > (uint32_t)cap || li >= n) return;
+    const uint32_t c = list[li];
+    int x, y;
+    tt_cell_xy(c, d, x, y);
+    const uint32_t cv = word[c];
+    const tt_uplift U = uplift[c];
+    const int prim = tt_slot_plate(cv, (int)((U.types >> 1) & 3u));
+    const int pl = k == 0 ? prim : tt_slot_plate(cv, (int)k - 1);
+    const size_t r = (size_t)k * cap + li;
+    if (k > 0 && (pl == TT_MAXP || pl == prim)) {
+        tcell[r] = TT_PROP_INVALID;
+        return;
+    }
+    int tx, ty;
+    tt_cont_prop(d, height, x, y, ps[pl].dirx, ps[pl].diry, ps[pl].pcx, ps[pl].pcy, height[c], tx, ty);
+    tcell[r] = (uint32_t)tt_wrapg(ty, d.H) * (uint32_t)d.W + (uint32_t)tt_wrapg(tx, d.W);
+    // a walk moves at most continentalPropMaxIters cells; offsets beyond
+    // +-127 (an iteration limit above 127) leave the LDS form (tt_prop_apply
+    // then falls back to the global pass)
+    const int ox = tx - x + 128, oy = ty - y + 128;
+    toff[r] = (ox < 0 || ox > 255 || oy < 0 || oy > 255) ? (uint16_t)0xFFFFu : (uint16_t)(ox | (oy << 8));
+}
+
+__global__ __launch_bounds__(1024) void k_prop_apply(tt_dev d, const uint32_t *list, const uint32_t *count,
+                                                     const uint32_t *word, const tt_uplift *uplift,
+                                                     const float *height, const tt_plate_step *ps, int cap,
+                                                     const uint32_t *tcell, const uint16_t *toff, uint32_t *tslot,
+                                                     uint32_t *slotOf, uint32_t *slotCell, tt_prop *prop)
+{
+    __shared__ uint32_t stInv[TT_PROP_SLOTS];
+    __shared__ float stCum[TT_PROP_SLOTS];
+    __shared__ float cCum[TT_PROP_CHUNK];
+    __shared__ uint32_t cSec[TT_PROP_CHUNK];      // plate ids of slots 0..3 (0xFF: no request)
+    __shared__ uint8_t cPrim[TT_PROP_CHUNK];
+    __shared__ uint16_t cSlot[5][TT_PROP_CHUNK];
+    __shared__ uint16_t cOff[5][TT_PROP_CHUNK];
+    __shared__ uint32_t nSlots;
+    __shared__ int overflow;
+    const int tid = threadIdx.x;
+    const uint32_t n = *count;
+    if (n == 0) return;
+    if (n > (uint32_t)cap) {
+        if (tid == 0) tt_propagate_serial_global(d, list, n, word, uplift, height, ps, prop);
+        return;
+    }
+    if (tid == 0) { nSlots = 0; overflow = 0; }
+    __syncthreads();
+    // 1. one owner request per distinct target cell (the smallest index)
+    for (uint32_t k = 0; k < 5; ++k)
+        for (uint32_t li = tid; li < n; li += 1024) {
+            const size_t r = (size_t)k * cap + li;
+            const uint32_t T = tcell[r];
+            if (T == TT_PROP_INVALID) continue;
+            if (toff[r] == 0xFFFFu) overflow = 1;
+            atomicMin(&slotOf[T], (uint32_t)r);
+        }
+    __syncthreads();
+    // 2. owners number the slots
+    for (uint32_t k = 0; k < 5; ++k)
+        for (uint32_t li = tid; li < n; li += 1024) {
+            const size_t r = (size_t)k * cap + li;
+            const uint32_t T = tcell[r];
+            if (T == TT_PROP_INVALID || slotOf[T] != (uint32_t)r) continue;
+            const uint32_t sl = atomicAdd(&nSlots, 1u);
+            tslot[r] = sl;
+            if (sl < TT_PROP_SLOTS) {
+                slotCell[sl] = T;
+                stInv[sl] = 0u;     // contested cells start from a cleared record
+                stCum[sl] = 0.0f;
+            }
+        }
+    __syncthreads();
+    // 3. the other requests take their owner's slot
+    for (uint32_t k = 0; k < 5; ++k)
+        for (uint32_t li = tid; li < n; li += 1024) {
+            const size_t r = (size_t)k * cap + li;
+            const uint32_t T = tcell[r];
+            if (T == TT_PROP_INVALID) continue;
+            const uint32_t o = slotOf[T];
+            if (o != (uint32_t)r) tslot[r] = tslot[o];
+        }
+    __syncthreads();
+    const bool lds = nSlots <= TT_PROP_SLOTS && !overflow;
+    if (lds) {
+        for (uint32_t base = 0; base < n; base += TT_PROP_CHUNK) {
+            // stage a chunk of sources
+            if (tid < TT_PROP_CHUNK && base + tid < n) {
+                const uint32_t li = base + tid;
+                const uint32_t c = list[li];
+                const uint32_t cv = word[c];
+                const tt_uplift U = uplift[c];
+                cCum[tid] = U.cum;
+                cPrim[tid] = (uint8_t)tt_slot_plate(cv, (int)((U.types >> 1) & 3u));
+                uint32_t sec = 0;
+                for (int k = 0; k < 5; ++k) {
+                    const size_t r = (size_t)k * cap + li;
+                    const bool v = tcell[r] != TT_PROP_INVALID;
+                    cSlot[k][tid] = v ? (uint16_t)tslot[r] : (uint16_t)0;
+                    cOff[k][tid] = v ? toff[r] : (uint16_t)0;
+                    if (k > 0) sec |= (v ? (uint32_t)tt_slot_plate(cv, k - 1) : 0xFFu) << (8 * (k - 1));
+                }
+                cSec[tid] = sec;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                const uint32_t m = min((uint32_t)TT_PROP_CHUNK, n - base);
+                for (uint32_t j = 0; j < m; ++j) {
+                    const int prim = cPrim[j];
+                    const uint32_t Spp = cSlot[0][j], kpp = cOff[0][j];
+                    uint32_t pk[6], ps_[6];
+                    int cnt[6], np = 0, total = 0;
+                    auto count_at = [&](uint32_t key, uint32_t sl, int inc) {
+                        for (int i = 0; i < np; ++i)
+                            if (pk[i] == key) { cnt[i] += inc; total += inc; return; }
+                        if (np < 6) { pk[np] = key; ps_[np] = sl; cnt[np] = inc; ++np; total += inc; }
+                    };
+                    const uint32_t secs = cSec[j];
+                    for (int ci = 0; ci < 4; ++ci) {
+                        const uint32_t sec = (secs >> (8 * ci)) & 0xFFu;
+                        if (sec == 0xFFu) continue;
+                        const uint32_t Ssp = cSlot[1 + ci][j], ksp = cOff[1 + ci][j];
+                        if (kpp == ksp) {
+                            uint32_t inv = stInv[Spp];
+                            if (tt_insert_pair(inv, prim, (int)sec)) { stInv[Spp] = inv; count_at(kpp, Spp, 2); }
+                        } else {
+                            uint32_t inv = stInv[Spp];
+                            if (tt_insert_pair(inv, prim, (int)sec)) { stInv[Spp] = inv; count_at(kpp, Spp, 1); }
+                            uint32_t inv2 = stInv[Ssp];
+                            if (tt_insert_pair(inv2, prim, (int)sec)) { stInv[Ssp] = inv2; count_at(ksp, Ssp, 1); }
+                        }
+                        count_at(kpp, Spp, 1);
+                        count_at(ksp, Ssp, 1);
+                    }
+                    const float cum = cCum[j];
+                    for (int i = 0; i < np; ++i) stCum[ps_[i]] = stCum[ps_[i]] + (cum / (float)total) * (float)cnt[i];
+                }
+            }
+            __syncthreads();
+        }
+        for (uint32_t sl = tid; sl < nSlots; sl += 1024) {
+            tt_prop *t = &prop[slotCell[sl]];
+            t->inv = stInv[sl];
+            t->cum = stCum[sl];
+        }
+    } else if (tid == 0) {
+        tt_propagate_serial_global(d, list, n, word, uplift, height, ps, prop);
+    }
+    __syncthreads();
+    // leave the slot map cleared for the next step
+    for (uint32_t k = 0; k < 5; ++k)
+        for (uint32_t li = tid; li < n; li += 1024) {
+            const uint32_t T = tcell[(size_t)k * cap + li];
+            if (T != TT_PROP_INVALID) slotOf[T] = TT_PROP_INVALID;
+        }
+}
+
+int tt_prop_cap(long long cells) { return (int)std::max(1024LL, std::min(16384LL, cells / 64)); }
+
 hipError_t tt_launch_propagate_serial(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *count,
                                       const uint32_t *word, const tt_uplift *uplift, const float *height,
-                                      const tt_plate_step *ps, tt_prop *prop)
+                                      const tt_plate_step *ps, tt_prop *prop, int cap, uint32_t *tcell,
+                                      uint16_t *toff, uint32_t *tslot, uint32_t *slotOf, uint32_t *slotCell)
 {
-    hipLaunchKernelGGL(k_propagate_serial, dim3(1), dim3(64), 0, s, d, list, count, word, uplift, height, ps, prop);
+    const unsigned blocks = (unsigned)(((long long)cap * 5 + 255) / 256);
+    hipLaunchKernelGGL(k_prop_targets, dim3(blocks), dim3(256), 0, s, d, list, count, word, uplift, height, ps, cap,
+                       tcell, toff);
+    hipLaunchKernelGGL(k_prop_apply, dim3(1), dim3(1024), 0, s, d, list, count, word, uplift, height, ps, cap, tcell,
+                       toff, tslot, slotOf, slotCell, prop);
     return hipGetLastError();
 }
 

@@ -189,6 +189,31 @@ __device__ __forceinline__ float tt_div_const(float x, float L, float rL)
     return __builtin_fmaf(-e, rL, q0);
 }
 
+// a / b from y = RN(1 / b), correctly rounded, for a and b in [2^-40, 2^40]
+// (the caller checks the range).  q0 = RN(a y) is within two ulps of a / b;
+// one correction q1 = RN(q0 + (a - b q0) y) leaves a relative error of about
+// 2^-46 before its rounding, so q1 is a faithful quotient (one of the two
+// floats around a / b); then a - b q1 is exact in one fused multiply-add and
+// Markstein's correction RN(q1 + (a - b q1) y) is the correctly rounded
+// quotient, given y = RN(1 / b) (P. Markstein, IBM J. Res. Dev. 34(1), 1990;
+// Cornea, Golliver, Markstein, ARITH 1999, Theorem 1).  The range keeps the
+// quotient, the residuals and their products with y normal.  Five
+// operations, against about ten for an IEEE division; worth it where one
+// reciprocal serves several quotients.
+__device__ __forceinline__ float tt_div_by_rcp(float a, float b, float y)
+{
+    const float q0 = a * y;
+    const float r0 = __builtin_fmaf(-q0, b, a);
+    const float q1 = __builtin_fmaf(r0, y, q0);
+    const float r1 = __builtin_fmaf(-q1, b, a);
+    return __builtin_fmaf(r1, y, q1);
+}
+
+__device__ __forceinline__ int tt_div_rcp_in_range(float x)
+{
+    return (int)(x >= 0x1p-40f) & (int)(x <= 0x1p40f);
+}
+
 // Square root of 2^-96 <= x <= FLT_MAX: the hardware root (within 1 ulp) moved
 // to a neighbour when the fused residual x - s' * s shows that the neighbour
 // s' is the better root.  This is the correction that the IEEE square root of

@@ -55,6 +55,7 @@ template <int TH>
 struct tt_therm_lds {
     float hs[TH + 4][TT_EW];
     float tot[TH + 2][TT_FW];
+    float rcp[TH + 2][TT_FW];   // RN(1 / tot) where tot > 0
     float rmin[4], rmax[4];
 };
 
@@ -157,7 +158,10 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
             return;
         }
     }
-    // sum of the downhill differences above the talus, on the tile + 1 halo
+    // sum of the downhill differences above the talus, on the tile + 1 halo,
+    // and its reciprocal: a cell's total divides its 8 outflows here and is
+    // the divisor of its neighbours' inflows from it
+    int wide = 0;
     for (int i = tid; i < (TH + 2) * TT_FW; i += 256) {
         const int ty = i / TT_FW, tx = i - ty * TT_FW;
         const int ly = ty + 1, lx = tx + 1;
@@ -172,8 +176,14 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
             t = (diff > talus) ? t + diff : t;
         }
         S.tot[ty][tx] = t;
+        S.rcp[ty][tx] = t > 0.0f ? 1.0f / t : 0.0f;
+        wide |= t > 0.0f && !(t <= 0x1p40f);
     }
-    __syncthreads();
+    // Every quotient is diff / total with talus < diff <= total (the total
+    // sums the diff with other positive terms).  With the talus at least
+    // 2^-40 and every total at most 2^40 both lie in tt_div_by_rcp's range,
+    // which then gives the IEEE quotient from the stored reciprocal.
+    const bool byRcp = !__syncthreads_or(wide) && A.talus >= 0x1p-40f;
     int changed = 0;
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
@@ -196,20 +206,39 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
         float cc = 0.0f;
         if (COMP) cc = A.ch[cidx];
         const float cc0 = cc;
-        if (tc > 0.0f) {
+        if (byRcp) {
+            if (tc > 0.0f) {
+                const float rc = S.rcp[ry + 1][lane + 1];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (dOut[k] > talus) {
+                        const float q = (amp * tt_div_by_rcp(dOut[k], tc, rc)) * str;
+                        v = COMP ? tt_kahan_add(v, cc, -q) : v - q;
+                    }
+            }
 #pragma unroll
             for (int k = 0; k < 8; ++k)
-                if (dOut[k] > talus) {
-                    const float q = (amp * (dOut[k] / tc)) * str;
-                    v = COMP ? tt_kahan_add(v, cc, -q) : v - q;
+                if (dIn[k] > talus && tIn[k] > 0.0f) {
+                    const int sy = ly - c_toff[k][1], sx = lx - c_toff[k][0];
+                    const float q = (amp * tt_div_by_rcp(dIn[k], tIn[k], S.rcp[sy - 1][sx - 1])) * str;
+                    v = COMP ? tt_kahan_add(v, cc, q) : v + q;
+                }
+        } else {
+            if (tc > 0.0f) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (dOut[k] > talus) {
+                        const float q = (amp * (dOut[k] / tc)) * str;
+                        v = COMP ? tt_kahan_add(v, cc, -q) : v - q;
+                    }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (dIn[k] > talus && tIn[k] > 0.0f) {
+                    const float q = (amp * (dIn[k] / tIn[k])) * str;
+                    v = COMP ? tt_kahan_add(v, cc, q) : v + q;
                 }
         }
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (dIn[k] > talus && tIn[k] > 0.0f) {
-                const float q = (amp * (dIn[k] / tIn[k])) * str;
-                v = COMP ? tt_kahan_add(v, cc, q) : v + q;
-            }
         if (COMP) {
             changed |= __float_as_uint(cc) != __float_as_uint(cc0);
             A.ch[cidx] = cc;

@@ -45,9 +45,15 @@ hipError_t tt_launch_flip(hipStream_t s, uint32_t *divBitmap, const uint32_t *ha
 hipError_t tt_launch_propagate_local(hipStream_t s, const tt_dev &d, const uint32_t *word, const tt_uplift *uplift,
                                      const uint8_t *srcFlag, const uint8_t *contested, tt_prop *prop, int *jfa,
                                      uint8_t *serialFlag);
+// cap: sources whose walks k_prop_targets computes (5 cap entries in tcell,
+// toff and tslot); slotOf: one word per cell, all 0xFFFFFFFF between calls;
+// slotCell: TT_PROP_SLOTS words (k_uplift.hip)
 hipError_t tt_launch_propagate_serial(hipStream_t s, const tt_dev &d, const uint32_t *list, const uint32_t *count,
                                       const uint32_t *word, const tt_uplift *uplift, const float *height,
-                                      const tt_plate_step *ps, tt_prop *prop);
+                                      const tt_plate_step *ps, tt_prop *prop, int cap, uint32_t *tcell,
+                                      uint16_t *toff, uint32_t *tslot, uint32_t *slotOf, uint32_t *slotCell);
+int tt_prop_cap(long long cells);
+#define TT_PROP_SLOT_WORDS 5120
 hipError_t tt_launch_combine_contested(hipStream_t s, const tt_dev &d, const uint32_t *word, const tt_uplift *uplift,
                                        const uint8_t *contested, const uint8_t *srcFlag, tt_prop *prop, int *jfa,
                                        tt_step_ctl *ctl);

@@ -64,6 +64,10 @@ struct tt_world {
     tt_prop *prop = nullptr;
     uint8_t *srcFlag = nullptr, *contested = nullptr, *serialFlag = nullptr;
     uint32_t *serialList = nullptr, *serialCount = nullptr;
+    // ordered continental propagation: walk targets of the listed sources,
+    // their LDS slots, the per-cell slot map (kept all-ones) and slot cells
+    uint32_t *propCell = nullptr, *propSlot = nullptr, *slotOf = nullptr, *slotCell = nullptr;
+    uint16_t *propOff = nullptr;
     int *jfa[2] = {nullptr, nullptr};
     tt_dist *dist = nullptr;
     tt_plate_step *ps = nullptr;

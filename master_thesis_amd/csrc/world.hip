@@ -138,6 +138,14 @@ size_t tt_world_layout(const tt_sim_config &c, bool alloc, tt_world *w)
     take((void **)&w->serialFlag, N);
     take((void **)&w->serialList, N * 4);
     take((void **)&w->serialCount, 16);
+    {
+        const size_t cap = (size_t)tt_prop_cap((long long)N);
+        take((void **)&w->propCell, 5 * cap * 4);
+        take((void **)&w->propOff, 5 * cap * 2);
+        take((void **)&w->propSlot, 5 * cap * 4);
+        take((void **)&w->slotOf, N * 4);
+        take((void **)&w->slotCell, TT_PROP_SLOT_WORDS * 4);
+    }
     take((void **)&w->dist, N * 8);
     take((void **)&w->pd, 256 * sizeof(tt_plate_data));
     take((void **)&w->ps, 256 * sizeof(tt_plate_step));
@@ -433,7 +441,8 @@ static int pre_launch(tt_world *w)
                                          w->serialFlag));
             CK(tt_compact_u8(s, w->serialFlag, (long long)N, w->serialList, w->serialCount, w->scanTmp));
             CK(tt_launch_propagate_serial(s, d, w->serialList, w->serialCount, w->word, w->uplift, hOld, w->ps,
-                                          w->prop));
+                                          w->prop, tt_prop_cap((long long)N), w->propCell, w->propOff,
+                                          w->propSlot, w->slotOf, w->slotCell));
             CK(tt_launch_combine_contested(s, d, w->word, w->uplift, w->contested, w->srcFlag, w->prop, w->jfa[0],
                                            w->ctl));
             if (w->cfg.uplift_mode == TT_UPLIFT_SEPARABLE) {
@@ -1182,7 +1191,9 @@ int tt_world_create(const tt_sim_config *cfg, const tt_kernel_settings *ks, tt_w
         set_error("noise table missing (master_thesis_amd/data/noise_r8.bin)");
         return TT_ERR_STATE;
     }
-    if (hipMemcpy(w->noise, noise.data(), 65536, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(w->noise, noise.data(), 65536, hipMemcpyHostToDevice) != hipSuccess ||
+        tt_fill_u32(w->stream, w->slotOf, 0xFFFFFFFFu, (long long)w->N) != hipSuccess ||
+        hipStreamSynchronize(w->stream) != hipSuccess) {
         tt_world_destroy(w);
         return TT_ERR_DEVICE;
     }
@@ -1535,9 +1546,10 @@ int tt_world_destroy(tt_world *w)
 
 int tt_math_eval_device(int32_t fn, const float *in, float *out, int64_t n)
 {
-    if (!in || !out || n < 0) return TT_ERR_INVALID;
+    if (!in || !out || n < 0 || fn < TT_MATH_EXPF || fn > TT_MATH_DIV_RCP) return TT_ERR_INVALID;
     if (n == 0) return TT_OK;
-    const int per = (fn == TT_MATH_SIMPLEX || fn == TT_MATH_DISCRETE) ? 3 : (fn == TT_MATH_FMODF || fn == TT_MATH_ATAN2) ? 2 : 1;
+    const int per = (fn == TT_MATH_SIMPLEX || fn == TT_MATH_DISCRETE) ? 3
+                    : (fn == TT_MATH_FMODF || fn == TT_MATH_ATAN2 || fn == TT_MATH_DIV_RCP) ? 2 : 1;
     float *din = nullptr, *dout = nullptr;
     CK(hipMalloc((void **)&din, (size_t)n * per * 4));
     if (hipMalloc((void **)&dout, (size_t)n * 4) != hipSuccess) { (void)hipFree(din); return TT_ERR_NOMEM; }

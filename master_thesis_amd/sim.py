@@ -277,7 +277,8 @@ def device_count() -> int:
 
 def math_eval_device(fn: int, x: np.ndarray) -> np.ndarray:
     x = np.ascontiguousarray(x, dtype=np.float32)
-    per = {_abi.TT_MATH_SIMPLEX: 3, _abi.TT_MATH_DISCRETE: 3, _abi.TT_MATH_FMODF: 2, _abi.TT_MATH_ATAN2: 2}.get(fn, 1)
+    per = {_abi.TT_MATH_SIMPLEX: 3, _abi.TT_MATH_DISCRETE: 3, _abi.TT_MATH_FMODF: 2, _abi.TT_MATH_ATAN2: 2,
+           _abi.TT_MATH_DIV_RCP: 2}.get(fn, 1)
     n = x.size // per
     out = np.empty(n, dtype=np.float32)
     check(lib().tt_math_eval_device(fn, x.ctypes.data, out.ctypes.data, n), "tt_math_eval_device")

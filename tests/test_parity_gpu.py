@@ -98,6 +98,37 @@ def test_device_math_matches_host(sim, O):
         assert a.tobytes() == b.tobytes(), f"function {fn}: {int(np.sum(a.view(np.uint32) != b.view(np.uint32)))}"
 
 
+def test_division_by_stored_reciprocal_is_ieee(sim):
+    """The thermal passes divide each transfer by a cell's total through
+    tt_div_by_rcp from RN(1 / total).  On the device it equals IEEE float
+    division (numpy's) over its range [2^-40, 2^40]: random significands and
+    exponents, divisors whose significand is all ones, quotients that are
+    powers of two or land next to one, and the thermal shape diff <= total."""
+    rng = np.random.default_rng(7)
+    n = 4_000_000
+
+    def floats(sig, ex):
+        return ((ex.astype(np.int64) + 127) << 23 | sig).astype(np.uint32).view(np.float32)
+
+    sa = rng.integers(0, 1 << 23, n)
+    sb = rng.integers(0, 1 << 23, n)
+    sb[: n // 8] = (1 << 23) - 1
+    sa[n // 8: n // 4] = sb[n // 8: n // 4]
+    sa[n // 4: 3 * n // 8] = (sb[n // 4: 3 * n // 8] + 1) % (1 << 23)
+    a = floats(sa, rng.integers(-40, 40, n))
+    b = floats(sb, rng.integers(-40, 40, n))
+    t = rng.uniform(0.5, 4000.0, n // 2).astype(np.float32)
+    d = (t * rng.uniform(0.0, 1.0, n // 2).astype(np.float32)).astype(np.float32)
+    d = np.maximum(d, np.float32(2.0 ** -40))
+    a = np.concatenate([a, d])
+    b = np.concatenate([b, t])
+    say(f"[div_rcp] n={a.size}")
+    got = sim.math_eval_device(_abi.TT_MATH_DIV_RCP, np.stack([a, b], 1).ravel())
+    want = (a / b).astype(np.float32)
+    bad = int(np.sum(got.view(np.uint32) != want.view(np.uint32)))
+    assert bad == 0, f"{bad} quotients differ"
+
+
 @pytest.mark.parametrize("W,H,P,seed", [(128, 128, 8, 3), (320, 64, 5, 11), (64, 64, 2, 1)])
 def test_synthetic_worlds(sim, O, W, H, P, seed):
     cfg, ks = configs.synthetic(W, H, P, (0.5, 3.0), seed, hydraulic=10, thermal=5, ccl_every=1)
@@ -115,6 +146,19 @@ def test_reference_init_worlds(sim, O, W, H, P, seed, gens):
         cfg.voronoi_gens[i] = v
     seen = run_parity(sim, O, cfg, ks, 6, f"reference {W}x{H}")
     assert sum(s.complex_sources for s in seen) >= 0
+
+
+def test_reference_world_propagation_at_scale(sim, O):
+    """A 1024^2 reference-init world (continental crust everywhere): hundreds
+    of continental sources march to remote targets each step, and the ordered
+    pass (parallel walks, targets in LDS slots, sources applied in cell order)
+    must leave every propagated record as the oracle's serial pass does."""
+    cfg, ks = configs.reference_defaults()
+    cfg.width, cfg.height, cfg.num_plates, cfg.seed, cfg.iterations_for_ccl = 1024, 1024, 24, 3, 1
+    cfg.num_voronoi_gens = 0
+    cfg.hydraulic_iterations, cfg.thermal_iterations = 5, 2
+    seen = run_parity(sim, O, cfg, ks, 3, "reference 1024x1024")
+    assert sum(s.complex_sources for s in seen) > 100
 
 
 def test_splits(sim, O):
