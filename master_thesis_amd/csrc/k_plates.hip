@@ -1061,18 +1061,17 @@ struct tt_round {
     int round;
 };
 
-__device__ __forceinline__ bool tt_round_skip(const tt_round &R, uint32_t tile)
+// The activity of tile and its 8 neighbours in the previous round, one
+// flag per calling thread (threads 0..8; the others return 0): combined with
+// __syncthreads_or, a block learns in one memory round trip whether to run.
+__device__ __forceinline__ int tt_round_neighbour_active(const tt_round &R, uint32_t tile, int t)
 {
-    if (R.round == 0) return false;
-    if (R.rchg[(R.round - 1) % TT_RCHG] == 0u) return true;
+    if (R.round == 0 || t >= 9) return 0;
     const uint8_t *prev = R.act + (size_t)((R.round - 1) & 1) * R.ntiles;
     const int tpc = R.ntiles / R.tpr;
     const int tx = (int)(tile % (uint32_t)R.tpr), ty = (int)(tile / (uint32_t)R.tpr);
-    int any = 0;
-    for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx)
-            any |= prev[tt_wrap1(ty + dy, tpc) * R.tpr + tt_wrap1(tx + dx, R.tpr)];
-    return any == 0;
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    return prev[tt_wrap1(ty + dy, tpc) * R.tpr + tt_wrap1(tx + dx, R.tpr)];
 }
 
 __device__ __forceinline__ void tt_round_mark(const tt_round &R, uint32_t tile, int wrote, int capped)
@@ -1108,17 +1107,24 @@ __global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *t
                                                      const float *height, float *eff, tt_round R)
 {
     const uint32_t tile = tiles[blockIdx.x];
-    if (tt_round_skip(R, tile)) {
-        if (threadIdx.x == 0 && R.round > 0) R.act[(size_t)(R.round & 1) * R.ntiles + tile] = 0;
-        return;
-    }
+    // Whether the tile runs this round (round 0, or a change in the previous
+    // round and an active tile among it and its neighbours) and the tile's
+    // inputs are loaded together: one memory round trip before the decision.
+    const uint32_t prevChg = R.round > 0 ? R.rchg[(R.round - 1) % TT_RCHG] : 1u;
+    const int nbActive = tt_round_neighbour_active(R, tile, (int)threadIdx.x);
     constexpr int PR = TH / 4;
+    constexpr int NE = ((TH + 2) * 66 + 255) / 256;
     __shared__ float e[TH + 2][67];
     const int tx0 = (int)(tile % (uint32_t)R.tpr) * 64, ty0 = (int)(tile / (uint32_t)R.tpr) * TH;
     const int W = d.W, H = d.H;
-    for (int i = threadIdx.x; i < (TH + 2) * 66; i += 256) {
-        const int r = i / 66, q = i - r * 66;
-        e[r][q] = eff[(size_t)tt_wrap1(ty0 + r - 1, H) * W + tt_wrap1(tx0 + q - 1, W)];
+    float ev[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        const int i = (int)threadIdx.x + 256 * k;
+        if (i < (TH + 2) * 66) {
+            const int r = i / 66, q = i - r * 66;
+            ev[k] = eff[(size_t)tt_wrap1(ty0 + r - 1, H) * W + tt_wrap1(tx0 + q - 1, W)];
+        }
     }
     const int lx = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float c1[PR], c2[PR], init[PR];
@@ -1130,6 +1136,16 @@ __global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *t
         in[j] = ids[c] == plate;
         c1[j] = height[c] * 1.0f;
         c2[j] = height[c] * SQRT2;
+    }
+    const int active = __syncthreads_or(nbActive);
+    if (R.round > 0 && (prevChg == 0u || !active)) {
+        if (threadIdx.x == 0) R.act[(size_t)(R.round & 1) * R.ntiles + tile] = 0;
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        const int i = (int)threadIdx.x + 256 * k;
+        if (i < (TH + 2) * 66) e[i / 66][i % 66] = ev[k];
     }
     __syncthreads();
 #pragma unroll

@@ -89,7 +89,24 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
     // A frozen tile keeps its heights through every pass of the step: the
     // first pass copies it to the other buffer, later passes find both
     // buffers equal and do nothing.
-    if (A.frozen && TH == TT_TILE_H && A.frozen[tile]) {
+    // A cell's update reads the heights within two cells (and its own
+    // compensation).  If no cell of this tile or of its eight neighbouring
+    // tiles changed in the previous pass, this pass computes what the previous
+    // one did: nothing changes, and the output buffer (the previous pass's
+    // input) already holds the heights.
+    // Both flags are loaded together (one memory round trip), and a tile that
+    // is not frozen issues its height loads before it learns whether it is
+    // unchanged.
+    const bool track = chgOut && A.pass > 0;
+    const bool fz = A.frozen && TH == TT_TILE_H && A.frozen[tile];
+    int any = 0;
+    if (track && tid < 9) {
+        const uint8_t *prev = A.chg + (size_t)((A.pass - 1) & 1) * (nbx * nby);
+        const int tx = x0 / TT_TILE_W, ty = y0 / TT_TILE_H;
+        const int dy = tid / 3 - 1, dx = tid % 3 - 1;
+        any = prev[tt_wrap1(ty + dy, nby) * nbx + tt_wrap1(tx + dx, nbx)];
+    }
+    if (fz) {
         if (A.pass == 0)
 #pragma unroll
             for (int r = 0; r < RPW; ++r) {
@@ -99,38 +116,36 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
         if (chgOut && tid == 0) *chgOut = 0;
         return;
     }
-    // A cell's update reads the heights within two cells (and its own
-    // compensation).  If no cell of this tile or of its eight neighbouring
-    // tiles changed in the previous pass, this pass computes what the previous
-    // one did: nothing changes, and the output buffer (the previous pass's
-    // input) already holds the heights.
-    if (chgOut && A.pass > 0) {
-        const uint8_t *prev = A.chg + (size_t)((A.pass - 1) & 1) * (nbx * nby);
-        const int tx = x0 / TT_TILE_W, ty = y0 / TT_TILE_H;
-        int any = 0;
-        if (tid < 9) {
-            const int dy = tid / 3 - 1, dx = tid % 3 - 1;
-            any = prev[tt_wrap1(ty + dy, nby) * nbx + tt_wrap1(tx + dx, nbx)];
-        }
-        if (!__syncthreads_or(any)) {
-            if (tid == 0) *chgOut = 0;
-            return;
-        }
-    }
-    float mn = tt_u2f(0x7f800000u), mx = tt_u2f(0xff800000u);
+    constexpr int NHR = (TH + 4 + 3) / 4;   // halo rows per wavefront
+    float ha[NHR], hb[NHR];
     {
         const int gxa = tt_wrap1(x0 + lane - 2, W), gxb = tt_wrap1(x0 + 64 + lane - 2, W);
-        for (int ly = wv; ly < TH + 4; ly += 4) {
-            const int row = tt_wrap1(y0 + ly - 2, H) * W;
-            const float a = A.hin[row + gxa];
-            S.hs[ly][lane] = a;
-            mn = tt_fminf(mn, a);
-            mx = tt_fmaxf(mx, a);
+#pragma unroll
+        for (int j = 0; j < NHR; ++j) {
+            const int ly = wv + 4 * j;
+            if (ly < TH + 4) {
+                const int row = tt_wrap1(y0 + ly - 2, H) * W;
+                ha[j] = A.hin[row + gxa];
+                if (lane < TT_EW - 64) hb[j] = A.hin[row + gxb];
+            }
+        }
+    }
+    if (track && !__syncthreads_or(any)) {
+        if (tid == 0) *chgOut = 0;
+        return;
+    }
+    float mn = tt_u2f(0x7f800000u), mx = tt_u2f(0xff800000u);
+#pragma unroll
+    for (int j = 0; j < NHR; ++j) {
+        const int ly = wv + 4 * j;
+        if (ly < TH + 4) {
+            S.hs[ly][lane] = ha[j];
+            mn = tt_fminf(mn, ha[j]);
+            mx = tt_fmaxf(mx, ha[j]);
             if (lane < TT_EW - 64) {
-                const float b = A.hin[row + gxb];
-                S.hs[ly][64 + lane] = b;
-                mn = tt_fminf(mn, b);
-                mx = tt_fmaxf(mx, b);
+                S.hs[ly][64 + lane] = hb[j];
+                mn = tt_fminf(mn, hb[j]);
+                mx = tt_fmaxf(mx, hb[j]);
             }
         }
     }
@@ -452,6 +467,13 @@ __device__ __forceinline__ void tt_flow_velocity_dir(const float4 &Sf, const flo
     }
 }
 
+// Wavefronts per hydraulic block: two own rows per wavefront at every tile
+// height.  A thread then needs at most 64 VGPRs, so a SIMD holds eight
+// wavefronts (a 16-row block has 29 KB of LDS: four 512-thread blocks fit a
+// CU); with four own rows per wavefront the 16-row block needed 85 VGPRs and a
+// SIMD held five.
+template <int TH> constexpr int tt_hyd_waves() { return TH / 2; }
+
 // One sub-iteration on one tile.  FIRST: the water input is the step's
 // state water and the sediment is the step's state; otherwise the water input
 // is the previous sub-iteration's water after its flow (before evaporation),
@@ -474,13 +496,6 @@ __device__ __forceinline__ void tt_flow_velocity_dir(const float4 &Sf, const flo
 // into a scalar register (row arithmetic runs on the scalar unit), and IN
 // (a tile whose two-cell halo does not cross the grid edge) drops the wrap of
 // every halo and flux-ring index.
-// Wavefronts per hydraulic block: two own rows per wavefront at every tile
-// height.  A thread then needs at most 64 VGPRs, so a SIMD holds eight
-// wavefronts (a 16-row block has 29 KB of LDS: four 512-thread blocks fit a
-// CU); with four own rows per wavefront the 16-row block needed 85 VGPRs and a
-// SIMD held five.
-template <int TH> constexpr int tt_hyd_waves() { return TH / 2; }
-
 template <int TH, bool FIRST, bool LAST, bool COMP = false, bool IN = false, int NW = 4>
 __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_lds<TH> &S, int x0, int y0)
 {
@@ -489,7 +504,9 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
     constexpr int RPW = TH / NW;                 // own rows per wavefront
     constexpr int NFI = (TH + 2) * TT_FW;        // flux items (tile plus a one-cell halo)
     constexpr int NFR = (NFI + NT - 1) / NT;     // flux items per thread
-    constexpr int NHR = (TH + 4 + NW - 1) / NW;  // halo rows per wavefront
+    constexpr int NHR = (TH + 4 + NW - 1) / NW;  // halo rows per wavefront (columns -2..61)
+    constexpr int NXE = 4 * (TH + 4);            // the halo's last 4 columns (62..65), packed
+    static_assert(NXE <= NT, "one extra-column element per thread");
     const int tid = threadIdx.x;
     const int W = A.W, H = A.H;
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -503,12 +520,13 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
 #pragma unroll
         for (int r = 0; r < RPW; ++r) wnp[r] = tt_at(A.win, (uint32_t)((y0 + wv * RPW + r) * W + gx));
     }
-    // 2. heights and water on the tile plus a two-cell halo: rows wv,
-    // wv + NW, ...; lanes 0..63 take columns 0..63, lanes 0..3 also columns
-    // 64..67
+    // 2. heights and water on the tile plus a two-cell halo.  Columns
+    // -2..61 of the halo, a row per wavefront pass (row addresses stay
+    // scalar); the 4 columns 62..65 of all rows are packed over the first
+    // lanes of the block, so that they cost two wavefront passes instead of a
+    // partial pass per row.
     const int gxa = IN ? x0 + lane - 2 : tt_wrap1(x0 + lane - 2, W);
-    const int gxb = IN ? x0 + 62 + lane : tt_wrap1(x0 + 64 + lane - 2, W);
-    float hm[NHR], hw[NHR], hm2[NHR], hw2[NHR];
+    float hm[NHR], hw[NHR];
 #pragma unroll
     for (int j = 0; j < NHR; ++j) {
         const int ly = wv + NW * j;
@@ -516,11 +534,16 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             const int row = (IN ? y0 + ly - 2 : tt_wrap1(y0 + ly - 2, H)) * W;
             hm[j] = tt_at(A.hin, (uint32_t)(row + gxa));
             hw[j] = tt_at(A.win, (uint32_t)(row + gxa));
-            if (lane < TT_EW - 64) {
-                hm2[j] = tt_at(A.hin, (uint32_t)(row + gxb));
-                hw2[j] = tt_at(A.win, (uint32_t)(row + gxb));
-            }
         }
+    }
+    const int xey = tid >> 2, xex = TT_EW - 4 + (tid & 3);
+    const int xgy = IN ? y0 + xey - 2 : tt_wrap1(y0 + xey - 2, H);
+    const int xgx = IN ? x0 + xex - 2 : tt_wrap1(x0 + xex - 2, W);
+    float xm = 0.0f, xw = 0.0f;
+    if (tid < NXE) {
+        const uint32_t o = __umul24((uint32_t)xgy, (uint32_t)W) + (uint32_t)xgx;
+        xm = tt_at(A.hin, o);
+        xw = tt_at(A.win, o);
     }
     // 3. incoming fluxes of the flux ring.  Item i = tid + NT k sits at row
     // i / TT_FW, column i % TT_FW of the ring; NT = (NT / TT_FW) TT_FW +
@@ -569,19 +592,19 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             const int gy = IN ? y0 + ly - 2 : tt_wrap1(y0 + ly - 2, H);
             // the water this sub-iteration starts from: the state water plus
             // rain (first), or the previous flow water evaporated plus rain
-            float m = hm[j], w = hw[j];
+            float w = hw[j];
             if (!FIRST) w = tt_evaporate(w, evap);
             w = w + tt_rain_at(gxa, gy, rseed, rain);
-            S.M[ly][lane] = m;
+            S.M[ly][lane] = hm[j];
             S.Wt[ly][lane] = w;
-            if (lane < TT_EW - 64) {
-                float w2 = hw2[j];
-                if (!FIRST) w2 = tt_evaporate(w2, evap);
-                w2 = w2 + tt_rain_at(gxb, gy, rseed, rain);
-                S.M[ly][64 + lane] = hm2[j];
-                S.Wt[ly][64 + lane] = w2;
-            }
         }
+    }
+    if (tid < NXE) {
+        float w = xw;
+        if (!FIRST) w = tt_evaporate(w, evap);
+        w = w + tt_rain_at(xgx, xgy, rseed, rain);
+        S.M[xey][xex] = xm;
+        S.Wt[xey][xex] = w;
     }
     if (!FIRST) {
         // the incoming fluxes, for the previous sub-iteration's velocities
