@@ -6,7 +6,7 @@ conversion returns R exactly.  The library ships the 256x256 R channel as raw
 bytes (row-major, 65536 bytes) so that nothing at run time needs the PNG or a
 PNG decoder.  Run once, where the reference checkout is available:
 
-    python master_thesis_amd/data/extract_noise.py /path/to/reference
+    python tests/golden/extract_noise.py /path/to/reference
 """
 import sys
 from pathlib import Path
@@ -23,7 +23,7 @@ def main(ref_root: str) -> None:
     r, g, b = arr[..., 0], arr[..., 1], arr[..., 2]
     if not (np.array_equal(r, g) and np.array_equal(r, b)):
         raise SystemExit("noise.png is not grey; stb luma would differ from R")
-    out = Path(__file__).with_name("noise_r8.bin")
+    out = Path(__file__).resolve().parents[2] / "master_thesis_amd" / "data" / "noise_r8.bin"
     out.write_bytes(r.astype(np.uint8).tobytes())
     print(f"wrote {out} ({r.size} bytes), mean {r.mean():.3f}")
 
