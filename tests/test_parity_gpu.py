@@ -210,10 +210,12 @@ def test_stage_masks(sim, O, mask):
     run_parity(sim, O, cfg, ks, 3, f"mask {mask:#x}")
 
 
-@pytest.mark.parametrize("talus", [1e-4, 4e-3])
+@pytest.mark.parametrize("talus", [1e-4, 4e-3, 1e-16])
 def test_thermal_transfers(sim, O, talus):
     """Talus far below the height differences: transfers in every tile (1e-4)
-    or in some tiles only (4e-3), next to tiles that take the no-transfer path."""
+    or in some tiles only (4e-3), next to tiles that take the no-transfer path.
+    A talus below 2^-40 (an angle of 1e-16 times the cell size of 1000) sends
+    every tile to the IEEE division instead of the stored reciprocals."""
     cfg, ks = configs.synthetic(192, 128, 5, (0.5, 3.0), 6, hydraulic=3, thermal=6, ccl_every=1)
     ks.thermalThresholdAngle = talus
     run_parity(sim, O, cfg, ks, 3, f"thermal talus {talus}")
