@@ -55,12 +55,22 @@ struct tt_pool {
     int busy = 0;
     bool stop = false;
 
+    // A new worker starts from the current generation: after a restart (the
+    // member count changed) gen is not 0, and a worker starting from 0 would
+    // run the last job again, whose lambda refers to the stack of a call
+    // that has returned.
     void start(int n, int device)
     {
+        uint64_t g0;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            g0 = gen;
+            busy = 0;
+        }
         for (int i = 0; i < n; ++i)
-            th.emplace_back([this, device]() {
+            th.emplace_back([this, device, g0]() {
                 (void)hipSetDevice(device);
-                uint64_t seen = 0;
+                uint64_t seen = g0;
                 for (;;) {
                     std::function<void()> f;
                     {
@@ -88,6 +98,7 @@ struct tt_pool {
         f();
         std::unique_lock<std::mutex> lk(mu);
         idle.wait(lk, [&] { return busy == 0; });
+        job = nullptr;   // its captures die with the caller's frame
     }
     void shutdown()
     {

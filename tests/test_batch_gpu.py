@@ -97,3 +97,33 @@ def test_batch_membership_changes(sim):
         assert not diff_fields(m, w, STATE_FIELDS), f"world {i}"
     for w in ws + solo:
         w.close()
+
+
+def test_host_threads_restart_as_membership_grows_and_shrinks(sim):
+    """The batch's host threads are restarted whenever the member count moves
+    the number of threads in use (here on every step: 1, 2, 3, 4, 3, 2, 1
+    members with up to 4 threads).  Every member still ends equal to its
+    solo twin: a restarted thread runs only the jobs issued after it started."""
+    cfgs = mixed_configs()[:4]
+    say("[batch restarts]")
+    ws = [sim.World(c, k).init() for c, k in cfgs]
+    solo = [sim.World(c, k).init() for c, k in cfgs]
+    steps = [0] * 4
+    with sim.Batch([ws[0]], threads=4) as b:
+        plan = [+1, +2, +3, -3, -2, -1, 0]
+        for s, change in enumerate(plan):
+            b.step(1)
+            for i in range(4):
+                if ws[i] in b.members:
+                    steps[i] += 1
+            say(f"[batch restarts] step {s}: {len(b)} members")
+            if change > 0:
+                b.add(ws[change])
+            elif change < 0:
+                b.remove(ws[-change])
+    for w, n in zip(solo, steps):
+        w.step(n)
+    for i, (m, w) in enumerate(zip(ws, solo)):
+        assert not diff_fields(m, w, STATE_FIELDS), f"world {i} after {steps[i]} steps"
+    for w in ws + solo:
+        w.close()
