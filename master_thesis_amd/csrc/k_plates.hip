@@ -1396,26 +1396,24 @@ __device__ __forceinline__ void tt_bt_fill(tt_bt_win &S, const tt_dev &d, const 
     const int g = lane & 15, r0 = lane >> 4;
     int col = wxw + 4 * g;
     col -= (col >= d.W) ? d.W : 0;
-    float4 e[16];
-    uint32_t v[16];
+    // global loads and LDS stores do not alias: the compiler issues all the
+    // loads before the first store
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         int row = wyw + r0 + 4 * k;
         row -= (row >= d.H) ? d.H : 0;
         const size_t q = (size_t)row * d.W + col;
-        e[k] = *(const float4 *)(eff + q);
-        v[k] = *(const uint32_t *)(ids + q);
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        *(float4 *)&S.e[r0 + 4 * k][4 * g] = e[k];
-        *(uint32_t *)&S.id[r0 + 4 * k][4 * g] = v[k];
+        *(float4 *)&S.e[r0 + 4 * k][4 * g] = *(const float4 *)(eff + q);
+        *(uint32_t *)&S.id[r0 + 4 * k][4 * g] = *(const uint32_t *)(ids + q);
     }
     __syncthreads();
 }
 
-__global__ __launch_bounds__(64) void k_backtrack(tt_dev d, const float *eff, uint8_t *ids, int px, int py, int newId,
-                                                  tt_step_ctl *ctl)
+// One wavefront runs alone: it may use every register of its SIMD, so the
+// window fill's 16 staged loads stay in registers (bounded to the default
+// occupancy target, they spilled to scratch, a memory round trip per refill).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_backtrack(
+    tt_dev d, const float *eff, uint8_t *ids, int px, int py, int newId, tt_step_ctl *ctl)
 {
     __shared__ tt_bt_win S;
     const int W = d.W, H = d.H;

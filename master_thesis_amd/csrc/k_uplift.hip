@@ -102,13 +102,15 @@ __device__ void tt_propagate_serial_global(const tt_dev &d, const uint32_t *list
         const float hc = height[c];
         int ppx, ppy;
         tt_cont_prop(d, height, x, y, ps[prim].dirx, ps[prim].diry, ps[prim].pcx, ps[prim].pcy, hc, ppx, ppy);
-        int posx[6], posy[6], cnt[6] = {0, 0, 0, 0, 0, 0};
+        int posx[6] = {0, 0, 0, 0, 0, 0}, posy[6] = {0, 0, 0, 0, 0, 0}, cnt[6] = {0, 0, 0, 0, 0, 0};
         int total = 0;
         auto count_at = [&](int qx, int qy, int inc) {
+            bool done = false;
+#pragma unroll
             for (int i = 0; i < 6; ++i)
-                if (cnt[i] == 0 || (posx[i] == qx && posy[i] == qy)) {
+                if (!done && (cnt[i] == 0 || (posx[i] == qx && posy[i] == qy))) {
                     posx[i] = qx; posy[i] = qy; cnt[i] += inc; total += inc;
-                    break;
+                    done = true;
                 }
         };
         for (int ci = 0; ci < 4; ++ci) {
@@ -130,8 +132,11 @@ __device__ void tt_propagate_serial_global(const tt_dev &d, const uint32_t *list
             count_at(ppx, ppy, 1);
             count_at(spx, spy, 1);
         }
+        bool end = false;
+#pragma unroll
         for (int i = 0; i < 6; ++i) {
-            if (cnt[i] == 0) break;
+            end = end || cnt[i] == 0;
+            if (end) continue;
             tt_prop *t = &prop[(size_t)tt_wrapg(posy[i], d.H) * d.W + tt_wrapg(posx[i], d.W)];
             t->cum = t->cum + (U.cum / (float)total) * (float)cnt[i];
         }
@@ -273,12 +278,23 @@ __global__ __launch_bounds__(1024) void k_prop_apply(tt_dev d, const uint32_t *l
                 for (uint32_t j = 0; j < m; ++j) {
                     const int prim = cPrim[j];
                     const uint32_t Spp = cSlot[0][j], kpp = cOff[0][j];
-                    uint32_t pk[6], ps_[6];
-                    int cnt[6], np = 0, total = 0;
+                    // positions by key in up to 6 slots; every index is a
+                    // constant after unrolling, so the slots stay in registers
+                    uint32_t pk[6] = {0, 0, 0, 0, 0, 0}, ps_[6] = {0, 0, 0, 0, 0, 0};
+                    int cnt[6] = {0, 0, 0, 0, 0, 0}, np = 0, total = 0;
                     auto count_at = [&](uint32_t key, uint32_t sl, int inc) {
-                        for (int i = 0; i < np; ++i)
-                            if (pk[i] == key) { cnt[i] += inc; total += inc; return; }
-                        if (np < 6) { pk[np] = key; ps_[np] = sl; cnt[np] = inc; ++np; total += inc; }
+                        bool done = false;
+#pragma unroll
+                        for (int i = 0; i < 6; ++i)
+                            if (!done && i < np && pk[i] == key) { cnt[i] += inc; done = true; }
+                        if (!done && np < 6) {
+#pragma unroll
+                            for (int i = 0; i < 6; ++i)
+                                if (i == np) { pk[i] = key; ps_[i] = sl; cnt[i] = inc; }
+                            ++np;
+                            done = true;
+                        }
+                        if (done) total += inc;
                     };
                     const uint32_t secs = cSec[j];
                     for (int ci = 0; ci < 4; ++ci) {
@@ -298,7 +314,9 @@ __global__ __launch_bounds__(1024) void k_prop_apply(tt_dev d, const uint32_t *l
                         count_at(ksp, Ssp, 1);
                     }
                     const float cum = cCum[j];
-                    for (int i = 0; i < np; ++i) stCum[ps_[i]] = stCum[ps_[i]] + (cum / (float)total) * (float)cnt[i];
+#pragma unroll
+                    for (int i = 0; i < 6; ++i)
+                        if (i < np) stCum[ps_[i]] = stCum[ps_[i]] + (cum / (float)total) * (float)cnt[i];
                 }
             }
             __syncthreads();

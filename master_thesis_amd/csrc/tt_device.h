@@ -80,10 +80,12 @@ struct tt_step_ctl {
 __host__ __device__ inline void tt_hyd_water_bufs(float *w0, float *w1, float *flow, int wc, int k, int K,
                                                   const float **win, float **wout)
 {
-    float *water[2] = {w0, w1};
-    float *X = water[wc ^ 1], *Y = flow;
-    *win = k == 0 ? water[wc] : (((k - 1) & 1) ? Y : X);
-    if (k == K - 1) *wout = K == 1 ? water[wc ^ 1] : water[wc];
+    // selects, not an indexed array: a runtime index into a local array
+    // puts it in scratch memory in device code
+    float *cur = wc ? w1 : w0, *other = wc ? w0 : w1;
+    float *X = other, *Y = flow;
+    *win = k == 0 ? cur : (((k - 1) & 1) ? Y : X);
+    if (k == K - 1) *wout = K == 1 ? other : cur;
     else *wout = (k & 1) ? Y : X;
 }
 
