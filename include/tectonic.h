@@ -275,7 +275,8 @@ int tt_world_init(tt_world *w);
  * lists longer than the 4 slots of tt_sim_config. */
 int tt_world_reset(tt_world *w, uint32_t seed, int32_t num_plates, const int32_t *voronoi_gens, int32_t num_gens);
 /* executeIteration() x n, plate_tectonic_sim.h:117; returns after the device
- * work is complete (the reference synchronises at sim.cu:117-119). */
+ * work is complete (the reference synchronises at sim.cu:117-119).
+ * TT_ERR_STATE for a member of a batch (it steps through the batch). */
 int tt_world_step(tt_world *w, int32_t n);
 /* The world's configuration and settings (e.g. of a world made by
  * tt_world_load or a server job); either pointer may be NULL. */
@@ -290,6 +291,7 @@ int tt_world_iteration(tt_world *w, uint32_t *iteration);
 int tt_world_set_iteration(tt_world *w, uint32_t iteration);
 /* Device bytes held by the world (persistent + transient), for admission. */
 int tt_world_bytes(const tt_sim_config *cfg, size_t *bytes);
+/* TT_ERR_STATE (and nothing freed) while the world is a member of a batch. */
 int tt_world_destroy(tt_world *w);
 /* Headless export (replaces saveFloatCudaTextureToDiskAsGray16 and
  * saveGrayscale8BitCudaTextureToDiskAsRgb, texture_save.cuh:12-18): a 16-bit
@@ -358,8 +360,9 @@ int tt_generate_plate_data(uint64_t *rng_state, const tt_voronoi_seed *centers, 
  * from a captured hipGraph); the other stages run per member on the member's
  * stream from a pool of host threads.  Each member's result is bit-identical to
  * stepping it alone.  Members must be initialised worlds of the batch's
- * device; a member must not be stepped on its own while in a batch.  Removing
- * a member does not destroy it. */
+ * device; while in a batch a member cannot be stepped on its own or destroyed
+ * (TT_ERR_STATE).  Removing a member, or destroying the batch, leaves it
+ * valid. */
 typedef struct tt_batch tt_batch;
 int tt_batch_create(tt_batch **out);
 int tt_batch_add(tt_batch *b, tt_world *w);

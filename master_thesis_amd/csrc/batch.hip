@@ -415,6 +415,7 @@ int tt_batch_add(tt_batch *b, tt_world *w)
     if (std::find(b->worlds.begin(), b->worlds.end(), w) != b->worlds.end()) return TT_OK;
     if ((int)b->worlds.size() >= TT_BATCH_MAX_WORLDS) { tt_set_error("batch is full"); return TT_ERR_UNSUPPORTED; }
     b->worlds.push_back(w);
+    ++w->batches;
     b->dirty = true;
     return TT_OK;
 }
@@ -425,6 +426,7 @@ int tt_batch_remove(tt_batch *b, tt_world *w)
     auto it = std::find(b->worlds.begin(), b->worlds.end(), w);
     if (it == b->worlds.end()) return TT_ERR_INVALID;
     b->worlds.erase(it);
+    --w->batches;
     b->dirty = true;
     return TT_OK;
 }
@@ -472,6 +474,7 @@ int tt_batch_destroy(tt_batch *b)
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     b->pool.shutdown();
+    for (tt_world *w : b->worlds) --w->batches;   // members stay valid
     drop_graph(b);
     free_tables(b);
     for (auto &e : b->preDone)

@@ -27,6 +27,7 @@ struct tt_world {
     int W = 0, H = 0, N = 0;
     uint32_t iter = 0;
     bool initialized = false;
+    int batches = 0;            // batches holding the world (it steps only through them)
     int device = 0;
     int rows = 0;            // rows of the hydraulic tiles (0 or -1 until resolved)
     bool tuning = false;     // the tile height is tuned on the next step with enough sub-iterations

@@ -1286,6 +1286,7 @@ int tt_world_step(tt_world *w, int32_t n)
 {
     if (!w || n < 0) return TT_ERR_INVALID;
     if (!w->initialized) { set_error("tt_world_step before tt_world_init"); return TT_ERR_STATE; }
+    if (w->batches > 0) { set_error("tt_world_step on a batch member (remove it from the batch first)"); return TT_ERR_STATE; }
     for (int i = 0; i < n; ++i) CKS(step_once(w));
     CK(hipStreamSynchronize(w->stream));
     return TT_OK;
@@ -1529,6 +1530,10 @@ int tt_world_stage_times(tt_world *w, tt_stage_times *out)
 int tt_world_destroy(tt_world *w)
 {
     if (!w) return TT_ERR_INVALID;
+    if (w->batches > 0) {
+        set_error("tt_world_destroy of a batch member (remove it from the batch first)");
+        return TT_ERR_STATE;
+    }
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     for (int i = 0; i < 8; ++i)
         if (w->ev[i]) (void)hipEventDestroy(w->ev[i]);

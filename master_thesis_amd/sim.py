@@ -58,7 +58,7 @@ class World:
 
     def close(self) -> None:
         if getattr(self, "h", None):
-            lib().tt_world_destroy(self.h)
+            check(lib().tt_world_destroy(self.h), "tt_world_destroy")
             self.h = None
 
     def __enter__(self):

@@ -99,6 +99,28 @@ def test_batch_membership_changes(sim):
         w.close()
 
 
+def test_member_cannot_step_alone_or_be_destroyed(sim):
+    """A batch member steps only through its batch: stepping or destroying it
+    on its own is refused with TT_ERR_STATE and leaves it intact.  Removing it,
+    or closing the batch, hands it back."""
+    (c0, k0), (c1, k1) = mixed_configs()[:2]
+    say("[batch member guard]")
+    a, b_ = sim.World(c0, k0).init(), sim.World(c1, k1).init()
+    with sim.Batch([a, b_]) as b:
+        b.step(1)
+        for call in (lambda: a.step(1), a.close):
+            with pytest.raises(RuntimeError, match="bad state"):
+                call()
+        assert a.h and a.iteration == 1
+        b.remove(a)
+        a.step(1)
+        assert a.iteration == 2
+    b_.step(1)
+    assert b_.iteration == 2
+    a.close()
+    b_.close()
+
+
 def test_host_threads_restart_as_membership_grows_and_shrinks(sim):
     """The batch's host threads are restarted whenever the member count moves
     the number of threads in use (here on every step: 1, 2, 3, 4, 3, 2, 1
