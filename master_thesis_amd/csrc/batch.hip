@@ -274,6 +274,7 @@ void fill_desc(tt_bworld &B, const tt_world *w, int T, int K)
     B.rL = w->rL;
     B.divL = w->divL;
     B.sqrtOk = w->sqrtOk;
+    B.cvtOk = w->cvtOk;
     B.sin003 = tt_sinf(0.03f);
     for (int i = 0; i < 2; ++i) {
         B.h[i] = w->h[i];

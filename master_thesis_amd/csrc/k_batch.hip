@@ -63,6 +63,7 @@ __global__ __launch_bounds__(64 * tt_hyd_waves<TH>()) void k_hydraulic_batch(con
     A.rL = B.rL;
     A.divL = B.divL;
     A.sqrtOk = B.sqrtOk;
+    A.cvtOk = B.cvtOk;
     A.sin003 = B.sin003;
     const int hc = B.hc1 ^ (k & 1), fc = B.fc ^ (k & 1);
     A.hin = B.h[hc];

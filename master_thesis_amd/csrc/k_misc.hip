@@ -556,32 +556,36 @@ hipError_t tt_check_divisor(hipStream_t s, float L, float rL, uint32_t *bad)
     return e != hipSuccess ? e : f;
 }
 
-// Every float x in [2^-96, FLT_MAX] (all 2^32 bit patterns visited): counts the
-// x for which tt_sqrt_normal differs in any bit from the IEEE square root.
+// All 2^32 bit patterns visited: bad[0] counts the x in [2^-96, FLT_MAX] for
+// which tt_sqrt_normal differs in any bit from the IEEE square root, bad[1] the
+// x (every float, NaN and infinities included) for which the hardware
+// conversion tt_f2u32_hw differs from tt_f2u32_sat.
 __global__ __launch_bounds__(256) void k_check_sqrt(uint32_t *__restrict__ bad)
 {
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
-    uint32_t n = 0;
+    uint32_t n = 0, m = 0;
     for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < (1ull << 32);
          i += stride) {
         const float x = __uint_as_float((uint32_t)i);
+        m += tt_f2u32_hw(x) != tt_f2u32_sat(x) ? 1u : 0u;
         if (!tt_sqrt_in_range(x)) continue;
         n += __float_as_uint(sqrtf(x)) != __float_as_uint(tt_sqrt_normal(x)) ? 1u : 0u;
     }
     if (n) atomicAdd(bad, n);
+    if (m) atomicAdd(bad + 1, m);
 }
 
-hipError_t tt_check_sqrt(hipStream_t s, uint32_t *bad)
+hipError_t tt_check_sqrt(hipStream_t s, uint32_t bad[2])
 {
     uint32_t *d = nullptr;
-    hipError_t e = hipMalloc((void **)&d, sizeof(uint32_t));
+    hipError_t e = hipMalloc((void **)&d, 2 * sizeof(uint32_t));
     if (e != hipSuccess) return e;
-    e = tt_fill_u32(s, d, 0u, 1);
+    e = tt_fill_u32(s, d, 0u, 2);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_check_sqrt, dim3(8192), dim3(256), 0, s, d);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(bad, d, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(bad, d, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     const hipError_t f = hipFree(d);
     return e != hipSuccess ? e : f;

@@ -17,6 +17,7 @@ struct tt_bworld {
     float rL;                                 // RN(1 / L)
     int divL;                                 // the division by L may take tt_div_const
     int sqrtOk;                               // square roots may take tt_sqrt_normal
+    int cvtOk;                                // the rain's conversion may take tt_f2u32_hw
     float sin003;                             // tt_sinf(0.03f)
     float *h[2];
     float *water[2];

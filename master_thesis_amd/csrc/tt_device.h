@@ -176,7 +176,21 @@ struct tt_dev {
     float rL;               // RN(1 / hydrationPipeLength)
     int divL;               // 1: tt_div_const is exact for the pipe length (checked)
     int sqrtOk;             // 1: tt_sqrt_normal is exact over its range (checked)
+    int cvtOk;              // 1: tt_f2u32_hw equals tt_f2u32_sat for every float (checked)
 };
+
+// float -> uint32 by the hardware conversion alone.  v_cvt_u32_f32 clamps:
+// negative values and NaN give 0, values of 2^32 and above give 0xffffffff,
+// which is tt_f2u32_sat's definition; whether the device agrees for all 2^32
+// bit patterns is checked once per process (k_check_sqrt) before a kernel uses
+// it.  Written as one instruction because the C conversion is undefined
+// outside [0, 2^32).
+__device__ __forceinline__ uint32_t tt_f2u32_hw(float f)
+{
+    uint32_t r;
+    asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(f));
+    return r;
+}
 
 // x / L as RN(x * rL) corrected by one fused multiply-add with the negated
 // residual L * q0 - x, rL = RN(1/L).

@@ -20,6 +20,8 @@
 // of float operations is the reference's; dt = 1 (plate_tectonic_sim.cu:338-363).
 #pragma once
 
+#include <type_traits>
+
 #include "tt_device.h"
 
 #define TT_EW (TT_TILE_W + 4)   // tile width with a two-cell halo
@@ -276,6 +278,7 @@ struct tt_hyd_args {
     float rL;             // RN(1 / L)
     int divL;             // 1: the division by L may take tt_div_const
     int sqrtOk;           // 1: square roots of normal arguments may take tt_sqrt_normal
+    int cvtOk;            // 1: the rain's saturating conversion may take tt_f2u32_hw
     float sin003;         // tt_sinf(0.03f): the capacity's sine below the slope threshold
     const float *hin;
     float *hout;
@@ -314,6 +317,7 @@ __host__ __device__ inline void tt_hyd_settings(const tt_dev &d, int rseed, tt_h
     A.rL = d.rL;
     A.divL = d.divL;
     A.sqrtOk = d.sqrtOk;
+    A.cvtOk = d.cvtOk;
     A.sin003 = tt_sinf(0.03f);
 }
 
@@ -415,10 +419,17 @@ __device__ __forceinline__ float tt_evaporate(float w, float evap)
 // For a cell (0 <= x, y < 2^24) the lattice point is the cell itself, and the
 // lattice sum ((x*1723 + 0*93241) + y*149812) + 3824 + seed of tt_random_grid
 // loses its middle term: adding +0 to the non-negative x*1723 changes nothing.
-__device__ __forceinline__ float tt_rain_at(int gx, int gy, int rseed, float rain)
+// tt_rain_from takes the sum's first two terms, a = (float)x * 1723 +
+// (float)y * 149812, and rseedf = (float)(seed + iter).  The noise value
+// tt_random_float(u) * 2 - 1 is formed as one fused multiply-add by 2^-31:
+// the scaling by 2^-32 and the doubling are exact, so both round the same
+// value once.  HW: the saturating conversion is the hardware's (checked).
+template <bool HW>
+__device__ __forceinline__ float tt_rain_from(float a, float rseedf, float rain)
 {
-    const float s = (((float)gx * 1723.0f + (float)gy * 149812.0f) + 3824.0f) + (float)rseed;
-    const float n = tt_random_float(tt_f2u32_sat(s)) * 2.0f - 1.0f;
+    const float s = (a + 3824.0f) + rseedf;
+    const uint32_t u = HW ? tt_f2u32_hw(s) : tt_f2u32_sat(s);
+    const float n = __builtin_fmaf((float)tt_hash(u), 0x1p-31f, -1.0f);
     return (tt_fmaxf(0.0f, n) * 1.0f) * rain;
 }
 
@@ -585,27 +596,35 @@ __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_l
             if (COMP) c4[r][0] = A.csprev[c];
         }
     }
+    // the water this sub-iteration starts from: the state water plus rain
+    // (first), or the previous flow water evaporated plus rain.  The row's
+    // lattice term of the rain noise is wave-uniform and comes from the scalar
+    // unit; the column's term is the lane's own, formed once.
+    const float rseedf = (float)rseed, xterm = (float)gxa * 1723.0f;
+    auto store_halo = [&](auto hw_cvt) {
+        constexpr bool HW = decltype(hw_cvt)::value;
 #pragma unroll
-    for (int j = 0; j < NHR; ++j) {
-        const int ly = wv + NW * j;
-        if (ly < TH + 4) {
-            const int gy = IN ? y0 + ly - 2 : tt_wrap1(y0 + ly - 2, H);
-            // the water this sub-iteration starts from: the state water plus
-            // rain (first), or the previous flow water evaporated plus rain
-            float w = hw[j];
-            if (!FIRST) w = tt_evaporate(w, evap);
-            w = w + tt_rain_at(gxa, gy, rseed, rain);
-            S.M[ly][lane] = hm[j];
-            S.Wt[ly][lane] = w;
+        for (int j = 0; j < NHR; ++j) {
+            const int ly = wv + NW * j;
+            if (ly < TH + 4) {
+                const int gy = IN ? y0 + ly - 2 : tt_wrap1(y0 + ly - 2, H);
+                float w = hw[j];
+                if (!FIRST) w = tt_evaporate(w, evap);
+                w = w + tt_rain_from<HW>(xterm + tt_lattice_z_term((uint32_t)gy), rseedf, rain);
+                S.M[ly][lane] = hm[j];
+                S.Wt[ly][lane] = w;
+            }
         }
-    }
-    if (tid < NXE) {
-        float w = xw;
-        if (!FIRST) w = tt_evaporate(w, evap);
-        w = w + tt_rain_at(xgx, xgy, rseed, rain);
-        S.M[xey][xex] = xm;
-        S.Wt[xey][xex] = w;
-    }
+        if (tid < NXE) {
+            float w = xw;
+            if (!FIRST) w = tt_evaporate(w, evap);
+            w = w + tt_rain_from<HW>((float)xgx * 1723.0f + (float)xgy * 149812.0f, rseedf, rain);
+            S.M[xey][xex] = xm;
+            S.Wt[xey][xex] = w;
+        }
+    };
+    if (A.cvtOk) store_halo(std::true_type{});
+    else store_halo(std::false_type{});
     if (!FIRST) {
         // the incoming fluxes, for the previous sub-iteration's velocities
 #pragma unroll

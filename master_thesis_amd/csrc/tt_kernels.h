@@ -130,5 +130,6 @@ hipError_t tt_launch_split_finalize(hipStream_t s, const tt_dev &d, const uint32
 hipError_t tt_launch_probe(hipStream_t s, int which, const float4 *a, float4 *b, float *sink, long long n);
 // counts (into *bad, host memory) the numerators for which tt_div_const(x, L, rL) differs from x / L
 hipError_t tt_check_divisor(hipStream_t s, float L, float rL, uint32_t *bad);
-// counts (into *bad, host memory) the arguments in [2^-96, FLT_MAX] where tt_sqrt_normal differs from sqrtf
-hipError_t tt_check_sqrt(hipStream_t s, uint32_t *bad);
+// counts (into bad[0], host memory) the arguments in [2^-96, FLT_MAX] where tt_sqrt_normal differs from
+// sqrtf, and (into bad[1]) the floats where tt_f2u32_hw differs from tt_f2u32_sat
+hipError_t tt_check_sqrt(hipStream_t s, uint32_t bad[2]);

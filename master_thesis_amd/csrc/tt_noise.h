@@ -39,6 +39,35 @@ TT_HD float tt_random_grid(int x, int y, int z, int seed)
     return tt_random_float(tt_f2u32_sat(s)) * 2.0f - 1.0f;
 }
 
+// (float)z * 149812.0f, the lattice term of tt_random_grid's third coordinate,
+// for 0 <= z < 28669 (the exact product z * 149812 stays below 2^32), from
+// integer operations only: the product rounded to 24 significant bits, ties
+// to even, and packed as a float.  A wave-uniform z then runs on the scalar
+// unit.
+TT_HD float tt_lattice_z_term(uint32_t z)
+{
+    const uint32_t p = z * 149812u;
+    if (p == 0u) return 0.0f;
+    const int nb = 32 - __builtin_clz(p);  // significant bits of p
+    uint32_t q;
+    int e;  // p ~ q * 2^e with 2^23 <= q < 2^24
+    if (nb <= 24) {
+        q = p << (24 - nb);
+        e = nb - 24;
+    } else {
+        const int sh = nb - 24;
+        q = p >> sh;
+        const uint32_t rem = p & ((1u << sh) - 1u), half = 1u << (sh - 1);
+        if (rem > half || (rem == half && (q & 1u))) ++q;
+        e = sh;
+        if (q == (1u << 24)) {
+            q >>= 1;
+            ++e;
+        }
+    }
+    return tt_u2f(((uint32_t)(150 + e) << 23) | (q & 0x007fffffu));
+}
+
 // nearest-lattice noise (cuda_noise.cuh:470-477).
 TT_HD float tt_discrete_noise(float px, float py, float pz, float scale, int seed)
 {

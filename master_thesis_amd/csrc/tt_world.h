@@ -107,6 +107,7 @@ struct tt_world {
     float rL = 0.0f;            // RN(1 / hydrationPipeLength)
     int divL = 0;               // 1: the hydraulic division by the pipe length may take tt_div_const
     int sqrtOk = 0;             // 1: hydraulic square roots of normal arguments may take tt_sqrt_normal
+    int cvtOk = 0;              // 1: the rain's saturating conversion may take tt_f2u32_hw
 
     // host mirrors
     tt_step_ctl *hctl = nullptr;

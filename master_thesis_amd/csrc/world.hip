@@ -205,6 +205,7 @@ static tt_dev make_dev(const tt_world *w)
     d.rL = w->rL;
     d.divL = w->divL;
     d.sqrtOk = w->sqrtOk;
+    d.cvtOk = w->cvtOk;
     tt_dev_set_divisor(d);
     return d;
 }
@@ -214,10 +215,12 @@ static tt_dev make_dev(const tt_world *w)
 // for every numerator it may receive; the verdict is kept per divisor for the
 // process.  Divisors outside [2^-20, 2^20] keep the IEEE division.  Likewise
 // tt_sqrt_normal replaces the IEEE square root once it has been checked over
-// its whole range (one verdict for the process).
+// its whole range, and the rain's hardware float -> uint32 conversion
+// (tt_f2u32_hw) replaces the branches of tt_f2u32_sat once it has been checked
+// over every float (one verdict each for the process).
 static std::mutex g_divMu;
 static std::map<uint32_t, int> g_divOk;
-static int g_sqrtOk = -1;
+static int g_sqrtOk = -1, g_cvtOk = -1;
 
 static int set_divisor(tt_world *w)
 {
@@ -227,11 +230,13 @@ static int set_divisor(tt_world *w)
     w->divL = 0;
     std::lock_guard<std::mutex> lk(g_divMu);
     if (g_sqrtOk < 0) {
-        uint32_t bad = 1;
-        CK(tt_check_sqrt(w->stream, &bad));
-        g_sqrtOk = bad == 0 ? 1 : 0;
+        uint32_t bad[2] = {1, 1};
+        CK(tt_check_sqrt(w->stream, bad));
+        g_sqrtOk = bad[0] == 0 ? 1 : 0;
+        g_cvtOk = bad[1] == 0 ? 1 : 0;
     }
     w->sqrtOk = g_sqrtOk;
+    w->cvtOk = g_cvtOk;
     if (!(L >= 0x1p-20f && L <= 0x1p20f)) return TT_OK;
     uint32_t key;
     std::memcpy(&key, &L, 4);
