@@ -9,6 +9,7 @@
 // back small control values where the reference's control flow depends on
 // device data (CCL fragment count, neighbour-fill rounds, split decision and
 // relaxation convergence).
+#include <unistd.h>
 #include <zlib.h>
 
 #include <algorithm>
@@ -978,19 +979,30 @@ void tune_load()
     }
 }
 
+// written to a temporary file and renamed over the old one, so that another
+// process never reads a half-written file
 void tune_save()
 {
     const char *path = std::getenv("TECTONIC_TUNE_FILE");
     if (!path) return;
-    std::ofstream f(path);
-    if (!f) return;
-    f << "{";
-    bool first = true;
-    for (const auto &kv : g_tune) {
-        f << (first ? "" : ", ") << "\"" << kv.first << "\": " << kv.second;
-        first = false;
+    const std::string tmp = std::string(path) + ".tmp" + std::to_string((long long)getpid());
+    {
+        std::ofstream f(tmp);
+        if (!f) return;
+        f << "{";
+        bool first = true;
+        for (const auto &kv : g_tune) {
+            f << (first ? "" : ", ") << "\"" << kv.first << "\": " << kv.second;
+            first = false;
+        }
+        f << "}\n";
+        if (!f) {
+            f.close();
+            std::remove(tmp.c_str());
+            return;
+        }
     }
-    f << "}\n";
+    if (std::rename(tmp.c_str(), path) != 0) std::remove(tmp.c_str());
 }
 }  // namespace
 
@@ -1437,15 +1449,28 @@ int tt_world_load(const char *path, tt_world **out)
         set_error(std::string(path) + " is not a world snapshot (version 1 or 2)");
         return TT_ERR_INVALID;
     }
-    // read and check every field before any device work
+    // read and check every field before any device work; a field may not
+    // claim more bytes than the file still holds (nothing is allocated for a
+    // length the file cannot back)
+    const long here = ftell(f);
+    uint64_t left = 0;
+    if (here >= 0 && fseek(f, 0, SEEK_END) == 0) {
+        const long end = ftell(f);
+        if (end >= here) left = (uint64_t)(end - here);
+    }
+    if (here < 0 || fseek(f, here, SEEK_SET) != 0) {
+        fclose(f);
+        set_error(std::string(path) + ": cannot seek");
+        return TT_ERR_INVALID;
+    }
     struct blob { int32_t id; std::vector<uint8_t> data; };
     std::vector<blob> blobs;
     for (uint32_t k = 0; k < hd.nfields; ++k) {
         int32_t id = 0;
         uint64_t bytes = 0;
         uint32_t crc = 0;
-        if (fread(&id, 4, 1, f) != 1 || fread(&bytes, 8, 1, f) != 1 || fread(&crc, 4, 1, f) != 1 ||
-            bytes > (uint64_t)16384 * 16384 * 16) {
+        if (left < 16 || fread(&id, 4, 1, f) != 1 || fread(&bytes, 8, 1, f) != 1 || fread(&crc, 4, 1, f) != 1 ||
+            bytes > (uint64_t)16384 * 16384 * 16 || bytes > left - 16) {
             fclose(f);
             set_error(std::string(path) + ": truncated field header");
             return TT_ERR_INVALID;
@@ -1459,6 +1484,7 @@ int tt_world_load(const char *path, tt_world **out)
             set_error(std::string(path) + ": field " + std::to_string(id) + " is truncated or corrupt");
             return TT_ERR_INVALID;
         }
+        left -= 16 + bytes;
         blobs.push_back(std::move(b));
     }
     fclose(f);

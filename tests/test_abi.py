@@ -2,6 +2,7 @@
 struct layouts and defaults match the reference parameter surface."""
 import ctypes as C
 import re
+import struct
 from pathlib import Path
 
 import numpy as np
@@ -130,3 +131,20 @@ def test_snapshot_load_rejects_bad_files(tectonic_lib, tmp_path):
     assert tectonic_lib.tt_world_load(str(p).encode(), C.byref(vp)) == _abi.TT_ERR_INVALID
     assert b"snapshot" in tectonic_lib.tt_last_error()
     assert tectonic_lib.tt_world_load(str(tmp_path / "missing.snap").encode(), C.byref(vp)) == _abi.TT_ERR_INVALID
+
+
+def test_snapshot_load_rejects_a_field_longer_than_the_file(tectonic_lib, tmp_path):
+    """A field header that claims more bytes than the file still holds is
+    rejected before anything is allocated for it."""
+    class Header(C.Structure):
+        _fields_ = [("cfg", _abi.SimConfig), ("ks", _abi.KernelSettings), ("iteration", C.c_uint32),
+                    ("stats", _abi.StepStats), ("nfields", C.c_uint32)]
+    hd = Header()
+    hd.nfields = 1
+    body = bytes(hd)
+    field = struct.pack("<iQI", 0, 4 << 30, 0) + b"\0" * 64   # claims 4 GiB, holds 64 bytes
+    p = tmp_path / "short.snap"
+    p.write_bytes(b"TTSNAP2\n" + struct.pack("<I", len(body)) + body + field)
+    vp = C.c_void_p()
+    assert tectonic_lib.tt_world_load(str(p).encode(), C.byref(vp)) == _abi.TT_ERR_INVALID
+    assert b"truncated field header" in tectonic_lib.tt_last_error()
