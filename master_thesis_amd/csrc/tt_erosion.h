@@ -2,10 +2,11 @@
 // (hydraulic erosion), shared by the one-world kernels (k_erosion.hip) and the
 // batched kernels over many worlds (k_batch.hip).
 //
-// Tiles are 64 columns by TH rows (TH = 8, 16 or 32), 256 threads: lane l of
-// wavefront w owns column l and rows w * TH/4 .. w * TH/4 + TH/4 - 1.  The
-// kernel owns the LDS and passes it in, so a kernel that can run several tile
-// variants still allocates it once.
+// Tiles are 64 columns by TH rows (TH = 8, 16 or 32); lane l of a wavefront
+// owns column l of its rows.  A thermal block has 4 wavefronts (TH/4 rows
+// each), a hydraulic block TH/2 (two rows each, tt_hyd_waves).  The kernel owns
+// the LDS and passes it in, so a kernel that can run several tile variants
+// still allocates it once.
 //
 // Thermal: thermalErosionKernel (plate_tectonics_kernel.cu:2735-2786) in the
 // canonical Jacobi form -- each pass reads the previous heights; a cell's new
@@ -502,11 +503,11 @@ template <int TH> constexpr int tt_hyd_waves() { return TH / 2; }
 // water with their halo, the flux ring, the own flow water); the sediment taps
 // follow the velocity and are in flight while the new fluxes are computed.
 //
-// The kernel is bound by its vector ALU work more than by HBM, so addressing
-// is kept off the vector units where it can be: the wavefront index is read
-// into a scalar register (row arithmetic runs on the scalar unit), and IN
-// (a tile whose two-cell halo does not cross the grid edge) drops the wrap of
-// every halo and flux-ring index.
+// The kernel sits at the balance of its vector ALU work and its HBM traffic,
+// so addressing is kept off the vector units where it can be: the wavefront
+// index is read into a scalar register (row arithmetic runs on the scalar
+// unit), and IN (a tile whose two-cell halo does not cross the grid edge)
+// drops the wrap of every halo and flux-ring index.
 template <int TH, bool FIRST, bool LAST, bool COMP = false, bool IN = false, int NW = 4>
 __device__ __forceinline__ void tt_hydraulic_tile(const tt_hyd_args &A, tt_hyd_lds<TH> &S, int x0, int y0)
 {
