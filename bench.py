@@ -121,7 +121,9 @@ def run_batch_mode(args) -> dict:
         b.step(warm)
         t_warm = b.stage_times()
         t0 = time.perf_counter()
-        b.step(args.batch_steps)
+        for a in range(0, args.batch_steps, 25):
+            b.step(min(25, args.batch_steps - a))
+            print(f"batched: {min(a + 25, args.batch_steps)}/{args.batch_steps} steps", file=sys.stderr, flush=True)
         tb = time.perf_counter() - t0
         t_all = b.stage_times()
     phases = {k: (t_all[k] - t_warm[k]) / args.batch_steps for k in ("ms_pre", "ms_erosion", "ms_post")}
@@ -131,8 +133,10 @@ def run_batch_mode(args) -> dict:
     for w in worlds:
         w.step(warm)
     t0 = time.perf_counter()
-    for w in worlds:
+    for i, w in enumerate(worlds):
         w.step(args.batch_steps)
+        if (i + 1) % 32 == 0:
+            print(f"one at a time: {i + 1}/{len(worlds)} worlds stepped", file=sys.stderr, flush=True)
     tp = time.perf_counter() - t0
     for w in worlds:
         w.close()
