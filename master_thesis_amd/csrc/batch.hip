@@ -3,8 +3,9 @@
 // A step of a batch runs, for every member:
 //   1. the pre-erosion phase (plate movement, collisions, uplift) on the
 //      member's own stream, from a pool of host threads;
-//   2. the erosion of all members at once: thermal pass t and hydraulic
-//      sub-iteration k of every member in one launch each (k_batch.hip), the
+//   2. the erosion of all members at once: thermal passes 2j, 2j + 1 (after
+//      an odd first pass) and hydraulic sub-iteration k of every member in
+//      one launch each (k_batch.hip), the
 //      whole pass sequence captured once as a hipGraph and replayed every
 //      step (its kernels read the per-step buffer indices from a descriptor
 //      table copied to the device before the replay);
@@ -255,7 +256,7 @@ void fill_desc(tt_bworld &B, const tt_world *w, int T, int K)
     B.T = T;
     B.K = K;
     B.hc0 = w->ehc;
-    B.hc1 = w->ehc ^ (T & 1);
+    B.hc1 = w->ehc ^ (tt_thermal_launches(T, false) & 1);
     B.wc = w->wc;
     B.fc = w->fc;
     B.sc = w->sc;
@@ -288,10 +289,13 @@ void fill_desc(tt_bworld &B, const tt_world *w, int T, int K)
     B.chg = w->tileChg;
 }
 
-int launch_erosion(tt_batch *b, int Tmax, int Kmax)
+// Tcode = 2 * (most pairs of thermal passes of a member) + (some member has an
+// odd pass count)
+int launch_erosion(tt_batch *b, int Tcode, int Kmax)
 {
-    for (int t = 0; t < Tmax; ++t)
-        BCK(tt_launch_thermal_batch(b->stream, b->dDesc, b->dThermTiles, b->nThermTiles, t));
+    if (Tcode & 1) BCK(tt_launch_thermal_batch(b->stream, b->dDesc, b->dThermTiles, b->nThermTiles, 0, false));
+    for (int j = 0; j < Tcode / 2; ++j)
+        BCK(tt_launch_thermal_batch(b->stream, b->dDesc, b->dThermTiles, b->nThermTiles, j, true));
     for (int k = 0; k < Kmax; ++k)
         for (int s = 0; s < 3; ++s)
             BCK(tt_launch_hydraulic_batch(b->stream, slot_rows(s), b->dDesc, b->dHydTiles[s], b->nHydTiles[s], k));
@@ -299,20 +303,20 @@ int launch_erosion(tt_batch *b, int Tmax, int Kmax)
 }
 
 // replays the captured pass sequence; captures it when the pass counts changed
-int run_erosion(tt_batch *b, int Tmax, int Kmax)
+int run_erosion(tt_batch *b, int Tcode, int Kmax)
 {
-    if (b->useGraph && (b->graphT != Tmax || b->graphK != Kmax)) {
+    if (b->useGraph && (b->graphT != Tcode || b->graphK != Kmax)) {
         drop_graph(b);
         hipGraph_t g = nullptr;
         bool ok = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (ok) {
-            const int st = launch_erosion(b, Tmax, Kmax);
+            const int st = launch_erosion(b, Tcode, Kmax);
             ok = hipStreamEndCapture(b->stream, &g) == hipSuccess && st == TT_OK && g != nullptr;
         }
         if (ok) ok = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) == hipSuccess;
         if (g) (void)hipGraphDestroy(g);
         if (ok) {
-            b->graphT = Tmax;
+            b->graphT = Tcode;
             b->graphK = Kmax;
         } else {
             (void)hipGetLastError();
@@ -324,7 +328,7 @@ int run_erosion(tt_batch *b, int Tmax, int Kmax)
         BCK(hipGraphLaunch(b->exec, b->stream));
         return TT_OK;
     }
-    return launch_erosion(b, Tmax, Kmax);
+    return launch_erosion(b, Tcode, Kmax);
 }
 
 int batch_step_once(tt_batch *b)
@@ -341,19 +345,20 @@ int batch_step_once(tt_batch *b)
         BCK(hipEventRecord(b->preDone[i], w->stream));
         return TT_OK;
     }));
-    int Tmax = 0, Kmax = 0;
+    int Pmax = 0, anyOdd = 0, Kmax = 0;
     for (int i = 0; i < n; ++i) {
         tt_world *w = b->worlds[i];
         BCK(hipStreamWaitEvent(b->stream, b->preDone[i], 0));
         b->Ts[i] = tt_step_thermal_count(w);
         b->Ks[i] = tt_step_hydraulic_count(w);
-        Tmax = std::max(Tmax, b->Ts[i]);
+        Pmax = std::max(Pmax, tt_thermal_pairs(b->Ts[i], false));
+        anyOdd |= b->Ts[i] & 1;
         Kmax = std::max(Kmax, b->Ks[i]);
         fill_desc(b->hDesc[i], w, b->Ts[i], b->Ks[i]);
     }
     BCK(hipEventRecord(b->tPre, b->stream));
     BCK(hipMemcpyAsync(b->dDesc, b->hDesc, (size_t)n * sizeof(tt_bworld), hipMemcpyHostToDevice, b->stream));
-    BCKS(run_erosion(b, Tmax, Kmax));
+    BCKS(run_erosion(b, 2 * Pmax + anyOdd, Kmax));
     BCK(hipEventRecord(b->erosionDone, b->stream));
     BCK(hipEventRecord(b->tEro, b->stream));
     for (int i = 0; i < n; ++i) {

@@ -3,8 +3,9 @@
 // A batch keeps, in device memory, one descriptor per member world (sizes,
 // settings, both buffers of every ping-pong field and the buffer indices at
 // the start of the step's erosion) and a tile table per tile height: entry
-// (world << 20) | tile.  One launch of k_thermal_batch runs thermal pass t of
-// every member, one launch of k_hydraulic_batch<TH> runs hydraulic
+// (world << 20) | tile.  One launch of k_thermal_batch runs a pair of thermal
+// passes of every member (after one launch of the single first pass of the
+// members with an odd pass count), one launch of k_hydraulic_batch<TH> runs hydraulic
 // sub-iteration k of every member whose tiles have TH rows.  A member whose
 // pass count is smaller skips the later launches.  The tile bodies are the
 // one-world kernels' (tt_erosion.h), so a world's bits do not depend on the
@@ -13,13 +14,19 @@
 #include "tt_erosion.h"
 #include "tt_batch.h"
 
+// TWO: launch j runs passes of a pair (launch index j + T % 2 of the member,
+// members with fewer pairs return); otherwise the launch runs the single first
+// pass of the members with an odd pass count (the others return).
+template <bool TWO>
 __global__ __launch_bounds__(256) void k_thermal_batch(const tt_bworld *__restrict__ worlds,
-                                                       const uint32_t *__restrict__ tiles, int t)
+                                                       const uint32_t *__restrict__ tiles, int j)
 {
-    __shared__ tt_therm_lds<TT_TILE_H> S;
+    __shared__ std::conditional_t<TWO, tt_therm2_lds<TT_TILE_H>, tt_therm_lds<TT_TILE_H>> S;
     const uint32_t e = tiles[tt_xcd_tile(blockIdx.x, gridDim.x)];
     const tt_bworld &B = worlds[e >> 20];
-    if (t >= B.T) return;
+    const int odd = B.T & 1;
+    if (TWO ? j >= B.T / 2 : !odd) return;
+    const int l = TWO ? j + odd : 0;
     const int tile = (int)(e & 0xFFFFFu);
     const int nbx = B.W / TT_TILE_W;
     tt_therm_args A;
@@ -28,13 +35,14 @@ __global__ __launch_bounds__(256) void k_thermal_batch(const tt_bworld *__restri
     A.talus = B.talus;
     A.amp = B.amp;
     A.str = B.str;
-    A.hin = B.h[B.hc0 ^ (t & 1)];
-    A.hout = B.h[B.hc0 ^ ((t + 1) & 1)];
+    A.hin = B.h[B.hc0 ^ (l & 1)];
+    A.hout = B.h[B.hc0 ^ ((l + 1) & 1)];
     A.ch = nullptr;
     A.frozen = B.frozen;
     A.chg = B.chg;
-    A.pass = t;
-    tt_thermal_tile<TT_TILE_H>(A, S, (tile % nbx) * TT_TILE_W, (tile / nbx) * TT_TILE_H);
+    A.pass = l;
+    if constexpr (TWO) tt_thermal_tile2<TT_TILE_H>(A, S, (tile % nbx) * TT_TILE_W, (tile / nbx) * TT_TILE_H);
+    else tt_thermal_tile<TT_TILE_H>(A, S, (tile % nbx) * TT_TILE_W, (tile / nbx) * TT_TILE_H);
 }
 
 template <int TH>
@@ -105,10 +113,12 @@ __global__ __launch_bounds__(64 * tt_hyd_waves<TH>()) void k_hydraulic_batch(con
     }
 }
 
-hipError_t tt_launch_thermal_batch(hipStream_t s, const tt_bworld *worlds, const uint32_t *tiles, int ntiles, int t)
+hipError_t tt_launch_thermal_batch(hipStream_t s, const tt_bworld *worlds, const uint32_t *tiles, int ntiles, int j,
+                                   bool two)
 {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_thermal_batch, dim3((unsigned)ntiles), dim3(256), 0, s, worlds, tiles, t);
+    if (two) hipLaunchKernelGGL(k_thermal_batch<true>, dim3((unsigned)ntiles), dim3(256), 0, s, worlds, tiles, j);
+    else hipLaunchKernelGGL(k_thermal_batch<false>, dim3((unsigned)ntiles), dim3(256), 0, s, worlds, tiles, j);
     return hipGetLastError();
 }
 

@@ -5,14 +5,11 @@
 #include "tt_kernels.h"
 #include "tt_erosion.h"
 
-template <int TH, bool COMP>
-__global__ __launch_bounds__(256) void k_thermal(tt_therm_args A)
+// Tile of a thermal workgroup: bands of 4 tile rows dealt round-robin to the
+// XCDs, so that neighbouring tiles share an L2 and the rough parts of the
+// terrain (the tiles that do not return early) spread over all XCDs.
+__device__ __forceinline__ int tt_therm_block(int nbx, int nby)
 {
-    __shared__ tt_therm_lds<TH> S;
-    const int nbx = A.W / TT_TILE_W, nby = A.H / TH;
-    // bands of 4 tile rows dealt round-robin to the XCDs: neighbouring tiles
-    // share an L2, and the rough parts of the terrain (the tiles that do not
-    // return early) spread over all XCDs
     const int nb = nbx * nby, band = 4 * nbx, nbands = nby / 4;
     int b = blockIdx.x;
     if ((nby & 3) == 0 && (nbands & 7) == 0) {
@@ -21,7 +18,25 @@ __global__ __launch_bounds__(256) void k_thermal(tt_therm_args A)
     } else {
         b = tt_xcd_tile(blockIdx.x, nb);
     }
+    return b;
+}
+
+template <int TH, bool COMP>
+__global__ __launch_bounds__(256) void k_thermal(tt_therm_args A)
+{
+    __shared__ tt_therm_lds<TH> S;
+    const int nbx = A.W / TT_TILE_W;
+    const int b = tt_therm_block(nbx, A.H / TH);
     tt_thermal_tile<TH, COMP>(A, S, (b % nbx) * TT_TILE_W, (b / nbx) * TH);
+}
+
+// two passes per launch (plain precision, 64x16 tiles)
+__global__ __launch_bounds__(256) void k_thermal2(tt_therm_args A)
+{
+    __shared__ tt_therm2_lds<TT_TILE_H> S;
+    const int nbx = A.W / TT_TILE_W;
+    const int b = tt_therm_block(nbx, A.H / TT_TILE_H);
+    tt_thermal_tile2<TT_TILE_H>(A, S, (b % nbx) * TT_TILE_W, (b / nbx) * TT_TILE_H);
 }
 
 template <int TH, bool FIRST, bool LAST, bool COMP>
@@ -99,7 +114,7 @@ hipError_t tt_launch_thermal_frozen(hipStream_t s, const tt_dev &d, int T, const
 }
 
 hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, int rows, const float *hin, float *hout, float *ch,
-                             const uint8_t *frozen, uint8_t *chg, int pass)
+                             const uint8_t *frozen, uint8_t *chg, int pass, bool two)
 {
     tt_therm_args A = tt_therm_make(d, hin, hout);
     A.ch = ch;
@@ -107,7 +122,10 @@ hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, int rows, const flo
     A.chg = rows == TT_TILE_H ? chg : nullptr;
     A.pass = pass;
     const unsigned blocks = (unsigned)((d.W / TT_TILE_W) * (d.H / rows));
-    if (ch) {
+    if (two) {
+        if (ch || rows != TT_TILE_H) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_thermal2, dim3(blocks), dim3(256), 0, s, A);
+    } else if (ch) {
         if (rows == 8) hipLaunchKernelGGL((k_thermal<8, true>), dim3(blocks), dim3(256), 0, s, A);
         else if (rows == 32) hipLaunchKernelGGL((k_thermal<32, true>), dim3(blocks), dim3(256), 0, s, A);
         else hipLaunchKernelGGL((k_thermal<16, true>), dim3(blocks), dim3(256), 0, s, A);

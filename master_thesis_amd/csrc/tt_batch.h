@@ -26,12 +26,15 @@ struct tt_bworld {
     float2 *vel;
     float *wn;              // water after the flow between sub-iterations
     const uint8_t *frozen;  // per 64x16 tile: unchanged by this step's thermal passes
-    uint8_t *chg;           // per 64x16 tile and pass parity: the thermal pass changed a cell
+    uint8_t *chg;           // per 64x16 tile and launch parity: the thermal launch changed a cell
 };
 
 // tile table entries: (world index << 20) | tile index within the world
 #define TT_BATCH_MAX_WORLDS 4096
 
-hipError_t tt_launch_thermal_batch(hipStream_t s, const tt_bworld *worlds, const uint32_t *tiles, int ntiles, int t);
+// two: pair j of every member's thermal passes; otherwise the single first pass
+// of the members with an odd pass count
+hipError_t tt_launch_thermal_batch(hipStream_t s, const tt_bworld *worlds, const uint32_t *tiles, int ntiles, int j,
+                                   bool two);
 hipError_t tt_launch_hydraulic_batch(hipStream_t s, int rows, const tt_bworld *worlds, const uint32_t *tiles,
                                      int ntiles, int k);

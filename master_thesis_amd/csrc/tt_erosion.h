@@ -270,6 +270,204 @@ __device__ __forceinline__ void tt_thermal_tile(const tt_therm_args &A, tt_therm
     }
 }
 
+// Two passes in one launch.  The tile is loaded with a four-cell halo; the
+// first pass computes the tile plus two cells (what the second pass reads) in
+// LDS, the second pass the tile.  Every cell's update is the one-pass update of
+// the same heights, so the result is bit for bit that of two single passes,
+// with half the loads, stores and launches.
+template <int TH>
+struct tt_therm2_lds {
+    float h0[TH + 8][TT_TILE_W + 8];    // input heights, four-cell halo
+    float h1[TH + 4][TT_EW];            // the first pass's heights, two-cell halo
+    float tot[TH + 6][TT_TILE_W + 6];   // totals of the pass being computed
+    float rcp[TH + 6][TT_TILE_W + 6];   // RN(1 / tot) where tot > 0
+    float rmin[4], rmax[4];
+};
+
+// Totals of the downhill differences above the talus, and their reciprocals,
+// on an R x C region whose cell (ty, tx) sits at hs[ty + 1][tx + 1]; the sum
+// runs over directions 0..7 as in tt_thermal_tile.  Returns whether some total
+// lies beyond 2^40 (outside tt_div_by_rcp's range).
+template <int R, int C, int HP, int TP>
+__device__ __forceinline__ int tt_therm_totals(const float (*hs)[HP], float (*tot)[TP], float (*rcp)[TP], float talus)
+{
+    int wide = 0;
+    for (int i = threadIdx.x; i < R * C; i += 256) {
+        const int ty = i / C, tx = i - ty * C;
+        const int ly = ty + 1, lx = tx + 1;
+        const float hc = hs[ly][lx];
+        float nb[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) nb[k] = hs[ly + c_toff[k][1]][lx + c_toff[k][0]];
+        float t = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float diff = hc - nb[k];
+            t = (diff > talus) ? t + diff : t;
+        }
+        tot[ty][tx] = t;
+        rcp[ty][tx] = t > 0.0f ? 1.0f / t : 0.0f;
+        wide |= t > 0.0f && !(t <= 0x1p40f);
+    }
+    return wide;
+}
+
+// New height of the cell at hs[ly][lx] from the heights hs and the totals of
+// the same pass (a cell's total at tot[ly - 1][lx - 1]): own outflows
+// (directions 0..7), then inflows (directions 0..7), as tt_thermal_tile.
+template <int HP, int TP>
+__device__ __forceinline__ float tt_therm_cell(const float (*hs)[HP], const float (*tot)[TP], const float (*rcp)[TP],
+                                               int ly, int lx, float talus, float amp, float str, bool byRcp)
+{
+    const float hc = hs[ly][lx];
+    const float tc = tot[ly - 1][lx - 1];
+    float dOut[8], dIn[8], tIn[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        dOut[k] = hc - hs[ly + c_toff[k][1]][lx + c_toff[k][0]];
+        const int sy = ly - c_toff[k][1], sx = lx - c_toff[k][0];
+        dIn[k] = hs[sy][sx] - hc;
+        tIn[k] = tot[sy - 1][sx - 1];
+    }
+    float v = hc;
+    if (byRcp) {
+        if (tc > 0.0f) {
+            const float rc = rcp[ly - 1][lx - 1];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (dOut[k] > talus) v = v - (amp * tt_div_by_rcp(dOut[k], tc, rc)) * str;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (dIn[k] > talus && tIn[k] > 0.0f) {
+                const int sy = ly - c_toff[k][1], sx = lx - c_toff[k][0];
+                v = v + (amp * tt_div_by_rcp(dIn[k], tIn[k], rcp[sy - 1][sx - 1])) * str;
+            }
+    } else {
+        if (tc > 0.0f) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (dOut[k] > talus) v = v - (amp * (dOut[k] / tc)) * str;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (dIn[k] > talus && tIn[k] > 0.0f) v = v + (amp * (dIn[k] / tIn[k])) * str;
+    }
+    return v;
+}
+
+// Passes p and p + 1 of the step on one 64 x TH tile (256 threads; not for
+// the compensated mode, whose passes run one at a time).  The frozen and
+// change flags keep their meaning with a launch in place of a pass: a launch
+// whose input equals the previous launch's input on the tile and its eight
+// neighbouring tiles (the two passes read heights within four cells) writes
+// what the previous launch wrote, which its output buffer already holds.
+template <int TH>
+__device__ __forceinline__ void tt_thermal_tile2(const tt_therm_args &A, tt_therm2_lds<TH> &S, int x0, int y0)
+{
+    static_assert(TH == TT_TILE_H, "frozen and change flags are kept per 64x16 tile");
+    constexpr int RPW = TH / 4;
+    constexpr int NHR = (TH + 8) / 4;   // halo rows per wavefront
+    constexpr int XC = 8;               // halo columns beyond the first 64 (60..67)
+    const int tid = threadIdx.x;
+    const int W = A.W, H = A.H;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int nbx = W / TT_TILE_W, nby = H / TT_TILE_H;
+    const int tile = (y0 / TT_TILE_H) * nbx + x0 / TT_TILE_W;
+    uint8_t *chgOut = A.chg ? A.chg + (size_t)(A.pass & 1) * (nbx * nby) + tile : nullptr;
+    const bool track = chgOut && A.pass > 0;
+    const bool fz = A.frozen && A.frozen[tile];
+    int any = 0;
+    if (track && tid < 9) {
+        const uint8_t *prev = A.chg + (size_t)((A.pass - 1) & 1) * (nbx * nby);
+        const int tx = x0 / TT_TILE_W, ty = y0 / TT_TILE_H;
+        const int dy = tid / 3 - 1, dx = tid % 3 - 1;
+        any = prev[tt_wrap1(ty + dy, nby) * nbx + tt_wrap1(tx + dx, nbx)];
+    }
+    if (fz) {
+        if (A.pass == 0)
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int c = (y0 + wv * RPW + r) * W + x0 + lane;
+                A.hout[c] = A.hin[c];
+            }
+        if (chgOut && tid == 0) *chgOut = 0;
+        return;
+    }
+    float ha[NHR], hb[NHR];
+    {
+        const int gxa = tt_wrap1(x0 + lane - 4, W), gxb = tt_wrap1(x0 + 60 + lane, W);
+#pragma unroll
+        for (int j = 0; j < NHR; ++j) {
+            const int row = tt_wrap1(y0 + wv + 4 * j - 4, H) * W;
+            ha[j] = A.hin[row + gxa];
+            if (lane < XC) hb[j] = A.hin[row + gxb];
+        }
+    }
+    if (track && !__syncthreads_or(any)) {
+        if (tid == 0) *chgOut = 0;
+        return;
+    }
+    float mn = tt_u2f(0x7f800000u), mx = tt_u2f(0xff800000u);
+#pragma unroll
+    for (int j = 0; j < NHR; ++j) {
+        const int ly = wv + 4 * j;
+        S.h0[ly][lane] = ha[j];
+        mn = tt_fminf(mn, ha[j]);
+        mx = tt_fmaxf(mx, ha[j]);
+        if (lane < XC) {
+            S.h0[ly][64 + lane] = hb[j];
+            mn = tt_fminf(mn, hb[j]);
+            mx = tt_fmaxf(mx, hb[j]);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = tt_fminf(mn, __shfl_xor(mn, o, 64));
+        mx = tt_fmaxf(mx, __shfl_xor(mx, o, 64));
+    }
+    if (lane == 0) { S.rmin[wv] = mn; S.rmax[wv] = mx; }
+    __syncthreads();
+    const float talus = A.talus, amp = A.amp, str = A.str;
+    // a tile plus four cells within the talus: neither pass moves anything
+    // there (tt_thermal_tile's argument, applied to each pass)
+    {
+        const float tmn = tt_fminf(tt_fminf(S.rmin[0], S.rmin[1]), tt_fminf(S.rmin[2], S.rmin[3]));
+        const float tmx = tt_fmaxf(tt_fmaxf(S.rmax[0], S.rmax[1]), tt_fmaxf(S.rmax[2], S.rmax[3]));
+        if (tmx - tmn <= talus) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int ry = wv * RPW + r;
+                A.hout[(y0 + ry) * W + x0 + lane] = S.h0[ry + 4][lane + 4];
+            }
+            if (chgOut && tid == 0) *chgOut = 0;
+            return;
+        }
+    }
+    const bool rcpOk = talus >= 0x1p-40f;
+    // first pass: totals on the tile plus three cells, new heights on the tile
+    // plus two
+    const bool byA = !__syncthreads_or(tt_therm_totals<TH + 6, TT_TILE_W + 6>(S.h0, S.tot, S.rcp, talus)) && rcpOk;
+    for (int i = tid; i < (TH + 4) * TT_EW; i += 256) {
+        const int ry = i / TT_EW, rx = i - ry * TT_EW;
+        S.h1[ry][rx] = tt_therm_cell(S.h0, S.tot, S.rcp, ry + 2, rx + 2, talus, amp, str, byA);
+    }
+    __syncthreads();
+    // second pass: totals on the tile plus one cell, new heights on the tile
+    const bool byB = !__syncthreads_or(tt_therm_totals<TH + 2, TT_FW>(S.h1, S.tot, S.rcp, talus)) && rcpOk;
+    int changed = 0;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int ry = wv * RPW + r;
+        const float v = tt_therm_cell(S.h1, S.tot, S.rcp, ry + 2, lane + 2, talus, amp, str, byB);
+        changed |= __float_as_uint(v) != __float_as_uint(S.h0[ry + 4][lane + 4]);
+        A.hout[(y0 + ry) * W + x0 + lane] = v;
+    }
+    if (chgOut) {
+        changed = __syncthreads_or(changed);
+        if (tid == 0) *chgOut = changed ? 1 : 0;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // hydraulic
 // ---------------------------------------------------------------------------

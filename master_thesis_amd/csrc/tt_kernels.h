@@ -67,10 +67,15 @@ hipError_t tt_launch_separable_uplift(hipStream_t s, const tt_dev &d, const tt_p
 int tt_tile_rows_valid(int rows, int H);
 // ch / csprev / csout: Kahan compensation arrays of the precision mode (null: plain)
 // frozen: per 64x16 tile, unchanged through all thermal passes of the step
-// (tt_launch_thermal_frozen); pass: index of the pass within the step
+// (tt_launch_thermal_frozen); pass: index of the launch within the step;
+// two: the launch runs two passes (64x16 tiles, no compensation)
 hipError_t tt_launch_thermal(hipStream_t s, const tt_dev &d, int rows, const float *hin, float *hout,
                              float *ch = nullptr, const uint8_t *frozen = nullptr, uint8_t *chg = nullptr,
-                             int pass = 0);
+                             int pass = 0, bool two = false);
+// Launches of T thermal passes: passes run in pairs, an odd first pass alone;
+// compensated passes (precision mode) one at a time.
+inline int tt_thermal_pairs(int T, bool comp) { return comp ? 0 : T / 2; }
+inline int tt_thermal_launches(int T, bool comp) { return T - tt_thermal_pairs(T, comp); }
 hipError_t tt_launch_thermal_frozen(hipStream_t s, const tt_dev &d, int T, const float *h, float2 *mm,
                                     uint8_t *frozen);
 hipError_t tt_launch_hydraulic(hipStream_t s, const tt_dev &d, int rows, int first, int last, int rseed,

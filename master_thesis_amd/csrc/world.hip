@@ -545,12 +545,12 @@ int tt_step_hydraulic_count(const tt_world *w)
     return (w->cfg.stage_mask & TT_STAGE_HYDRAULIC) ? w->cfg.hydraulic_iterations : 0;
 }
 
-// buffer indices after T thermal and K hydraulic passes (the launches of
-// tt_step_erode); returns the sediment buffer holding the last sub-iteration's
-// output (input of the closing transport) in *sedLast
+// buffer indices after T thermal passes (tt_thermal_launches of them) and K
+// hydraulic sub-iterations; returns the sediment buffer holding the last
+// sub-iteration's output (input of the closing transport) in *sedLast
 void tt_step_erode_indices(tt_world *w, int T, int K, int *sedLast)
 {
-    w->ehc ^= (T + K) & 1;
+    w->ehc ^= (tt_thermal_launches(T, w->ch != nullptr) + K) & 1;
     w->wc = tt_hyd_water_index_after(w->wc, K);
     w->fc ^= K & 1;
     *sedLast = K > 0 ? (w->sc ^ ((K - 1) & 1)) : w->sc;
@@ -584,8 +584,11 @@ static int step_erode(tt_world *w)
     const int T = tt_step_thermal_count(w), K = tt_step_hydraulic_count(w);
     int hc = w->ehc;
     CKS(tt_step_thermal_prep(w));
-    for (int it = 0; it < T; ++it) {
-        CK(tt_launch_thermal(s, d, TT_TILE_H, w->h[hc], w->h[1 - hc], w->ch, w->tileFrozen, w->tileChg, it));
+    // an odd first pass alone, then pairs of passes per launch
+    const int TL = tt_thermal_launches(T, w->ch != nullptr), singles = TL - tt_thermal_pairs(T, w->ch != nullptr);
+    for (int l = 0; l < TL; ++l) {
+        CK(tt_launch_thermal(s, d, TT_TILE_H, w->h[hc], w->h[1 - hc], w->ch, w->tileFrozen, w->tileChg, l,
+                             l >= singles));
         hc = 1 - hc;
     }
     CKS(record(w, 3));
@@ -692,7 +695,7 @@ int tt_step_post(tt_world *w)
         CK(hipEventElapsedTime(&tot, w->ev[0], w->ev[7]));
         w->times.ms_total += tot;
         w->times.hydraulic_launches += (float)tt_step_hydraulic_count(w);
-        w->times.thermal_launches += (float)tt_step_thermal_count(w);
+        w->times.thermal_launches += (float)tt_thermal_launches(tt_step_thermal_count(w), w->ch != nullptr);
         w->times.steps += 1.0f;
     }
     return TT_OK;
