@@ -72,28 +72,36 @@ hipError_t tt_launch_plate_movement(hipStream_t s, const tt_dev &d, tt_plate_dat
 // writer recorded is MAX_PLATE_COUNT - its new owner, i.e. the accreted id is
 // the writer's owner after resolution.
 // ---------------------------------------------------------------------------
+// Four cells per thread (N is a multiple of 4, the fields 16-byte aligned):
+// the word, id and accretion accesses are 16- and 4-byte vectors.
 __global__ __launch_bounds__(256) void k_accretion_apply(tt_dev d, const uint32_t *accw, const float *height,
                                                          const uint8_t *idsIn, uint8_t *idsOut, uint8_t *accretion)
 {
-    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long c = ((long long)blockIdx.x * 256 + threadIdx.x) * 4;
     if (c >= d.N) return;
-    const uint32_t a = accw[c];
-    uint8_t id = idsIn[c];
-    uint8_t val = 0;
-    if (a != 0u) {
-        const uint8_t owner = idsIn[a - 1u];
-        val = (uint8_t)(TT_MAXP - owner);
-        const uint8_t aid = (uint8_t)(TT_MAXP - val);
-        if (aid != TT_MAXP && height[c] >= d.ks.continentalCrustThreshold) id = aid;
+    const uint4 a4 = *reinterpret_cast<const uint4 *>(accw + c);
+    const uchar4 i4 = *reinterpret_cast<const uchar4 *>(idsIn + c);
+    const uint32_t a[4] = {a4.x, a4.y, a4.z, a4.w};
+    uint8_t id[4] = {i4.x, i4.y, i4.z, i4.w}, val[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        val[k] = 0;
+        if (a[k] != 0u) {
+            const uint8_t owner = idsIn[a[k] - 1u];
+            val[k] = (uint8_t)(TT_MAXP - owner);
+            const uint8_t aid = (uint8_t)(TT_MAXP - val[k]);
+            if (aid != TT_MAXP && height[c + k] >= d.ks.continentalCrustThreshold) id[k] = aid;
+        }
     }
-    idsOut[c] = id;
-    accretion[c] = val;
+    *reinterpret_cast<uchar4 *>(idsOut + c) = make_uchar4(id[0], id[1], id[2], id[3]);
+    *reinterpret_cast<uchar4 *>(accretion + c) = make_uchar4(val[0], val[1], val[2], val[3]);
 }
 
 hipError_t tt_launch_accretion_apply(hipStream_t s, const tt_dev &d, const uint32_t *accw, const float *height,
                                      const uint8_t *idsIn, uint8_t *idsOut, uint8_t *accretion)
 {
-    const unsigned blocks = (unsigned)(((long long)d.N + 255) / 256);
+    if (d.N & 3) return hipErrorInvalidValue;
+    const unsigned blocks = (unsigned)(((long long)d.N / 4 + 255) / 256);
     hipLaunchKernelGGL(k_accretion_apply, dim3(blocks), dim3(256), 0, s, d, accw, height, idsIn, idsOut, accretion);
     return hipGetLastError();
 }
