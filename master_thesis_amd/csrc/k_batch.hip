@@ -66,6 +66,7 @@ __global__ __launch_bounds__(64 * tt_hyd_waves<TH>()) void k_hydraulic_batch(con
     A.ke = B.ke;
     A.kd = B.kd;
     A.fk = B.fk;
+    A.cpxMagic = 0;
     A.g = B.g;
     A.L = B.L;
     A.rL = B.rL;

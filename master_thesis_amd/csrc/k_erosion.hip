@@ -21,22 +21,30 @@ __device__ __forceinline__ int tt_therm_block(int nbx, int nby)
     return b;
 }
 
-// Tile of a hydraulic workgroup.  The dispatcher deals consecutive workgroups
-// to the 8 XCDs round-robin.  A grid of at least 4096 tiles passes through the
-// chip in several waves of resident blocks: there each XCD owns a block of
-// nbx / 8 tile columns and all XCDs advance through the rows together, so a
-// wave of blocks covers one band of rows (fewer DRAM pages open) and a tile's
-// neighbours above, below and beside it share its L2 except at the edges of
-// the column block.  A smaller grid is resident almost at once, and the XCD
-// map of contiguous tile ranges keeps neighbouring rows in one L2.  Every map
-// is a bijection on the tiles, so the results do not depend on it.
-__device__ __forceinline__ int tt_hyd_block(int nbx, int nby)
+// Tile (column, row) of a hydraulic workgroup.  The dispatcher deals
+// consecutive workgroups to the 8 XCDs round-robin.  A grid of at least 4096
+// tiles passes through the chip in several waves of resident blocks: there each
+// XCD owns a block of nbx / 8 tile columns and all XCDs advance through the
+// rows together, so a wave of blocks covers one band of rows (fewer DRAM pages
+// open) and a tile's neighbours above, below and beside it share its L2 except
+// at the edges of the column block.  A smaller grid is resident almost at once,
+// and the XCD map of contiguous tile ranges keeps neighbouring rows in one L2.
+// Every map is a bijection on the tiles, so the results do not depend on it.
+// The column-block map's row division is a multiply-high by a host constant,
+// so the map runs on the scalar unit.
+__device__ __forceinline__ void tt_hyd_tile_xy(int nbx, int nby, uint32_t cpxMagic, int &tx, int &ty)
 {
     const int nb = nbx * nby, blk = (int)blockIdx.x;
-    if (nb < 4096) return tt_xcd_tile(blk, nb);
-    if (nbx & 7) return blk;
-    const int cpx = nbx >> 3, x = blk & 7, k = blk >> 3;
-    return (k / cpx) * nbx + x * cpx + k % cpx;
+    if (nb >= 4096 && (nbx & 7) == 0) {
+        const int cpx = nbx >> 3, x = blk & 7, k = blk >> 3;
+        const int r = cpx == 1 ? k : (int)__umulhi((uint32_t)k, cpxMagic);
+        tx = x * cpx + (k - r * cpx);
+        ty = r;
+        return;
+    }
+    const int b = nb >= 4096 ? blk : tt_xcd_tile(blk, nb);
+    tx = b % nbx;
+    ty = b / nbx;
 }
 
 template <int TH, bool COMP>
@@ -62,9 +70,9 @@ __global__ __launch_bounds__(64 * tt_hyd_waves<TH>()) void k_hydraulic(tt_hyd_ar
 {
     constexpr int NW = tt_hyd_waves<TH>();
     __shared__ tt_hyd_lds<TH> S;
-    const int nbx = A.W / TT_TILE_W, nby = A.H / TH;
-    const int b = tt_hyd_block(nbx, nby);
-    const int x0 = (b % nbx) * TT_TILE_W, y0 = (b / nbx) * TH;
+    int tx, ty;
+    tt_hyd_tile_xy(A.W / TT_TILE_W, A.H / TH, A.cpxMagic, tx, ty);
+    const int x0 = tx * TT_TILE_W, y0 = ty * TH;
     if (x0 >= 2 && x0 + TT_TILE_W + 2 <= A.W && y0 >= 2 && y0 + TH + 2 <= A.H)
         tt_hydraulic_tile<TH, FIRST, LAST, COMP, true, NW>(A, S, x0, y0);
     else

@@ -479,6 +479,7 @@ struct tt_hyd_args {
     int sqrtOk;           // 1: square roots of normal arguments may take tt_sqrt_normal
     int cvtOk;            // 1: the rain's saturating conversion may take tt_f2u32_hw
     float sin003;         // tt_sinf(0.03f): the capacity's sine below the slope threshold
+    uint32_t cpxMagic;    // ceil(2^32 / (W / 512)) for the column-block tile map (0: W < 1024)
     const float *hin;
     float *hout;
     const float *win;     // first sub-iteration: the state water; later: the previous one's flow water
@@ -518,6 +519,10 @@ __host__ __device__ inline void tt_hyd_settings(const tt_dev &d, int rseed, tt_h
     A.sqrtOk = d.sqrtOk;
     A.cvtOk = d.cvtOk;
     A.sin003 = tt_sinf(0.03f);
+    // k / cpx = umulhi(k, ceil(2^32 / cpx)) for every block index k < 2^17 and
+    // cpx in 2..32 (checked exhaustively on the host)
+    const uint32_t cpx = (uint32_t)(d.W / TT_TILE_W) >> 3;
+    A.cpxMagic = cpx >= 2 ? 0xFFFFFFFFu / cpx + 1u : 0u;
 }
 
 // Squared slopes up to this bound have a square root below 0.03f (the root is
