@@ -41,6 +41,8 @@ __global__ __launch_bounds__(256) void k_thermal_batch(const tt_bworld *__restri
     A.frozen = B.frozen;
     A.chg = B.chg;
     A.pass = l;
+    A.bandMagic = 0;
+    A.nbxMagic = 0;
     if constexpr (TWO) tt_thermal_tile2<TT_TILE_H>(A, S, (tile % nbx) * TT_TILE_W, (tile / nbx) * TT_TILE_H);
     else tt_thermal_tile<TT_TILE_H>(A, S, (tile % nbx) * TT_TILE_W, (tile / nbx) * TT_TILE_H);
 }

@@ -5,20 +5,45 @@
 #include "tt_kernels.h"
 #include "tt_erosion.h"
 
-// Tile of a thermal workgroup: bands of 4 tile rows dealt round-robin to the
-// XCDs, so that neighbouring tiles share an L2 and the rough parts of the
-// terrain (the tiles that do not return early) spread over all XCDs.
-__device__ __forceinline__ int tt_therm_block(int nbx, int nby)
+// Tile (column, row) of a thermal workgroup: bands of 4 tile rows dealt
+// round-robin to the XCDs, so that neighbouring tiles share an L2 and the rough
+// parts of the terrain (the tiles that do not return early) spread over all
+// XCDs.  Workgroup j of XCD x takes band x + 8 (j / band) and tile j % band of
+// it (band = 4 nbx tiles); the divisions are multiply-highs by host constants,
+// so the map runs on the scalar unit.
+__device__ __forceinline__ void tt_therm_tile_xy(const tt_therm_args &A, int nbx, int nby, int &tx, int &ty)
 {
-    const int nb = nbx * nby, band = 4 * nbx, nbands = nby / 4;
-    int b = blockIdx.x;
-    if ((nby & 3) == 0 && (nbands & 7) == 0) {
-        const int x = b & 7, j = b >> 3;
-        b = (x + 8 * (j / band)) * band + j % band;
+    const int blk = (int)blockIdx.x, band = 4 * nbx;
+    if ((nby & 3) == 0 && ((nby >> 2) & 7) == 0) {
+        const int x = blk & 7, j = blk >> 3;
+        const int q = (int)__umulhi((uint32_t)j, A.bandMagic);
+        const int m = j - q * band;
+        const int r = nbx == 1 ? m : (int)__umulhi((uint32_t)m, A.nbxMagic);
+        tx = m - r * nbx;
+        ty = (x + 8 * q) * 4 + r;
     } else {
-        b = tt_xcd_tile(blockIdx.x, nb);
+        const int b = tt_xcd_tile(blk, nbx * nby);
+        tx = b % nbx;
+        ty = b / nbx;
     }
-    return b;
+}
+
+template <int TH, bool COMP>
+__global__ __launch_bounds__(256) void k_thermal(tt_therm_args A)
+{
+    __shared__ tt_therm_lds<TH> S;
+    int tx, ty;
+    tt_therm_tile_xy(A, A.W / TT_TILE_W, A.H / TH, tx, ty);
+    tt_thermal_tile<TH, COMP>(A, S, tx * TT_TILE_W, ty * TH);
+}
+
+// two passes per launch (plain precision, 64x16 tiles)
+__global__ __launch_bounds__(256) void k_thermal2(tt_therm_args A)
+{
+    __shared__ tt_therm2_lds<TT_TILE_H> S;
+    int tx, ty;
+    tt_therm_tile_xy(A, A.W / TT_TILE_W, A.H / TT_TILE_H, tx, ty);
+    tt_thermal_tile2<TT_TILE_H>(A, S, tx * TT_TILE_W, ty * TT_TILE_H);
 }
 
 // Tile (column, row) of a hydraulic workgroup.  The dispatcher deals
@@ -45,24 +70,6 @@ __device__ __forceinline__ void tt_hyd_tile_xy(int nbx, int nby, uint32_t cpxMag
     const int b = nb >= 4096 ? blk : tt_xcd_tile(blk, nb);
     tx = b % nbx;
     ty = b / nbx;
-}
-
-template <int TH, bool COMP>
-__global__ __launch_bounds__(256) void k_thermal(tt_therm_args A)
-{
-    __shared__ tt_therm_lds<TH> S;
-    const int nbx = A.W / TT_TILE_W;
-    const int b = tt_therm_block(nbx, A.H / TH);
-    tt_thermal_tile<TH, COMP>(A, S, (b % nbx) * TT_TILE_W, (b / nbx) * TH);
-}
-
-// two passes per launch (plain precision, 64x16 tiles)
-__global__ __launch_bounds__(256) void k_thermal2(tt_therm_args A)
-{
-    __shared__ tt_therm2_lds<TT_TILE_H> S;
-    const int nbx = A.W / TT_TILE_W;
-    const int b = tt_therm_block(nbx, A.H / TT_TILE_H);
-    tt_thermal_tile2<TT_TILE_H>(A, S, (b % nbx) * TT_TILE_W, (b / nbx) * TT_TILE_H);
 }
 
 template <int TH, bool FIRST, bool LAST, bool COMP>

@@ -43,6 +43,8 @@ struct tt_therm_args {
     const uint8_t *frozen;  // per 64x16 tile: unchanged by every pass of this step (null: unknown)
     uint8_t *chg;           // per 64x16 tile and pass parity: the pass changed a cell (null: not tracked)
     int pass;               // pass index within the step
+    uint32_t bandMagic;     // ceil(2^32 / (4 nbx)) and ceil(2^32 / nbx) (0 for nbx = 1): the
+    uint32_t nbxMagic;      // tile map's divisions as multiply-highs (nbx = W / 64)
 };
 
 // Kahan-compensated sum += d: c carries the low-order part lost so far
@@ -76,6 +78,11 @@ __host__ __device__ inline tt_therm_args tt_therm_make(const tt_dev &d, const fl
     A.frozen = nullptr;
     A.chg = nullptr;
     A.pass = 0;
+    // exact for every block index below 2^16 and every nbx up to 256 (checked
+    // exhaustively on the host)
+    const uint32_t nbx = (uint32_t)(d.W / TT_TILE_W);
+    A.bandMagic = 0xFFFFFFFFu / (4u * nbx) + 1u;
+    A.nbxMagic = nbx >= 2 ? 0xFFFFFFFFu / nbx + 1u : 0u;
     return A;
 }
 
