@@ -1063,11 +1063,21 @@ __global__ __launch_bounds__(256) void k_split_tiles(tt_dev d, const uint8_t *id
 // (two host batches), so a round can read its predecessor across batches.
 struct tt_round {
     uint32_t *rchg;
-    uint8_t *act;   // 2 x tiles
-    int ntiles;     // tiles in the whole grid
-    int tpr;        // tiles per row
+    uint8_t *act;       // 2 x tiles
+    int ntiles;         // tiles in the whole grid
+    int tpr;            // tiles per row
+    int tpc;            // tiles per column
+    uint32_t tprMagic;  // ceil(2^32 / tpr) (0 for tpr = 1): tile / tpr as a multiply-high,
+                        // exact for every tile index below 2^19 and tpr up to 256
     int round;
 };
+
+// (column, row) of a tile of the split grid, on the scalar unit
+__device__ __forceinline__ void tt_round_tile_xy(const tt_round &R, uint32_t tile, int &tx, int &ty)
+{
+    ty = R.tpr == 1 ? (int)tile : (int)__umulhi(tile, R.tprMagic);
+    tx = (int)tile - ty * R.tpr;
+}
 
 // The activity of tile and its 8 neighbours in the previous round, one
 // flag per calling thread (threads 0..8; the others return 0): combined with
@@ -1076,10 +1086,10 @@ __device__ __forceinline__ int tt_round_neighbour_active(const tt_round &R, uint
 {
     if (R.round == 0 || t >= 9) return 0;
     const uint8_t *prev = R.act + (size_t)((R.round - 1) & 1) * R.ntiles;
-    const int tpc = R.ntiles / R.tpr;
-    const int tx = (int)(tile % (uint32_t)R.tpr), ty = (int)(tile / (uint32_t)R.tpr);
+    int tx, ty;
+    tt_round_tile_xy(R, tile, tx, ty);
     const int dy = t / 3 - 1, dx = t % 3 - 1;
-    return prev[tt_wrap1(ty + dy, tpc) * R.tpr + tt_wrap1(tx + dx, R.tpr)];
+    return prev[tt_wrap1(ty + dy, R.tpc) * R.tpr + tt_wrap1(tx + dx, R.tpr)];
 }
 
 __device__ __forceinline__ void tt_round_mark(const tt_round &R, uint32_t tile, int wrote, int capped)
@@ -1123,7 +1133,9 @@ __global__ __launch_bounds__(256) void k_relax_tiles(tt_dev d, const uint32_t *t
     constexpr int PR = TH / 4;
     constexpr int NE = ((TH + 2) * 66 + 255) / 256;
     __shared__ float e[TH + 2][67];
-    const int tx0 = (int)(tile % (uint32_t)R.tpr) * 64, ty0 = (int)(tile / (uint32_t)R.tpr) * TH;
+    int ttx, tty;
+    tt_round_tile_xy(R, tile, ttx, tty);
+    const int tx0 = ttx * 64, ty0 = tty * TH;
     const int W = d.W, H = d.H;
     float ev[NE];
 #pragma unroll
@@ -1221,7 +1233,9 @@ static tt_round make_round(const tt_dev &d, uint32_t *rchg, uint8_t *act, int r)
     R.rchg = rchg;
     R.act = act;
     R.tpr = d.W / 64;
-    R.ntiles = R.tpr * (d.H / tt_split_tile_h(d));
+    R.tpc = d.H / tt_split_tile_h(d);
+    R.ntiles = R.tpr * R.tpc;
+    R.tprMagic = R.tpr >= 2 ? 0xFFFFFFFFu / (uint32_t)R.tpr + 1u : 0u;
     R.round = r;
     return R;
 }

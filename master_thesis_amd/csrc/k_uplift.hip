@@ -425,6 +425,13 @@ __device__ __forceinline__ int tt_tdist2(int p, int s, int wh)
     return __builtin_amdgcn_sdot2(t, t, 0, false);
 }
 
+// Toroidal wrap, remainder and quotient for the jump flood.  On power-of-two
+// sides (P2) they are masks and shifts; otherwise the general forms, whose
+// integer divisions the compiler expands on the vector unit in every thread.
+template <bool P2> __device__ __forceinline__ int tt_jwrap(int v, int n) { return P2 ? (v & (n - 1)) : tt_wrapg(v, n); }
+template <bool P2> __device__ __forceinline__ int tt_jmod(int v, int n) { return P2 ? (v & (n - 1)) : v % n; }
+template <bool P2> __device__ __forceinline__ int tt_jdiv(int v, int n) { return P2 ? (v >> __builtin_ctz((unsigned)n)) : v / n; }
+
 // JFAPass (plate_tectonics_kernel.cu:1512-1571): candidates at (ox, oy) * step
 // in scan order oy = -1..1, ox = -1..1; a strictly smaller squared distance
 // replaces the current seed.  4 consecutive cells of a row per thread: every
@@ -446,7 +453,7 @@ __global__ __launch_bounds__(256) void k_jfa_pass4(tt_dev d, const int4 *__restr
     tt_cell_xy(q << 2, d, x, y);
     const int W = d.W, H = d.H;
     const int wh = (H << 16) | W;
-    const int sx = step % W, sy = step % H;
+    const int sx = tt_jmod<P2>(step, W), sy = tt_jmod<P2>(step, H);
     const int rows[3] = {tt_wrap1(y - sy, H) * W, y * W, tt_wrap1(y + sy, H) * W};
     // window: groups at x - 4, x, x + 4; otherwise at x - s, x, x + s
     constexpr bool WINDOW = SX > 0;
@@ -506,17 +513,17 @@ __global__ __launch_bounds__(256) void k_jfa_chain(tt_dev d, const int4 *__restr
     const int W = d.W, H = d.H;
     const uint32_t g4 = (uint32_t)(W >> 2);
     if (q >= (uint32_t)(d.N >> 2) / R) return;
-    const int x = (int)(q % g4) * 4;
-    const int chain = (int)(q / g4);
-    const int sy = step % H, sx = step % W;
-    const int y0 = (chain / sy) * (R * sy) + chain % sy;
+    const int x = tt_jmod<P2>((int)q, (int)g4) * 4;
+    const int chain = tt_jdiv<P2>((int)q, (int)g4);
+    const int sy = tt_jmod<P2>(step, H), sx = tt_jmod<P2>(step, W);
+    const int y0 = tt_jdiv<P2>(chain, sy) * (R * sy) + tt_jmod<P2>(chain, sy);
     const int wh = (H << 16) | W;
     const int cols[3] = {tt_wrap1(x - sx, W), x, tt_wrap1(x + sx, W)};
     // rows y0 - s, y0, ..., y0 + R s
     int t[R + 2][3][4];
 #pragma unroll
     for (int j = 0; j < R + 2; ++j) {
-        const int row = tt_wrapg(y0 + (j - 1) * sy, H) * W;
+        const int row = tt_jwrap<P2>(y0 + (j - 1) * sy, H) * W;
 #pragma unroll
         for (int g = 0; g < 3; ++g) {
             const int4 v = in[(row + cols[g]) >> 2];
