@@ -190,10 +190,12 @@ int rebuild(tt_batch *b)
     for (size_t i = 0; i < n; ++i) {
         const tt_world *w = b->worlds[i];
         const uint32_t tag = (uint32_t)i << 20;
-        const int nbx = w->W / TT_TILE_W;
-        for (int t = 0; t < nbx * (w->H / TT_TILE_H); ++t) therm.push_back(tag | (uint32_t)t);
+        const int nbx = w->W / TT_TILE_W;   // at most 256 (8 bits); rows at most 2048 (12 bits)
+        for (int ty = 0; ty < w->H / TT_TILE_H; ++ty)
+            for (int tx = 0; tx < nbx; ++tx) therm.push_back(tag | ((uint32_t)ty << 8) | (uint32_t)tx);
         const int slot = rows_slot(w->rows);
-        for (int t = 0; t < nbx * (w->H / w->rows); ++t) hyd[slot].push_back(tag | (uint32_t)t);
+        for (int ty = 0; ty < w->H / w->rows; ++ty)
+            for (int tx = 0; tx < nbx; ++tx) hyd[slot].push_back(tag | ((uint32_t)ty << 8) | (uint32_t)tx);
     }
     BCKS(upload_table(b, therm, &b->dThermTiles));
     b->nThermTiles = (int)therm.size();

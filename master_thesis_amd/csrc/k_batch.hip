@@ -3,7 +3,8 @@
 // A batch keeps, in device memory, one descriptor per member world (sizes,
 // settings, both buffers of every ping-pong field and the buffer indices at
 // the start of the step's erosion) and a tile table per tile height: entry
-// (world << 20) | tile.  One launch of k_thermal_batch runs a pair of thermal
+// (world << 20) | (tile row << 8) | tile column, so a workgroup finds its tile
+// without dividing by the member's tiles per row.  One launch of k_thermal_batch runs a pair of thermal
 // passes of every member (after one launch of the single first pass of the
 // members with an odd pass count), one launch of k_hydraulic_batch<TH> runs hydraulic
 // sub-iteration k of every member whose tiles have TH rows.  A member whose
@@ -27,8 +28,7 @@ __global__ __launch_bounds__(256) void k_thermal_batch(const tt_bworld *__restri
     const int odd = B.T & 1;
     if (TWO ? j >= B.T / 2 : !odd) return;
     const int l = TWO ? j + odd : 0;
-    const int tile = (int)(e & 0xFFFFFu);
-    const int nbx = B.W / TT_TILE_W;
+    const int tx = (int)(e & 0xFFu), ty = (int)((e >> 8) & 0xFFFu);
     tt_therm_args A;
     A.W = B.W;
     A.H = B.H;
@@ -43,8 +43,8 @@ __global__ __launch_bounds__(256) void k_thermal_batch(const tt_bworld *__restri
     A.pass = l;
     A.bandMagic = 0;
     A.nbxMagic = 0;
-    if constexpr (TWO) tt_thermal_tile2<TT_TILE_H>(A, S, (tile % nbx) * TT_TILE_W, (tile / nbx) * TT_TILE_H);
-    else tt_thermal_tile<TT_TILE_H>(A, S, (tile % nbx) * TT_TILE_W, (tile / nbx) * TT_TILE_H);
+    if constexpr (TWO) tt_thermal_tile2<TT_TILE_H>(A, S, tx * TT_TILE_W, ty * TT_TILE_H);
+    else tt_thermal_tile<TT_TILE_H>(A, S, tx * TT_TILE_W, ty * TT_TILE_H);
 }
 
 template <int TH>
@@ -56,8 +56,7 @@ __global__ __launch_bounds__(64 * tt_hyd_waves<TH>()) void k_hydraulic_batch(con
     const uint32_t e = tiles[tt_xcd_tile(blockIdx.x, gridDim.x)];
     const tt_bworld &B = worlds[e >> 20];
     if (k >= B.K) return;
-    const int tile = (int)(e & 0xFFFFFu);
-    const int nbx = B.W / TT_TILE_W;
+    const int tx = (int)(e & 0xFFu), ty = (int)((e >> 8) & 0xFFFu);
     tt_hyd_args A;
     A.W = B.W;
     A.H = B.H;
@@ -93,7 +92,7 @@ __global__ __launch_bounds__(64 * tt_hyd_waves<TH>()) void k_hydraulic_batch(con
         A.sprev = B.sed[B.sc ^ ((k - 1) & 1)];
         A.sout = B.sed[B.sc ^ (k & 1)];
     }
-    const int x0 = (tile % nbx) * TT_TILE_W, y0 = (tile / nbx) * TH;
+    const int x0 = tx * TT_TILE_W, y0 = ty * TH;
     const bool first = k == 0, last = k == B.K - 1;
     // interior tiles (two-cell halo inside the grid) take the wrap-free body
     const bool in = x0 >= 2 && x0 + TT_TILE_W + 2 <= B.W && y0 >= 2 && y0 + TH + 2 <= B.H;

@@ -29,7 +29,7 @@ struct tt_bworld {
     uint8_t *chg;           // per 64x16 tile and launch parity: the thermal launch changed a cell
 };
 
-// tile table entries: (world index << 20) | tile index within the world
+// tile table entries: (world index << 20) | (tile row << 8) | tile column
 #define TT_BATCH_MAX_WORLDS 4096
 
 // two: pair j of every member's thermal passes; otherwise the single first pass
