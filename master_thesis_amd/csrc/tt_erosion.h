@@ -608,15 +608,15 @@ __device__ __forceinline__ float tt_bilinear(const float *__restrict__ s, float 
     return tt_bilinear_mix(s[t.a], s[t.b], s[t.c], s[t.d], t.wx, t.wy);
 }
 
-// tt_sinf for x >= 0.03 (finite or not): when the reduction's quadrant count
-// is 0 for every lane of the wavefront, the reduced argument is x itself
-// (x - 0 * pi/2 pieces) and tt_sinf is the sine polynomial of x; the other
-// wavefronts take tt_sinf.  Same bits either way.
+// tt_sinf for x >= 0.03 (finite or not): when every lane of the wavefront has
+// x < 0.78, the reduction's quadrant count rint(x * 2/pi) is 0 for every lane
+// (0.78 * 2/pi < 0.497, far from the rounding boundary 0.5), so the reduced
+// argument is x itself (x - 0 * pi/2 pieces) and tt_sinf is the sine
+// polynomial of x; the other wavefronts (NaN included) take tt_sinf.  Same
+// bits either way; the float compare replaces three fp64 operations per cell.
 __device__ __forceinline__ float tt_sinf_wave(float x)
 {
-    const double xd = (double)x;
-    const double k = tt_rint_small(xd * 6.36619772367581382433e-01);
-    if (__builtin_amdgcn_ballot_w64(!(k == 0.0)) == 0) return (float)tt_sin_poly(xd);
+    if (__builtin_amdgcn_ballot_w64(!(x < 0.78f)) == 0) return (float)tt_sin_poly((double)x);
     return tt_sinf(x);
 }
 

@@ -99,3 +99,16 @@ def test_noise_restatement_random(om):
     # simplex stays inside [-1, 1]
     many = om.math_eval(_abi.TT_MATH_SIMPLEX, rng.uniform(0, 5000, 300000).astype(np.float32))
     assert np.abs(many).max() <= 1.0
+
+
+def test_sine_quadrant_count_is_zero_below_the_wavefront_bound():
+    """The hydraulic kernel's wavefront sine takes the bare polynomial when every
+    lane's argument is below 0.78f, which needs rint(x * 2/pi) == 0 there (the
+    reduction then leaves x unchanged).  The double product and rint are both
+    monotone in x, so the largest float below the bound decides it; the bound
+    must keep a margin (pi/4 itself would not)."""
+    two_over_pi = 6.36619772367581382433e-01
+    x_max = float(np.nextafter(np.float32(0.78), np.float32(0.0)))
+    assert np.rint(x_max * two_over_pi) == 0.0
+    assert x_max * two_over_pi < 0.497
+    assert np.rint(float(np.float32(np.pi / 2)) * two_over_pi) == 1.0
